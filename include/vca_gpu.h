@@ -1,0 +1,185 @@
+/*
+ * vca_gpu.h -- C ABI of the MI355X-native VCA solver core (libvca_gpu.so).
+ *
+ * Plain pointers and sizes only.  Complex numbers are complex128 stored as
+ * interleaved (re, im) doubles, arrays with several indices are Fortran
+ * column-major exactly as the reference passes them (so a Fortran caller
+ * passes c_loc(array) of its own arrays).  Every entry point returns a
+ * status code (VCA_OK or a negative error) and never aborts the process;
+ * vca_last_error() gives the message of the last failure on the calling
+ * thread.  Handles are independent: no hidden global solver state.
+ *
+ * Reference interfaces replaced (QcmPlab/VCA):
+ *   vca_init_solver      VCA_MAIN.f90:51-76     (vca_init_solver_serial)
+ *   vca_solve            VCA_MAIN.f90:124-215   (vca_solve_serial)
+ *   vca_get_sft_potential VCA_IO/get_sft_potential.f90:1-4
+ *   vca_set_hcluster     VCA_AUX_FUNX.f90:124-131 (set_Hcluster_nnn)
+ *   vca_set_bath         VCA_BATH_SETUP.f90:112-138
+ *   vca_sector_create /
+ *   vca_sector_destroy   VCA_HAMILTONIAN.f90:38-186 (build_Hv_sector /
+ *                        delete_Hv_sector)
+ *   vca_hxv              VCA_VARS_GLOBAL.f90:83-89 (cc_sparse_HxV operator,
+ *                        bound to directMatVec_main,
+ *                        VCA_HAMILTONIAN_DIRECT_HxV.f90:42-78)
+ *   vca_lanczos_gs       SciFortran sp_lanc_eigh as called at VCA_DIAG.f90:172-184
+ *   vca_lanczos_tridiag  SciFortran sp_lanc_tridiag as called at
+ *                        VCA_GF_NORMAL.f90:218-223
+ *   vca_gf_cluster       VCA_IO/gf_cluster.f90:1-64 (vca_gf_cluster_scalar)
+ *   vca_gf_poles         VCA_VARS_GLOBAL.f90:61-71 (GFmatrix weights/poles)
+ *   vca_omega_integrand  VCA_OMEGA.f90:37-75     (sum_kmesh, batched in omega)
+ */
+#ifndef VCA_GPU_H
+#define VCA_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define VCA_OK 0
+#define VCA_ERR_ARG (-1)    /* bad argument or shape            */
+#define VCA_ERR_HIP (-2)    /* HIP runtime error                */
+#define VCA_ERR_NOMEM (-3)  /* device or host allocation failed */
+#define VCA_ERR_STATE (-4)  /* call out of order                */
+#define VCA_ERR_NOCONV (-5) /* iteration did not converge       */
+
+/* precision of the H x v and Lanczos vectors */
+#define VCA_FP64 0  /* complex128 everywhere (drop-in)                      */
+#define VCA_MIXED 1 /* complex64 vectors and H x v, FP64 alpha/beta, FP64
+                       refinement of the ground state                      */
+
+/* ground-state method, reference LANC_METHOD (VCA_DIAG.f90:98-107) */
+#define VCA_LANC_LANCZOS 0 /* one state per sector                        */
+#define VCA_LANC_ARPACK 1  /* up to lanc_nstates_sector states per sector
+                              (dense sectors only in this build)           */
+
+/* frequency integration of Omega */
+#define VCA_OMEGA_QUAD 0      /* adaptive Gauss-Kronrod on [0,inf), T=0     */
+#define VCA_OMEGA_MATSUBARA 1 /* (2/beta) sum over omega_nfreq Matsubara
+                                 frequencies                               */
+
+/* Input variables of the path (VCA_INPUT_VARS.f90:112-172 names in
+ * comments).  vca_default_params fills the reference defaults. */
+typedef struct vca_params {
+  int32_t nlat;          /* NLAT      */
+  int32_t norb;          /* NORB      */
+  int32_t nspin;         /* NSPIN     */
+  int32_t nlat_bath;     /* NLAT_BATH */
+  int32_t norb_bath;     /* NORB_BATH */
+  int32_t hfmode;        /* HFMODE    */
+  int32_t hfshift;       /* HFSHIFT   */
+  int32_t gf_symmetric;  /* VCA_GF_SYMMETRIC */
+  double uloc[5];        /* ULOC      */
+  double ust;            /* UST       */
+  double jh;             /* JH        */
+  double xmu;            /* XMU       */
+  double beta;           /* BETA      */
+  int32_t lanc_method;   /* LANC_METHOD: VCA_LANC_*            */
+  int32_t lanc_nstates_sector; /* LANC_NSTATES_SECTOR          */
+  int32_t lanc_niter;    /* LANC_NITER                         */
+  int32_t lanc_ngfiter;  /* LANC_NGFITER                       */
+  int32_t lanc_dim_threshold; /* LANC_DIM_THRESHOLD            */
+  double lanc_tolerance; /* LANC_TOLERANCE                     */
+  double gs_threshold;   /* GS_THRESHOLD                       */
+  int32_t precision;     /* VCA_FP64 / VCA_MIXED               */
+  int32_t omega_mode;    /* VCA_OMEGA_*                        */
+  int32_t omega_nfreq;   /* Matsubara frequencies (grid mode)  */
+  double quad_epsabs;    /* quadrature tolerances (quad mode)  */
+  double quad_epsrel;
+  int32_t nsectors;      /* 0: scan every sector; else the list below, the
+                            equivalent of VCA_SECTORS=T with a
+                            sectors_list.restart file (VCA_DIAG.f90:287-319) */
+  const int32_t *sectors; /* nsectors pairs (nup, ndw)                     */
+} vca_params;
+
+/* Per-solve results and timings (seconds) in addition to the SFT potential. */
+typedef struct vca_results {
+  double sft_potential;  /* Omega = omegaprime - omega_integral           */
+  double omegaprime;     /* E0 at T=0                                     */
+  double omega_integral;
+  double e0_mixed;       /* MIXED: E0 before FP64 refinement              */
+  int32_t nstates;       /* ground states kept (zeta function)            */
+  int32_t nchains;       /* excitation chains run                         */
+  int64_t hxv_count;     /* H x v applications (all precisions)           */
+  double hxv_states;     /* sum over applications of the sector dimension */
+  double hxv_seconds;    /* GPU time in H x v kernels (events)            */
+  double t_diag, t_gf, t_omega;
+  int64_t omega_evals;   /* (k, omega) log-det evaluations                */
+  double omega_seconds;  /* GPU time in the k x omega kernels             */
+} vca_results;
+
+typedef struct vca_solver vca_solver;
+typedef struct vca_sector vca_sector;
+
+void vca_default_params(vca_params *p);
+const char *vca_last_error(void);
+
+/* VCA_MAIN.f90:51-76.  Fixes the dimensions; binds device 0. */
+int vca_init_solver(const vca_params *p, vca_solver **out);
+void vca_finalize_solver(vca_solver *s);
+
+/* VCA_AUX_FUNX.f90:124-131: Hloc complex128 [nlat,nlat,nspin,nspin,norb,norb] */
+int vca_set_hcluster(vca_solver *s, const double *hloc);
+/* VCA_BATH_SETUP.f90:112-138: bath_h [nlat_b,nlat_b,nspin,nspin,norb_b,norb_b],
+ * bath_v [nlat,nlat_b,nspin,nspin,norb,norb_b]; applies the hfmode shift. */
+int vca_set_bath(vca_solver *s, const double *bath_h, const double *bath_v);
+
+/* VCA_MAIN.f90:124-215.  hk complex128 [nlat,nlat,nspin,nspin,norb,norb,nk];
+ * bath_h/bath_v may be NULL when the model has no bath. */
+int vca_solve(vca_solver *s, const double *hloc, const double *hk, int64_t nk,
+              const double *bath_h, const double *bath_v);
+/* VCA_IO/get_sft_potential.f90:1-4 */
+int vca_get_sft_potential(const vca_solver *s, double *sft_potential);
+int vca_get_results(const vca_solver *s, vca_results *out);
+
+/* ---- operator level (the cc_sparse_HxV plugin point) ---------------- */
+/* Sector (nup, ndw) of the current Hloc/bath; device tables built on GPU. */
+int vca_sector_create(vca_solver *s, int32_t nup, int32_t ndw,
+                      int32_t precision, vca_sector **out);
+void vca_sector_destroy(vca_sector *sec);
+int vca_sector_dims(const vca_sector *sec, int64_t *dim, int32_t *dimup,
+                    int32_t *dimdw);
+/* Hv = H v, host complex128 arrays of length n = dim (state index
+ * i = iup + idw*DimUp, 0-based).  In MIXED precision the product runs on
+ * complex64 and is widened back. */
+int vca_hxv(vca_sector *sec, int64_t n, const double *v, double *hv);
+/* Same on device buffers already resident in HBM (complex128 for FP64,
+ * complex64 for MIXED sectors), enqueued on the sector's stream;
+ * repeat > 1 applies the product repeat times (for timing).  Returns the
+ * average kernel time in *ms when ms is not NULL. */
+int vca_hxv_device(vca_sector *sec, const void *v, void *hv, int32_t repeat,
+                   float *ms);
+/* One-spin hop table in coordinate form (rows, cols: 0-based string
+ * indices; amps complex128), spin 1 = up, 2 = down.  With rows == NULL only
+ * the count is returned in *nnz.  Connectivity/sign check of the device
+ * tables. */
+int vca_sector_spin_coo(const vca_sector *sec, int32_t spin, int64_t *nnz,
+                        int32_t *rows, int32_t *cols, double *amps);
+/* Plain Lanczos ground state of the sector, FP64 result; vec (host
+ * complex128, length dim) may be NULL. */
+int vca_lanczos_gs(vca_sector *sec, double tol, int32_t niter, double *e0,
+                   double *vec, int32_t *iters);
+/* Lanczos tridiagonalisation from host start vector v0 (need not be
+ * normalised), nlanc steps: alpha[nlanc], beta[nlanc] (beta[0] = 0). */
+int vca_lanczos_tridiag(vca_sector *sec, const double *v0, int32_t nlanc,
+                        double *alpha, double *beta);
+
+/* ---- after vca_solve: Green's function and Omega integrand ---------- */
+/* Lehmann data of element (ilat,jlat,ispin,iorb,jorb) (1-based), state and
+ * channel (1-based): *n poles; weights complex128[n], poles double[n]. */
+int vca_gf_poles(const vca_solver *s, int32_t ilat, int32_t jlat,
+                 int32_t ispin, int32_t iorb, int32_t jorb, int32_t istate,
+                 int32_t ichan, int32_t *n, double *weights, double *poles);
+/* G'(z) -> gf complex128 [nlat,nlat,nspin,nspin,norb,norb] */
+int vca_gf_cluster(const vca_solver *s, double zre, double zim, double *gf);
+/* f(z_m) = (1/Nk) sum_k log|det(1 + (Delta + Hloc - H_k) G'(z_m))| for nz
+ * complex128 frequencies z (imaginary axis or any z off the real axis). */
+int vca_omega_integrand(vca_solver *s, int32_t nz, const double *z,
+                        int32_t precision, double *f);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VCA_GPU_H */

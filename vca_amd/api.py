@@ -1,0 +1,273 @@
+"""Python mirror of the reference's public interface for the solver core.
+
+Reference names (QcmPlab/VCA):
+  vca_init_solver        VCA_MAIN.f90:51-76
+  vca_solve              VCA_MAIN.f90:124-215
+  vca_get_sft_potential  VCA_IO/get_sft_potential.f90:1-4
+  vca_gf_cluster         VCA_IO/gf_cluster.f90:1-64
+  operator (cc_sparse_HxV, spHtimesV_p)  VCA_VARS_GLOBAL.f90:83-89,137
+
+Arrays are complex128 with the reference's shapes and Fortran order:
+Hloc [Nlat,Nlat,Nspin,Nspin,Norb,Norb], Hk [...,Nk], bath_h
+[Nlat_bath,Nlat_bath,Nspin,Nspin,Norb_bath,Norb_bath], bath_v
+[Nlat,Nlat_bath,Nspin,Nspin,Norb,Norb_bath].  Errors raise VcaError where
+the reference would `stop`.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import (VCA_FP64, VCA_MIXED, VCA_LANC_LANCZOS, VCA_LANC_ARPACK,  # noqa: F401
+                   VCA_OMEGA_QUAD, VCA_OMEGA_MATSUBARA, VcaError, check)
+
+
+@dataclass
+class VcaParams:
+    """Input variables of the path (VCA_INPUT_VARS.f90:112-172); the
+    defaults are the reference's."""
+    nlat: int = 1
+    norb: int = 1
+    nspin: int = 1
+    nlat_bath: int = 0
+    norb_bath: int = 0
+    uloc: tuple = (2.0, 0.0, 0.0, 0.0, 0.0)
+    ust: float = 0.0
+    jh: float = 0.0
+    xmu: float = 0.0
+    beta: float = 1000.0
+    hfmode: bool = True
+    hfshift: bool = True
+    gf_symmetric: bool = False
+    lanc_method: str = "arpack"      # "arpack" | "lanczos"
+    lanc_nstates_sector: int = 6
+    lanc_niter: int = 512
+    lanc_ngfiter: int = 200
+    lanc_dim_threshold: int = 256
+    lanc_tolerance: float = 1e-12
+    gs_threshold: float = 1e-9
+    precision: str = "fp64"          # "fp64" | "mixed"
+    omega_mode: str = "quad"         # "quad" | "matsubara"
+    omega_nfreq: int = 4096
+    quad_epsabs: float = 1e-10
+    quad_epsrel: float = 1e-12
+    sectors: list | None = None      # [(nup, ndw), ...] or None for all
+    _keep: list = field(default_factory=list, repr=False)
+
+    @property
+    def ns(self):
+        return self.nlat * self.norb + self.nlat_bath * self.norb_bath
+
+    @property
+    def nlso(self):
+        return self.nlat * self.nspin * self.norb
+
+    def to_c(self) -> _lib.CParams:
+        c = _lib.CParams()
+        _lib.load().vca_default_params(ctypes.byref(c))
+        c.nlat, c.norb, c.nspin = self.nlat, self.norb, self.nspin
+        c.nlat_bath, c.norb_bath = self.nlat_bath, self.norb_bath
+        c.hfmode, c.hfshift = int(self.hfmode), int(self.hfshift)
+        c.gf_symmetric = int(self.gf_symmetric)
+        u = (list(self.uloc) + [0.0] * 5)[:5]
+        for i in range(5):
+            c.uloc[i] = float(u[i])
+        c.ust, c.jh, c.xmu, c.beta = self.ust, self.jh, self.xmu, self.beta
+        c.lanc_method = {"arpack": VCA_LANC_ARPACK, "lanczos": VCA_LANC_LANCZOS}[self.lanc_method]
+        c.lanc_nstates_sector = self.lanc_nstates_sector
+        c.lanc_niter, c.lanc_ngfiter = self.lanc_niter, self.lanc_ngfiter
+        c.lanc_dim_threshold = self.lanc_dim_threshold
+        c.lanc_tolerance, c.gs_threshold = self.lanc_tolerance, self.gs_threshold
+        c.precision = {"fp64": VCA_FP64, "mixed": VCA_MIXED}[self.precision]
+        c.omega_mode = {"quad": VCA_OMEGA_QUAD, "matsubara": VCA_OMEGA_MATSUBARA}[self.omega_mode]
+        c.omega_nfreq = self.omega_nfreq
+        c.quad_epsabs, c.quad_epsrel = self.quad_epsabs, self.quad_epsrel
+        if self.sectors:
+            arr = (ctypes.c_int32 * (2 * len(self.sectors)))(
+                *[int(x) for s in self.sectors for x in s])
+            self._keep = [arr]
+            c.nsectors = len(self.sectors)
+            c.sectors = ctypes.cast(arr, ctypes.POINTER(ctypes.c_int32))
+        return c
+
+
+def _carr(a, shape):
+    a = np.asfortranarray(np.asarray(a, dtype=np.complex128))
+    if a.shape != tuple(shape):
+        raise VcaError(-1, f"wrong shape {a.shape}, expected {tuple(shape)}")
+    return a
+
+
+class Sector:
+    """One (Nup, Ndw) sector with device-resident hop tables: the operator
+    the reference binds to spHtimesV_p (build_Hv_sector/delete_Hv_sector,
+    VCA_HAMILTONIAN.f90:38-186)."""
+
+    def __init__(self, solver: "VcaSolver", nup: int, ndw: int, precision="fp64"):
+        self._lib = _lib.load()
+        self.solver = solver
+        h = ctypes.c_void_p()
+        prec = {"fp64": VCA_FP64, "mixed": VCA_MIXED}[precision]
+        check(self._lib.vca_sector_create(solver._h, nup, ndw, prec, ctypes.byref(h)))
+        self._h = h
+        dim, du, dd = ctypes.c_int64(), ctypes.c_int32(), ctypes.c_int32()
+        check(self._lib.vca_sector_dims(h, ctypes.byref(dim), ctypes.byref(du), ctypes.byref(dd)))
+        self.dim, self.dimup, self.dimdw = dim.value, du.value, dd.value
+
+    def close(self):
+        if self._h:
+            self._lib.vca_sector_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def hxv(self, v):
+        """Hv = H v (cc_sparse_HxV semantics: Nloc = dim)."""
+        v = np.ascontiguousarray(v, dtype=np.complex128)
+        if v.shape != (self.dim,):
+            raise VcaError(-1, f"vector of length {v.size}, sector dimension {self.dim}")
+        hv = np.empty_like(v)
+        check(self._lib.vca_hxv(self._h, self.dim, v.ctypes.data, hv.ctypes.data))
+        return hv
+
+    def hxv_device(self, v_ptr, hv_ptr, repeat=1):
+        ms = ctypes.c_float()
+        check(self._lib.vca_hxv_device(self._h, ctypes.c_void_p(v_ptr), ctypes.c_void_p(hv_ptr),
+                                       repeat, ctypes.byref(ms)))
+        return ms.value
+
+    def spin_coo(self, spin):
+        nnz = ctypes.c_int64()
+        check(self._lib.vca_sector_spin_coo(self._h, spin, ctypes.byref(nnz), None, None, None))
+        n = nnz.value
+        rows = np.zeros(n, np.int32)
+        cols = np.zeros(n, np.int32)
+        amps = np.zeros(n, np.complex128)
+        check(self._lib.vca_sector_spin_coo(self._h, spin, ctypes.byref(nnz), rows.ctypes.data,
+                                            cols.ctypes.data, amps.ctypes.data))
+        return rows, cols, amps
+
+    def lanczos_gs(self, tol=1e-12, niter=512, want_vec=True):
+        e0 = ctypes.c_double()
+        it = ctypes.c_int32()
+        vec = np.zeros(self.dim, np.complex128) if want_vec else None
+        check(self._lib.vca_lanczos_gs(self._h, tol, niter, ctypes.byref(e0),
+                                       vec.ctypes.data if want_vec else None, ctypes.byref(it)))
+        return e0.value, vec, it.value
+
+    def lanczos_tridiag(self, v0, nlanc):
+        v0 = np.ascontiguousarray(v0, dtype=np.complex128)
+        alpha = np.zeros(nlanc)
+        beta = np.zeros(nlanc)
+        check(self._lib.vca_lanczos_tridiag(self._h, v0.ctypes.data, nlanc, alpha.ctypes.data,
+                                            beta.ctypes.data))
+        return alpha, beta
+
+
+class VcaSolver:
+    """Handle returned by vca_init_solver."""
+
+    def __init__(self, params: VcaParams):
+        self._lib = _lib.load()
+        self.params = params
+        self._cp = params.to_c()
+        h = ctypes.c_void_p()
+        check(self._lib.vca_init_solver(ctypes.byref(self._cp), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.vca_finalize_solver(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- shapes ---------------------------------------------------------
+    def _shape_hloc(self):
+        p = self.params
+        return (p.nlat, p.nlat, p.nspin, p.nspin, p.norb, p.norb)
+
+    def _shape_bath(self):
+        p = self.params
+        return ((p.nlat_bath, p.nlat_bath, p.nspin, p.nspin, p.norb_bath, p.norb_bath),
+                (p.nlat, p.nlat_bath, p.nspin, p.nspin, p.norb, p.norb_bath))
+
+    # --- input ----------------------------------------------------------
+    def vca_set_hcluster(self, hloc):
+        a = _carr(hloc, self._shape_hloc())
+        check(self._lib.vca_set_hcluster(self._h, a.ctypes.data))
+
+    def vca_set_bath(self, bath_h, bath_v):
+        sh, sv = self._shape_bath()
+        bh, bv = _carr(bath_h, sh), _carr(bath_v, sv)
+        check(self._lib.vca_set_bath(self._h, bh.ctypes.data, bv.ctypes.data))
+
+    def sector(self, nup, ndw, precision="fp64") -> Sector:
+        return Sector(self, nup, ndw, precision)
+
+    # --- solve ----------------------------------------------------------
+    def vca_solve(self, hloc, hk, bath_h=None, bath_v=None):
+        a = _carr(hloc, self._shape_hloc())
+        hk = np.asfortranarray(np.asarray(hk, dtype=np.complex128))
+        if hk.ndim != 7 or hk.shape[:6] != self._shape_hloc():
+            raise VcaError(-1, "STOP: wrong lattice matrix dimensions")
+        bh = bv = None
+        if bath_h is not None and bath_v is not None:
+            sh, sv = self._shape_bath()
+            bh, bv = _carr(bath_h, sh), _carr(bath_v, sv)
+        check(self._lib.vca_solve(self._h, a.ctypes.data, hk.ctypes.data, hk.shape[6],
+                                  bh.ctypes.data if bh is not None else None,
+                                  bv.ctypes.data if bv is not None else None))
+        return self.vca_get_sft_potential()
+
+    def vca_get_sft_potential(self) -> float:
+        out = ctypes.c_double()
+        check(self._lib.vca_get_sft_potential(self._h, ctypes.byref(out)))
+        return out.value
+
+    def results(self) -> dict:
+        r = _lib.CResults()
+        check(self._lib.vca_get_results(self._h, ctypes.byref(r)))
+        return {k: getattr(r, k) for k, _ in _lib.CResults._fields_}
+
+    # --- Green's function and Omega integrand ---------------------------
+    def gf_poles(self, ilat, jlat, ispin, iorb, jorb, istate, ichan):
+        n = ctypes.c_int32()
+        args = (self._h, ilat, jlat, ispin, iorb, jorb, istate, ichan, ctypes.byref(n))
+        check(self._lib.vca_gf_poles(*args, None, None))
+        w = np.zeros(n.value, np.complex128)
+        p = np.zeros(n.value)
+        check(self._lib.vca_gf_poles(*args, w.ctypes.data, p.ctypes.data))
+        return w, p
+
+    def vca_gf_cluster(self, z):
+        g = np.zeros(self._shape_hloc(), dtype=np.complex128, order="F")
+        check(self._lib.vca_gf_cluster(self._h, float(np.real(z)), float(np.imag(z)),
+                                       g.ctypes.data))
+        return g
+
+    def omega_integrand(self, z, precision="fp64"):
+        z = np.ascontiguousarray(np.atleast_1d(z), dtype=np.complex128)
+        f = np.zeros(z.size)
+        prec = {"fp64": VCA_FP64, "mixed": VCA_MIXED}[precision]
+        check(self._lib.vca_omega_integrand(self._h, z.size, z.ctypes.data, prec, f.ctypes.data))
+        return f
+
+
+def vca_init_solver(params: VcaParams, bath_h=None, bath_v=None) -> VcaSolver:
+    """VCA_MAIN.f90:51-76 (the optional bath arrays only fix shapes there)."""
+    s = VcaSolver(params)
+    if bath_h is not None and bath_v is not None:
+        s._shape_bath()
+    return s

@@ -1,0 +1,677 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the VCA solver core.
+//
+// H x v (reference: directMatVec_main, VCA_HAMILTONIAN_DIRECT_HxV.f90:42-78
+// with HxV_local/HxV_up/HxV_dw, matrix elements of the stored path
+// VCA_HAMILTONIAN/stored/H_cluster_{local,up,dw}.f90):
+//   Hv(iup,idw) = [E_up(iup) + E_dw(idw) + cross(up,dw)] v(iup,idw)
+//               + sum_q H_up(iup, jup_q) v(jup_q, idw)      (column gather)
+//               + sum_q H_dw(idw, jdw_q) v(iup, jdw_q)      (column stream)
+// One workgroup owns one output column idw (or several short ones).  The
+// input column v(:, idw) is staged in LDS when it fits, so the up-spin
+// gathers hit LDS; the down-spin terms read whole neighbouring columns
+// v(:, jdw) with 64-lane coalesced loads, the hop list of the column being
+// wavefront-uniform.  Fermion signs and target ranks come from the tables
+// built by k_build_spin (popcount parity and combinatorial rank).
+//
+// Lanczos step (SciFortran sp_lanc_tridiag / sp_lanc_eigh as called at
+// VCA_GF_NORMAL.f90:218-223 and VCA_DIAG.f90:172-184): vectors are kept
+// unnormalised, u_k = nrm_k v_k; k_hxv computes H v_k = H u_k / nrm_k and
+// alpha_k = <v_k|H v_k> with a deterministic grid reduction finished by the
+// last workgroup; k_lanczos_update forms u_{k+1} = Hv - alpha v_k -
+// beta_k v_{k-1} and beta_{k+1} = |u_{k+1}|.  Step index, alpha, beta and
+// norms live in device memory, so a run of steps needs no host sync.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+
+#include "vca_internal.hpp"
+
+namespace vca {
+
+template <typename R> struct C2;
+template <> struct C2<double> { using T = double2; };
+template <> struct C2<float> { using T = float2; };
+template <typename R> using cplx = typename C2<R>::T;
+
+constexpr int kBlock = 256;
+// dynamic LDS per workgroup for the staged column and the amplitude tables
+constexpr size_t kLdsBudget = 60 * 1024;
+
+// ------------------------------------------------------------ reductions
+__device__ inline double wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block sum, result valid in thread 0.  blockDim.x must be kBlock.
+__device__ inline double block_sum(double v) {
+  __shared__ double s_part[kBlock / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_part[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < kBlock / 64; ++w) r += s_part[w];
+  return r;
+}
+
+// Grid-wide deterministic sum: every block publishes its partial, the last
+// block to arrive sums all partials in block order.  Returns true in thread
+// 0 of the last block, with the total in *total.
+__device__ inline bool grid_sum(double v, double *partials, uint32_t *counter,
+                                double *total) {
+  __shared__ bool s_last;
+  double b = block_sum(v);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = b;
+    __threadfence();
+    uint32_t prev = atomicAdd(counter, 1u);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double s = 0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
+    s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s = block_sum(s);
+  if (threadIdx.x == 0) {
+    *total = s;
+    *counter = 0u;
+    return true;
+  }
+  return false;
+}
+
+// ------------------------------------------------------------ tables ----
+__global__ void k_build_spin(int ns, int npart, int dim, int nterms, const int2 *ab,
+                             const uint32_t *binom, const double *eps,
+                             const double *pair, double e_add, uint32_t *map,
+                             double *energy, int32_t *cnt, uint32_t *ent) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  const uint32_t s = unrank_string(ns, npart, (uint32_t)i, binom);
+  map[i] = s;
+  energy[i] = spin_energy(s, ns, eps, pair) + e_add;
+  cnt[i] = row_hops(s, nterms, ab, binom, ent + i, dim);
+}
+
+// ------------------------------------------------------------ H x v -----
+struct HxvArgs {
+  const void *v0, *v1;  // input (Lanczos: u_k in buffer k&1)
+  void *hv;
+  int dimup, dimdw, cpb;
+  const uint32_t *map_up, *map_dw;
+  const double *e_up, *e_dw;
+  const int32_t *cnt_up, *cnt_dw;
+  const uint32_t *ent_up, *ent_dw;
+  const void *amp_up, *amp_dw;
+  int namp_up, namp_dw;
+  CrossTerm cross;
+  LanczosDev *lz;  // null: plain product
+  double *partials;
+};
+
+__device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t sd) {
+  if (c.norb == 1) return c.wx[0][0] * (double)__builtin_popcount(su & sd & c.site_mask);
+  double e = 0;
+  for (int io = 0; io < c.norb; ++io)
+    for (int jo = 0; jo < c.norb; ++jo)
+      e += c.wx[io][jo] * (double)__builtin_popcount((su >> io) & (sd >> jo) & c.site_mask);
+  return e;
+}
+
+template <typename R, bool REAL_AMP>
+__device__ inline void cmac(R &ar, R &ai, const cplx<R> &h, const cplx<R> &y) {
+  if (REAL_AMP) {
+    ar = fma(h.x, y.x, ar);
+    ai = fma(h.x, y.y, ai);
+  } else {
+    ar = fma(h.x, y.x, ar);
+    ar = fma(-h.y, y.y, ar);
+    ai = fma(h.x, y.y, ai);
+    ai = fma(h.y, y.x, ai);
+  }
+}
+
+template <typename R, bool REAL_AMP, bool STAGE, bool ONECOL>
+__global__ __launch_bounds__(kBlock) void k_hxv(const HxvArgs a) {
+  using C = cplx<R>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  C *s_amp_up = reinterpret_cast<C *>(smem);
+  C *s_amp_dw = s_amp_up + a.namp_up;
+  C *s_col = s_amp_dw + a.namp_dw;
+
+  const C *g_amp_up = static_cast<const C *>(a.amp_up);
+  const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
+  for (int t = threadIdx.x; t < a.namp_up; t += kBlock) s_amp_up[t] = g_amp_up[t];
+  for (int t = threadIdx.x; t < a.namp_dw; t += kBlock) s_amp_dw[t] = g_amp_dw[t];
+
+  int k = 0;
+  bool done = false;
+  R scale = (R)1;
+  const C *vin = static_cast<const C *>(a.v0);
+  if (a.lz) {
+    k = a.lz->k;
+    done = a.lz->done != 0;
+    if (k & 1) vin = static_cast<const C *>(a.v1);
+    if (!done) scale = (R)(1.0 / a.lz->nrm[k]);
+  }
+  C *hv = static_cast<C *>(a.hv);
+  const int dimup = a.dimup;
+  const int64_t col0 = (int64_t)blockIdx.x * a.cpb;
+  const int ncol = (int)min((int64_t)a.cpb, (int64_t)a.dimdw - col0);
+  const int nel = ncol * dimup;
+  const C *vcol = vin + col0 * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    if (STAGE)
+      for (int f = threadIdx.x; f < nel; f += kBlock) s_col[f] = vcol[f];
+    __syncthreads();
+    for (int f = threadIdx.x; f < nel; f += kBlock) {
+      int c, iup;
+      if (ONECOL) {
+        c = 0;
+        iup = f;
+      } else {
+        c = f / dimup;
+        iup = f - c * dimup;
+      }
+      const int idw = (int)col0 + c;
+      const C x = STAGE ? s_col[f] : vcol[f];
+      const double ed = a.e_up[iup] + a.e_dw[idw] +
+                        cross_energy(a.cross, a.map_up[iup], a.map_dw[idw]);
+      R ar = (R)ed * x.x, ai = (R)ed * x.y;
+      // up-spin hops: gather inside the column
+      const int nu = a.cnt_up[iup];
+      const C *colp = STAGE ? (s_col + c * dimup) : (vcol + (int64_t)c * dimup);
+      for (int q = 0; q < nu; ++q) {
+        const uint32_t e = a.ent_up[(int64_t)q * dimup + iup];
+        const C y = colp[e & kColMask];
+        cmac<R, REAL_AMP>(ar, ai, s_amp_up[e >> kColBits], y);
+      }
+      // down-spin hops: whole neighbouring columns, coalesced in iup
+      const int nd = a.cnt_dw[idw];
+      for (int q = 0; q < nd; ++q) {
+        const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
+        const C y = vin[(int64_t)(e & kColMask) * dimup + iup];
+        cmac<R, REAL_AMP>(ar, ai, s_amp_dw[e >> kColBits], y);
+      }
+      C out;
+      out.x = ar * scale;
+      out.y = ai * scale;
+      hv[col0 * dimup + f] = out;
+      if (a.lz) dot += (double)(x.x * scale) * (double)out.x + (double)(x.y * scale) * (double)out.y;
+    }
+  }
+  if (a.lz) {
+    double tot;
+    if (grid_sum(dot, a.partials, &a.lz->counter[0], &tot)) a.lz->alpha[k] = done ? 0.0 : tot;
+  }
+}
+
+// ------------------------------------------------------ Lanczos update --
+struct UpdArgs {
+  void *u0, *u1;  // u_k in buffer k&1, u_{k-1} (and u_{k+1}) in the other
+  const void *hv;
+  void *x;        // eigenvector accumulator or null
+  int64_t n;
+  LanczosDev *lz;
+  double *partials;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_lanczos_update(const UpdArgs a) {
+  using C = cplx<R>;
+  LanczosDev *lz = a.lz;
+  const int k = lz->k;
+  const bool done = lz->done != 0;
+  double ss = 0.0;
+  if (!done) {
+    const C *uk = static_cast<const C *>((k & 1) ? a.u1 : a.u0);
+    C *up = static_cast<C *>((k & 1) ? a.u0 : a.u1);
+    const C *hv = static_cast<const C *>(a.hv);
+    const R ca = (R)(lz->alpha[k] / lz->nrm[k]);
+    const R cb = k > 0 ? (R)(lz->beta[k] / lz->nrm[k - 1]) : (R)0;
+    C *x = static_cast<C *>(a.x);
+    const R cy = x ? (R)(lz->y[k] / lz->nrm[k]) : (R)0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
+      const C u = uk[i], h = hv[i];
+      C w;
+      if (k > 0) {
+        const C p = up[i];
+        w.x = h.x - ca * u.x - cb * p.x;
+        w.y = h.y - ca * u.y - cb * p.y;
+      } else {
+        w.x = h.x - ca * u.x;
+        w.y = h.y - ca * u.y;
+      }
+      up[i] = w;
+      ss += (double)w.x * (double)w.x + (double)w.y * (double)w.y;
+      if (x) {
+        C xv = x[i];
+        xv.x += cy * u.x;
+        xv.y += cy * u.y;
+        x[i] = xv;
+      }
+    }
+  }
+  double tot;
+  if (grid_sum(ss, a.partials, &lz->counter[1], &tot)) {
+    if (!done) {
+      const double b = sqrt(tot);
+      if (!(b > 1e-300)) {  // exact breakdown (or NaN)
+        lz->beta[k + 1] = 0.0;
+        lz->nrm[k + 1] = 1.0;
+        lz->done = 1;
+      } else {
+        lz->beta[k + 1] = b;
+        lz->nrm[k + 1] = b;
+      }
+    } else {
+      lz->beta[k + 1] = 0.0;
+      lz->nrm[k + 1] = 1.0;
+    }
+    lz->k = k + 1;
+  }
+}
+
+// ------------------------------------------------------ vector helpers --
+// splitmix64 hash of (seed, i) -> [-1, 1), the start vector of the GS
+// Lanczos (same numbers as the oracle's start_vector before normalisation).
+__device__ inline double hash_unit(uint64_t seed, uint64_t i) {
+  const uint64_t g = 0x9E3779B97F4A7C15ull;
+  uint64_t z = seed * g + (i + 1ull) * g;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_start(void *u, int64_t n, uint64_t seed) {
+  using C = cplx<R>;
+  C *p = static_cast<C *>(u);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    C c;
+    c.x = (R)hash_unit(seed, (uint64_t)i);
+    c.y = (R)0;
+    p[i] = c;
+  }
+}
+
+// nrm[0] = |u|, k = 0, done = 0 (resets a Lanczos run on buffer 0).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_norm0(const void *u, int64_t n, LanczosDev *lz,
+                                                  double *partials) {
+  using C = cplx<R>;
+  const C *p = static_cast<const C *>(u);
+  double ss = 0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const C c = p[i];
+    ss += (double)c.x * c.x + (double)c.y * c.y;
+  }
+  double tot;
+  if (grid_sum(ss, partials, &lz->counter[2], &tot)) {
+    lz->nrm[0] = sqrt(tot);
+    lz->beta[0] = 0.0;
+    lz->k = 0;
+    lz->done = 0;
+  }
+}
+
+// x /= |x|
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_normalize(void *u, int64_t n, LanczosDev *lz,
+                                                      double *partials) {
+  using C = cplx<R>;
+  C *p = static_cast<C *>(u);
+  double ss = 0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const C c = p[i];
+    ss += (double)c.x * c.x + (double)c.y * c.y;
+  }
+  // the last block publishes the norm in y[0]; k_scale_by then divides
+  double tot;
+  if (grid_sum(ss, partials, &lz->counter[3], &tot)) lz->y[0] = sqrt(tot);
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_scale_by(void *u, int64_t n, const double *den) {
+  using C = cplx<R>;
+  C *p = static_cast<C *>(u);
+  const R s = (R)(1.0 / *den);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    C c = p[i];
+    c.x *= s;
+    c.y *= s;
+    p[i] = c;
+  }
+}
+
+template <typename RS, typename RD>
+__global__ __launch_bounds__(kBlock) void k_convert(const void *src, void *dst, int64_t n) {
+  const cplx<RS> *s = static_cast<const cplx<RS> *>(src);
+  cplx<RD> *d = static_cast<cplx<RD> *>(dst);
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    cplx<RD> c;
+    c.x = (RD)s[i].x;
+    c.y = (RD)s[i].y;
+    d[i] = c;
+  }
+}
+
+// --------------------------------------------------- excitation vectors --
+// Gather form of O|psi> (VCA_GF_NORMAL.f90:184-198 and 624-660): for every
+// target state j of sector J and every operator of the channel (all on the
+// same spin), the source string is the target string with the operator
+// undone; the sign is the parity of occupied orbitals below the position
+// in the source string (cdg/c of VCA_SETUP.f90:724-750).  Each target has
+// at most one source per operator, so no atomics are needed.
+struct ExcArgs {
+  void *out;
+  const void *psi;
+  int dimup_j, dimdw_j, dimup_i, dimdw_i;
+  const uint32_t *map_j;  // the J-sector map of the acted-on spin
+  const uint32_t *binom;
+  int nops, spin;
+  int pos[2], dagger[2];
+  double cre[2], cim[2];
+  LanczosDev *lz;
+  double *partials;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
+  using C = cplx<R>;
+  const C *psi = static_cast<const C *>(a.psi);
+  C *out = static_cast<C *>(a.out);
+  const int64_t n = (int64_t)a.dimup_j * a.dimdw_j;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  double ss = 0;
+  for (int64_t j = (int64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
+    const int jup = (int)(j % a.dimup_j), jdw = (int)(j / a.dimup_j);
+    const uint32_t t = a.map_j[a.spin == 1 ? jup : jdw];
+    double vr = 0, vi = 0;
+    for (int o = 0; o < a.nops; ++o) {
+      const int pos = a.pos[o];
+      const uint32_t bit = 1u << pos;
+      uint32_t s;
+      if (a.dagger[o]) {
+        if (!(t & bit)) continue;
+        s = t & ~bit;
+      } else {
+        if (t & bit) continue;
+        s = t | bit;
+      }
+      const double sg = parity_below(s, pos) ? -1.0 : 1.0;
+      const uint32_t r = rank_string(s, a.binom);
+      const int64_t i = a.spin == 1 ? (int64_t)r + (int64_t)jdw * a.dimup_i
+                                    : (int64_t)jup + (int64_t)r * a.dimup_i;
+      const C p = psi[i];
+      // coef * sg * psi
+      vr += sg * (a.cre[o] * (double)p.x - a.cim[o] * (double)p.y);
+      vi += sg * (a.cre[o] * (double)p.y + a.cim[o] * (double)p.x);
+    }
+    C c;
+    c.x = (R)vr;
+    c.y = (R)vi;
+    out[j] = c;
+    ss += (double)c.x * c.x + (double)c.y * c.y;
+  }
+  double tot;
+  if (grid_sum(ss, a.partials, &a.lz->counter[2], &tot)) {
+    a.lz->nrm[0] = sqrt(tot);
+    a.lz->beta[0] = 0.0;
+    a.lz->k = 0;
+    a.lz->done = 0;
+  }
+}
+
+// ================================================== host launch wrappers
+static int grid_for(int64_t n) {
+  int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
+  if (g < 1) g = 1;
+  if (g > 4096) g = 4096;
+  return (int)g;
+}
+
+void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int npart,
+                       const uint32_t *d_binom, double e_add) {
+  Binom bn;
+  sp.npart = npart;
+  sp.dim = (int)bn(ns, npart);
+  sp.nterms = (int)m.terms.size();
+  sp.real_amp = m.real_amp;
+  const int bound = npart * (ns - npart);
+  sp.maxnnz = std::max(1, std::min(sp.nterms, bound));
+  const hipStream_t st = sec.stream;
+  std::vector<int2> ab(std::max(1, sp.nterms));
+  std::vector<double2> amp64(2 * std::max(1, sp.nterms));
+  std::vector<float2> amp32(2 * std::max(1, sp.nterms));
+  for (int t = 0; t < sp.nterms; ++t) {
+    ab[t] = make_int2(m.terms[t].a, m.terms[t].b);
+    amp64[2 * t] = make_double2(m.terms[t].re, m.terms[t].im);
+    amp64[2 * t + 1] = make_double2(-m.terms[t].re, -m.terms[t].im);
+    amp32[2 * t] = make_float2((float)m.terms[t].re, (float)m.terms[t].im);
+    amp32[2 * t + 1] = make_float2(-(float)m.terms[t].re, -(float)m.terms[t].im);
+  }
+  std::vector<double> pair((size_t)kMaxNs * kMaxNs);
+  for (int a = 0; a < kMaxNs; ++a)
+    for (int b = 0; b < kMaxNs; ++b) pair[(size_t)a * kMaxNs + b] = m.pair[a][b];
+  int2 *d_ab;
+  double *d_eps, *d_pair;
+  VCA_HIP(hipMalloc(&sp.map, sizeof(uint32_t) * sp.dim));
+  VCA_HIP(hipMalloc(&sp.energy, sizeof(double) * sp.dim));
+  VCA_HIP(hipMalloc(&sp.cnt, sizeof(int32_t) * sp.dim));
+  VCA_HIP(hipMalloc(&sp.ent, sizeof(uint32_t) * (size_t)sp.dim * sp.maxnnz));
+  VCA_HIP(hipMalloc(&sp.amp64, sizeof(double2) * amp64.size()));
+  VCA_HIP(hipMalloc(&sp.amp32, sizeof(float2) * amp32.size()));
+  VCA_HIP(hipMalloc(&d_ab, sizeof(int2) * ab.size()));
+  VCA_HIP(hipMalloc(&d_eps, sizeof(double) * kMaxNs));
+  VCA_HIP(hipMalloc(&d_pair, sizeof(double) * pair.size()));
+  VCA_HIP(hipMemcpyAsync(sp.amp64, amp64.data(), sizeof(double2) * amp64.size(),
+                         hipMemcpyHostToDevice, st));
+  VCA_HIP(hipMemcpyAsync(sp.amp32, amp32.data(), sizeof(float2) * amp32.size(),
+                         hipMemcpyHostToDevice, st));
+  VCA_HIP(hipMemcpyAsync(d_ab, ab.data(), sizeof(int2) * ab.size(), hipMemcpyHostToDevice, st));
+  VCA_HIP(hipMemcpyAsync(d_eps, m.eps, sizeof(double) * kMaxNs, hipMemcpyHostToDevice, st));
+  VCA_HIP(hipMemcpyAsync(d_pair, pair.data(), sizeof(double) * pair.size(),
+                         hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_build_spin, dim3((sp.dim + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     ns, npart, sp.dim, sp.nterms, d_ab, d_binom, d_eps, d_pair, e_add,
+                     sp.map, sp.energy, sp.cnt, sp.ent);
+  VCA_HIP(hipGetLastError());
+  VCA_HIP(hipStreamSynchronize(st));
+  VCA_HIP(hipFree(d_ab));
+  VCA_HIP(hipFree(d_eps));
+  VCA_HIP(hipFree(d_pair));
+}
+
+static void ensure_partials(Sector &sec, int nblocks) {
+  if (sec.partials_cap >= nblocks) return;
+  if (sec.partials) VCA_HIP(hipFree(sec.partials));
+  const int cap = std::max(nblocks, 4096);
+  VCA_HIP(hipMalloc(&sec.partials, sizeof(double) * cap));
+  sec.partials_cap = cap;
+}
+
+template <typename R, bool REAL_AMP>
+static void launch_hxv_t(Sector &sec, HxvArgs &a, bool stage, bool onecol, int grid,
+                         size_t smem) {
+  const hipStream_t st = sec.stream;
+  if (stage) {
+    if (onecol)
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, true, true>), dim3(grid), dim3(kBlock), smem, st, a);
+    else
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, true, false>), dim3(grid), dim3(kBlock), smem, st, a);
+  } else {
+    if (onecol)
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, false, true>), dim3(grid), dim3(kBlock), smem, st, a);
+    else
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, false, false>), dim3(grid), dim3(kBlock), smem, st, a);
+  }
+}
+
+void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos) {
+  HxvArgs a{};
+  a.v0 = v0;
+  a.v1 = v1;
+  a.hv = hv;
+  a.dimup = sec.up.dim;
+  a.dimdw = sec.dw.dim;
+  a.map_up = sec.up.map;
+  a.map_dw = sec.dw.map;
+  a.e_up = sec.up.energy;
+  a.e_dw = sec.dw.energy;
+  a.cnt_up = sec.up.cnt;
+  a.cnt_dw = sec.dw.cnt;
+  a.ent_up = sec.up.ent;
+  a.ent_dw = sec.dw.ent;
+  const bool fp64 = sec.precision == VCA_FP64;
+  a.amp_up = fp64 ? (const void *)sec.up.amp64 : (const void *)sec.up.amp32;
+  a.amp_dw = fp64 ? (const void *)sec.dw.amp64 : (const void *)sec.dw.amp32;
+  a.namp_up = 2 * sec.up.nterms;
+  a.namp_dw = 2 * sec.dw.nterms;
+  a.cross = sector_cross(sec);
+  a.lz = lanczos ? sec.lz : nullptr;
+  const size_t esz = fp64 ? sizeof(double2) : sizeof(float2);
+  // columns per block: fill 256 lanes when columns are short
+  int cpb = 1;
+  if (a.dimup < kBlock) cpb = std::max(1, kBlock / a.dimup);
+  a.cpb = cpb;
+  const int grid = (int)((a.dimdw + cpb - 1) / cpb);
+  const size_t amp_bytes = (size_t)(a.namp_up + a.namp_dw) * esz;
+  const size_t col_bytes = (size_t)cpb * a.dimup * esz;
+  const bool stage = amp_bytes + col_bytes <= kLdsBudget;
+  const size_t smem = amp_bytes + (stage ? col_bytes : 0);
+  if (lanczos) ensure_partials(sec, grid);
+  a.partials = sec.partials;
+  const bool real_amp = sec.up.real_amp && sec.dw.real_amp;
+  const bool onecol = cpb == 1;
+  if (fp64) {
+    if (real_amp) launch_hxv_t<double, true>(sec, a, stage, onecol, grid, smem);
+    else launch_hxv_t<double, false>(sec, a, stage, onecol, grid, smem);
+  } else {
+    if (real_amp) launch_hxv_t<float, true>(sec, a, stage, onecol, grid, smem);
+    else launch_hxv_t<float, false>(sec, a, stage, onecol, grid, smem);
+  }
+  VCA_HIP(hipGetLastError());
+}
+
+void launch_lanczos_update(Sector &sec, bool accumulate_x) {
+  UpdArgs a{};
+  a.u0 = sec.buf[0];
+  a.u1 = sec.buf[1];
+  a.hv = sec.buf[2];
+  a.x = accumulate_x ? sec.xvec : nullptr;
+  a.n = sec.dim;
+  a.lz = sec.lz;
+  const int grid = grid_for(sec.dim);
+  ensure_partials(sec, grid);
+  a.partials = sec.partials;
+  if (sec.precision == VCA_FP64)
+    hipLaunchKernelGGL(k_lanczos_update<double>, dim3(grid), dim3(kBlock), 0, sec.stream, a);
+  else
+    hipLaunchKernelGGL(k_lanczos_update<float>, dim3(grid), dim3(kBlock), 0, sec.stream, a);
+  VCA_HIP(hipGetLastError());
+}
+
+void launch_start_vector(Sector &sec, uint64_t seed) {
+  const int grid = grid_for(sec.dim);
+  if (sec.precision == VCA_FP64)
+    hipLaunchKernelGGL(k_start<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
+                       sec.dim, seed);
+  else
+    hipLaunchKernelGGL(k_start<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
+                       sec.dim, seed);
+  VCA_HIP(hipGetLastError());
+}
+
+void launch_norm0(Sector &sec) {
+  const int grid = grid_for(sec.dim);
+  ensure_partials(sec, grid);
+  if (sec.precision == VCA_FP64)
+    hipLaunchKernelGGL(k_norm0<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
+                       sec.dim, sec.lz, sec.partials);
+  else
+    hipLaunchKernelGGL(k_norm0<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
+                       sec.dim, sec.lz, sec.partials);
+  VCA_HIP(hipGetLastError());
+}
+
+void launch_scale_x(Sector &sec) {
+  const int grid = grid_for(sec.dim);
+  ensure_partials(sec, grid);
+  // the norm lands in lz->y[0] (y is no longer needed after pass 2)
+  const double *den = sec.lzh.y;
+  if (sec.precision == VCA_FP64) {
+    hipLaunchKernelGGL(k_normalize<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
+                       sec.dim, sec.lz, sec.partials);
+    hipLaunchKernelGGL(k_scale_by<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
+                       sec.dim, den);
+  } else {
+    hipLaunchKernelGGL(k_normalize<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
+                       sec.dim, sec.lz, sec.partials);
+    hipLaunchKernelGGL(k_scale_by<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
+                       sec.dim, den);
+  }
+  VCA_HIP(hipGetLastError());
+}
+
+void launch_convert(const void *src, int src_prec, void *dst, int dst_prec, int64_t n,
+                    hipStream_t st) {
+  const int grid = grid_for(n);
+  if (src_prec == VCA_FP64 && dst_prec == VCA_FP64)
+    VCA_HIP(hipMemcpyAsync(dst, src, sizeof(double2) * n, hipMemcpyDeviceToDevice, st));
+  else if (src_prec == VCA_FP64)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid), dim3(kBlock), 0, st, src, dst, n);
+  else if (dst_prec == VCA_FP64)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid), dim3(kBlock), 0, st, src, dst, n);
+  else
+    VCA_HIP(hipMemcpyAsync(dst, src, sizeof(float2) * n, hipMemcpyDeviceToDevice, st));
+  VCA_HIP(hipGetLastError());
+}
+
+void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int nops,
+                   const int *spin, const int *pos, const int *dagger, const double *coef_re,
+                   const double *coef_im) {
+  ExcArgs a{};
+  a.out = jsec.buf[0];
+  a.psi = psi;
+  a.dimup_j = jsec.up.dim;
+  a.dimdw_j = jsec.dw.dim;
+  a.dimup_i = dimup_i;
+  a.dimdw_i = dimdw_i;
+  a.spin = spin[0];
+  a.map_j = a.spin == 1 ? jsec.up.map : jsec.dw.map;
+  a.binom = solver_binom(jsec.solver);
+  a.nops = nops;
+  for (int o = 0; o < nops; ++o) {
+    a.pos[o] = pos[o];
+    a.dagger[o] = dagger[o];
+    a.cre[o] = coef_re[o];
+    a.cim[o] = coef_im[o];
+  }
+  a.lz = jsec.lz;
+  const int grid = grid_for(jsec.dim);
+  ensure_partials(jsec, grid);
+  a.partials = jsec.partials;
+  if (jsec.precision == VCA_FP64)
+    hipLaunchKernelGGL(k_excite<double>, dim3(grid), dim3(kBlock), 0, jsec.stream, a);
+  else
+    hipLaunchKernelGGL(k_excite<float>, dim3(grid), dim3(kBlock), 0, jsec.stream, a);
+  VCA_HIP(hipGetLastError());
+}
+
+}  // namespace vca

@@ -1,0 +1,497 @@
+// omega.hip -- the Omega integrand on the GPU (VCA_OMEGA.f90:37-75).
+//
+//   f(z) = (1/Nk) sum_k log|det(1 + (Delta(z) + Hloc - H_k) G'(z))|
+//
+// is evaluated through the identity
+//   det(1 + A_k G') = det(G') det(G'^{-1} + A_k),
+// so that per frequency one small inverse gives B(z) = G'^{-1} + Delta + Hloc
+// and per (k, z) only an LU of B(z) - H_k remains (no n^3 product per k).
+// When Hloc, H_k and the bath are spin-diagonal, the determinant further
+// factorises over spin blocks of size Nlat*Norb (G' is spin-diagonal in the
+// reference, VCA_IO/gf_cluster.f90:23-49).
+//
+// Kernels:
+//   k_pole_sum   G_raw(z, e) = sum_p w_p / (z - p_p) over the Lehmann poles
+//                of element e (all states and channels),
+//                VCA_IO/gf_cluster.f90:23-49
+//   k_prep       recombination G_ij = (G^mix - (1 - i chan4)(G_ii + G_jj))/2
+//                (gf_cluster.f90:50-62), Delta(z) = V (z+mu-h)^-1 V^+
+//                (VCA_BATH_FUNCTIONS.f90:127-150), G'^{-1}, log|det G'|,
+//                B(z); one workgroup per (z, spin block), Gauss-Jordan in LDS
+//   k_logdet     per (k, z): LU with partial pivoting of B(z) - H_k, one
+//                matrix per 16-lane group (row per lane, pivot row broadcast
+//                by lane permutes), H_k rows held in registers across a
+//                chunk of frequencies; FP64 log accumulation
+//   k_reduce_z   deterministic sum of the k-block partials per z
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "omega.hpp"
+
+namespace vca {
+
+template <typename R> struct OC2;
+template <> struct OC2<double> { using T = double2; };
+template <> struct OC2<float> { using T = float2; };
+template <typename R> using ocplx = typename OC2<R>::T;
+
+constexpr int kOBlock = 256;
+constexpr int kNB = 16;  // padded block size of the LU kernel
+
+// ----------------------------------------------------------- pole sums --
+__global__ __launch_bounds__(kOBlock) void k_pole_sum(int nz, const double2 *z, int nelem,
+                                                      const int64_t *off, const double2 *w,
+                                                      const double *pole, double2 *graw) {
+  const int64_t t = (int64_t)blockIdx.x * kOBlock + threadIdx.x;
+  if (t >= (int64_t)nz * nelem) return;
+  const int e = (int)(t % nelem), iz = (int)(t / nelem);
+  const double2 zz = z[iz];
+  double gr = 0, gi = 0;
+  for (int64_t p = off[e]; p < off[e + 1]; ++p) {
+    // w / (z - p)
+    const double dr = zz.x - pole[p], di = zz.y;
+    const double den = 1.0 / (dr * dr + di * di);
+    const double2 ww = w[p];
+    gr += (ww.x * dr + ww.y * di) * den;
+    gi += (ww.y * dr - ww.x * di) * den;
+  }
+  graw[t] = make_double2(gr, gi);
+}
+
+// -------------------------------------------------------- per-z prep ----
+// Gauss-Jordan inverse with partial pivoting of the n x n complex matrix in
+// LDS (row-major a[r*ld + c]); returns log|det| (valid in all threads).
+__device__ double gj_inverse(int n, double2 *a, double2 *inv, int ld, double *s_logdet,
+                             int *s_piv) {
+  const int tid = threadIdx.x;
+  for (int e = tid; e < n * n; e += blockDim.x) {
+    const int r = e / n, c = e % n;
+    inv[r * ld + c] = make_double2(r == c ? 1.0 : 0.0, 0.0);
+  }
+  if (tid == 0) *s_logdet = 0.0;
+  __syncthreads();
+  for (int k = 0; k < n; ++k) {
+    if (tid == 0) {
+      int p = k;
+      double best = -1.0;
+      for (int r = k; r < n; ++r) {
+        const double2 v = a[r * ld + k];
+        const double m = v.x * v.x + v.y * v.y;
+        if (m > best) {
+          best = m;
+          p = r;
+        }
+      }
+      *s_piv = p;
+      *s_logdet += 0.5 * log(best);
+    }
+    __syncthreads();
+    const int p = *s_piv;
+    if (p != k)
+      for (int c = tid; c < n; c += blockDim.x) {
+        double2 t1 = a[k * ld + c];
+        a[k * ld + c] = a[p * ld + c];
+        a[p * ld + c] = t1;
+        double2 t2 = inv[k * ld + c];
+        inv[k * ld + c] = inv[p * ld + c];
+        inv[p * ld + c] = t2;
+      }
+    __syncthreads();
+    const double2 piv = a[k * ld + k];
+    const double dd = 1.0 / (piv.x * piv.x + piv.y * piv.y);
+    const double2 pinv = make_double2(piv.x * dd, -piv.y * dd);
+    __syncthreads();
+    for (int c = tid; c < n; c += blockDim.x) {
+      const double2 x = a[k * ld + c], y = inv[k * ld + c];
+      a[k * ld + c] = make_double2(x.x * pinv.x - x.y * pinv.y, x.x * pinv.y + x.y * pinv.x);
+      inv[k * ld + c] = make_double2(y.x * pinv.x - y.y * pinv.y, y.x * pinv.y + y.y * pinv.x);
+    }
+    __syncthreads();
+    for (int e = tid; e < n * n; e += blockDim.x) {
+      const int r = e / n, c = e % n;
+      if (r == k) continue;
+      const double2 f = a[r * ld + k];
+      const double2 x = a[k * ld + c], y = inv[k * ld + c];
+      // defer the column-k update of a until all threads read f
+      a[r * ld + c] = (c == k) ? a[r * ld + c]
+                               : make_double2(a[r * ld + c].x - (f.x * x.x - f.y * x.y),
+                                              a[r * ld + c].y - (f.x * x.y + f.y * x.x));
+      inv[r * ld + c] = make_double2(inv[r * ld + c].x - (f.x * y.x - f.y * y.y),
+                                     inv[r * ld + c].y - (f.x * y.y + f.y * y.x));
+    }
+    __syncthreads();
+    for (int r = tid; r < n; r += blockDim.x)
+      if (r != k) a[r * ld + k] = make_double2(0.0, 0.0);
+    __syncthreads();
+  }
+  return *s_logdet;
+}
+
+struct PrepArgs {
+  int nz, nspin_blk, nb, nlat, norb, nspin_gf;
+  double chan4, xmu;
+  const double2 *z;
+  const double2 *graw;  // [nz][nelem], element order (ispin, ilat, jlat, iorb, jorb)
+  int nelem;
+  const double2 *hloc_blk;  // [nspin_blk][nb][nb] row-major
+  int nbb;                  // bath block size (0: no bath)
+  const double2 *hb_blk;    // [nspin_blk][nbb][nbb]
+  const double2 *vb_blk;    // [nspin_blk][nb][nbb]
+  double2 *bmat;            // out [nz][nspin_blk][nb][nb] row-major
+  double *logdet_g;         // out [nz][nspin_blk]
+};
+
+__global__ __launch_bounds__(64) void k_prep(const PrepArgs a) {
+  __shared__ double2 s_g[kNB * kNB], s_inv[kNB * kNB];
+  __shared__ double2 s_hb[kNB * kNB], s_hbinv[kNB * kNB];
+  __shared__ double s_ld;
+  __shared__ int s_piv;
+  const int iz = blockIdx.x, sb = blockIdx.y, tid = threadIdx.x;
+  const int nb = a.nb, norb = a.norb, nlat = a.nlat;
+  const double2 zz = a.z[iz];
+  const double2 *g = a.graw + (int64_t)iz * a.nelem;
+  // G' of the block, lso order inside the block: i = iorb + ilat*Norb
+  // (+ spin offset when one block holds both spins)
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e / nb, c = e % nb;
+    const int ls = nlat * norb;
+    const int sr = (a.nspin_blk == 1 && a.nspin_gf == 2) ? r / ls : sb;
+    const int sc = (a.nspin_blk == 1 && a.nspin_gf == 2) ? c / ls : sb;
+    double2 v = make_double2(0.0, 0.0);
+    if (sr == sc) {
+      const int rr = r % ls, cc = c % ls;
+      const int il = rr / norb, io = rr % norb, jl = cc / norb, jo = cc % norb;
+      auto elem = [&](int i1, int j1, int o1, int o2) {
+        return g[(((sr * nlat + i1) * nlat + j1) * norb + o1) * norb + o2];
+      };
+      v = elem(il, jl, io, jo);
+      if (!(il == jl && io == jo)) {
+        const double2 gii = elem(il, il, io, io), gjj = elem(jl, jl, jo, jo);
+        // (1 - i chan4) (gii + gjj)
+        const double sx = gii.x + gjj.x, sy = gii.y + gjj.y;
+        const double tx = sx + a.chan4 * sy, ty = sy - a.chan4 * sx;
+        v = make_double2(0.5 * (v.x - tx), 0.5 * (v.y - ty));
+      }
+    }
+    s_g[r * kNB + c] = v;
+  }
+  __syncthreads();
+  const double ldg = gj_inverse(nb, s_g, s_inv, kNB, &s_ld, &s_piv);
+  // Delta = V (z + mu - h)^-1 V^+
+  if (a.nbb > 0) {
+    const int nbb = a.nbb;
+    const double2 *hb = a.hb_blk + (int64_t)sb * nbb * nbb;
+    for (int e = tid; e < nbb * nbb; e += blockDim.x) {
+      const int r = e / nbb, c = e % nbb;
+      double2 h = hb[r * nbb + c];
+      double2 v = make_double2(-h.x, -h.y);
+      if (r == c) {
+        v.x += zz.x + a.xmu;
+        v.y += zz.y;
+      }
+      s_hb[r * kNB + c] = v;
+    }
+    __syncthreads();
+    gj_inverse(nbb, s_hb, s_hbinv, kNB, &s_ld, &s_piv);
+    const double2 *vb = a.vb_blk + (int64_t)sb * nb * nbb;
+    for (int e = tid; e < nb * nb; e += blockDim.x) {
+      const int r = e / nb, c = e % nb;
+      double dr = 0, di = 0;
+      for (int p = 0; p < nbb; ++p) {
+        const double2 vr = vb[r * nbb + p];
+        for (int q = 0; q < nbb; ++q) {
+          const double2 m = s_hbinv[p * kNB + q];
+          const double2 vc = vb[c * nbb + q];  // conj(V(c,q)) below
+          // vr * m * conj(vc)
+          const double tx = vr.x * m.x - vr.y * m.y, ty = vr.x * m.y + vr.y * m.x;
+          dr += tx * vc.x + ty * vc.y;
+          di += ty * vc.x - tx * vc.y;
+        }
+      }
+      s_inv[r * kNB + c].x += dr;
+      s_inv[r * kNB + c].y += di;
+    }
+    __syncthreads();
+  }
+  const double2 *hl = a.hloc_blk + (int64_t)sb * nb * nb;
+  double2 *out = a.bmat + ((int64_t)iz * a.nspin_blk + sb) * kNB * kNB;
+  for (int e = tid; e < kNB * kNB; e += blockDim.x) {
+    const int r = e / kNB, c = e % kNB;
+    double2 v;
+    if (r < nb && c < nb) {
+      v = s_inv[r * kNB + c];
+      v.x += hl[r * nb + c].x;
+      v.y += hl[r * nb + c].y;
+    } else {
+      v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+    }
+    out[e] = v;
+  }
+  if (tid == 0) a.logdet_g[(int64_t)iz * a.nspin_blk + sb] = ldg;
+}
+
+// ------------------------------------------------------- k x z log-det --
+// Each 16-lane group owns one matrix M = B(z) - H_k, lane r holding row r.
+// Partial pivoting: the pivot row is the active row of largest |M(r,k)|
+// (ties to the lowest row), found with a 16-lane butterfly, then broadcast
+// to the group.  Rows already used as pivots go inactive, so no row swap is
+// needed; log|det| = sum log|pivot| is independent of the row order.
+template <typename R>
+__global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int z0, int zchunk,
+                                                    int nspin_blk, const ocplx<R> *hk,
+                                                    const ocplx<R> *bmat, double *partials) {
+  using C = ocplx<R>;
+  const int lane = threadIdx.x & 63;
+  const int gl = lane & 15;
+  const int grp = threadIdx.x >> 4;  // 0..15 within the block
+  const int kb = blockIdx.x;
+  const int sb = blockIdx.z;
+  const int k = kb * 16 + grp;
+  const bool kvalid = k < nk;
+  __shared__ double s_red[16];
+  // H_k row in registers (identity padding outside the block and for k >= nk)
+  R hr[kNB], hi[kNB];
+  {
+    const C *row = hk + (((int64_t)(kvalid ? k : 0) * nspin_blk + sb) * kNB + gl) * kNB;
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      const C v = row[j];
+      hr[j] = kvalid ? v.x : (R)0;
+      hi[j] = kvalid ? v.y : (R)0;
+    }
+  }
+  const int zbeg = z0 + blockIdx.y * zchunk;
+  const int zend = min(zbeg + zchunk, nz);
+  for (int iz = zbeg; iz < zend; ++iz) {
+    R ar[kNB], ai[kNB];
+    const C *brow = bmat + (((int64_t)iz * nspin_blk + sb) * kNB + gl) * kNB;
+#pragma unroll
+    for (int j = 0; j < kNB; ++j) {
+      const C b = brow[j];
+      if (kvalid) {
+        ar[j] = b.x - hr[j];
+        ai[j] = b.y - hi[j];
+      } else {
+        ar[j] = (gl == j) ? (R)1 : (R)0;
+        ai[j] = (R)0;
+      }
+    }
+    bool active = true;
+    double ld = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < kNB; ++kk) {
+      float mag = active ? (float)(ar[kk] * ar[kk] + ai[kk] * ai[kk]) : -1.0f;
+      int idx = gl;
+#pragma unroll
+      for (int off = 8; off > 0; off >>= 1) {
+        const float m2 = __shfl_xor(mag, off, 16);
+        const int i2 = __shfl_xor(idx, off, 16);
+        if (m2 > mag || (m2 == mag && i2 < idx)) {
+          mag = m2;
+          idx = i2;
+        }
+      }
+      const int src = (lane & ~15) | idx;
+      R pr[kNB], pim[kNB];
+#pragma unroll
+      for (int j = kk; j < kNB; ++j) {
+        pr[j] = __shfl(ar[j], src, 64);
+        pim[j] = __shfl(ai[j], src, 64);
+      }
+      const double pm = (double)pr[kk] * pr[kk] + (double)pim[kk] * pim[kk];
+      ld += 0.5 * log(pm);
+      if (gl == idx) active = false;
+      if (active) {
+        // l = a[kk] / p[kk]
+        const R inv = (R)1 / (pr[kk] * pr[kk] + pim[kk] * pim[kk]);
+        const R lr = (ar[kk] * pr[kk] + ai[kk] * pim[kk]) * inv;
+        const R li = (ai[kk] * pr[kk] - ar[kk] * pim[kk]) * inv;
+#pragma unroll
+        for (int j = kk + 1; j < kNB; ++j) {
+          ar[j] -= lr * pr[j] - li * pim[j];
+          ai[j] -= lr * pim[j] + li * pr[j];
+        }
+      }
+    }
+    // ld is identical in the 16 lanes of a group; one lane per group adds
+    if (gl == 0) s_red[grp] = ld;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double s = 0;
+      for (int g = 0; g < 16; ++g) s += s_red[g];
+      partials[((int64_t)iz * nspin_blk + sb) * gridDim.x + kb] = s;
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(64) void k_reduce_z(int nz, int z0, int nspin_blk, int nkb,
+                                                 int nk, const double *partials,
+                                                 const double *logdet_g, double *f) {
+  const int iz = z0 + blockIdx.x;
+  if (iz >= nz) return;
+  double s = 0;
+  for (int sb = 0; sb < nspin_blk; ++sb) {
+    double t = 0;
+    const double *p = partials + ((int64_t)iz * nspin_blk + sb) * nkb;
+    for (int b = threadIdx.x; b < nkb; b += 64) t += p[b];
+    for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+    s += t / nk + logdet_g[(int64_t)iz * nspin_blk + sb];
+  }
+  if (threadIdx.x == 0) f[iz] = s;
+}
+
+// ================================================================ host ===
+OmegaDev::~OmegaDev() { release(); }
+
+void OmegaDev::release() {
+  for (void *p : {(void *)d_off, (void *)d_w, (void *)d_pole, (void *)d_hk64, (void *)d_hk32,
+                  (void *)d_hloc, (void *)d_hb, (void *)d_vb, (void *)d_z, (void *)d_graw,
+                  (void *)d_b64, (void *)d_b32, (void *)d_ldg, (void *)d_part, (void *)d_f})
+    if (p) (void)hipFree(p);
+  d_off = nullptr;
+  d_w = nullptr;
+  d_pole = nullptr;
+  d_hk64 = nullptr;
+  d_hk32 = nullptr;
+  d_hloc = nullptr;
+  d_hb = nullptr;
+  d_vb = nullptr;
+  d_z = nullptr;
+  d_graw = nullptr;
+  d_b64 = nullptr;
+  d_b32 = nullptr;
+  d_ldg = nullptr;
+  d_part = nullptr;
+  d_f = nullptr;
+  zcap = 0;
+}
+
+void OmegaDev::upload(const OmegaHost &h, hipStream_t st) {
+  release();
+  host = h;
+  stream = st;
+  const int64_t np = (int64_t)h.w.size();
+  VCA_HIP(hipMalloc(&d_off, sizeof(int64_t) * h.off.size()));
+  VCA_HIP(hipMalloc(&d_w, sizeof(double2) * std::max<int64_t>(np, 1)));
+  VCA_HIP(hipMalloc(&d_pole, sizeof(double) * std::max<int64_t>(np, 1)));
+  VCA_HIP(hipMemcpyAsync(d_off, h.off.data(), sizeof(int64_t) * h.off.size(),
+                         hipMemcpyHostToDevice, st));
+  if (np) {
+    VCA_HIP(hipMemcpyAsync(d_w, h.w.data(), sizeof(double2) * np, hipMemcpyHostToDevice, st));
+    VCA_HIP(hipMemcpyAsync(d_pole, h.pole.data(), sizeof(double) * np, hipMemcpyHostToDevice,
+                           st));
+  }
+  const size_t hk_n = (size_t)h.nk * h.nspin_blk * kNB * kNB;
+  std::vector<float2> hk32(hk_n);
+  for (size_t i = 0; i < hk_n; ++i)
+    hk32[i] = make_float2((float)h.hk[i].x, (float)h.hk[i].y);
+  VCA_HIP(hipMalloc(&d_hk64, sizeof(double2) * hk_n));
+  VCA_HIP(hipMalloc(&d_hk32, sizeof(float2) * hk_n));
+  VCA_HIP(hipMemcpyAsync(d_hk64, h.hk.data(), sizeof(double2) * hk_n, hipMemcpyHostToDevice,
+                         st));
+  VCA_HIP(hipMemcpyAsync(d_hk32, hk32.data(), sizeof(float2) * hk_n, hipMemcpyHostToDevice, st));
+  VCA_HIP(hipMalloc(&d_hloc, sizeof(double2) * h.hloc.size()));
+  VCA_HIP(hipMemcpyAsync(d_hloc, h.hloc.data(), sizeof(double2) * h.hloc.size(),
+                         hipMemcpyHostToDevice, st));
+  if (h.nbb > 0) {
+    VCA_HIP(hipMalloc(&d_hb, sizeof(double2) * h.hb.size()));
+    VCA_HIP(hipMalloc(&d_vb, sizeof(double2) * h.vb.size()));
+    VCA_HIP(hipMemcpyAsync(d_hb, h.hb.data(), sizeof(double2) * h.hb.size(),
+                           hipMemcpyHostToDevice, st));
+    VCA_HIP(hipMemcpyAsync(d_vb, h.vb.data(), sizeof(double2) * h.vb.size(),
+                           hipMemcpyHostToDevice, st));
+  }
+  VCA_HIP(hipStreamSynchronize(st));
+}
+
+void OmegaDev::ensure_z(int nz) {
+  if (nz <= zcap) return;
+  for (void *p : {(void *)d_z, (void *)d_graw, (void *)d_b64, (void *)d_b32, (void *)d_ldg,
+                  (void *)d_part, (void *)d_f})
+    if (p) VCA_HIP(hipFree(p));
+  zcap = std::max(nz, 64);
+  const int nkb = (host.nk + 15) / 16;
+  VCA_HIP(hipMalloc(&d_z, sizeof(double2) * zcap));
+  VCA_HIP(hipMalloc(&d_graw, sizeof(double2) * (size_t)zcap * host.nelem));
+  VCA_HIP(hipMalloc(&d_b64, sizeof(double2) * (size_t)zcap * host.nspin_blk * kNB * kNB));
+  VCA_HIP(hipMalloc(&d_b32, sizeof(float2) * (size_t)zcap * host.nspin_blk * kNB * kNB));
+  VCA_HIP(hipMalloc(&d_ldg, sizeof(double) * (size_t)zcap * host.nspin_blk));
+  VCA_HIP(hipMalloc(&d_part, sizeof(double) * (size_t)zcap * host.nspin_blk * nkb));
+  VCA_HIP(hipMalloc(&d_f, sizeof(double) * zcap));
+}
+
+__global__ void k_b_to_f32(int64_t n, const double2 *s, float2 *d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = make_float2((float)s[i].x, (float)s[i].y);
+}
+
+void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, float *ms_logdet) {
+  if (nz <= 0) return;
+  ensure_z(nz);
+  const hipStream_t st = stream;
+  VCA_HIP(hipMemcpyAsync(d_z, z, sizeof(double2) * nz, hipMemcpyHostToDevice, st));
+  {
+    const int64_t nt = (int64_t)nz * host.nelem;
+    hipLaunchKernelGGL(k_pole_sum, dim3((unsigned)((nt + kOBlock - 1) / kOBlock)),
+                       dim3(kOBlock), 0, st, nz, d_z, host.nelem, d_off, d_w, d_pole, d_graw);
+    VCA_HIP(hipGetLastError());
+  }
+  PrepArgs pa{};
+  pa.nz = nz;
+  pa.nspin_blk = host.nspin_blk;
+  pa.nb = host.nb;
+  pa.nlat = host.nlat;
+  pa.norb = host.norb;
+  pa.nspin_gf = host.nspin;
+  pa.chan4 = host.chan4;
+  pa.xmu = host.xmu;
+  pa.z = d_z;
+  pa.graw = d_graw;
+  pa.nelem = host.nelem;
+  pa.hloc_blk = d_hloc;
+  pa.nbb = host.nbb;
+  pa.hb_blk = d_hb;
+  pa.vb_blk = d_vb;
+  pa.bmat = d_b64;
+  pa.logdet_g = d_ldg;
+  hipLaunchKernelGGL(k_prep, dim3(nz, host.nspin_blk), dim3(64), 0, st, pa);
+  VCA_HIP(hipGetLastError());
+  const int nkb = (host.nk + 15) / 16;
+  const int zchunk = 16;
+  const dim3 grid(nkb, (nz + zchunk - 1) / zchunk, host.nspin_blk);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (ms_logdet) {
+    VCA_HIP(hipEventCreate(&e0));
+    VCA_HIP(hipEventCreate(&e1));
+  }
+  if (precision == VCA_FP64) {
+    if (e0) VCA_HIP(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_logdet<double>, grid, dim3(kOBlock), 0, st, host.nk, nz, 0, zchunk,
+                       host.nspin_blk, d_hk64, d_b64, d_part);
+  } else {
+    const int64_t nb = (int64_t)nz * host.nspin_blk * kNB * kNB;
+    hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, nb,
+                       d_b64, d_b32);
+    if (e0) VCA_HIP(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_logdet<float>, grid, dim3(kOBlock), 0, st, host.nk, nz, 0, zchunk,
+                       host.nspin_blk, d_hk32, d_b32, d_part);
+  }
+  VCA_HIP(hipGetLastError());
+  if (e1) VCA_HIP(hipEventRecord(e1, st));
+  hipLaunchKernelGGL(k_reduce_z, dim3(nz), dim3(64), 0, st, nz, 0, host.nspin_blk, nkb,
+                     host.nk, d_part, d_ldg, d_f);
+  VCA_HIP(hipGetLastError());
+  VCA_HIP(hipMemcpyAsync(f, d_f, sizeof(double) * nz, hipMemcpyDeviceToHost, st));
+  VCA_HIP(hipStreamSynchronize(st));
+  if (ms_logdet) {
+    VCA_HIP(hipEventElapsedTime(ms_logdet, e0, e1));
+    VCA_HIP(hipEventDestroy(e0));
+    VCA_HIP(hipEventDestroy(e1));
+  }
+}
+
+}  // namespace vca

@@ -1,0 +1,1266 @@
+// solver.cpp -- host orchestration and C ABI of the VCA solver core.
+//
+// The flow of vca_solve follows vca_solve_serial (VCA_MAIN.f90:124-215):
+//   1. input ingestion: Hloc, H_k, bath (VCA_AUX_FUNX.f90:124-164,
+//      VCA_BATH_SETUP.f90:112-138) -> one-body term tables per spin;
+//   2. sector scan and T=0 ground-state selection (VCA_DIAG.f90:83-235):
+//      dense Hermitian eigensolve for small sectors, GPU Lanczos otherwise;
+//   3. excitation chains per kept state (VCA_GF_NORMAL.f90:37-104,121-958):
+//      start vector built on the GPU, GPU Lanczos, host tridiagonal
+//      eigensolve -> Lehmann poles and weights (:970-1022);
+//   4. Omega integrand on the GPU for batches of frequencies and the
+//      frequency quadrature on the host (VCA_OMEGA.f90:123-188);
+//   5. Omega = E0 - integral (VCA_MAIN.f90:183-197).
+#include <algorithm>
+#include <array>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "omega.hpp"
+#include "vca_internal.hpp"
+
+namespace vca {
+
+// ------------------------------------------------------------- errors ---
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+void fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  throw Error{code};
+}
+
+Binom::Binom() {
+  for (int n = 0; n < kBinomN; ++n)
+    for (int k = 0; k < kBinomN; ++k) {
+      uint32_t v;
+      if (k == 0) v = 1;
+      else if (n == 0) v = 0;
+      else v = c[(n - 1) * kBinomN + (k - 1)] + c[(n - 1) * kBinomN + k];
+      c[n * kBinomN + k] = v;
+    }
+}
+
+// ------------------------------------------------------------- solver ---
+struct GfChan {
+  std::vector<zc> w;
+  std::vector<double> p;
+};
+using ElemKey = std::array<int, 5>;  // ilat, jlat, ispin, iorb, jorb (1-based)
+
+struct KeptState {
+  double e;
+  int nup, ndw;
+  void *d_vec;  // complex128 on device
+};
+
+struct Solver {
+  vca_params p{};
+  std::vector<int32_t> sectors;
+  int ns = 0, nlso = 0;
+  std::vector<zc> hloc, bath_h, bath_v, hk;
+  int64_t nk = 0;
+  bool has_hloc = false, has_bath = false;
+  ModelData md;
+  bool md_valid = false;
+  uint32_t *d_binom = nullptr;
+  hipStream_t stream = nullptr;
+  vca_results res{};
+  bool solved = false;
+  std::map<ElemKey, std::vector<std::vector<GfChan>>> gf;
+  double emin = 0, zeta = 1, max_exc = 0;
+  std::unique_ptr<OmegaDev> om;
+  bool om_ready = false;
+  std::vector<KeptState> states;
+  ~Solver();
+};
+
+static void free_states(Solver &s) {
+  for (auto &st : s.states)
+    if (st.d_vec) (void)hipFree(st.d_vec);
+  s.states.clear();
+}
+
+Solver::~Solver() {
+  free_states(*this);
+  om.reset();
+  if (d_binom) (void)hipFree(d_binom);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+const uint32_t *solver_binom(const Solver *s) { return s->d_binom; }
+
+Sector::~Sector() {
+  for (DevSpin *sp : {&up, &dw})
+    for (void *q : {(void *)sp->map, (void *)sp->energy, (void *)sp->cnt, (void *)sp->ent,
+                    (void *)sp->amp64, (void *)sp->amp32})
+      if (q) (void)hipFree(q);
+  for (void *q : {buf[0], buf[1], buf[2], xvec, (void *)partials, (void *)lz, (void *)lz_arrays})
+    if (q) (void)hipFree(q);
+  if (stream) (void)hipStreamDestroy(stream);
+}
+
+CrossTerm sector_cross(const Sector &sec) {
+  const ModelData &md = sec.solver->md;
+  CrossTerm c{};
+  for (int i = 0; i < 5; ++i)
+    for (int j = 0; j < 5; ++j) c.wx[i][j] = md.wx[i][j];
+  c.site_mask = md.site_mask;
+  c.norb = md.norb;
+  return c;
+}
+
+// rank-6 column-major index, 0-based indices
+static inline size_t idx6(int i1, int i2, int i3, int i4, int i5, int i6, int d1, int d2,
+                          int d3, int d4, int d5) {
+  return (size_t)i1 +
+         (size_t)d1 * (i2 + (size_t)d2 * (i3 + (size_t)d3 * (i4 + (size_t)d4 * (i5 + (size_t)d5 * i6))));
+}
+
+// Derive the term tables from Hloc and the bath, in the insertion order of
+// the stored path (H_cluster_up.f90:108-186, H_cluster_dw.f90:8-84) and the
+// diagonal decomposition of H_cluster_local.f90:11-90.
+static void derive_model(Solver &s) {
+  const vca_params &p = s.p;
+  ModelData &md = s.md;
+  md = ModelData{};
+  md.ns = s.ns;
+  md.nlat = p.nlat;
+  md.norb = p.norb;
+  md.nspin = p.nspin;
+  md.nlat_bath = p.nlat_bath;
+  md.norb_bath = p.norb_bath;
+  const int nl = p.nlat, no = p.norb, nsp = p.nspin, nlb = p.nlat_bath, nob = p.norb_bath;
+  const bool bath = nlb > 0 && nob > 0;
+  auto H = [&](int il, int jl, int si, int sj, int io, int jo) {
+    return s.hloc[idx6(il, jl, si, sj, io, jo, nl, nl, nsp, nsp, no)];
+  };
+  auto BH = [&](int il, int jl, int si, int sj, int io, int jo) {
+    return s.bath_h[idx6(il, jl, si, sj, io, jo, nlb, nlb, nsp, nsp, nob)];
+  };
+  auto BV = [&](int il, int jl, int si, int sj, int io, int jo) {
+    return s.bath_v[idx6(il, jl, si, sj, io, jo, nl, nlb, nsp, nsp, no)];
+  };
+  auto imp = [&](int il, int io) { return io + il * no; };          // 0-based
+  auto bpos = [&](int ib, int ob) { return nl * no + ib * nob + ob; };  // 0-based
+  const double hf_orb = p.hfmode ? 1.0 : 0.0;
+  for (int sp = 0; sp < 2; ++sp) {
+    const int si = sp == 0 ? 0 : nsp - 1;  // block (1,1) or (Nspin,Nspin)
+    SpinModel &m = md.spin[sp];
+    m.terms.clear();
+    m.real_amp = true;
+    for (int a = 0; a < kMaxNs; ++a) {
+      m.eps[a] = 0;
+      for (int b = 0; b < kMaxNs; ++b) m.pair[a][b] = 0;
+    }
+    auto add = [&](int a, int b, zc v) {
+      if (v == zc(0.0)) return;
+      if (a == b) return;
+      if (v.imag() != 0.0) m.real_amp = false;
+      m.terms.push_back(Term{a, b, v.real(), v.imag()});
+    };
+    for (int il = 0; il < nl; ++il)
+      for (int jl = 0; jl < nl; ++jl)
+        for (int io = 0; io < no; ++io)
+          for (int jo = 0; jo < no; ++jo) add(imp(il, io), imp(jl, jo), H(il, jl, si, si, io, jo));
+    if (bath) {
+      for (int il = 0; il < nlb; ++il)
+        for (int jl = 0; jl < nlb; ++jl)
+          for (int io = 0; io < nob; ++io)
+            for (int jo = 0; jo < nob; ++jo)
+              add(bpos(il, io), bpos(jl, jo), BH(il, jl, si, si, io, jo));
+      for (int il = 0; il < nl; ++il)
+        for (int io = 0; io < no; ++io)
+          for (int jl = 0; jl < nlb; ++jl)
+            for (int jo = 0; jo < nob; ++jo) {
+              const zc v = BV(il, jl, si, si, io, jo);
+              add(imp(il, io), bpos(jl, jo), v);
+              add(bpos(jl, jo), imp(il, io), std::conj(v));
+            }
+    }
+    if ((int)m.terms.size() > kMaxTerms)
+      fail(VCA_ERR_ARG, "%zu one-body terms exceed the table limit %d", m.terms.size(), kMaxTerms);
+    // linear energies
+    for (int il = 0; il < nl; ++il)
+      for (int io = 0; io < no; ++io) {
+        const zc d = H(il, il, si, si, io, io);
+        if (d.imag() != 0.0) fail(VCA_ERR_ARG, "Hloc has a complex diagonal element");
+        m.eps[imp(il, io)] = d.real() - p.xmu -
+                             hf_orb * (0.5 * p.uloc[io] + 0.5 * (2.0 * p.ust - p.jh) * (no - 1));
+        for (int jo = io + 1; jo < no; ++jo) m.pair[imp(il, io)][imp(il, jo)] = p.ust - p.jh;
+      }
+    if (bath)
+      for (int ib = 0; ib < nlb; ++ib)
+        for (int ob = 0; ob < nob; ++ob) {
+          const zc d = BH(ib, ib, si, si, ob, ob);
+          if (d.imag() != 0.0) fail(VCA_ERR_ARG, "bath_h has a complex diagonal element");
+          m.eps[bpos(ib, ob)] = d.real() - p.xmu;
+        }
+  }
+  for (int io = 0; io < no; ++io)
+    for (int jo = 0; jo < no; ++jo) md.wx[io][jo] = io == jo ? p.uloc[io] : p.ust;
+  md.site_mask = 0;
+  for (int il = 0; il < nl; ++il) md.site_mask |= 1u << (il * no);
+  md.e_const = 0;
+  if (p.hfmode) {
+    for (int il = 0; il < nl; ++il) {
+      for (int io = 0; io < no; ++io) md.e_const += 0.25 * p.uloc[io];
+      md.e_const += 0.25 * (2.0 * p.ust - p.jh) * (no * (no - 1) / 2);
+    }
+  }
+  s.md_valid = true;
+}
+
+// ------------------------------------------------------------ sectors ---
+static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec) {
+  if (!s.md_valid) fail(VCA_ERR_STATE, "Hloc not set");
+  if (nup < 0 || ndw < 0 || nup > s.ns || ndw > s.ns)
+    fail(VCA_ERR_ARG, "sector (%d,%d) out of range for Ns=%d", nup, ndw, s.ns);
+  auto sec = std::make_unique<Sector>();
+  sec->solver = &s;
+  sec->nup = nup;
+  sec->ndw = ndw;
+  sec->precision = prec;
+  VCA_HIP(hipStreamCreate(&sec->stream));
+  build_spin_tables(*sec, sec->up, s.md.spin[0], s.ns, nup, s.d_binom, s.md.e_const);
+  build_spin_tables(*sec, sec->dw, s.md.spin[1], s.ns, ndw, s.d_binom, 0.0);
+  sec->dim = (int64_t)sec->up.dim * sec->dw.dim;
+  return sec;
+}
+
+static void ensure_lanczos(Sector &sec, int cap) {
+  if (!sec.buf[0]) {
+    for (int i = 0; i < 3; ++i) VCA_HIP(hipMalloc(&sec.buf[i], sec.vec_bytes()));
+    VCA_HIP(hipMalloc(&sec.xvec, sec.vec_bytes()));
+  }
+  if (!sec.lz) VCA_HIP(hipMalloc(&sec.lz, sizeof(LanczosDev)));
+  if (sec.lz_cap < cap) {
+    if (sec.lz_arrays) VCA_HIP(hipFree(sec.lz_arrays));
+    const size_t n = (size_t)cap + 4;
+    VCA_HIP(hipMalloc(&sec.lz_arrays, sizeof(double) * 4 * n));
+    VCA_HIP(hipMemsetAsync(sec.lz_arrays, 0, sizeof(double) * 4 * n, sec.stream));
+    sec.lzh = LanczosDev{};
+    sec.lzh.alpha = sec.lz_arrays;
+    sec.lzh.beta = sec.lz_arrays + n;
+    sec.lzh.nrm = sec.lz_arrays + 2 * n;
+    sec.lzh.y = sec.lz_arrays + 3 * n;
+    sec.lz_cap = cap;
+  }
+  VCA_HIP(hipMemcpyAsync(sec.lz, &sec.lzh, sizeof(LanczosDev), hipMemcpyHostToDevice, sec.stream));
+}
+
+static void run_steps(Sector &sec, int nsteps, bool accx, Solver *s) {
+  for (int i = 0; i < nsteps; ++i) {
+    launch_hxv(sec, sec.buf[0], sec.buf[1], sec.buf[2], true);
+    launch_lanczos_update(sec, accx);
+  }
+  if (s) {
+    s->res.hxv_count += nsteps;
+    s->res.hxv_states += (double)nsteps * (double)sec.dim;
+  }
+}
+
+static void fetch(Sector &sec, const double *dsrc, int i0, int n, double *dst) {
+  if (n <= 0) return;
+  VCA_HIP(hipMemcpyAsync(dst + i0, dsrc + i0, sizeof(double) * n, hipMemcpyDeviceToHost,
+                         sec.stream));
+}
+
+struct GsOut {
+  double e0;
+  int m;
+  bool converged;
+};
+
+// Plain Lanczos ground state on sec (its precision), start = device vector
+// in that precision or null for the hash start vector.  The converged
+// eigenvector is left normalised in sec.xvec.  rel_tol: stop when
+// |dE| < tol*max(1,|E|) instead of |dE| < tol.
+static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, bool rel_tol,
+                        Solver *s) {
+  niter = (int)std::min<int64_t>(niter, sec.dim);
+  ensure_lanczos(sec, niter + 2);
+  auto init = [&]() {
+    if (start)
+      VCA_HIP(hipMemcpyAsync(sec.buf[0], start, sec.vec_bytes(), hipMemcpyDeviceToDevice,
+                             sec.stream));
+    else
+      launch_start_vector(sec, 5001);
+    launch_norm0(sec);
+  };
+  init();
+  std::vector<double> al(niter + 4, 0.0), be(niter + 4, 0.0);
+  const int chunk = 16;
+  int k0 = 0, m = 0;
+  double e_old = std::numeric_limits<double>::infinity(), e_m = 0;
+  bool conv = false;
+  while (k0 < niter && !conv) {
+    const int n = std::min(chunk, niter - k0);
+    run_steps(sec, n, false, s);
+    fetch(sec, sec.lzh.alpha, k0, n, al.data());
+    fetch(sec, sec.lzh.beta, k0, n + 1, be.data());
+    VCA_HIP(hipStreamSynchronize(sec.stream));
+    for (int j = k0; j < k0 + n; ++j) {
+      const int mm = j + 1;
+      const double e = tridiag_lowest(mm, al.data(), be.data());
+      const double thr = rel_tol ? tol * std::max(1.0, std::fabs(e)) : tol;
+      if (std::fabs(e - e_old) < thr || be[j + 1] == 0.0) {
+        m = mm;
+        e_m = e;
+        conv = true;
+        break;
+      }
+      e_old = e;
+    }
+    k0 += n;
+  }
+  if (!conv) {
+    m = niter;
+    e_m = tridiag_lowest(m, al.data(), be.data());
+  }
+  // second pass: x = sum_k y_k v_k
+  std::vector<double> w(m), z((size_t)m * m);
+  tridiag_eig(m, al.data(), be.data(), w.data(), z.data());
+  VCA_HIP(hipMemcpyAsync(sec.lzh.y, z.data(), sizeof(double) * m, hipMemcpyHostToDevice,
+                         sec.stream));
+  init();
+  VCA_HIP(hipMemsetAsync(sec.xvec, 0, sec.vec_bytes(), sec.stream));
+  run_steps(sec, m, true, s);
+  launch_scale_x(sec);
+  VCA_HIP(hipStreamSynchronize(sec.stream));
+  return GsOut{e_m, m, conv};
+}
+
+// MIXED: FP32 Lanczos, then an FP64 Lanczos restarted from the FP32 ground
+// state.  Returns the FP64 result; the eigenvector ends in sec64.xvec.
+static GsOut gs_mixed(Solver &s, Sector &sec32, Sector &sec64, double tol, int niter,
+                      double *e32) {
+  GsOut g32 = gs_lanczos(sec32, 1e-7, niter, nullptr, true, &s);
+  if (e32) *e32 = g32.e0;
+  void *start64 = nullptr;
+  VCA_HIP(hipMalloc(&start64, sec64.vec_bytes()));
+  launch_convert(sec32.xvec, VCA_MIXED, start64, VCA_FP64, sec32.dim, sec32.stream);
+  VCA_HIP(hipStreamSynchronize(sec32.stream));
+  GsOut g64 = gs_lanczos(sec64, tol, niter, start64, false, &s);
+  VCA_HIP(hipFree(start64));
+  return g64;
+}
+
+// Lanczos tridiagonalisation from the start vector already in buf[0] with
+// its norm in nrm[0] (launch_excite or launch_norm0), nlanc steps.
+static void tridiag_run(Sector &sec, int nlanc, double *alpha, double *beta, double *norm0,
+                        Solver *s) {
+  run_steps(sec, nlanc, false, s);
+  fetch(sec, sec.lzh.alpha, 0, nlanc, alpha);
+  fetch(sec, sec.lzh.beta, 0, nlanc, beta);
+  fetch(sec, sec.lzh.nrm, 0, 1, norm0);
+  VCA_HIP(hipStreamSynchronize(sec.stream));
+}
+
+// --------------------------------------------------------- dense path ---
+static HostSpinTable host_spin_table(const SpinModel &m, int ns, int npart, double e_add) {
+  Binom bn;
+  HostSpinTable t;
+  t.npart = npart;
+  t.dim = (int)bn(ns, npart);
+  const int nterms = (int)m.terms.size();
+  t.maxnnz = std::max(1, std::min(nterms, npart * (ns - npart)));
+  t.map.resize(t.dim);
+  t.energy.resize(t.dim);
+  t.cnt.resize(t.dim);
+  t.ent.assign((size_t)t.dim * t.maxnnz, 0u);
+  std::vector<int2> ab(std::max(1, nterms));
+  for (int i = 0; i < nterms; ++i) ab[i] = make_int2(m.terms[i].a, m.terms[i].b);
+  std::vector<double> pair((size_t)kMaxNs * kMaxNs);
+  for (int a = 0; a < kMaxNs; ++a)
+    for (int b = 0; b < kMaxNs; ++b) pair[(size_t)a * kMaxNs + b] = m.pair[a][b];
+  for (int i = 0; i < t.dim; ++i) {
+    const uint32_t str = unrank_string(ns, npart, (uint32_t)i, bn.c);
+    t.map[i] = str;
+    t.energy[i] = spin_energy(str, ns, m.eps, pair.data()) + e_add;
+    t.cnt[i] = row_hops(str, nterms, ab.data(), bn.c, t.ent.data() + i, t.dim);
+  }
+  return t;
+}
+
+static double cross_host(const ModelData &md, uint32_t su, uint32_t sd) {
+  double e = 0;
+  for (int io = 0; io < md.norb; ++io)
+    for (int jo = 0; jo < md.norb; ++jo)
+      e += md.wx[io][jo] * (double)__builtin_popcount((su >> io) & (sd >> jo) & md.site_mask);
+  return e;
+}
+
+static std::vector<zc> dense_h(const Solver &s, int nup, int ndw) {
+  const ModelData &md = s.md;
+  HostSpinTable tu = host_spin_table(md.spin[0], s.ns, nup, md.e_const);
+  HostSpinTable td = host_spin_table(md.spin[1], s.ns, ndw, 0.0);
+  const int64_t du = tu.dim, dd = td.dim, dim = du * dd;
+  std::vector<zc> h((size_t)(dim * dim), zc(0.0));
+  auto amp = [&](const SpinModel &m, uint32_t e) {
+    const uint32_t ai = e >> kColBits;
+    const Term &t = m.terms[ai >> 1];
+    zc v(t.re, t.im);
+    return (ai & 1u) ? -v : v;
+  };
+  for (int64_t idw = 0; idw < dd; ++idw)
+    for (int64_t iup = 0; iup < du; ++iup) {
+      const int64_t i = iup + idw * du;
+      h[i + i * dim] += tu.energy[iup] + td.energy[idw] + cross_host(md, tu.map[iup], td.map[idw]);
+      for (int q = 0; q < tu.cnt[iup]; ++q) {
+        const uint32_t e = tu.ent[(size_t)q * du + iup];
+        const int64_t j = (int64_t)(e & kColMask) + idw * du;
+        h[i + j * dim] += amp(md.spin[0], e);
+      }
+      for (int q = 0; q < td.cnt[idw]; ++q) {
+        const uint32_t e = td.ent[(size_t)q * dd + idw];
+        const int64_t j = iup + (int64_t)(e & kColMask) * du;
+        h[i + j * dim] += amp(md.spin[1], e);
+      }
+    }
+  return h;
+}
+
+// --------------------------------------------------------- GS search ----
+static void diagonalize(Solver &s) {
+  const vca_params &p = s.p;
+  free_states(s);
+  double oldzero = 1000.0;
+  std::vector<std::pair<int, int>> list;
+  if (s.sectors.empty()) {
+    for (int nup = 0; nup <= s.ns; ++nup)
+      for (int ndw = 0; ndw <= s.ns; ++ndw) list.push_back({nup, ndw});
+  } else {
+    for (size_t i = 0; i + 1 < s.sectors.size(); i += 2)
+      list.push_back({s.sectors[i], s.sectors[i + 1]});
+  }
+  Binom bn;
+  for (auto [nup, ndw] : list) {
+    const int64_t dim = (int64_t)bn(s.ns, nup) * bn(s.ns, ndw);
+    const int neigen =
+        p.lanc_method == VCA_LANC_LANCZOS ? 1 : (int)std::min<int64_t>(dim, p.lanc_nstates_sector);
+    const bool lanc_solve = !(neigen == dim || dim <= p.lanc_dim_threshold);
+    std::vector<double> evals;
+    std::vector<void *> evecs;  // device complex128
+    if (!lanc_solve) {
+      std::vector<zc> h = dense_h(s, nup, ndw);
+      std::vector<double> w(dim);
+      herm_eig((int)dim, h.data(), w.data());
+      for (int i = 0; i < neigen; ++i) {
+        evals.push_back(w[i]);
+        void *d = nullptr;
+        VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
+        VCA_HIP(hipMemcpy(d, h.data() + (size_t)i * dim, sizeof(double2) * dim,
+                          hipMemcpyHostToDevice));
+        evecs.push_back(d);
+      }
+    } else {
+      if (neigen > 1)
+        fail(VCA_ERR_ARG,
+             "sector (%d,%d): %d states per Lanczos sector requested; this build's GPU "
+             "ground-state search returns one state per sector (LANC_METHOD=lanczos)",
+             nup, ndw, neigen);
+      GsOut g{};
+      void *d = nullptr;
+      VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
+      if (p.precision == VCA_MIXED) {
+        auto s32 = make_sector(s, nup, ndw, VCA_MIXED);
+        auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+        double e32;
+        g = gs_mixed(s, *s32, *s64, p.lanc_tolerance, p.lanc_niter, &e32);
+        s.res.e0_mixed = e32;
+        VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
+      } else {
+        auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+        g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s);
+        VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
+      }
+      evals.push_back(g.e0);
+      evecs.push_back(d);
+    }
+    // T=0 selection, VCA_DIAG.f90:224-235
+    for (size_t i = 0; i < evals.size(); ++i) {
+      const double en = evals[i];
+      if (en < oldzero - 10.0 * p.gs_threshold) {
+        oldzero = en;
+        free_states(s);
+        s.states.push_back(KeptState{en, nup, ndw, evecs[i]});
+        evecs[i] = nullptr;
+      } else if (std::fabs(en - oldzero) <= p.gs_threshold) {
+        oldzero = std::min(oldzero, en);
+        s.states.push_back(KeptState{en, nup, ndw, evecs[i]});
+        evecs[i] = nullptr;
+      }
+    }
+    for (void *d : evecs)
+      if (d) VCA_HIP(hipFree(d));
+  }
+  if (s.states.empty()) fail(VCA_ERR_NOCONV, "no ground state found");
+  s.emin = s.states[0].e;
+  for (auto &st : s.states) s.emin = std::min(s.emin, st.e);
+  s.zeta = (double)s.states.size();  // vca_post_diag at T=0 (VCA_DIAG.f90:363)
+  s.res.nstates = (int)s.states.size();
+  s.res.omegaprime = s.emin;
+}
+
+// ----------------------------------------------------------- GF chains --
+struct ChainOp {
+  int spin, pos, dagger;
+  double cre, cim;
+};
+
+static void build_gf(Solver &s) {
+  const vca_params &p = s.p;
+  s.gf.clear();
+  s.max_exc = -std::numeric_limits<double>::infinity();
+  const int prec = p.precision;
+  std::map<std::array<int, 2>, std::unique_ptr<Sector>> cache;
+  auto get_sector = [&](int nup, int ndw) -> Sector & {
+    auto &slot = cache[{nup, ndw}];
+    if (!slot) slot = make_sector(s, nup, ndw, prec);
+    return *slot;
+  };
+  const double chan_scale_re[4] = {1, 1, 0, 0}, chan_scale_im[4] = {0, 0, -1, -1};
+  for (size_t ist = 0; ist < s.states.size(); ++ist) {
+    const KeptState &st = s.states[ist];
+    Binom bn;
+    const int64_t dim_i = (int64_t)bn(s.ns, st.nup) * bn(s.ns, st.ndw);
+    const int dimup_i = (int)bn(s.ns, st.nup), dimdw_i = (int)bn(s.ns, st.ndw);
+    void *psi = st.d_vec;
+    void *psi32 = nullptr;
+    if (prec == VCA_MIXED) {
+      VCA_HIP(hipMalloc(&psi32, sizeof(float2) * dim_i));
+      launch_convert(st.d_vec, VCA_FP64, psi32, VCA_MIXED, dim_i, s.stream);
+      VCA_HIP(hipStreamSynchronize(s.stream));
+      psi = psi32;
+    }
+    // One chain: O|psi> with ops (same spin, same dagger), weight factor
+    // (chan_re + i chan_im) * norm2, pole sign isign.
+    auto chain = [&](const std::vector<ChainOp> &ops, int ichan, int isign) -> GfChan {
+      GfChan out;
+      const int spin = ops[0].spin, d = ops[0].dagger ? 1 : -1;
+      const int jnup = st.nup + (spin == 1 ? d : 0), jndw = st.ndw + (spin == 2 ? d : 0);
+      if (jnup < 0 || jndw < 0 || jnup > s.ns || jndw > s.ns) return out;
+      Sector &js = get_sector(jnup, jndw);
+      const int nlanc = (int)std::min<int64_t>(js.dim, p.lanc_ngfiter);
+      ensure_lanczos(js, nlanc + 2);
+      int sp[2], pos[2], dg[2];
+      double cre[2], cim[2];
+      for (size_t o = 0; o < ops.size(); ++o) {
+        sp[o] = ops[o].spin;
+        pos[o] = ops[o].pos;
+        dg[o] = ops[o].dagger;
+        cre[o] = ops[o].cre;
+        cim[o] = ops[o].cim;
+      }
+      // the excitation reads psi on the J sector's stream
+      VCA_HIP(hipStreamSynchronize(s.stream));
+      launch_excite(js, dimup_i, dimdw_i, psi, (int)ops.size(), sp, pos, dg, cre, cim);
+      std::vector<double> al(nlanc), be(nlanc);
+      double nrm0 = 0;
+      tridiag_run(js, nlanc, al.data(), be.data(), &nrm0, &s);
+      const double norm2 = nrm0 * nrm0;
+      s.res.nchains += 1;
+      if (!(norm2 > 0.0)) return out;
+      std::vector<double> w(nlanc), z((size_t)nlanc * nlanc);
+      tridiag_eig(nlanc, al.data(), be.data(), w.data(), z.data());
+      const zc peso_bz = zc(chan_scale_re[ichan - 1], chan_scale_im[ichan - 1]) * norm2 / s.zeta;
+      out.w.resize(nlanc);
+      out.p.resize(nlanc);
+      for (int j = 0; j < nlanc; ++j) {
+        const double de = w[j] - st.e;
+        s.max_exc = std::max(s.max_exc, de);
+        const double z1 = z[(size_t)j * nlanc];
+        out.w[j] = peso_bz * z1 * z1;
+        out.p[j] = isign * de;
+      }
+      return out;
+    };
+    for (int ispin = 1; ispin <= p.nspin; ++ispin)
+      for (int iorb = 1; iorb <= p.norb; ++iorb)
+        for (int isite = 1; isite <= p.nlat; ++isite) {
+          const int is = iorb - 1 + (isite - 1) * p.norb;  // 0-based position
+          const int spin = ispin;  // acts on the up string for ispin=1
+          auto &diag = s.gf[{isite, isite, ispin, iorb, iorb}];
+          diag.resize(s.states.size());
+          diag[ist].push_back(chain({{spin, is, 1, 1.0, 0.0}}, 1, +1));
+          diag[ist].push_back(chain({{spin, is, 0, 1.0, 0.0}}, 2, -1));
+          for (int jsite = 1; jsite <= p.nlat; ++jsite)
+            for (int jorb = 1; jorb <= p.norb; ++jorb) {
+              if (isite == jsite && iorb == jorb) continue;
+              const int js_ = jorb - 1 + (jsite - 1) * p.norb;
+              auto &el = s.gf[{isite, jsite, ispin, iorb, jorb}];
+              el.resize(s.states.size());
+              el[ist].push_back(chain({{spin, is, 1, 1.0, 0.0}, {spin, js_, 1, 1.0, 0.0}}, 1, +1));
+              el[ist].push_back(chain({{spin, is, 0, 1.0, 0.0}, {spin, js_, 0, 1.0, 0.0}}, 2, -1));
+              if (!p.gf_symmetric) {
+                el[ist].push_back(
+                    chain({{spin, is, 1, 1.0, 0.0}, {spin, js_, 1, 0.0, 1.0}}, 3, +1));
+                el[ist].push_back(
+                    chain({{spin, is, 0, 1.0, 0.0}, {spin, js_, 0, 0.0, -1.0}}, 4, -1));
+              }
+            }
+        }
+    if (psi32) VCA_HIP(hipFree(psi32));
+  }
+}
+
+// -------------------------------------------------------------- Omega ---
+static zc hk_at(const Solver &s, int il, int jl, int si, int sj, int io, int jo, int64_t k) {
+  const int nl = s.p.nlat, nsp = s.p.nspin, no = s.p.norb;
+  const size_t per_k = (size_t)nl * nl * nsp * nsp * no * no;
+  return s.hk[idx6(il, jl, si, sj, io, jo, nl, nl, nsp, nsp, no) + per_k * (size_t)k];
+}
+
+static void prepare_omega(Solver &s) {
+  const vca_params &p = s.p;
+  const int nl = p.nlat, no = p.norb, nsp = p.nspin, nlb = p.nlat_bath, nob = p.norb_bath;
+  const bool bath = nlb > 0 && nob > 0;
+  // spin-block structure
+  bool blockdiag = true;
+  for (int si = 0; si < nsp && blockdiag; ++si)
+    for (int sj = 0; sj < nsp && blockdiag; ++sj) {
+      if (si == sj) continue;
+      for (int il = 0; il < nl; ++il)
+        for (int jl = 0; jl < nl; ++jl)
+          for (int io = 0; io < no; ++io)
+            for (int jo = 0; jo < no; ++jo) {
+              if (s.hloc[idx6(il, jl, si, sj, io, jo, nl, nl, nsp, nsp, no)] != zc(0.0))
+                blockdiag = false;
+              for (int64_t k = 0; k < s.nk && blockdiag; ++k)
+                if (hk_at(s, il, jl, si, sj, io, jo, k) != zc(0.0)) blockdiag = false;
+            }
+      if (bath) {
+        for (int il = 0; il < nlb; ++il)
+          for (int jl = 0; jl < nlb; ++jl)
+            for (int io = 0; io < nob; ++io)
+              for (int jo = 0; jo < nob; ++jo)
+                if (s.bath_h[idx6(il, jl, si, sj, io, jo, nlb, nlb, nsp, nsp, nob)] != zc(0.0))
+                  blockdiag = false;
+        for (int il = 0; il < nl; ++il)
+          for (int jl = 0; jl < nlb; ++jl)
+            for (int io = 0; io < no; ++io)
+              for (int jo = 0; jo < nob; ++jo)
+                if (s.bath_v[idx6(il, jl, si, sj, io, jo, nl, nlb, nsp, nsp, no)] != zc(0.0))
+                  blockdiag = false;
+      }
+    }
+  OmegaHost h;
+  h.nlat = nl;
+  h.norb = no;
+  h.nspin = nsp;
+  h.nspin_blk = blockdiag ? nsp : 1;
+  h.nb = blockdiag ? nl * no : nl * no * nsp;
+  h.nbb = bath ? (blockdiag ? nlb * nob : nlb * nob * nsp) : 0;
+  if (h.nb > 16 || h.nbb > 16)
+    fail(VCA_ERR_ARG, "Omega: block size %d / bath block %d above 16 is not supported", h.nb,
+         h.nbb);
+  h.nk = (int)s.nk;
+  h.chan4 = p.gf_symmetric ? 0.0 : 1.0;
+  h.xmu = p.xmu;
+  // pole store in element order (ispin, ilat, jlat, iorb, jorb)
+  h.nelem = nsp * nl * nl * no * no;
+  h.off.assign(h.nelem + 1, 0);
+  for (int sp = 0; sp < nsp; ++sp)
+    for (int il = 0; il < nl; ++il)
+      for (int jl = 0; jl < nl; ++jl)
+        for (int io = 0; io < no; ++io)
+          for (int jo = 0; jo < no; ++jo) {
+            const int e = (((sp * nl + il) * nl + jl) * no + io) * no + jo;
+            auto it = s.gf.find({il + 1, jl + 1, sp + 1, io + 1, jo + 1});
+            if (it != s.gf.end())
+              for (auto &chans : it->second)
+                for (auto &c : chans)
+                  for (size_t j = 0; j < c.w.size(); ++j) {
+                    h.w.push_back(make_double2(c.w[j].real(), c.w[j].imag()));
+                    h.pole.push_back(c.p[j]);
+                  }
+            h.off[e + 1] = (int64_t)h.w.size();
+          }
+  // lso position inside a block: io + il*no (+ spin*nl*no when spin-mixed)
+  auto lso = [&](int il, int io, int sp) { return io + il * no + (blockdiag ? 0 : sp * nl * no); };
+  h.hk.assign((size_t)s.nk * h.nspin_blk * 256, make_double2(0, 0));
+  h.hloc.assign((size_t)h.nspin_blk * h.nb * h.nb, make_double2(0, 0));
+  for (int si = 0; si < nsp; ++si)
+    for (int sj = 0; sj < nsp; ++sj) {
+      if (blockdiag && si != sj) continue;
+      const int sb = blockdiag ? si : 0;
+      for (int il = 0; il < nl; ++il)
+        for (int jl = 0; jl < nl; ++jl)
+          for (int io = 0; io < no; ++io)
+            for (int jo = 0; jo < no; ++jo) {
+              const int r = lso(il, io, si), c = lso(jl, jo, sj);
+              const zc hv = s.hloc[idx6(il, jl, si, sj, io, jo, nl, nl, nsp, nsp, no)];
+              h.hloc[((size_t)sb * h.nb + r) * h.nb + c] = make_double2(hv.real(), hv.imag());
+              for (int64_t k = 0; k < s.nk; ++k) {
+                const zc v = hk_at(s, il, jl, si, sj, io, jo, k);
+                h.hk[(((size_t)k * h.nspin_blk + sb) * 16 + r) * 16 + c] =
+                    make_double2(v.real(), v.imag());
+              }
+            }
+      if (bath) {
+        if (sb == 0 && h.hb.empty()) {
+          h.hb.assign((size_t)h.nspin_blk * h.nbb * h.nbb, make_double2(0, 0));
+          h.vb.assign((size_t)h.nspin_blk * h.nb * h.nbb, make_double2(0, 0));
+        }
+        auto blso = [&](int ib, int ob, int sp) {
+          return ob + ib * nob + (blockdiag ? 0 : sp * nlb * nob);
+        };
+        for (int ib = 0; ib < nlb; ++ib)
+          for (int jb = 0; jb < nlb; ++jb)
+            for (int ob = 0; ob < nob; ++ob)
+              for (int pb = 0; pb < nob; ++pb) {
+                const zc v = s.bath_h[idx6(ib, jb, si, sj, ob, pb, nlb, nlb, nsp, nsp, nob)];
+                h.hb[((size_t)sb * h.nbb + blso(ib, ob, si)) * h.nbb + blso(jb, pb, sj)] =
+                    make_double2(v.real(), v.imag());
+              }
+        for (int il = 0; il < nl; ++il)
+          for (int jb = 0; jb < nlb; ++jb)
+            for (int io = 0; io < no; ++io)
+              for (int pb = 0; pb < nob; ++pb) {
+                const zc v = s.bath_v[idx6(il, jb, si, sj, io, pb, nl, nlb, nsp, nsp, no)];
+                h.vb[((size_t)sb * h.nb + lso(il, io, si)) * h.nbb + blso(jb, pb, sj)] =
+                    make_double2(v.real(), v.imag());
+              }
+      }
+    }
+  if (!s.om) s.om = std::make_unique<OmegaDev>();
+  s.om->upload(h, s.stream);
+  s.om_ready = true;
+}
+
+static std::vector<double> integrand(Solver &s, const std::vector<double2> &z, int prec) {
+  std::vector<double> f(z.size());
+  const int batch = 1024;
+  for (size_t i = 0; i < z.size(); i += batch) {
+    const int n = (int)std::min<size_t>(batch, z.size() - i);
+    float ms = 0;
+    s.om->integrand(n, z.data() + i, prec, f.data() + i, &ms);
+    s.res.omega_seconds += ms * 1e-3;
+    s.res.omega_evals += (int64_t)n * s.nk;
+  }
+  return f;
+}
+
+// Gauss-Kronrod 7/15 abscissae and weights (Kronrod 1965, as tabulated in
+// QUADPACK's qk15): x descending from the end point to the centre.
+static const double kXgk[8] = {0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+                               0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+                               0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+                               0.207784955007898467600689403773245, 0.000000000000000000000000000000000};
+static const double kWgk[8] = {0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+                               0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+                               0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+                               0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+static const double kWg[4] = {0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+                              0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+
+struct Panel {
+  double a, b, val, err;
+};
+
+// Nodes of the panel [a, b] in t, mapped to x = (1-t)/t (the QAGI map of
+// [0, inf) onto (0, 1]); fills 15 x values.
+static void panel_nodes(double a, double b, double *x) {
+  const double c = 0.5 * (a + b), hl = 0.5 * (b - a);
+  for (int j = 0; j < 7; ++j) {
+    const double t1 = c - hl * kXgk[j], t2 = c + hl * kXgk[j];
+    x[j] = (1.0 - t1) / t1;
+    x[14 - j] = (1.0 - t2) / t2;
+  }
+  x[7] = (1.0 - c) / c;
+}
+
+static void panel_eval(Panel &pn, const double *f /*15 values at the nodes*/) {
+  const double c = 0.5 * (pn.a + pn.b), hl = 0.5 * (pn.b - pn.a);
+  double g[15];
+  for (int j = 0; j < 7; ++j) {
+    const double t1 = c - hl * kXgk[j], t2 = c + hl * kXgk[j];
+    g[j] = f[j] / (t1 * t1);
+    g[14 - j] = f[14 - j] / (t2 * t2);
+  }
+  g[7] = f[7] / (c * c);
+  double rk = kWgk[7] * g[7], rg = kWg[3] * g[7];
+  for (int j = 0; j < 7; ++j) {
+    rk += kWgk[j] * (g[j] + g[14 - j]);
+    if (j & 1) rg += kWg[j / 2] * (g[j] + g[14 - j]);
+  }
+  pn.val = rk * hl;
+  pn.err = std::fabs((rk - rg) * hl);
+}
+
+// int_0^inf f(x) dx with f(x) = integrand at z = i x, adaptive bisection in
+// t with every panel of above-average error split in each round.
+static double quad_imag_axis(Solver &s, int prec) {
+  const double epsabs = s.p.quad_epsabs, epsrel = s.p.quad_epsrel;
+  std::vector<Panel> panels;
+  auto evaluate = [&](std::vector<Panel> &ps) {
+    std::vector<double2> z(ps.size() * 15);
+    std::vector<double> x(15);
+    for (size_t i = 0; i < ps.size(); ++i) {
+      panel_nodes(ps[i].a, ps[i].b, x.data());
+      for (int j = 0; j < 15; ++j) z[i * 15 + j] = make_double2(0.0, x[j]);
+    }
+    std::vector<double> f = integrand(s, z, prec);
+    for (size_t i = 0; i < ps.size(); ++i) panel_eval(ps[i], f.data() + i * 15);
+  };
+  std::vector<Panel> init = {{0.0, 1.0, 0, 0}};
+  evaluate(init);
+  panels = init;
+  for (int round = 0; round < 200; ++round) {
+    double tot = 0, err = 0;
+    for (auto &pn : panels) {
+      tot += pn.val;
+      err += pn.err;
+    }
+    const double tol = std::max(epsabs, epsrel * std::fabs(tot));
+    if (err <= tol || panels.size() > 20000) break;
+    const double thr = tol / (double)panels.size();
+    std::vector<Panel> keep, fresh;
+    // split the largest-error panel and all panels above the average share
+    size_t imax = 0;
+    for (size_t i = 1; i < panels.size(); ++i)
+      if (panels[i].err > panels[imax].err) imax = i;
+    for (size_t i = 0; i < panels.size(); ++i) {
+      const Panel &pn = panels[i];
+      if (i == imax || pn.err > thr) {
+        const double m = 0.5 * (pn.a + pn.b);
+        fresh.push_back({pn.a, m, 0, 0});
+        fresh.push_back({m, pn.b, 0, 0});
+      } else {
+        keep.push_back(pn);
+      }
+    }
+    evaluate(fresh);
+    keep.insert(keep.end(), fresh.begin(), fresh.end());
+    panels.swap(keep);
+  }
+  double tot = 0;
+  for (auto &pn : panels) tot += pn.val;
+  return tot;
+}
+
+static void omega_stage(Solver &s) {
+  const vca_params &p = s.p;
+  prepare_omega(s);
+  double integral;
+  if (p.omega_mode == VCA_OMEGA_MATSUBARA) {
+    std::vector<double2> z(p.omega_nfreq);
+    for (int n = 0; n < p.omega_nfreq; ++n)
+      z[n] = make_double2(0.0, (2.0 * n + 1.0) * M_PI / p.beta);
+    std::vector<double> f = integrand(s, z, p.precision);
+    double sum = 0;
+    for (double v : f) sum += v;
+    integral = (3.0 - p.nspin) * 2.0 / p.beta * sum;
+  } else {
+    integral = (3.0 - p.nspin) * quad_imag_axis(s, p.precision) / M_PI;
+  }
+  s.res.omega_integral = integral;
+  s.res.sft_potential = s.emin - integral;
+}
+
+static double now() {
+  return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch())
+      .count();
+}
+
+}  // namespace vca
+
+// ================================================================ C ABI ==
+using namespace vca;
+
+#define VCA_GUARD_BEGIN try {
+#define VCA_GUARD_END                         \
+  }                                           \
+  catch (const Error &e) {                    \
+    return e.code;                            \
+  }                                           \
+  catch (const std::bad_alloc &) {            \
+    set_error("host allocation failed");      \
+    return VCA_ERR_NOMEM;                     \
+  }                                           \
+  catch (...) {                               \
+    set_error("unexpected exception");        \
+    return VCA_ERR_ARG;                       \
+  }                                           \
+  return VCA_OK;
+
+extern "C" {
+
+void vca_default_params(vca_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->nlat = 1;
+  p->norb = 1;
+  p->nspin = 1;
+  p->hfmode = 1;
+  p->hfshift = 1;
+  p->uloc[0] = 2.0;
+  p->beta = 1000.0;
+  p->lanc_method = VCA_LANC_ARPACK;
+  p->lanc_nstates_sector = 6;
+  p->lanc_niter = 512;
+  p->lanc_ngfiter = 200;
+  p->lanc_dim_threshold = 256;
+  p->lanc_tolerance = 1e-12;
+  p->gs_threshold = 1e-9;
+  p->precision = VCA_FP64;
+  p->omega_mode = VCA_OMEGA_QUAD;
+  p->omega_nfreq = 4096;
+  p->quad_epsabs = 1e-10;
+  p->quad_epsrel = 1e-12;
+}
+
+const char *vca_last_error(void) { return g_err.c_str(); }
+
+int vca_init_solver(const vca_params *p, vca_solver **out) {
+  VCA_GUARD_BEGIN
+  if (!p || !out) fail(VCA_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (p->nlat < 1 || p->norb < 1 || p->norb > 5 || (p->nspin != 1 && p->nspin != 2) ||
+      p->nlat_bath < 0 || p->norb_bath < 0)
+    fail(VCA_ERR_ARG, "bad dimensions nlat=%d norb=%d nspin=%d", p->nlat, p->norb, p->nspin);
+  const int ns = p->nlat * p->norb + p->nlat_bath * p->norb_bath;
+  if (ns > kMaxNs) fail(VCA_ERR_ARG, "Ns=%d above the supported %d", ns, kMaxNs);
+  if (p->precision != VCA_FP64 && p->precision != VCA_MIXED)
+    fail(VCA_ERR_ARG, "unknown precision %d", p->precision);
+  auto s = std::make_unique<Solver>();
+  s->p = *p;
+  if (p->nsectors > 0) {
+    if (!p->sectors) fail(VCA_ERR_ARG, "nsectors > 0 with a null sector list");
+    s->sectors.assign(p->sectors, p->sectors + 2 * p->nsectors);
+  }
+  s->p.sectors = nullptr;
+  s->ns = ns;
+  s->nlso = p->nlat * p->nspin * p->norb;
+  int ndev = 0;
+  VCA_HIP(hipGetDeviceCount(&ndev));
+  if (ndev < 1) fail(VCA_ERR_HIP, "no HIP device");
+  VCA_HIP(hipSetDevice(0));
+  VCA_HIP(hipStreamCreate(&s->stream));
+  Binom bn;
+  VCA_HIP(hipMalloc(&s->d_binom, sizeof(bn.c)));
+  VCA_HIP(hipMemcpy(s->d_binom, bn.c, sizeof(bn.c), hipMemcpyHostToDevice));
+  *out = reinterpret_cast<vca_solver *>(s.release());
+  VCA_GUARD_END
+}
+
+void vca_finalize_solver(vca_solver *s) { delete reinterpret_cast<Solver *>(s); }
+
+int vca_set_hcluster(vca_solver *sv, const double *hloc) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s || !hloc) fail(VCA_ERR_ARG, "null argument");
+  const size_t n = (size_t)s->nlso * s->nlso;
+  s->hloc.assign(reinterpret_cast<const zc *>(hloc), reinterpret_cast<const zc *>(hloc) + n);
+  s->has_hloc = true;
+  if (s->p.nlat_bath * s->p.norb_bath == 0 || s->has_bath) derive_model(*s);
+  VCA_GUARD_END
+}
+
+int vca_set_bath(vca_solver *sv, const double *bath_h, const double *bath_v) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s || !bath_h || !bath_v) fail(VCA_ERR_ARG, "null argument");
+  const vca_params &p = s->p;
+  const int nbl = p.nlat_bath * p.nspin * p.norb_bath;
+  const size_t nh = (size_t)nbl * nbl, nv = (size_t)s->nlso * nbl;
+  if (nbl == 0) fail(VCA_ERR_ARG, "model has no bath");
+  s->bath_h.assign(reinterpret_cast<const zc *>(bath_h), reinterpret_cast<const zc *>(bath_h) + nh);
+  s->bath_v.assign(reinterpret_cast<const zc *>(bath_v), reinterpret_cast<const zc *>(bath_v) + nv);
+  // VCA_BATH_SETUP.f90:127-136
+  if (p.hfmode && !p.hfshift) {
+    const int nlb = p.nlat_bath, nsp = p.nspin, nob = p.norb_bath;
+    for (int il = 0; il < nlb; ++il)
+      for (int io = 0; io < nob; ++io)
+        for (int sp = 0; sp < nsp; ++sp)
+          s->bath_h[idx6(il, il, sp, sp, io, io, nlb, nlb, nsp, nsp, nob)] -=
+              0.5 * p.uloc[io] + 0.5 * p.ust * (p.norb - 1) + 0.5 * (p.ust - p.jh) * (p.norb - 1);
+  }
+  s->has_bath = true;
+  if (s->has_hloc) derive_model(*s);
+  VCA_GUARD_END
+}
+
+int vca_solve(vca_solver *sv, const double *hloc, const double *hk, int64_t nk,
+              const double *bath_h, const double *bath_v) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s || !hloc || !hk || nk < 1) fail(VCA_ERR_ARG, "null argument or nk < 1");
+  const bool bath = s->p.nlat_bath * s->p.norb_bath > 0;
+  if (bath && (!bath_h || !bath_v)) fail(VCA_ERR_ARG, "model has a bath: bath_h/bath_v needed");
+  s->res = vca_results{};
+  s->solved = false;
+  s->om_ready = false;
+  if (bath) {
+    int rc = vca_set_bath(sv, bath_h, bath_v);
+    if (rc) return rc;
+  }
+  int rc = vca_set_hcluster(sv, hloc);
+  if (rc) return rc;
+  const size_t per_k = (size_t)s->nlso * s->nlso;
+  s->hk.assign(reinterpret_cast<const zc *>(hk), reinterpret_cast<const zc *>(hk) + per_k * nk);
+  s->nk = nk;
+  double t0 = now();
+  diagonalize(*s);
+  double t1 = now();
+  build_gf(*s);
+  double t2 = now();
+  omega_stage(*s);
+  double t3 = now();
+  s->res.t_diag = t1 - t0;
+  s->res.t_gf = t2 - t1;
+  s->res.t_omega = t3 - t2;
+  s->solved = true;
+  VCA_GUARD_END
+}
+
+int vca_get_sft_potential(const vca_solver *sv, double *out) {
+  VCA_GUARD_BEGIN
+  const Solver *s = reinterpret_cast<const Solver *>(sv);
+  if (!s || !out) fail(VCA_ERR_ARG, "null argument");
+  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  *out = s->res.sft_potential;
+  VCA_GUARD_END
+}
+
+int vca_get_results(const vca_solver *sv, vca_results *out) {
+  VCA_GUARD_BEGIN
+  const Solver *s = reinterpret_cast<const Solver *>(sv);
+  if (!s || !out) fail(VCA_ERR_ARG, "null argument");
+  *out = s->res;
+  VCA_GUARD_END
+}
+
+int vca_sector_create(vca_solver *sv, int32_t nup, int32_t ndw, int32_t precision,
+                      vca_sector **out) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s || !out) fail(VCA_ERR_ARG, "null argument");
+  if (precision != VCA_FP64 && precision != VCA_MIXED) fail(VCA_ERR_ARG, "bad precision");
+  *out = reinterpret_cast<vca_sector *>(make_sector(*s, nup, ndw, precision).release());
+  VCA_GUARD_END
+}
+
+void vca_sector_destroy(vca_sector *sec) { delete reinterpret_cast<Sector *>(sec); }
+
+int vca_sector_dims(const vca_sector *sv, int64_t *dim, int32_t *dimup, int32_t *dimdw) {
+  VCA_GUARD_BEGIN
+  const Sector *sec = reinterpret_cast<const Sector *>(sv);
+  if (!sec) fail(VCA_ERR_ARG, "null sector");
+  if (dim) *dim = sec->dim;
+  if (dimup) *dimup = sec->up.dim;
+  if (dimdw) *dimdw = sec->dw.dim;
+  VCA_GUARD_END
+}
+
+int vca_hxv(vca_sector *sv, int64_t n, const double *v, double *hv) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !v || !hv) fail(VCA_ERR_ARG, "null argument");
+  if (n != sec->dim) fail(VCA_ERR_ARG, "vector length %lld != sector dimension %lld",
+                          (long long)n, (long long)sec->dim);
+  void *d64 = nullptr, *din = nullptr, *dout = nullptr;
+  const size_t b64 = sizeof(double2) * n;
+  VCA_HIP(hipMalloc(&d64, b64));
+  VCA_HIP(hipMalloc(&din, sec->vec_bytes()));
+  VCA_HIP(hipMalloc(&dout, sec->vec_bytes()));
+  VCA_HIP(hipMemcpyAsync(d64, v, b64, hipMemcpyHostToDevice, sec->stream));
+  launch_convert(d64, VCA_FP64, din, sec->precision, n, sec->stream);
+  launch_hxv(*sec, din, din, dout, false);
+  launch_convert(dout, sec->precision, d64, VCA_FP64, n, sec->stream);
+  VCA_HIP(hipMemcpyAsync(hv, d64, b64, hipMemcpyDeviceToHost, sec->stream));
+  VCA_HIP(hipStreamSynchronize(sec->stream));
+  VCA_HIP(hipFree(d64));
+  VCA_HIP(hipFree(din));
+  VCA_HIP(hipFree(dout));
+  VCA_GUARD_END
+}
+
+int vca_hxv_device(vca_sector *sv, const void *v, void *hv, int32_t repeat, float *ms) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !v || !hv) fail(VCA_ERR_ARG, "null argument");
+  if (repeat < 1) repeat = 1;
+  hipEvent_t e0, e1;
+  VCA_HIP(hipEventCreate(&e0));
+  VCA_HIP(hipEventCreate(&e1));
+  VCA_HIP(hipEventRecord(e0, sec->stream));
+  for (int r = 0; r < repeat; ++r) launch_hxv(*sec, v, v, hv, false);
+  VCA_HIP(hipEventRecord(e1, sec->stream));
+  VCA_HIP(hipEventSynchronize(e1));
+  if (ms) {
+    float t = 0;
+    VCA_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeat;
+  }
+  VCA_HIP(hipEventDestroy(e0));
+  VCA_HIP(hipEventDestroy(e1));
+  VCA_GUARD_END
+}
+
+int vca_sector_spin_coo(const vca_sector *sv, int32_t spin, int64_t *nnz, int32_t *rows,
+                        int32_t *cols, double *amps) {
+  VCA_GUARD_BEGIN
+  const Sector *sec = reinterpret_cast<const Sector *>(sv);
+  if (!sec || !nnz || (spin != 1 && spin != 2)) fail(VCA_ERR_ARG, "bad argument");
+  const DevSpin &sp = spin == 1 ? sec->up : sec->dw;
+  const SpinModel &m = sec->solver->md.spin[spin - 1];
+  std::vector<int32_t> cnt(sp.dim);
+  std::vector<uint32_t> ent((size_t)sp.dim * sp.maxnnz);
+  VCA_HIP(hipMemcpy(cnt.data(), sp.cnt, sizeof(int32_t) * sp.dim, hipMemcpyDeviceToHost));
+  VCA_HIP(hipMemcpy(ent.data(), sp.ent, sizeof(uint32_t) * ent.size(), hipMemcpyDeviceToHost));
+  int64_t n = 0;
+  for (int i = 0; i < sp.dim; ++i)
+    for (int q = 0; q < cnt[i]; ++q) {
+      if (rows) {
+        const uint32_t e = ent[(size_t)q * sp.dim + i];
+        const uint32_t ai = e >> kColBits;
+        const Term &t = m.terms[ai >> 1];
+        const double sg = (ai & 1u) ? -1.0 : 1.0;
+        rows[n] = i;
+        cols[n] = (int32_t)(e & kColMask);
+        amps[2 * n] = sg * t.re;
+        amps[2 * n + 1] = sg * t.im;
+      }
+      ++n;
+    }
+  *nnz = n;
+  VCA_GUARD_END
+}
+
+int vca_lanczos_gs(vca_sector *sv, double tol, int32_t niter, double *e0, double *vec,
+                   int32_t *iters) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !e0) fail(VCA_ERR_ARG, "null argument");
+  GsOut g{};
+  std::unique_ptr<Sector> twin;
+  Sector *res_sec = sec;
+  if (sec->precision == VCA_MIXED) {
+    twin = make_sector(*sec->solver, sec->nup, sec->ndw, VCA_FP64);
+    g = gs_mixed(*sec->solver, *sec, *twin, tol, niter, nullptr);
+    res_sec = twin.get();
+  } else {
+    g = gs_lanczos(*sec, tol, niter, nullptr, false, nullptr);
+  }
+  *e0 = g.e0;
+  if (iters) *iters = g.m;
+  if (vec)
+    VCA_HIP(hipMemcpy(vec, res_sec->xvec, sizeof(double2) * sec->dim, hipMemcpyDeviceToHost));
+  VCA_GUARD_END
+}
+
+int vca_lanczos_tridiag(vca_sector *sv, const double *v0, int32_t nlanc, double *alpha,
+                        double *beta) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !v0 || !alpha || !beta || nlanc < 1) fail(VCA_ERR_ARG, "bad argument");
+  ensure_lanczos(*sec, nlanc + 2);
+  void *d64 = nullptr;
+  VCA_HIP(hipMalloc(&d64, sizeof(double2) * sec->dim));
+  VCA_HIP(hipMemcpyAsync(d64, v0, sizeof(double2) * sec->dim, hipMemcpyHostToDevice,
+                         sec->stream));
+  launch_convert(d64, VCA_FP64, sec->buf[0], sec->precision, sec->dim, sec->stream);
+  launch_norm0(*sec);
+  double nrm0;
+  tridiag_run(*sec, nlanc, alpha, beta, &nrm0, nullptr);
+  VCA_HIP(hipFree(d64));
+  VCA_GUARD_END
+}
+
+int vca_gf_poles(const vca_solver *sv, int32_t ilat, int32_t jlat, int32_t ispin, int32_t iorb,
+                 int32_t jorb, int32_t istate, int32_t ichan, int32_t *n, double *weights,
+                 double *poles) {
+  VCA_GUARD_BEGIN
+  const Solver *s = reinterpret_cast<const Solver *>(sv);
+  if (!s || !n) fail(VCA_ERR_ARG, "null argument");
+  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  auto it = s->gf.find({ilat, jlat, ispin, iorb, jorb});
+  if (it == s->gf.end()) fail(VCA_ERR_ARG, "no such element");
+  if (istate < 1 || istate > (int)it->second.size()) fail(VCA_ERR_ARG, "bad state index");
+  const auto &chans = it->second[istate - 1];
+  if (ichan < 1 || ichan > (int)chans.size()) fail(VCA_ERR_ARG, "bad channel index");
+  const GfChan &c = chans[ichan - 1];
+  *n = (int32_t)c.w.size();
+  if (weights)
+    for (size_t j = 0; j < c.w.size(); ++j) {
+      weights[2 * j] = c.w[j].real();
+      weights[2 * j + 1] = c.w[j].imag();
+    }
+  if (poles)
+    for (size_t j = 0; j < c.p.size(); ++j) poles[j] = c.p[j];
+  VCA_GUARD_END
+}
+
+int vca_gf_cluster(const vca_solver *sv, double zre, double zim, double *gfo) {
+  VCA_GUARD_BEGIN
+  const Solver *s = reinterpret_cast<const Solver *>(sv);
+  if (!s || !gfo) fail(VCA_ERR_ARG, "null argument");
+  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  const vca_params &p = s->p;
+  const int nl = p.nlat, no = p.norb, nsp = p.nspin;
+  const zc z(zre, zim);
+  std::vector<zc> g((size_t)s->nlso * s->nlso, zc(0.0));
+  auto at = [&](int il, int jl, int sp, int io, int jo) -> zc & {
+    return g[idx6(il, jl, sp, sp, io, jo, nl, nl, nsp, nsp, no)];
+  };
+  for (auto &kv : s->gf) {
+    zc acc = 0;
+    for (auto &chans : kv.second)
+      for (auto &c : chans)
+        for (size_t j = 0; j < c.w.size(); ++j) acc += c.w[j] / (z - c.p[j]);
+    at(kv.first[0] - 1, kv.first[1] - 1, kv.first[2] - 1, kv.first[3] - 1, kv.first[4] - 1) = acc;
+  }
+  const double chan4 = p.gf_symmetric ? 0.0 : 1.0;
+  const zc f(1.0, -chan4);
+  for (int sp = 0; sp < nsp; ++sp)
+    for (int io = 0; io < no; ++io)
+      for (int jo = 0; jo < no; ++jo)
+        for (int il = 0; il < nl; ++il)
+          for (int jl = 0; jl < nl; ++jl) {
+            if (il == jl && io == jo) continue;
+            at(il, jl, sp, io, jo) =
+                0.5 * (at(il, jl, sp, io, jo) - f * at(il, il, sp, io, io) - f * at(jl, jl, sp, jo, jo));
+          }
+  std::memcpy(gfo, g.data(), sizeof(zc) * g.size());
+  VCA_GUARD_END
+}
+
+int vca_omega_integrand(vca_solver *sv, int32_t nz, const double *z, int32_t precision,
+                        double *f) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s || !z || !f || nz < 1) fail(VCA_ERR_ARG, "bad argument");
+  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  if (!s->om_ready) prepare_omega(*s);
+  std::vector<double2> zz(nz);
+  for (int i = 0; i < nz; ++i) zz[i] = make_double2(z[2 * i], z[2 * i + 1]);
+  std::vector<double> out = integrand(*s, zz, precision);
+  std::memcpy(f, out.data(), sizeof(double) * nz);
+  VCA_GUARD_END
+}
+
+}  // extern "C"

@@ -1,0 +1,238 @@
+// vca_internal.hpp -- shared types of the MI355X VCA solver core.
+//
+// Layout of a sector (nup, ndw) in HBM:
+//   state i = iup + idw * DimUp (0-based), the reference's layout
+//   (VCA_SETUP.f90:505-518); a vector is complex128 (FP64) or complex64
+//   (MIXED), interleaved (re, im).
+//   Per spin sigma: map[dim] (ascending strings of popcount n, the order of
+//   build_sector, VCA_SETUP.f90:679-684), energy[dim] (one-spin diagonal
+//   energy), cnt[dim] (hops per row) and ent[maxnnz][dim] (k-major ELL of
+//   the one-spin hop matrix H_sigma(row, col): col in bits 0..20, signed
+//   amplitude index 2*term+neg in bits 21..31).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <complex>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/vca_gpu.h"
+
+namespace vca {
+
+using zc = std::complex<double>;
+
+constexpr int kMaxNs = 24;        // strings fit 32 bits; dims fit 21 bits
+constexpr int kColBits = 21;
+constexpr uint32_t kColMask = (1u << kColBits) - 1u;
+constexpr int kMaxTerms = 1024;   // 2*terms must fit the 11 high bits
+constexpr int kBinomN = kMaxNs + 1;
+
+struct Error {
+  int code;
+};
+[[noreturn]] void fail(int code, const char *fmt, ...);
+void set_error(const char *fmt, ...);
+
+#define VCA_HIP(call)                                                       \
+  do {                                                                      \
+    hipError_t e_ = (call);                                                 \
+    if (e_ != hipSuccess)                                                   \
+      ::vca::fail(VCA_ERR_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #call,   \
+                  hipGetErrorString(e_));                                   \
+  } while (0)
+
+// ---------------------------------------------------------------- model --
+// One off-diagonal one-body term c^+_a c_b (0-based orbital positions) of
+// one spin, with its amplitude.
+struct Term {
+  int a, b;
+  double re, im;
+};
+
+struct SpinModel {
+  std::vector<Term> terms;      // nonzero terms, reference insertion order
+  double eps[kMaxNs];           // linear energy per orbital (incl. -xmu, hf)
+  double pair[kMaxNs][kMaxNs];  // same-spin density-density, a < b
+  bool real_amp;
+};
+
+// Hamiltonian of the cluster in the form the kernels consume:
+//   H = sum_sigma sum_{a!=b} T_sigma(a,b) c^+_a c_b
+//     + sum_sigma E_sigma(string_sigma) + cross(up, dw) + e_const
+// cross(up,dw) = sum_{io,jo} wx[io][jo] popc((up>>io) & (dw>>jo) & site_mask)
+struct ModelData {
+  int ns = 0, nlat = 0, norb = 0, nspin = 0, nlat_bath = 0, norb_bath = 0;
+  SpinModel spin[2];
+  double wx[5][5] = {};
+  uint32_t site_mask = 0;
+  double e_const = 0;
+};
+
+// Binomial table C(n, k), n, k < kBinomN, row-major [n][k].
+struct Binom {
+  uint32_t c[kBinomN * kBinomN];
+  Binom();
+  uint32_t operator()(int n, int k) const {
+    return (k < 0 || k > n || n < 0) ? 0u : c[n * kBinomN + k];
+  }
+};
+
+// ------------------------------------------------- Fock-space arithmetic --
+// Rank of a string among the strings of equal popcount in ascending order:
+// rank(s) = sum_k C(p_k, k) over set-bit positions p_1 < p_2 < ... (k from
+// 1).  Equals binary_search(map, s) - 1 (VCA_SETUP.f90:946-963) on the
+// ascending map, checked by brute force in the tests.
+__host__ __device__ inline uint32_t rank_string(uint32_t s, const uint32_t *binom) {
+  uint32_t r = 0;
+  int k = 1;
+  while (s) {
+    int p = __builtin_ctz(s);
+    r += binom[p * kBinomN + k];
+    ++k;
+    s &= s - 1u;
+  }
+  return r;
+}
+
+__host__ __device__ inline uint32_t unrank_string(int ns, int n, uint32_t r,
+                                                  const uint32_t *binom) {
+  uint32_t s = 0;
+  int p = ns - 1;
+  for (int k = n; k >= 1; --k) {
+    while (binom[p * kBinomN + k] > r) --p;
+    s |= 1u << p;
+    r -= binom[p * kBinomN + k];
+    --p;
+  }
+  return s;
+}
+
+// Parity of the number of set bits of s strictly below position pos.
+__host__ __device__ inline uint32_t parity_below(uint32_t s, int pos) {
+  return (uint32_t)__builtin_popcount(s & ((1u << pos) - 1u)) & 1u;
+}
+
+// Hops into row string s: for every term (a <- b) with a occupied and b
+// empty in s, the column string j = s - a + b and the sign of
+// c^+_a c_b acting on j, computed as the reference does: c(b) on j then
+// c^+(a) on the result (VCA_SETUP.f90:724-750), each sign the parity of the
+// occupied orbitals below the position.  Writes packed entries to
+// out[q*stride] and returns their count.
+__host__ __device__ inline int row_hops(uint32_t s, int nterms, const int2 *ab,
+                                        const uint32_t *binom, uint32_t *out,
+                                        int64_t stride) {
+  int q = 0;
+  for (int t = 0; t < nterms; ++t) {
+    const int a = ab[t].x, b = ab[t].y;
+    if (((s >> a) & 1u) == 0u || ((s >> b) & 1u) != 0u) continue;
+    const uint32_t j = (s ^ (1u << a)) | (1u << b);
+    const uint32_t k1 = j & ~(1u << b);
+    const uint32_t neg = parity_below(j, b) ^ parity_below(k1, a);
+    const uint32_t col = rank_string(j, binom);
+    out[(int64_t)q * stride] = col | ((uint32_t)(2 * t) + neg) << kColBits;
+    ++q;
+  }
+  return q;
+}
+
+__host__ __device__ inline double spin_energy(uint32_t s, int ns, const double *eps,
+                                              const double *pair /*[ns][ns]*/) {
+  double e = 0;
+  for (int a = 0; a < ns; ++a)
+    if ((s >> a) & 1u) {
+      e += eps[a];
+      for (int b = a + 1; b < ns; ++b)
+        if ((s >> b) & 1u) e += pair[a * kMaxNs + b];
+    }
+  return e;
+}
+
+// Host copy of one spin table (dense path and CPU tests).
+struct HostSpinTable {
+  int npart = 0, dim = 0, maxnnz = 0;
+  std::vector<uint32_t> map;
+  std::vector<double> energy;
+  std::vector<int32_t> cnt;
+  std::vector<uint32_t> ent;
+};
+
+// ------------------------------------------------------------ device ----
+struct LanczosDev {
+  int32_t k;     // current step
+  int32_t done;  // exact breakdown seen
+  uint32_t counter[4];
+  double *alpha, *beta, *nrm, *y;  // device arrays
+};
+
+struct DevSpin {
+  int npart = 0, dim = 0, maxnnz = 0, nterms = 0;
+  bool real_amp = true;
+  uint32_t *map = nullptr;
+  double *energy = nullptr;
+  int32_t *cnt = nullptr;
+  uint32_t *ent = nullptr;
+  double2 *amp64 = nullptr;  // [2*nterms]: +amp, -amp
+  float2 *amp32 = nullptr;
+};
+
+struct CrossTerm {
+  double wx[5][5];
+  uint32_t site_mask;
+  int norb;
+};
+
+struct Solver;
+
+struct Sector {
+  Solver *solver = nullptr;
+  int nup = 0, ndw = 0, precision = VCA_FP64;
+  int64_t dim = 0;
+  DevSpin up, dw;
+  hipStream_t stream = nullptr;
+  // Lanczos workspace (lazily allocated)
+  int lz_cap = 0;
+  void *buf[3] = {nullptr, nullptr, nullptr};  // u_even, u_odd, hv
+  void *xvec = nullptr;                        // eigenvector accumulator
+  double *partials = nullptr;
+  int partials_cap = 0;
+  LanczosDev *lz = nullptr;  // device struct
+  LanczosDev lzh{};          // host copy of the device array pointers
+  double *lz_arrays = nullptr;
+  size_t vec_bytes() const {
+    return (size_t)dim * (precision == VCA_FP64 ? sizeof(double2) : sizeof(float2));
+  }
+  ~Sector();
+};
+
+// Kernel launch wrappers (kernels.hip)
+void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns,
+                       int npart, const uint32_t *d_binom, double e_add);
+CrossTerm sector_cross(const Sector &sec);
+const uint32_t *solver_binom(const Solver *s);
+void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos);
+void launch_lanczos_update(Sector &sec, bool accumulate_x);
+void launch_start_vector(Sector &sec, uint64_t seed);
+void launch_norm0(Sector &sec);  // nrm[0] = ||buf[0]||, k = 0, done = 0
+void launch_scale_x(Sector &sec);  // xvec /= ||xvec||
+void launch_convert(const void *src, int src_prec, void *dst, int dst_prec,
+                    int64_t n, hipStream_t st);
+void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int nops,
+                   const int *spin, const int *pos, const int *dagger,
+                   const double *coef_re, const double *coef_im);
+CrossTerm make_cross(const ModelData &md);
+
+// Host linear algebra (linalg.cpp)
+// Symmetric tridiagonal eigenproblem, diag d[n], offdiag e[n] with e[i]
+// coupling i-1 and i (e[0] unused).  Eigenvalues ascending in w; if z is
+// not null, z is n x n column-major eigenvectors.
+void tridiag_eig(int n, const double *d, const double *e, double *w, double *z);
+// Lowest eigenvalue of the leading m x m block.
+double tridiag_lowest(int m, const double *d, const double *e);
+// Complex Hermitian eigenproblem (column-major a, n x n, overwritten by the
+// eigenvectors), eigenvalues ascending in w.
+void herm_eig(int n, zc *a, double *w);
+
+}  // namespace vca
