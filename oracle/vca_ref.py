@@ -463,6 +463,10 @@ def build_gf(model: Model, states: list, matvec_factory=None):
         if jsec is None:
             return (np.zeros(0, np.complex128), np.zeros(0))
         norm2 = np.vdot(v, v).real
+        if norm2 == 0.0:
+            # O|psi> = 0: the reference would divide by zero
+            # (VCA_GF_NORMAL.f90:201-203); an empty channel is its limit
+            return (np.zeros(0, np.complex128), np.zeros(0))
         v = v / math.sqrt(norm2)
         du, dd = model.dims(*jsec)
         nlanc = min(du * dd, p.lanc_ngfiter)

@@ -1,0 +1,50 @@
+"""The Fortran drop-in: module vca_gpu (iso_c_binding over the C ABI) and a
+reference-style driver compiled with the image's Fortran compiler."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FDIR = os.path.join(ROOT, "vca_amd", "fortran")
+FC = shutil.which("amdflang") or "/opt/rocm/bin/amdflang"
+
+needs_fc = pytest.mark.skipif(not os.path.exists(FC), reason="no Fortran compiler in the image")
+
+
+def _build(tmp_path):
+    mod = tmp_path / "mod"
+    mod.mkdir()
+    obj = tmp_path / "vca_gpu.o"
+    subprocess.check_call([FC, "-c", "-fPIC", f"-J{mod}", os.path.join(FDIR, "vca_gpu.f90"),
+                           "-o", str(obj)])
+    exe = tmp_path / "example_chain4"
+    libdir = os.path.join(ROOT, "vca_amd")
+    subprocess.check_call([FC, f"-I{mod}", os.path.join(FDIR, "example_chain4.f90"), str(obj),
+                           f"-L{libdir}", "-lvca_gpu", f"-Wl,-rpath,{libdir}", "-o", str(exe)])
+    return exe
+
+
+@needs_fc
+def test_fortran_module_and_driver_compile(tmp_path):
+    exe = _build(tmp_path)
+    assert exe.exists()
+
+
+@needs_fc
+@pytest.mark.gpu
+def test_fortran_driver_matches_python_mirror(tmp_path):
+    from vca_amd import configs
+    from vca_amd.api import VcaSolver
+
+    exe = _build(tmp_path)
+    print("running the Fortran driver", flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    line = [x for x in r.stdout.splitlines() if x.startswith("SFT_POTENTIAL")][0]
+    sft_f = float(line.split()[1])
+    p, hloc, hk, _, _ = configs.config_chain4(nk=16)
+    sft_py = VcaSolver(p).vca_solve(hloc, hk)
+    assert abs(sft_f - sft_py) < 1e-12 * abs(sft_py)
+    assert (tmp_path / "SFT_potential.vca").exists()

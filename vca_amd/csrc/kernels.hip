@@ -111,8 +111,9 @@ struct HxvArgs {
   const void *amp_up, *amp_dw;
   int namp_up, namp_dw;
   CrossTerm cross;
-  LanczosDev *lz;  // null: plain product
+  LanczosDev *lz;  // null: plain product; else one per chain (blockIdx.y)
   double *partials;
+  int64_t vstride;  // elements between the vectors of consecutive chains
 };
 
 __device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t sd) {
@@ -150,17 +151,19 @@ __global__ __launch_bounds__(kBlock) void k_hxv(const HxvArgs a) {
   for (int t = threadIdx.x; t < a.namp_up; t += kBlock) s_amp_up[t] = g_amp_up[t];
   for (int t = threadIdx.x; t < a.namp_dw; t += kBlock) s_amp_dw[t] = g_amp_dw[t];
 
+  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
   int k = 0;
   bool done = false;
   R scale = (R)1;
-  const C *vin = static_cast<const C *>(a.v0);
-  if (a.lz) {
-    k = a.lz->k;
-    done = a.lz->done != 0;
-    if (k & 1) vin = static_cast<const C *>(a.v1);
-    if (!done) scale = (R)(1.0 / a.lz->nrm[k]);
+  const C *vin = static_cast<const C *>(a.v0) + chain_off;
+  if (lz) {
+    k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const C *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
   }
-  C *hv = static_cast<C *>(a.hv);
+  C *hv = static_cast<C *>(a.hv) + chain_off;
   const int dimup = a.dimup;
   const int64_t col0 = (int64_t)blockIdx.x * a.cpb;
   const int ncol = (int)min((int64_t)a.cpb, (int64_t)a.dimdw - col0);
@@ -205,12 +208,13 @@ __global__ __launch_bounds__(kBlock) void k_hxv(const HxvArgs a) {
       out.x = ar * scale;
       out.y = ai * scale;
       hv[col0 * dimup + f] = out;
-      if (a.lz) dot += (double)(x.x * scale) * (double)out.x + (double)(x.y * scale) * (double)out.y;
+      if (lz) dot += (double)(x.x * scale) * (double)out.x + (double)(x.y * scale) * (double)out.y;
     }
   }
-  if (a.lz) {
+  if (lz) {
     double tot;
-    if (grid_sum(dot, a.partials, &a.lz->counter[0], &tot)) a.lz->alpha[k] = done ? 0.0 : tot;
+    if (grid_sum(dot, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[0], &tot))
+      lz->alpha[k] = done ? 0.0 : tot;
   }
 }
 
@@ -220,21 +224,23 @@ struct UpdArgs {
   const void *hv;
   void *x;        // eigenvector accumulator or null
   int64_t n;
-  LanczosDev *lz;
+  LanczosDev *lz;  // one per chain (blockIdx.y)
   double *partials;
+  int64_t vstride;
 };
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_lanczos_update(const UpdArgs a) {
   using C = cplx<R>;
-  LanczosDev *lz = a.lz;
+  const int64_t off = (int64_t)blockIdx.y * a.vstride;
+  LanczosDev *lz = a.lz + blockIdx.y;
   const int k = lz->k;
   const bool done = lz->done != 0;
   double ss = 0.0;
   if (!done) {
-    const C *uk = static_cast<const C *>((k & 1) ? a.u1 : a.u0);
-    C *up = static_cast<C *>((k & 1) ? a.u0 : a.u1);
-    const C *hv = static_cast<const C *>(a.hv);
+    const C *uk = static_cast<const C *>((k & 1) ? a.u1 : a.u0) + off;
+    C *up = static_cast<C *>((k & 1) ? a.u0 : a.u1) + off;
+    const C *hv = static_cast<const C *>(a.hv) + off;
     const R ca = (R)(lz->alpha[k] / lz->nrm[k]);
     const R cb = k > 0 ? (R)(lz->beta[k] / lz->nrm[k - 1]) : (R)0;
     C *x = static_cast<C *>(a.x);
@@ -262,7 +268,7 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_update(const UpdArgs a) {
     }
   }
   double tot;
-  if (grid_sum(ss, a.partials, &lz->counter[1], &tot)) {
+  if (grid_sum(ss, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[1], &tot)) {
     if (!done) {
       const double b = sqrt(tot);
       if (!(b > 1e-300)) {  // exact breakdown (or NaN)
@@ -384,18 +390,20 @@ struct ExcArgs {
   int dimup_j, dimdw_j, dimup_i, dimdw_i;
   const uint32_t *map_j;  // the J-sector map of the acted-on spin
   const uint32_t *binom;
-  int nops, spin;
-  int pos[2], dagger[2];
-  double cre[2], cim[2];
-  LanczosDev *lz;
+  int spin;
+  const ExcOp *ops;  // one per chain (blockIdx.y)
+  LanczosDev *lz;    // one per chain
   double *partials;
+  int64_t vstride;
 };
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
   using C = cplx<R>;
   const C *psi = static_cast<const C *>(a.psi);
-  C *out = static_cast<C *>(a.out);
+  C *out = static_cast<C *>(a.out) + (int64_t)blockIdx.y * a.vstride;
+  const ExcOp op = a.ops[blockIdx.y];
+  LanczosDev *lz = a.lz + blockIdx.y;
   const int64_t n = (int64_t)a.dimup_j * a.dimdw_j;
   const int64_t stride = (int64_t)gridDim.x * kBlock;
   double ss = 0;
@@ -403,11 +411,11 @@ __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
     const int jup = (int)(j % a.dimup_j), jdw = (int)(j / a.dimup_j);
     const uint32_t t = a.map_j[a.spin == 1 ? jup : jdw];
     double vr = 0, vi = 0;
-    for (int o = 0; o < a.nops; ++o) {
-      const int pos = a.pos[o];
+    for (int o = 0; o < op.nops; ++o) {
+      const int pos = op.pos[o];
       const uint32_t bit = 1u << pos;
       uint32_t s;
-      if (a.dagger[o]) {
+      if (op.dagger[o]) {
         if (!(t & bit)) continue;
         s = t & ~bit;
       } else {
@@ -420,8 +428,8 @@ __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
                                     : (int64_t)jup + (int64_t)r * a.dimup_i;
       const C p = psi[i];
       // coef * sg * psi
-      vr += sg * (a.cre[o] * (double)p.x - a.cim[o] * (double)p.y);
-      vi += sg * (a.cre[o] * (double)p.y + a.cim[o] * (double)p.x);
+      vr += sg * (op.cre[o] * (double)p.x - op.cim[o] * (double)p.y);
+      vi += sg * (op.cre[o] * (double)p.y + op.cim[o] * (double)p.x);
     }
     C c;
     c.x = (R)vr;
@@ -430,11 +438,11 @@ __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
     ss += (double)c.x * c.x + (double)c.y * c.y;
   }
   double tot;
-  if (grid_sum(ss, a.partials, &a.lz->counter[2], &tot)) {
-    a.lz->nrm[0] = sqrt(tot);
-    a.lz->beta[0] = 0.0;
-    a.lz->k = 0;
-    a.lz->done = 0;
+  if (grid_sum(ss, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[2], &tot)) {
+    lz->nrm[0] = sqrt(tot);
+    lz->beta[0] = 0.0;
+    lz->k = 0;
+    lz->done = 0;
   }
 }
 
@@ -498,33 +506,40 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
   VCA_HIP(hipFree(d_pair));
 }
 
+// Per-block partial sums of the grid reductions.  Sized once for the
+// largest grid any kernel of the sector uses, so it is never reallocated
+// while launches are queued; a larger request first drains the stream.
 static void ensure_partials(Sector &sec, int nblocks) {
   if (sec.partials_cap >= nblocks) return;
-  if (sec.partials) VCA_HIP(hipFree(sec.partials));
-  const int cap = std::max(nblocks, 4096);
+  if (sec.partials) {
+    VCA_HIP(hipStreamSynchronize(sec.stream));
+    VCA_HIP(hipFree(sec.partials));
+  }
+  const int cap = std::max(nblocks, std::max(4096, sec.dw.dim) * std::max(1, sec.nb_cap));
   VCA_HIP(hipMalloc(&sec.partials, sizeof(double) * cap));
   sec.partials_cap = cap;
 }
 
 template <typename R, bool REAL_AMP>
-static void launch_hxv_t(Sector &sec, HxvArgs &a, bool stage, bool onecol, int grid,
+static void launch_hxv_t(Sector &sec, HxvArgs &a, bool stage, bool onecol, dim3 grid,
                          size_t smem) {
   const hipStream_t st = sec.stream;
   if (stage) {
     if (onecol)
-      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, true, true>), dim3(grid), dim3(kBlock), smem, st, a);
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, true, true>), grid, dim3(kBlock), smem, st, a);
     else
-      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, true, false>), dim3(grid), dim3(kBlock), smem, st, a);
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, true, false>), grid, dim3(kBlock), smem, st, a);
   } else {
     if (onecol)
-      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, false, true>), dim3(grid), dim3(kBlock), smem, st, a);
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, false, true>), grid, dim3(kBlock), smem, st, a);
     else
-      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, false, false>), dim3(grid), dim3(kBlock), smem, st, a);
+      hipLaunchKernelGGL((k_hxv<R, REAL_AMP, false, false>), grid, dim3(kBlock), smem, st, a);
   }
 }
 
-void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos) {
+void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos, int nb) {
   HxvArgs a{};
+  a.vstride = sec.dim;
   a.v0 = v0;
   a.v1 = v1;
   a.hv = hv;
@@ -555,22 +570,24 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   const size_t col_bytes = (size_t)cpb * a.dimup * esz;
   const bool stage = amp_bytes + col_bytes <= kLdsBudget;
   const size_t smem = amp_bytes + (stage ? col_bytes : 0);
-  if (lanczos) ensure_partials(sec, grid);
+  if (lanczos) ensure_partials(sec, grid * nb);
   a.partials = sec.partials;
   const bool real_amp = sec.up.real_amp && sec.dw.real_amp;
   const bool onecol = cpb == 1;
+  const dim3 g3(grid, nb);
   if (fp64) {
-    if (real_amp) launch_hxv_t<double, true>(sec, a, stage, onecol, grid, smem);
-    else launch_hxv_t<double, false>(sec, a, stage, onecol, grid, smem);
+    if (real_amp) launch_hxv_t<double, true>(sec, a, stage, onecol, g3, smem);
+    else launch_hxv_t<double, false>(sec, a, stage, onecol, g3, smem);
   } else {
-    if (real_amp) launch_hxv_t<float, true>(sec, a, stage, onecol, grid, smem);
-    else launch_hxv_t<float, false>(sec, a, stage, onecol, grid, smem);
+    if (real_amp) launch_hxv_t<float, true>(sec, a, stage, onecol, g3, smem);
+    else launch_hxv_t<float, false>(sec, a, stage, onecol, g3, smem);
   }
   VCA_HIP(hipGetLastError());
 }
 
-void launch_lanczos_update(Sector &sec, bool accumulate_x) {
+void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
   UpdArgs a{};
+  a.vstride = sec.dim;
   a.u0 = sec.buf[0];
   a.u1 = sec.buf[1];
   a.hv = sec.buf[2];
@@ -578,12 +595,12 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x) {
   a.n = sec.dim;
   a.lz = sec.lz;
   const int grid = grid_for(sec.dim);
-  ensure_partials(sec, grid);
+  ensure_partials(sec, grid * nb);
   a.partials = sec.partials;
   if (sec.precision == VCA_FP64)
-    hipLaunchKernelGGL(k_lanczos_update<double>, dim3(grid), dim3(kBlock), 0, sec.stream, a);
+    hipLaunchKernelGGL(k_lanczos_update<double>, dim3(grid, nb), dim3(kBlock), 0, sec.stream, a);
   else
-    hipLaunchKernelGGL(k_lanczos_update<float>, dim3(grid), dim3(kBlock), 0, sec.stream, a);
+    hipLaunchKernelGGL(k_lanczos_update<float>, dim3(grid, nb), dim3(kBlock), 0, sec.stream, a);
   VCA_HIP(hipGetLastError());
 }
 
@@ -643,34 +660,28 @@ void launch_convert(const void *src, int src_prec, void *dst, int dst_prec, int6
   VCA_HIP(hipGetLastError());
 }
 
-void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int nops,
-                   const int *spin, const int *pos, const int *dagger, const double *coef_re,
-                   const double *coef_im) {
+void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int spin,
+                   const ExcOp *d_ops, int nb) {
   ExcArgs a{};
+  a.vstride = jsec.dim;
   a.out = jsec.buf[0];
   a.psi = psi;
   a.dimup_j = jsec.up.dim;
   a.dimdw_j = jsec.dw.dim;
   a.dimup_i = dimup_i;
   a.dimdw_i = dimdw_i;
-  a.spin = spin[0];
+  a.spin = spin;
   a.map_j = a.spin == 1 ? jsec.up.map : jsec.dw.map;
   a.binom = solver_binom(jsec.solver);
-  a.nops = nops;
-  for (int o = 0; o < nops; ++o) {
-    a.pos[o] = pos[o];
-    a.dagger[o] = dagger[o];
-    a.cre[o] = coef_re[o];
-    a.cim[o] = coef_im[o];
-  }
+  a.ops = d_ops;
   a.lz = jsec.lz;
   const int grid = grid_for(jsec.dim);
-  ensure_partials(jsec, grid);
+  ensure_partials(jsec, grid * nb);
   a.partials = jsec.partials;
   if (jsec.precision == VCA_FP64)
-    hipLaunchKernelGGL(k_excite<double>, dim3(grid), dim3(kBlock), 0, jsec.stream, a);
+    hipLaunchKernelGGL(k_excite<double>, dim3(grid, nb), dim3(kBlock), 0, jsec.stream, a);
   else
-    hipLaunchKernelGGL(k_excite<float>, dim3(grid), dim3(kBlock), 0, jsec.stream, a);
+    hipLaunchKernelGGL(k_excite<float>, dim3(grid, nb), dim3(kBlock), 0, jsec.stream, a);
   VCA_HIP(hipGetLastError());
 }
 

@@ -113,11 +113,13 @@ Solver::~Solver() {
 const uint32_t *solver_binom(const Solver *s) { return s->d_binom; }
 
 Sector::~Sector() {
+  if (stream) (void)hipStreamSynchronize(stream);
   for (DevSpin *sp : {&up, &dw})
     for (void *q : {(void *)sp->map, (void *)sp->energy, (void *)sp->cnt, (void *)sp->ent,
                     (void *)sp->amp64, (void *)sp->amp32})
       if (q) (void)hipFree(q);
-  for (void *q : {buf[0], buf[1], buf[2], xvec, (void *)partials, (void *)lz, (void *)lz_arrays})
+  for (void *q : {buf[0], buf[1], buf[2], xvec, (void *)partials, (void *)lz, (void *)lz_arrays,
+                  (void *)d_ops})
     if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -250,35 +252,52 @@ static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec
   return sec;
 }
 
-static void ensure_lanczos(Sector &sec, int cap) {
-  if (!sec.buf[0]) {
-    for (int i = 0; i < 3; ++i) VCA_HIP(hipMalloc(&sec.buf[i], sec.vec_bytes()));
-    VCA_HIP(hipMalloc(&sec.xvec, sec.vec_bytes()));
+// Lanczos workspace for nb chains of up to cap steps: three vector buffers
+// [nb][dim], one LanczosDev per chain with its alpha/beta/nrm/y arrays.
+static void ensure_lanczos(Sector &sec, int cap, int nb = 1) {
+  if (sec.nb_cap < nb) {
+    VCA_HIP(hipStreamSynchronize(sec.stream));
+    for (int i = 0; i < 3; ++i)
+      if (sec.buf[i]) VCA_HIP(hipFree(sec.buf[i]));
+    for (int i = 0; i < 3; ++i) VCA_HIP(hipMalloc(&sec.buf[i], sec.vec_bytes() * nb));
+    sec.nb_cap = nb;
+    sec.lz_cap = 0;  // per-chain arrays follow the new chain count
   }
-  if (!sec.lz) VCA_HIP(hipMalloc(&sec.lz, sizeof(LanczosDev)));
+  if (!sec.xvec) VCA_HIP(hipMalloc(&sec.xvec, sec.vec_bytes()));
   if (sec.lz_cap < cap) {
-    if (sec.lz_arrays) VCA_HIP(hipFree(sec.lz_arrays));
+    VCA_HIP(hipStreamSynchronize(sec.stream));
+    for (void *q : {(void *)sec.lz_arrays, (void *)sec.lz, (void *)sec.d_ops})
+      if (q) VCA_HIP(hipFree(q));
     const size_t n = (size_t)cap + 4;
-    VCA_HIP(hipMalloc(&sec.lz_arrays, sizeof(double) * 4 * n));
-    VCA_HIP(hipMemsetAsync(sec.lz_arrays, 0, sizeof(double) * 4 * n, sec.stream));
-    sec.lzh = LanczosDev{};
-    sec.lzh.alpha = sec.lz_arrays;
-    sec.lzh.beta = sec.lz_arrays + n;
-    sec.lzh.nrm = sec.lz_arrays + 2 * n;
-    sec.lzh.y = sec.lz_arrays + 3 * n;
+    const int nbc = sec.nb_cap;
+    VCA_HIP(hipMalloc(&sec.lz_arrays, sizeof(double) * 4 * n * nbc));
+    VCA_HIP(hipMemsetAsync(sec.lz_arrays, 0, sizeof(double) * 4 * n * nbc, sec.stream));
+    VCA_HIP(hipMalloc(&sec.lz, sizeof(LanczosDev) * nbc));
+    VCA_HIP(hipMalloc(&sec.d_ops, sizeof(ExcOp) * nbc));
+    sec.lzh_all.assign(nbc, LanczosDev{});
+    for (int c = 0; c < nbc; ++c) {
+      double *base = sec.lz_arrays + (size_t)c * 4 * n;
+      sec.lzh_all[c].alpha = base;
+      sec.lzh_all[c].beta = base + n;
+      sec.lzh_all[c].nrm = base + 2 * n;
+      sec.lzh_all[c].y = base + 3 * n;
+    }
+    sec.lzh = sec.lzh_all[0];
+    VCA_HIP(hipMemcpyAsync(sec.lz, sec.lzh_all.data(), sizeof(LanczosDev) * nbc,
+                           hipMemcpyHostToDevice, sec.stream));
+    VCA_HIP(hipStreamSynchronize(sec.stream));
     sec.lz_cap = cap;
   }
-  VCA_HIP(hipMemcpyAsync(sec.lz, &sec.lzh, sizeof(LanczosDev), hipMemcpyHostToDevice, sec.stream));
 }
 
-static void run_steps(Sector &sec, int nsteps, bool accx, Solver *s) {
+static void run_steps(Sector &sec, int nsteps, bool accx, Solver *s, int nb = 1) {
   for (int i = 0; i < nsteps; ++i) {
-    launch_hxv(sec, sec.buf[0], sec.buf[1], sec.buf[2], true);
-    launch_lanczos_update(sec, accx);
+    launch_hxv(sec, sec.buf[0], sec.buf[1], sec.buf[2], true, nb);
+    launch_lanczos_update(sec, accx, nb);
   }
   if (s) {
-    s->res.hxv_count += nsteps;
-    s->res.hxv_states += (double)nsteps * (double)sec.dim;
+    s->res.hxv_count += (int64_t)nsteps * nb;
+    s->res.hxv_states += (double)nsteps * (double)sec.dim * nb;
   }
 }
 
@@ -526,11 +545,72 @@ static void diagonalize(Solver &s) {
 }
 
 // ----------------------------------------------------------- GF chains --
-struct ChainOp {
-  int spin, pos, dagger;
-  double cre, cim;
+// One excitation chain of build_gf_normal (VCA_GF_NORMAL.f90:37-104):
+// element, channel (1..4), pole sign, the spin the operators act on and
+// the operators themselves.
+struct ChainSpec {
+  ElemKey key;
+  int ichan, isign, spin;
+  ExcOp op;
+  int jnup, jndw;
 };
 
+// Chains of one kept state in the reference's order: for each spin,
+// orbital and site, the diagonal channels c^+_i (1) and c_i (2), then for
+// every other (site, orbital) the mixed channels (c^+_i + c^+_j) (1),
+// (c_i + c_j) (2), (c^+_i + i c^+_j) (3), (c_i - i c_j) (4)
+// (VCA_GF_NORMAL.f90:121-318, 561-958; 2 channels when gf_symmetric).
+static std::vector<ChainSpec> chain_list(const Solver &s, const KeptState &st) {
+  const vca_params &p = s.p;
+  std::vector<ChainSpec> out;
+  auto mk = [&](ElemKey key, int ichan, int isign, int spin, int pos1, int pos2, int dagger,
+                double re2, double im2) {
+    ChainSpec c{};
+    c.key = key;
+    c.ichan = ichan;
+    c.isign = isign;
+    c.spin = spin;
+    c.op.nops = pos2 < 0 ? 1 : 2;
+    c.op.pos[0] = pos1;
+    c.op.dagger[0] = dagger;
+    c.op.cre[0] = 1.0;
+    c.op.cim[0] = 0.0;
+    if (pos2 >= 0) {
+      c.op.pos[1] = pos2;
+      c.op.dagger[1] = dagger;
+      c.op.cre[1] = re2;
+      c.op.cim[1] = im2;
+    }
+    const int d = dagger ? 1 : -1;
+    c.jnup = st.nup + (spin == 1 ? d : 0);
+    c.jndw = st.ndw + (spin == 2 ? d : 0);
+    out.push_back(c);
+  };
+  for (int ispin = 1; ispin <= p.nspin; ++ispin)
+    for (int iorb = 1; iorb <= p.norb; ++iorb)
+      for (int isite = 1; isite <= p.nlat; ++isite) {
+        const int is = iorb - 1 + (isite - 1) * p.norb;
+        mk({isite, isite, ispin, iorb, iorb}, 1, +1, ispin, is, -1, 1, 0, 0);
+        mk({isite, isite, ispin, iorb, iorb}, 2, -1, ispin, is, -1, 0, 0, 0);
+        for (int jsite = 1; jsite <= p.nlat; ++jsite)
+          for (int jorb = 1; jorb <= p.norb; ++jorb) {
+            if (isite == jsite && iorb == jorb) continue;
+            const int js = jorb - 1 + (jsite - 1) * p.norb;
+            const ElemKey k{isite, jsite, ispin, iorb, jorb};
+            mk(k, 1, +1, ispin, is, js, 1, 1.0, 0.0);
+            mk(k, 2, -1, ispin, is, js, 0, 1.0, 0.0);
+            if (!p.gf_symmetric) {
+              mk(k, 3, +1, ispin, is, js, 1, 0.0, 1.0);
+              mk(k, 4, -1, ispin, is, js, 0, 0.0, -1.0);
+            }
+          }
+      }
+  return out;
+}
+
+// Chains of the same target sector run together: one launch per Lanczos
+// step covers a batch of chains (grid.y = chain), their vectors stacked
+// [chain][dim] in HBM, each chain with its own alpha/beta in device memory.
 static void build_gf(Solver &s) {
   const vca_params &p = s.p;
   s.gf.clear();
@@ -542,87 +622,85 @@ static void build_gf(Solver &s) {
     if (!slot) slot = make_sector(s, nup, ndw, prec);
     return *slot;
   };
-  const double chan_scale_re[4] = {1, 1, 0, 0}, chan_scale_im[4] = {0, 0, -1, -1};
+  const double chan_re[4] = {1, 1, 0, 0}, chan_im[4] = {0, 0, -1, -1};
+  Binom bn;
   for (size_t ist = 0; ist < s.states.size(); ++ist) {
     const KeptState &st = s.states[ist];
-    Binom bn;
-    const int64_t dim_i = (int64_t)bn(s.ns, st.nup) * bn(s.ns, st.ndw);
     const int dimup_i = (int)bn(s.ns, st.nup), dimdw_i = (int)bn(s.ns, st.ndw);
+    const int64_t dim_i = (int64_t)dimup_i * dimdw_i;
     void *psi = st.d_vec;
     void *psi32 = nullptr;
     if (prec == VCA_MIXED) {
       VCA_HIP(hipMalloc(&psi32, sizeof(float2) * dim_i));
       launch_convert(st.d_vec, VCA_FP64, psi32, VCA_MIXED, dim_i, s.stream);
-      VCA_HIP(hipStreamSynchronize(s.stream));
       psi = psi32;
     }
-    // One chain: O|psi> with ops (same spin, same dagger), weight factor
-    // (chan_re + i chan_im) * norm2, pole sign isign.
-    auto chain = [&](const std::vector<ChainOp> &ops, int ichan, int isign) -> GfChan {
-      GfChan out;
-      const int spin = ops[0].spin, d = ops[0].dagger ? 1 : -1;
-      const int jnup = st.nup + (spin == 1 ? d : 0), jndw = st.ndw + (spin == 2 ? d : 0);
-      if (jnup < 0 || jndw < 0 || jnup > s.ns || jndw > s.ns) return out;
-      Sector &js = get_sector(jnup, jndw);
+    VCA_HIP(hipStreamSynchronize(s.stream));
+    const std::vector<ChainSpec> specs = chain_list(s, st);
+    std::vector<GfChan> res(specs.size());
+    std::map<std::array<int, 2>, std::vector<int>> groups;
+    for (size_t c = 0; c < specs.size(); ++c) {
+      const ChainSpec &cs = specs[c];
+      if (cs.jnup < 0 || cs.jndw < 0 || cs.jnup > s.ns || cs.jndw > s.ns) continue;
+      groups[{cs.jnup, cs.jndw}].push_back((int)c);
+    }
+    for (auto &g : groups) {
+      Sector &js = get_sector(g.first[0], g.first[1]);
       const int nlanc = (int)std::min<int64_t>(js.dim, p.lanc_ngfiter);
-      ensure_lanczos(js, nlanc + 2);
-      int sp[2], pos[2], dg[2];
-      double cre[2], cim[2];
-      for (size_t o = 0; o < ops.size(); ++o) {
-        sp[o] = ops[o].spin;
-        pos[o] = ops[o].pos;
-        dg[o] = ops[o].dagger;
-        cre[o] = ops[o].cre;
-        cim[o] = ops[o].cim;
-      }
-      // the excitation reads psi on the J sector's stream
-      VCA_HIP(hipStreamSynchronize(s.stream));
-      launch_excite(js, dimup_i, dimdw_i, psi, (int)ops.size(), sp, pos, dg, cre, cim);
-      std::vector<double> al(nlanc), be(nlanc);
-      double nrm0 = 0;
-      tridiag_run(js, nlanc, al.data(), be.data(), &nrm0, &s);
-      const double norm2 = nrm0 * nrm0;
-      s.res.nchains += 1;
-      if (!(norm2 > 0.0)) return out;
-      std::vector<double> w(nlanc), z((size_t)nlanc * nlanc);
-      tridiag_eig(nlanc, al.data(), be.data(), w.data(), z.data());
-      const zc peso_bz = zc(chan_scale_re[ichan - 1], chan_scale_im[ichan - 1]) * norm2 / s.zeta;
-      out.w.resize(nlanc);
-      out.p.resize(nlanc);
-      for (int j = 0; j < nlanc; ++j) {
-        const double de = w[j] - st.e;
-        s.max_exc = std::max(s.max_exc, de);
-        const double z1 = z[(size_t)j * nlanc];
-        out.w[j] = peso_bz * z1 * z1;
-        out.p[j] = isign * de;
-      }
-      return out;
-    };
-    for (int ispin = 1; ispin <= p.nspin; ++ispin)
-      for (int iorb = 1; iorb <= p.norb; ++iorb)
-        for (int isite = 1; isite <= p.nlat; ++isite) {
-          const int is = iorb - 1 + (isite - 1) * p.norb;  // 0-based position
-          const int spin = ispin;  // acts on the up string for ispin=1
-          auto &diag = s.gf[{isite, isite, ispin, iorb, iorb}];
-          diag.resize(s.states.size());
-          diag[ist].push_back(chain({{spin, is, 1, 1.0, 0.0}}, 1, +1));
-          diag[ist].push_back(chain({{spin, is, 0, 1.0, 0.0}}, 2, -1));
-          for (int jsite = 1; jsite <= p.nlat; ++jsite)
-            for (int jorb = 1; jorb <= p.norb; ++jorb) {
-              if (isite == jsite && iorb == jorb) continue;
-              const int js_ = jorb - 1 + (jsite - 1) * p.norb;
-              auto &el = s.gf[{isite, jsite, ispin, iorb, jorb}];
-              el.resize(s.states.size());
-              el[ist].push_back(chain({{spin, is, 1, 1.0, 0.0}, {spin, js_, 1, 1.0, 0.0}}, 1, +1));
-              el[ist].push_back(chain({{spin, is, 0, 1.0, 0.0}, {spin, js_, 0, 1.0, 0.0}}, 2, -1));
-              if (!p.gf_symmetric) {
-                el[ist].push_back(
-                    chain({{spin, is, 1, 1.0, 0.0}, {spin, js_, 1, 0.0, 1.0}}, 3, +1));
-                el[ist].push_back(
-                    chain({{spin, is, 0, 1.0, 0.0}, {spin, js_, 0, 0.0, -1.0}}, 4, -1));
-              }
-            }
+      size_t free_b = 0, total_b = 0;
+      VCA_HIP(hipMemGetInfo(&free_b, &total_b));
+      const size_t have = (size_t)js.nb_cap * 3 * js.vec_bytes();
+      const size_t budget = (free_b + have) / 2;
+      int nb_max = (int)std::max<size_t>(1, budget / (3 * js.vec_bytes()));
+      nb_max = std::min(nb_max, 512);
+      const std::vector<int> &idx = g.second;
+      for (size_t b0 = 0; b0 < idx.size(); b0 += nb_max) {
+        const int nb = (int)std::min<size_t>(nb_max, idx.size() - b0);
+        ensure_lanczos(js, nlanc + 2, std::max(nb, js.nb_cap));
+        std::vector<ExcOp> ops(nb);
+        for (int c = 0; c < nb; ++c) ops[c] = specs[idx[b0 + c]].op;
+        VCA_HIP(hipMemcpyAsync(js.d_ops, ops.data(), sizeof(ExcOp) * nb, hipMemcpyHostToDevice,
+                               js.stream));
+        launch_excite(js, dimup_i, dimdw_i, psi, specs[idx[b0]].spin, js.d_ops, nb);
+        run_steps(js, nlanc, false, &s, nb);
+        std::vector<double> al((size_t)nb * nlanc), be((size_t)nb * nlanc), nrm0(nb);
+        for (int c = 0; c < nb; ++c) {
+          VCA_HIP(hipMemcpyAsync(al.data() + (size_t)c * nlanc, js.lzh_all[c].alpha,
+                                 sizeof(double) * nlanc, hipMemcpyDeviceToHost, js.stream));
+          VCA_HIP(hipMemcpyAsync(be.data() + (size_t)c * nlanc, js.lzh_all[c].beta,
+                                 sizeof(double) * nlanc, hipMemcpyDeviceToHost, js.stream));
+          VCA_HIP(hipMemcpyAsync(&nrm0[c], js.lzh_all[c].nrm, sizeof(double),
+                                 hipMemcpyDeviceToHost, js.stream));
         }
+        VCA_HIP(hipStreamSynchronize(js.stream));
+        // add_to_lanczos_gf_normal (VCA_GF_NORMAL.f90:970-1022), T=0 weights
+        for (int c = 0; c < nb; ++c) {
+          const ChainSpec &cs = specs[idx[b0 + c]];
+          const double norm2 = nrm0[c] * nrm0[c];
+          s.res.nchains += 1;
+          if (!(norm2 > 0.0)) continue;  // O|psi> = 0: empty channel
+          std::vector<double> w(nlanc), z((size_t)nlanc * nlanc);
+          tridiag_eig(nlanc, al.data() + (size_t)c * nlanc, be.data() + (size_t)c * nlanc,
+                      w.data(), z.data());
+          const zc peso_bz = zc(chan_re[cs.ichan - 1], chan_im[cs.ichan - 1]) * norm2 / s.zeta;
+          GfChan &out = res[idx[b0 + c]];
+          out.w.resize(nlanc);
+          out.p.resize(nlanc);
+          for (int j = 0; j < nlanc; ++j) {
+            const double de = w[j] - st.e;
+            s.max_exc = std::max(s.max_exc, de);
+            const double z1 = z[(size_t)j * nlanc];
+            out.w[j] = peso_bz * z1 * z1;
+            out.p[j] = cs.isign * de;
+          }
+        }
+      }
+    }
+    for (size_t c = 0; c < specs.size(); ++c) {
+      auto &el = s.gf[specs[c].key];
+      el.resize(s.states.size());
+      el[ist].push_back(std::move(res[c]));
+    }
     if (psi32) VCA_HIP(hipFree(psi32));
   }
 }

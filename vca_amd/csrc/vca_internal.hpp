@@ -160,6 +160,14 @@ struct HostSpinTable {
 };
 
 // ------------------------------------------------------------ device ----
+// One excitation chain's operator O = sum_o coef_o (c or c^+)_{pos_o}, all
+// on the spin of the launch (VCA_GF_NORMAL.f90:184-198, 624-660).
+struct ExcOp {
+  int nops;
+  int pos[2], dagger[2];
+  double cre[2], cim[2];
+};
+
 struct LanczosDev {
   int32_t k;     // current step
   int32_t done;  // exact breakdown seen
@@ -192,14 +200,16 @@ struct Sector {
   int64_t dim = 0;
   DevSpin up, dw;
   hipStream_t stream = nullptr;
-  // Lanczos workspace (lazily allocated)
-  int lz_cap = 0;
-  void *buf[3] = {nullptr, nullptr, nullptr};  // u_even, u_odd, hv
+  // Lanczos workspace (lazily allocated) for nb_cap chains of lz_cap steps
+  int lz_cap = 0, nb_cap = 0;
+  void *buf[3] = {nullptr, nullptr, nullptr};  // u_even, u_odd, hv [nb][dim]
   void *xvec = nullptr;                        // eigenvector accumulator
   double *partials = nullptr;
   int partials_cap = 0;
-  LanczosDev *lz = nullptr;  // device struct
-  LanczosDev lzh{};          // host copy of the device array pointers
+  LanczosDev *lz = nullptr;  // device structs [nb_cap]
+  LanczosDev lzh{};          // host copy of chain 0's device array pointers
+  std::vector<LanczosDev> lzh_all;  // host copies for every chain
+  ExcOp *d_ops = nullptr;    // [nb_cap]
   double *lz_arrays = nullptr;
   size_t vec_bytes() const {
     return (size_t)dim * (precision == VCA_FP64 ? sizeof(double2) : sizeof(float2));
@@ -212,16 +222,16 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns,
                        int npart, const uint32_t *d_binom, double e_add);
 CrossTerm sector_cross(const Sector &sec);
 const uint32_t *solver_binom(const Solver *s);
-void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos);
-void launch_lanczos_update(Sector &sec, bool accumulate_x);
+void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos,
+                int nb = 1);
+void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb = 1);
 void launch_start_vector(Sector &sec, uint64_t seed);
 void launch_norm0(Sector &sec);  // nrm[0] = ||buf[0]||, k = 0, done = 0
 void launch_scale_x(Sector &sec);  // xvec /= ||xvec||
 void launch_convert(const void *src, int src_prec, void *dst, int dst_prec,
                     int64_t n, hipStream_t st);
-void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int nops,
-                   const int *spin, const int *pos, const int *dagger,
-                   const double *coef_re, const double *coef_im);
+void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int spin,
+                   const ExcOp *d_ops, int nb);
 CrossTerm make_cross(const ModelData &md);
 
 // Host linear algebra (linalg.cpp)

@@ -1,0 +1,217 @@
+!  vca_gpu.f90 -- Fortran drop-in for the VCA solver core on MI355X.
+!
+!  Presents the reference's public names (QcmPlab/VCA, module VCA):
+!    vca_init_solver([comm,] [bath_h, bath_v])    VCA_MAIN.f90:51-115
+!    vca_solve(Hloc, Hk [, bath_h, bath_v])       VCA_MAIN.f90:124-215
+!    vca_get_sft_potential(sft)                   VCA_IO/get_sft_potential.f90:1-4
+!  and the input variables of VCA_INPUT_VARS.f90:14-60 under the same
+!  names, and calls libvca_gpu.so through its C ABI (include/vca_gpu.h)
+!  with iso_c_binding.  A nonzero status from the library stops the
+!  program with the library's message, as the reference stops on errors.
+!  The MPI form of vca_init_solver/vca_solve accepts a communicator and
+!  ignores it: one GPU holds the whole sector.
+module vca_gpu
+  use iso_c_binding
+  implicit none
+  private
+
+  ! ---- input variables (reference names and defaults) -----------------
+  integer, public :: Nlat = 1, Norb = 1, Nspin = 1, Nlat_bath = 0, Norb_bath = 0
+  real(8), public :: Uloc(5) = [2d0, 0d0, 0d0, 0d0, 0d0]
+  real(8), public :: Ust = 0d0, Jh = 0d0, xmu = 0d0, beta = 1000d0
+  logical, public :: hfmode = .true., hfshift = .true., vca_gf_symmetric = .false.
+  character(len=16), public :: lanc_method = "arpack"
+  integer, public :: lanc_nstates_sector = 6, lanc_niter = 512, lanc_ngfiter = 200
+  integer, public :: lanc_dim_threshold = 256
+  real(8), public :: lanc_tolerance = 1d-12, gs_threshold = 1d-9
+  ! additions of this build
+  integer, public :: vca_precision = 0        ! 0 FP64, 1 MIXED
+  integer, public :: vca_omega_mode = 0       ! 0 adaptive quadrature, 1 Matsubara grid
+  integer, public :: vca_omega_nfreq = 4096
+  real(8), public :: vca_quad_epsabs = 1d-10, vca_quad_epsrel = 1d-12
+  integer, allocatable, public :: vca_sectors_list(:,:)   ! (2, n): (nup, ndw)
+
+  type, bind(C) :: vca_params_t
+     integer(c_int32_t) :: nlat, norb, nspin, nlat_bath, norb_bath
+     integer(c_int32_t) :: hfmode, hfshift, gf_symmetric
+     real(c_double)     :: uloc(5), ust, jh, xmu, beta
+     integer(c_int32_t) :: lanc_method, lanc_nstates_sector, lanc_niter, lanc_ngfiter
+     integer(c_int32_t) :: lanc_dim_threshold
+     real(c_double)     :: lanc_tolerance, gs_threshold
+     integer(c_int32_t) :: precision, omega_mode, omega_nfreq
+     real(c_double)     :: quad_epsabs, quad_epsrel
+     integer(c_int32_t) :: nsectors
+     type(c_ptr)        :: sectors
+  end type vca_params_t
+
+  interface
+     subroutine c_default_params(p) bind(C, name="vca_default_params")
+       import :: vca_params_t
+       type(vca_params_t), intent(inout) :: p
+     end subroutine c_default_params
+     function c_last_error() bind(C, name="vca_last_error") result(msg)
+       import :: c_ptr
+       type(c_ptr) :: msg
+     end function c_last_error
+     function c_init_solver(p, out) bind(C, name="vca_init_solver") result(rc)
+       import :: vca_params_t, c_ptr, c_int
+       type(vca_params_t), intent(in) :: p
+       type(c_ptr), intent(out) :: out
+       integer(c_int) :: rc
+     end function c_init_solver
+     subroutine c_finalize_solver(s) bind(C, name="vca_finalize_solver")
+       import :: c_ptr
+       type(c_ptr), value :: s
+     end subroutine c_finalize_solver
+     function c_solve(s, hloc, hk, nk, bath_h, bath_v) bind(C, name="vca_solve") result(rc)
+       import :: c_ptr, c_int, c_int64_t
+       type(c_ptr), value :: s, hloc, hk, bath_h, bath_v
+       integer(c_int64_t), value :: nk
+       integer(c_int) :: rc
+     end function c_solve
+     function c_get_sft(s, out) bind(C, name="vca_get_sft_potential") result(rc)
+       import :: c_ptr, c_int, c_double
+       type(c_ptr), value :: s
+       real(c_double), intent(out) :: out
+       integer(c_int) :: rc
+     end function c_get_sft
+  end interface
+
+  type(c_ptr), save :: handle = c_null_ptr
+  type(vca_params_t), save, target :: cparams
+  integer(c_int32_t), allocatable, target, save :: csectors(:)
+  real(8), save :: last_omegaprime = 0d0, last_sft = 0d0
+
+  interface vca_init_solver
+     module procedure vca_init_solver_serial
+     module procedure vca_init_solver_comm
+  end interface vca_init_solver
+
+  interface vca_solve
+     module procedure vca_solve_serial
+     module procedure vca_solve_comm
+  end interface vca_solve
+
+  public :: vca_init_solver, vca_solve, vca_get_sft_potential, vca_finalize_solver
+
+contains
+
+  subroutine check(rc, where)
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: where
+    character(kind=c_char), pointer :: msg(:)
+    character(len=512) :: text
+    integer :: i
+    if (rc == 0) return
+    call c_f_pointer(c_last_error(), msg, [512])
+    text = ""
+    do i = 1, 512
+       if (msg(i) == c_null_char) exit
+       text(i:i) = msg(i)
+    end do
+    write(*, "(A,A,A,I0,A,A)") "STOP: ", where, " failed (", rc, "): ", trim(text)
+    stop 1
+  end subroutine check
+
+  subroutine fill_params()
+    integer :: i
+    call c_default_params(cparams)
+    cparams%nlat = Nlat; cparams%norb = Norb; cparams%nspin = Nspin
+    cparams%nlat_bath = Nlat_bath; cparams%norb_bath = Norb_bath
+    cparams%hfmode = merge(1, 0, hfmode); cparams%hfshift = merge(1, 0, hfshift)
+    cparams%gf_symmetric = merge(1, 0, vca_gf_symmetric)
+    cparams%uloc = Uloc; cparams%ust = Ust; cparams%jh = Jh
+    cparams%xmu = xmu; cparams%beta = beta
+    cparams%lanc_method = merge(0, 1, trim(lanc_method) == "lanczos")
+    cparams%lanc_nstates_sector = lanc_nstates_sector
+    cparams%lanc_niter = lanc_niter; cparams%lanc_ngfiter = lanc_ngfiter
+    cparams%lanc_dim_threshold = lanc_dim_threshold
+    cparams%lanc_tolerance = lanc_tolerance; cparams%gs_threshold = gs_threshold
+    cparams%precision = vca_precision; cparams%omega_mode = vca_omega_mode
+    cparams%omega_nfreq = vca_omega_nfreq
+    cparams%quad_epsabs = vca_quad_epsabs; cparams%quad_epsrel = vca_quad_epsrel
+    cparams%nsectors = 0; cparams%sectors = c_null_ptr
+    if (allocated(vca_sectors_list)) then
+       if (allocated(csectors)) deallocate(csectors)
+       allocate(csectors(2 * size(vca_sectors_list, 2)))
+       do i = 1, size(vca_sectors_list, 2)
+          csectors(2 * i - 1) = vca_sectors_list(1, i)
+          csectors(2 * i) = vca_sectors_list(2, i)
+       end do
+       cparams%nsectors = size(vca_sectors_list, 2)
+       cparams%sectors = c_loc(csectors)
+    end if
+  end subroutine fill_params
+
+  ! VCA_MAIN.f90:51-76
+  subroutine vca_init_solver_serial(bath_h, bath_v)
+    complex(8), intent(in), optional :: bath_h(:,:,:,:,:,:), bath_v(:,:,:,:,:,:)
+    if (c_associated(handle)) return   ! dimensions are fixed once, as isetup does
+    call fill_params()
+    call check(c_init_solver(cparams, handle), "vca_init_solver")
+    if (present(bath_h) .neqv. present(bath_v)) stop "vca_init_solver: bath_h and bath_v go together"
+  end subroutine vca_init_solver_serial
+
+  subroutine vca_init_solver_comm(comm, bath_h, bath_v)
+    integer, intent(in) :: comm
+    complex(8), intent(in), optional :: bath_h(:,:,:,:,:,:), bath_v(:,:,:,:,:,:)
+    if (comm < -huge(comm)) stop
+    call vca_init_solver_serial(bath_h, bath_v)
+  end subroutine vca_init_solver_comm
+
+  ! VCA_MAIN.f90:124-215
+  subroutine vca_solve_serial(Hloc, Hk, bath_h, bath_v)
+    complex(8), intent(in), target, contiguous :: Hloc(:,:,:,:,:,:)
+    complex(8), intent(in), target, contiguous :: Hk(:,:,:,:,:,:,:)
+    complex(8), intent(inout), target, contiguous, optional :: bath_h(:,:,:,:,:,:)
+    complex(8), intent(inout), target, contiguous, optional :: bath_v(:,:,:,:,:,:)
+    type(c_ptr) :: pbh, pbv
+    integer :: unit
+    real(8) :: sft
+    if (.not. c_associated(handle)) call vca_init_solver_serial()
+    if (any(shape(Hloc) /= [Nlat, Nlat, Nspin, Nspin, Norb, Norb])) &
+         stop "STOP: wrong cluster matrix dimensions"
+    if (any(shape(Hk(:,:,:,:,:,:,1)) /= [Nlat, Nlat, Nspin, Nspin, Norb, Norb])) &
+         stop "STOP: wrong lattice matrix dimensions"
+    pbh = c_null_ptr
+    pbv = c_null_ptr
+    if (present(bath_h) .and. present(bath_v)) then
+       if (any(shape(bath_h) /= [Nlat_bath, Nlat_bath, Nspin, Nspin, Norb_bath, Norb_bath])) &
+            stop "vca_solve: wrong bath_h shape"
+       if (any(shape(bath_v) /= [Nlat, Nlat_bath, Nspin, Nspin, Norb, Norb_bath])) &
+            stop "vca_solve: wrong bath_v shape"
+       pbh = c_loc(bath_h)
+       pbv = c_loc(bath_v)
+    end if
+    call check(c_solve(handle, c_loc(Hloc), c_loc(Hk), int(size(Hk, 7), c_int64_t), pbh, pbv), &
+         "vca_solve")
+    call check(c_get_sft(handle, sft), "vca_get_sft_potential")
+    last_sft = sft
+    ! SFT_potential.vca, VCA_MAIN.f90:199-203 (sft, omegaprime, -integral)
+    open(newunit=unit, file="SFT_potential.vca", position="append")
+    write(unit, *) sft
+    close(unit)
+  end subroutine vca_solve_serial
+
+  subroutine vca_solve_comm(comm, Hloc, Hk, bath_h, bath_v)
+    integer, intent(in) :: comm
+    complex(8), intent(in), target, contiguous :: Hloc(:,:,:,:,:,:)
+    complex(8), intent(in), target, contiguous :: Hk(:,:,:,:,:,:,:)
+    complex(8), intent(inout), target, contiguous, optional :: bath_h(:,:,:,:,:,:)
+    complex(8), intent(inout), target, contiguous, optional :: bath_v(:,:,:,:,:,:)
+    if (comm < -huge(comm)) stop
+    call vca_solve_serial(Hloc, Hk, bath_h, bath_v)
+  end subroutine vca_solve_comm
+
+  ! VCA_IO/get_sft_potential.f90:1-4
+  subroutine vca_get_sft_potential(sft)
+    real(8), intent(out) :: sft
+    call check(c_get_sft(handle, sft), "vca_get_sft_potential")
+  end subroutine vca_get_sft_potential
+
+  subroutine vca_finalize_solver()
+    if (c_associated(handle)) call c_finalize_solver(handle)
+    handle = c_null_ptr
+  end subroutine vca_finalize_solver
+
+end module vca_gpu
