@@ -44,6 +44,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
+    ap.add_argument("--only", choices=["all", "hxv", "omega"], default="all",
+                    help="hxv: only the H x v timing plus a 256 MiB copy kernel that "
+                         "calibrates HBM byte counters (for rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
@@ -83,6 +86,15 @@ def main():
     sec32, ms32, wall32 = time_hxv("mixed")
     dim = sec64.dim
     log(f"H x v FP64: {ms64 * 1e3:.1f} us/apply; MIXED: {ms32 * 1e3:.1f} us/apply; dim {dim}")
+    if args.only == "hxv":
+        # calibration kernel for the HBM byte counters: a copy of 256 MiB
+        a = torch.ones(1 << 25, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"hxv_fp64_us": ms64 * 1e3, "hxv_mixed_us": ms32 * 1e3, "dim": dim,
+                          "copy_bytes_each_way": a.numel() * 8}), flush=True)
+        return
 
     # --------------------------------------------------- ground state ----
     t0 = time.perf_counter()

@@ -134,10 +134,12 @@ def _oracle_solve(p, hloc, hk, bh=None, bv=None, **kw):
     return R.vca_solve(M.to_oracle(p), hloc, hk, bh, bv, **kw)
 
 
-@pytest.mark.parametrize("name", ["chain4", "bath", "bhz"])
+@pytest.mark.parametrize("name", ["chain4", "chain4_u0_tp07", "bath", "bhz"])
 def test_vca_solve_pipeline(name):
     if name == "chain4":
         p, hloc, hk, bh, bv = configs.config_chain4(nk=16)
+    elif name == "chain4_u0_tp07":
+        p, hloc, hk, bh, bv = M.chain(nlat=4, u=0.0, nk=16, tprime=-0.7)
     elif name == "bath":
         p, hloc, hk, bh, bv = M.bath_model(nlat=2, nb=2)
     else:
@@ -172,3 +174,21 @@ def test_sft_exact_at_u0():
         hkl = R.hk_to_lso(M.to_oracle(p), hk)
         exact = (3 - p.nspin) * np.mean([np.sum(np.minimum(np.linalg.eigvalsh(h), 0)) for h in hkl])
         assert abs(omega - exact) < 1e-8 * abs(exact), (omega, exact)
+
+
+def test_graph_replays_match_eager_launches(monkeypatch):
+    """Chunks of Lanczos steps replayed from a captured graph (several
+    replays, with eager work and syncs in between) give bit-identical
+    alpha/beta to the same steps launched one by one."""
+    p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0)
+    s = _solver(p, hloc)
+    sec = s.sector(5, 5)
+    v0 = _rand(sec.dim, 9)
+    _say("graph vs eager, chain10 (5,5), 50 steps x 3 runs")
+    monkeypatch.setenv("VCA_NO_GRAPHS", "1")
+    a_e, b_e = sec.lanczos_tridiag(v0, 50)
+    monkeypatch.setenv("VCA_NO_GRAPHS", "0")
+    for _ in range(3):
+        a_g, b_g = sec.lanczos_tridiag(v0, 50)
+        assert np.array_equal(a_g, a_e) and np.array_equal(b_g, b_e)
+        sec.hxv(v0)  # eager work between the replays

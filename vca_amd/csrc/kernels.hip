@@ -114,6 +114,7 @@ struct HxvArgs {
   LanczosDev *lz;  // null: plain product; else one per chain (blockIdx.y)
   double *partials;
   int64_t vstride;  // elements between the vectors of consecutive chains
+  int dw_maxnnz;    // capacity of the staged down-spin hop list
 };
 
 __device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t sd) {
@@ -214,6 +215,170 @@ __global__ __launch_bounds__(kBlock) void k_hxv(const HxvArgs a) {
   if (lz) {
     double tot;
     if (grid_sum(dot, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[0], &tot))
+      lz->alpha[k] = done ? 0.0 : tot;
+  }
+}
+
+// Block sum for a block of BS threads, result valid in thread 0.
+template <int BS>
+__device__ inline double block_sum_t(double v) {
+  __shared__ double s_part[BS / 64];
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_part[wid] = v;
+  __syncthreads();
+  double r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < BS / 64; ++w) r += s_part[w];
+  return r;
+}
+
+template <int BS>
+__device__ inline bool grid_sum_t(double v, double *partials, uint32_t *counter,
+                                  double *total) {
+  __shared__ bool s_last;
+  double b = block_sum_t<BS>(v);
+  if (threadIdx.x == 0) {
+    partials[blockIdx.x] = b;
+    __threadfence();
+    uint32_t prev = atomicAdd(counter, 1u);
+    s_last = (prev == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (!s_last) return false;
+  __threadfence();
+  double s = 0;
+  for (int i = threadIdx.x; i < (int)gridDim.x; i += BS)
+    s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  s = block_sum_t<BS>(s);
+  if (threadIdx.x == 0) {
+    *total = s;
+    *counter = 0u;
+    return true;
+  }
+  return false;
+}
+
+// Workgroup b -> output column.  Workgroups are dealt round-robin to the 8
+// XCDs, so column ranges are assigned per XCD: each XCD's L2 then holds
+// the neighbour columns of one contiguous range of columns.
+__device__ inline int xcd_column(int b, int ncols) {
+  const int x = b & 7, pos = b >> 3;
+  const int q = ncols >> 3, r = ncols & 7;
+  return x * q + min(x, r) + pos;
+}
+
+constexpr int kRows = 4;  // rows per thread processed together
+
+// One workgroup of BS lanes per output column idw (DimUp >= BS/2):
+//  * the down-spin hop list of the column (<= maxnnz entries) is staged in
+//    LDS as (column offset, signed amplitude);
+//  * the input column v(:, idw) is staged in LDS when it fits (STAGE), so
+//    the diagonal and the up-spin gathers read LDS;
+//  * each lane handles kRows rows at once, so the down-spin loop issues
+//    kRows independent coalesced loads of v(:, jdw) per hop.
+template <typename R, bool REAL_AMP, bool STAGE, int BS>
+__global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
+  using C = cplx<R>;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int maxnd = a.dw_maxnnz;
+  // every region starts on a 16-byte boundary (128-bit LDS accesses)
+  auto align16 = [](unsigned char *p) {
+    return reinterpret_cast<unsigned char *>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
+  };
+  C *s_amp_up = reinterpret_cast<C *>(smem);
+  C *s_dw_amp = reinterpret_cast<C *>(align16(reinterpret_cast<unsigned char *>(s_amp_up + a.namp_up)));
+  int64_t *s_dw_off = reinterpret_cast<int64_t *>(align16(reinterpret_cast<unsigned char *>(s_dw_amp + maxnd)));
+  C *s_col = reinterpret_cast<C *>(align16(reinterpret_cast<unsigned char *>(s_dw_off + maxnd)));
+  __shared__ int s_nd;
+
+  const int dimup = a.dimup;
+  const int idw = xcd_column(blockIdx.x, a.dimdw);
+  const C *g_amp_up = static_cast<const C *>(a.amp_up);
+  const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
+  for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
+  if (threadIdx.x == 0) s_nd = a.cnt_dw[idw];
+  __syncthreads();
+  const int nd = s_nd;
+  for (int q = threadIdx.x; q < nd; q += BS) {
+    const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
+    s_dw_off[q] = (int64_t)(e & kColMask) * dimup;
+    s_dw_amp[q] = g_amp_dw[e >> kColBits];
+  }
+
+  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
+  int k = 0;
+  bool done = false;
+  R scale = (R)1;
+  const C *vin = static_cast<const C *>(a.v0) + chain_off;
+  if (lz) {
+    k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const C *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  C *hv = static_cast<C *>(a.hv) + chain_off + (int64_t)idw * dimup;
+  const C *vcol = vin + (int64_t)idw * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    if (STAGE)
+      for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
+    __syncthreads();
+    const double edw = a.e_dw[idw];
+    const uint32_t sdw = a.map_dw[idw];
+    for (int base = threadIdx.x; base < dimup; base += BS * kRows) {
+      int row[kRows];
+      R ar[kRows], ai[kRows];
+      C x[kRows];
+#pragma unroll
+      for (int j = 0; j < kRows; ++j) {
+        row[j] = min(base + j * BS, dimup - 1);  // clamped: loads stay in bounds
+        x[j] = STAGE ? s_col[row[j]] : vcol[row[j]];
+        const double ed = a.e_up[row[j]] + edw + cross_energy(a.cross, a.map_up[row[j]], sdw);
+        ar[j] = (R)ed * x[j].x;
+        ai[j] = (R)ed * x[j].y;
+      }
+      // up-spin hops: gathers inside the column
+#pragma unroll
+      for (int j = 0; j < kRows; ++j) {
+        const int nu = (base + j * BS < dimup) ? a.cnt_up[row[j]] : 0;
+        for (int q = 0; q < nu; ++q) {
+          const uint32_t e = a.ent_up[(int64_t)q * dimup + row[j]];
+          const C y = STAGE ? s_col[e & kColMask] : vcol[e & kColMask];
+          cmac<R, REAL_AMP>(ar[j], ai[j], s_amp_up[e >> kColBits], y);
+        }
+      }
+      // down-spin hops: kRows independent coalesced loads per hop
+      for (int q = 0; q < nd; ++q) {
+        const int64_t off = s_dw_off[q];
+        const C h = s_dw_amp[q];
+        C y[kRows];
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) y[j] = vin[off + row[j]];
+#pragma unroll
+        for (int j = 0; j < kRows; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h, y[j]);
+      }
+#pragma unroll
+      for (int j = 0; j < kRows; ++j) {
+        if (base + j * BS < dimup) {
+          C out;
+          out.x = ar[j] * scale;
+          out.y = ai[j] * scale;
+          hv[row[j]] = out;
+          if (lz)
+            dot += (double)(x[j].x * scale) * (double)out.x +
+                   (double)(x[j].y * scale) * (double)out.y;
+        }
+      }
+    }
+  }
+  if (lz) {
+    double tot;
+    if (grid_sum_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[0],
+                       &tot))
       lz->alpha[k] = done ? 0.0 : tot;
   }
 }
@@ -513,11 +678,66 @@ static void ensure_partials(Sector &sec, int nblocks) {
   if (sec.partials_cap >= nblocks) return;
   if (sec.partials) {
     VCA_HIP(hipStreamSynchronize(sec.stream));
+    clear_graphs(sec);  // recorded launches hold the old pointer
     VCA_HIP(hipFree(sec.partials));
   }
   const int cap = std::max(nblocks, std::max(4096, sec.dw.dim) * std::max(1, sec.nb_cap));
   VCA_HIP(hipMalloc(&sec.partials, sizeof(double) * cap));
   sec.partials_cap = cap;
+}
+
+// Launch shape of H x v on a sector: one workgroup per column (k_hxv_col)
+// when columns are long, several columns per workgroup (k_hxv) otherwise.
+struct HxvShape {
+  bool onecol_kernel;
+  int bs, cpb, grid;
+};
+static HxvShape hxv_shape(const Sector &sec) {
+  HxvShape h{};
+  if (sec.up.dim >= 128) {
+    h.onecol_kernel = true;
+    h.bs = sec.up.dim >= 4096 ? 1024 : 256;
+    h.cpb = 1;
+    h.grid = sec.dw.dim;
+  } else {
+    h.onecol_kernel = false;
+    h.bs = kBlock;
+    h.cpb = std::max(1, kBlock / sec.up.dim);
+    h.grid = (sec.dw.dim + h.cpb - 1) / h.cpb;
+  }
+  return h;
+}
+
+// Allocates the partial-sum buffer for the largest grid of a Lanczos step
+// on nb chains, so that launches recorded into a graph never allocate.
+void ensure_step_partials(Sector &sec, int nb) {
+  ensure_partials(sec, std::max(hxv_shape(sec).grid, grid_for(sec.dim)) * nb);
+}
+
+constexpr size_t kLdsBig = 150 * 1024;  // one 1024-lane workgroup per CU
+
+template <typename R, bool REAL_AMP, bool STAGE, int BS>
+static void launch_col(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static bool attr_set = false;
+  auto fn = k_hxv_col<R, REAL_AMP, STAGE, BS>;
+  if (!attr_set) {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
+}
+
+template <typename R, bool REAL_AMP>
+static void launch_col_t(Sector &sec, const HxvArgs &a, bool stage, int bs, dim3 grid,
+                         size_t smem) {
+  if (bs == 1024) {
+    if (stage) launch_col<R, REAL_AMP, true, 1024>(sec, a, grid, smem);
+    else launch_col<R, REAL_AMP, false, 1024>(sec, a, grid, smem);
+  } else {
+    if (stage) launch_col<R, REAL_AMP, true, 256>(sec, a, grid, smem);
+    else launch_col<R, REAL_AMP, false, 256>(sec, a, grid, smem);
+  }
 }
 
 template <typename R, bool REAL_AMP>
@@ -561,18 +781,37 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   a.cross = sector_cross(sec);
   a.lz = lanczos ? sec.lz : nullptr;
   const size_t esz = fp64 ? sizeof(double2) : sizeof(float2);
-  // columns per block: fill 256 lanes when columns are short
-  int cpb = 1;
-  if (a.dimup < kBlock) cpb = std::max(1, kBlock / a.dimup);
+  const HxvShape shp = hxv_shape(sec);
+  const int cpb = shp.cpb;
   a.cpb = cpb;
-  const int grid = (int)((a.dimdw + cpb - 1) / cpb);
+  const int grid = shp.grid;
+  a.dw_maxnnz = sec.dw.maxnnz;
+  const bool real_amp = sec.up.real_amp && sec.dw.real_amp;
+  if (shp.onecol_kernel) {
+    if (lanczos) ensure_partials(sec, grid * nb);
+    a.partials = sec.partials;
+    const size_t fixed = (size_t)a.namp_up * esz + (size_t)a.dw_maxnnz * (esz + 8) + 48;
+    const size_t col_bytes = (size_t)a.dimup * esz;
+    const size_t budget = shp.bs == 1024 ? kLdsBig : kLdsBudget;
+    const bool stage = fixed + col_bytes <= budget;
+    const size_t smem = fixed + (stage ? col_bytes : 0);
+    const dim3 g3(grid, nb);
+    if (fp64) {
+      if (real_amp) launch_col_t<double, true>(sec, a, stage, shp.bs, g3, smem);
+      else launch_col_t<double, false>(sec, a, stage, shp.bs, g3, smem);
+    } else {
+      if (real_amp) launch_col_t<float, true>(sec, a, stage, shp.bs, g3, smem);
+      else launch_col_t<float, false>(sec, a, stage, shp.bs, g3, smem);
+    }
+    VCA_HIP(hipGetLastError());
+    return;
+  }
   const size_t amp_bytes = (size_t)(a.namp_up + a.namp_dw) * esz;
   const size_t col_bytes = (size_t)cpb * a.dimup * esz;
   const bool stage = amp_bytes + col_bytes <= kLdsBudget;
   const size_t smem = amp_bytes + (stage ? col_bytes : 0);
   if (lanczos) ensure_partials(sec, grid * nb);
   a.partials = sec.partials;
-  const bool real_amp = sec.up.real_amp && sec.dw.real_amp;
   const bool onecol = cpb == 1;
   const dim3 g3(grid, nb);
   if (fp64) {

@@ -2,13 +2,14 @@
 //
 //   f(z) = (1/Nk) sum_k log|det(1 + (Delta(z) + Hloc - H_k) G'(z))|
 //
-// is evaluated through the identity
-//   det(1 + A_k G') = det(G') det(G'^{-1} + A_k),
-// so that per frequency one small inverse gives B(z) = G'^{-1} + Delta + Hloc
-// and per (k, z) only an LU of B(z) - H_k remains (no n^3 product per k).
-// When Hloc, H_k and the bath are spin-diagonal, the determinant further
-// factorises over spin blocks of size Nlat*Norb (G' is spin-diagonal in the
-// reference, VCA_IO/gf_cluster.f90:23-49).
+// in the reference's own form: per frequency, P(z) = (Delta + Hloc) G'(z)
+// once, then per (k, z) the contraction M_k = 1 + P(z) - H_k G'(z) and an
+// LU of M_k.  (The algebraically equal det(G') det(G'^-1 + A_k) avoids the
+// product but cancels catastrophically at large |z|, where both logarithms
+// are ~ +-n log|z| and f ~ 1/|z|^2: the QAGI map samples exactly there.)
+// When Hloc, H_k and the bath are spin-diagonal the determinant factorises
+// over spin blocks of size Nlat*Norb (G' is spin-diagonal in the reference,
+// VCA_IO/gf_cluster.f90:23-49).
 //
 // Kernels:
 //   k_pole_sum   G_raw(z, e) = sum_p w_p / (z - p_p) over the Lehmann poles
@@ -16,12 +17,12 @@
 //                VCA_IO/gf_cluster.f90:23-49
 //   k_prep       recombination G_ij = (G^mix - (1 - i chan4)(G_ii + G_jj))/2
 //                (gf_cluster.f90:50-62), Delta(z) = V (z+mu-h)^-1 V^+
-//                (VCA_BATH_FUNCTIONS.f90:127-150), G'^{-1}, log|det G'|,
-//                B(z); one workgroup per (z, spin block), Gauss-Jordan in LDS
-//   k_logdet     per (k, z): LU with partial pivoting of B(z) - H_k, one
-//                matrix per 16-lane group (row per lane, pivot row broadcast
-//                by lane permutes), H_k rows held in registers across a
-//                chunk of frequencies; FP64 log accumulation
+//                (VCA_BATH_FUNCTIONS.f90:127-150, Gauss-Jordan in LDS), and
+//                P(z) = (Delta + Hloc) G'; one workgroup per (z, spin block)
+//   k_logdet     per (k, z): M_k = 1 + P - H_k G' with H_k's rows held in
+//                registers across a chunk of frequencies and G'(z), P(z)
+//                staged in LDS; LU with partial pivoting, one matrix per
+//                16-lane group (row per lane); FP64 log accumulation
 //   k_reduce_z   deterministic sum of the k-block partials per z
 #include <hip/hip_runtime.h>
 
@@ -140,8 +141,8 @@ struct PrepArgs {
   int nbb;                  // bath block size (0: no bath)
   const double2 *hb_blk;    // [nspin_blk][nbb][nbb]
   const double2 *vb_blk;    // [nspin_blk][nb][nbb]
-  double2 *bmat;            // out [nz][nspin_blk][nb][nb] row-major
-  double *logdet_g;         // out [nz][nspin_blk]
+  double2 *gmat;            // out G'(z)  [nz][nspin_blk][16][16], zero padded
+  double2 *pmat;            // out P(z) = (Delta + Hloc) G'(z), same layout
 };
 
 __global__ __launch_bounds__(64) void k_prep(const PrepArgs a) {
@@ -179,7 +180,12 @@ __global__ __launch_bounds__(64) void k_prep(const PrepArgs a) {
     s_g[r * kNB + c] = v;
   }
   __syncthreads();
-  const double ldg = gj_inverse(nb, s_g, s_inv, kNB, &s_ld, &s_piv);
+  // s_inv := Hloc (+ Delta)
+  const double2 *hl = a.hloc_blk + (int64_t)sb * nb * nb;
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e / nb, c = e % nb;
+    s_inv[r * kNB + c] = hl[r * nb + c];
+  }
   // Delta = V (z + mu - h)^-1 V^+
   if (a.nbb > 0) {
     const int nbb = a.nbb;
@@ -214,35 +220,39 @@ __global__ __launch_bounds__(64) void k_prep(const PrepArgs a) {
       s_inv[r * kNB + c].x += dr;
       s_inv[r * kNB + c].y += di;
     }
-    __syncthreads();
   }
-  const double2 *hl = a.hloc_blk + (int64_t)sb * nb * nb;
-  double2 *out = a.bmat + ((int64_t)iz * a.nspin_blk + sb) * kNB * kNB;
+  __syncthreads();
+  // P = (Hloc + Delta) G'; both outputs zero padded to 16 x 16
+  double2 *gout = a.gmat + ((int64_t)iz * a.nspin_blk + sb) * kNB * kNB;
+  double2 *pout = a.pmat + ((int64_t)iz * a.nspin_blk + sb) * kNB * kNB;
   for (int e = tid; e < kNB * kNB; e += blockDim.x) {
     const int r = e / kNB, c = e % kNB;
-    double2 v;
+    double2 gv = make_double2(0.0, 0.0), pv = make_double2(0.0, 0.0);
     if (r < nb && c < nb) {
-      v = s_inv[r * kNB + c];
-      v.x += hl[r * nb + c].x;
-      v.y += hl[r * nb + c].y;
-    } else {
-      v = make_double2(r == c ? 1.0 : 0.0, 0.0);
+      gv = s_g[r * kNB + c];
+      for (int m = 0; m < nb; ++m) {
+        const double2 x = s_inv[r * kNB + m], y = s_g[m * kNB + c];
+        pv.x += x.x * y.x - x.y * y.y;
+        pv.y += x.x * y.y + x.y * y.x;
+      }
     }
-    out[e] = v;
+    gout[e] = gv;
+    pout[e] = pv;
   }
-  if (tid == 0) a.logdet_g[(int64_t)iz * a.nspin_blk + sb] = ldg;
 }
 
 // ------------------------------------------------------- k x z log-det --
-// Each 16-lane group owns one matrix M = B(z) - H_k, lane r holding row r.
-// Partial pivoting: the pivot row is the active row of largest |M(r,k)|
-// (ties to the lowest row), found with a 16-lane butterfly, then broadcast
-// to the group.  Rows already used as pivots go inactive, so no row swap is
-// needed; log|det| = sum log|pivot| is independent of the row order.
+// Each 16-lane group owns one matrix M = 1 + P(z) - H_k G'(z), lane r
+// holding row r: the lane forms its row of H_k G' from its register copy
+// of H_k's row and G'(z) read from LDS (the same address for the whole
+// wavefront: broadcast reads).  Partial pivoting: the pivot row is the
+// active row of largest |M(r,k)| (ties to the lowest row), found with a
+// 16-lane butterfly, then broadcast to the group; rows already used as
+// pivots go inactive, so no swap is needed and log|det| = sum log|pivot|.
 template <typename R>
-__global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int z0, int zchunk,
-                                                    int nspin_blk, const ocplx<R> *hk,
-                                                    const ocplx<R> *bmat, double *partials) {
+__global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, int nspin_blk,
+                                                    const ocplx<R> *hk, const ocplx<R> *gmat,
+                                                    const ocplx<R> *pmat, double *partials) {
   using C = ocplx<R>;
   const int lane = threadIdx.x & 63;
   const int gl = lane & 15;
@@ -252,7 +262,7 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int z0, int 
   const int k = kb * 16 + grp;
   const bool kvalid = k < nk;
   __shared__ double s_red[16];
-  // H_k row in registers (identity padding outside the block and for k >= nk)
+  __shared__ C s_gz[kNB * kNB], s_pz[kNB * kNB];
   R hr[kNB], hi[kNB];
   {
     const C *row = hk + (((int64_t)(kvalid ? k : 0) * nspin_blk + sb) * kNB + gl) * kNB;
@@ -263,20 +273,33 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int z0, int 
       hi[j] = kvalid ? v.y : (R)0;
     }
   }
-  const int zbeg = z0 + blockIdx.y * zchunk;
+  const int zbeg = blockIdx.y * zchunk;
   const int zend = min(zbeg + zchunk, nz);
   for (int iz = zbeg; iz < zend; ++iz) {
+    const int64_t zoff = ((int64_t)iz * nspin_blk + sb) * kNB * kNB;
+    __syncthreads();
+    for (int e = threadIdx.x; e < kNB * kNB; e += kOBlock) {
+      s_gz[e] = gmat[zoff + e];
+      s_pz[e] = pmat[zoff + e];
+    }
+    __syncthreads();
     R ar[kNB], ai[kNB];
-    const C *brow = bmat + (((int64_t)iz * nspin_blk + sb) * kNB + gl) * kNB;
 #pragma unroll
     for (int j = 0; j < kNB; ++j) {
-      const C b = brow[j];
-      if (kvalid) {
-        ar[j] = b.x - hr[j];
-        ai[j] = b.y - hi[j];
-      } else {
-        ar[j] = (gl == j) ? (R)1 : (R)0;
-        ai[j] = (R)0;
+      const C pz = s_pz[gl * kNB + j];
+      ar[j] = (gl == j ? (R)1 : (R)0) + pz.x;
+      ai[j] = pz.y;
+    }
+    // - H_k G' (m is uniform across the wavefront: the row element is
+    // picked from registers with an indexed move, not spilled)
+#pragma unroll 1
+    for (int m = 0; m < kNB; ++m) {
+      const R xr = hr[m], xi = hi[m];
+#pragma unroll
+      for (int j = 0; j < kNB; ++j) {
+        const C g = s_gz[m * kNB + j];
+        ar[j] -= xr * g.x - xi * g.y;
+        ai[j] -= xr * g.y + xi * g.x;
       }
     }
     bool active = true;
@@ -305,7 +328,6 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int z0, int 
       ld += 0.5 * log(pm);
       if (gl == idx) active = false;
       if (active) {
-        // l = a[kk] / p[kk]
         const R inv = (R)1 / (pr[kk] * pr[kk] + pim[kk] * pim[kk]);
         const R lr = (ar[kk] * pr[kk] + ai[kk] * pim[kk]) * inv;
         const R li = (ai[kk] * pr[kk] - ar[kk] * pim[kk]) * inv;
@@ -316,22 +338,20 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int z0, int 
         }
       }
     }
-    // ld is identical in the 16 lanes of a group; one lane per group adds
-    if (gl == 0) s_red[grp] = ld;
+    // ld is identical in the 16 lanes of a group; invalid k contribute 0
+    if (gl == 0) s_red[grp] = kvalid ? ld : 0.0;
     __syncthreads();
     if (threadIdx.x == 0) {
-      double s = 0;
-      for (int g = 0; g < 16; ++g) s += s_red[g];
-      partials[((int64_t)iz * nspin_blk + sb) * gridDim.x + kb] = s;
+      double t = 0;
+      for (int g = 0; g < 16; ++g) t += s_red[g];
+      partials[((int64_t)iz * nspin_blk + sb) * gridDim.x + kb] = t;
     }
-    __syncthreads();
   }
 }
 
-__global__ __launch_bounds__(64) void k_reduce_z(int nz, int z0, int nspin_blk, int nkb,
-                                                 int nk, const double *partials,
-                                                 const double *logdet_g, double *f) {
-  const int iz = z0 + blockIdx.x;
+__global__ __launch_bounds__(64) void k_reduce_z(int nz, int nspin_blk, int nkb, int nk,
+                                                 const double *partials, double *f) {
+  const int iz = blockIdx.x;
   if (iz >= nz) return;
   double s = 0;
   for (int sb = 0; sb < nspin_blk; ++sb) {
@@ -339,7 +359,7 @@ __global__ __launch_bounds__(64) void k_reduce_z(int nz, int z0, int nspin_blk, 
     const double *p = partials + ((int64_t)iz * nspin_blk + sb) * nkb;
     for (int b = threadIdx.x; b < nkb; b += 64) t += p[b];
     for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
-    s += t / nk + logdet_g[(int64_t)iz * nspin_blk + sb];
+    s += t / nk;
   }
   if (threadIdx.x == 0) f[iz] = s;
 }
@@ -350,7 +370,8 @@ OmegaDev::~OmegaDev() { release(); }
 void OmegaDev::release() {
   for (void *p : {(void *)d_off, (void *)d_w, (void *)d_pole, (void *)d_hk64, (void *)d_hk32,
                   (void *)d_hloc, (void *)d_hb, (void *)d_vb, (void *)d_z, (void *)d_graw,
-                  (void *)d_b64, (void *)d_b32, (void *)d_ldg, (void *)d_part, (void *)d_f})
+                  (void *)d_g64, (void *)d_g32, (void *)d_p64, (void *)d_p32, (void *)d_part,
+                  (void *)d_f})
     if (p) (void)hipFree(p);
   d_off = nullptr;
   d_w = nullptr;
@@ -362,9 +383,10 @@ void OmegaDev::release() {
   d_vb = nullptr;
   d_z = nullptr;
   d_graw = nullptr;
-  d_b64 = nullptr;
-  d_b32 = nullptr;
-  d_ldg = nullptr;
+  d_g64 = nullptr;
+  d_g32 = nullptr;
+  d_p64 = nullptr;
+  d_p32 = nullptr;
   d_part = nullptr;
   d_f = nullptr;
   zcap = 0;
@@ -410,16 +432,18 @@ void OmegaDev::upload(const OmegaHost &h, hipStream_t st) {
 
 void OmegaDev::ensure_z(int nz) {
   if (nz <= zcap) return;
-  for (void *p : {(void *)d_z, (void *)d_graw, (void *)d_b64, (void *)d_b32, (void *)d_ldg,
-                  (void *)d_part, (void *)d_f})
+  for (void *p : {(void *)d_z, (void *)d_graw, (void *)d_g64, (void *)d_g32, (void *)d_p64,
+                  (void *)d_p32, (void *)d_part, (void *)d_f})
     if (p) VCA_HIP(hipFree(p));
   zcap = std::max(nz, 64);
   const int nkb = (host.nk + 15) / 16;
   VCA_HIP(hipMalloc(&d_z, sizeof(double2) * zcap));
   VCA_HIP(hipMalloc(&d_graw, sizeof(double2) * (size_t)zcap * host.nelem));
-  VCA_HIP(hipMalloc(&d_b64, sizeof(double2) * (size_t)zcap * host.nspin_blk * kNB * kNB));
-  VCA_HIP(hipMalloc(&d_b32, sizeof(float2) * (size_t)zcap * host.nspin_blk * kNB * kNB));
-  VCA_HIP(hipMalloc(&d_ldg, sizeof(double) * (size_t)zcap * host.nspin_blk));
+  const size_t nm = (size_t)zcap * host.nspin_blk * kNB * kNB;
+  VCA_HIP(hipMalloc(&d_g64, sizeof(double2) * nm));
+  VCA_HIP(hipMalloc(&d_p64, sizeof(double2) * nm));
+  VCA_HIP(hipMalloc(&d_g32, sizeof(float2) * nm));
+  VCA_HIP(hipMalloc(&d_p32, sizeof(float2) * nm));
   VCA_HIP(hipMalloc(&d_part, sizeof(double) * (size_t)zcap * host.nspin_blk * nkb));
   VCA_HIP(hipMalloc(&d_f, sizeof(double) * zcap));
 }
@@ -456,8 +480,8 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   pa.nbb = host.nbb;
   pa.hb_blk = d_hb;
   pa.vb_blk = d_vb;
-  pa.bmat = d_b64;
-  pa.logdet_g = d_ldg;
+  pa.gmat = d_g64;
+  pa.pmat = d_p64;
   hipLaunchKernelGGL(k_prep, dim3(nz, host.nspin_blk), dim3(64), 0, st, pa);
   VCA_HIP(hipGetLastError());
   const int nkb = (host.nk + 15) / 16;
@@ -470,20 +494,22 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   }
   if (precision == VCA_FP64) {
     if (e0) VCA_HIP(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_logdet<double>, grid, dim3(kOBlock), 0, st, host.nk, nz, 0, zchunk,
-                       host.nspin_blk, d_hk64, d_b64, d_part);
+    hipLaunchKernelGGL(k_logdet<double>, grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
+                       host.nspin_blk, d_hk64, d_g64, d_p64, d_part);
   } else {
-    const int64_t nb = (int64_t)nz * host.nspin_blk * kNB * kNB;
-    hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nb + 255) / 256)), dim3(256), 0, st, nb,
-                       d_b64, d_b32);
+    const int64_t nm = (int64_t)nz * host.nspin_blk * kNB * kNB;
+    hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, nm,
+                       d_g64, d_g32);
+    hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, nm,
+                       d_p64, d_p32);
     if (e0) VCA_HIP(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_logdet<float>, grid, dim3(kOBlock), 0, st, host.nk, nz, 0, zchunk,
-                       host.nspin_blk, d_hk32, d_b32, d_part);
+    hipLaunchKernelGGL(k_logdet<float>, grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
+                       host.nspin_blk, d_hk32, d_g32, d_p32, d_part);
   }
   VCA_HIP(hipGetLastError());
   if (e1) VCA_HIP(hipEventRecord(e1, st));
-  hipLaunchKernelGGL(k_reduce_z, dim3(nz), dim3(64), 0, st, nz, 0, host.nspin_blk, nkb,
-                     host.nk, d_part, d_ldg, d_f);
+  hipLaunchKernelGGL(k_reduce_z, dim3(nz), dim3(64), 0, st, nz, host.nspin_blk, nkb, host.nk,
+                     d_part, d_f);
   VCA_HIP(hipGetLastError());
   VCA_HIP(hipMemcpyAsync(f, d_f, sizeof(double) * nz, hipMemcpyDeviceToHost, st));
   VCA_HIP(hipStreamSynchronize(st));

@@ -36,9 +36,9 @@ struct OmegaDev {
   double2 *d_hk64 = nullptr;
   float2 *d_hk32 = nullptr;
   double2 *d_hloc = nullptr, *d_hb = nullptr, *d_vb = nullptr;
-  double2 *d_z = nullptr, *d_graw = nullptr, *d_b64 = nullptr;
-  float2 *d_b32 = nullptr;
-  double *d_ldg = nullptr, *d_part = nullptr, *d_f = nullptr;
+  double2 *d_z = nullptr, *d_graw = nullptr, *d_g64 = nullptr, *d_p64 = nullptr;
+  float2 *d_g32 = nullptr, *d_p32 = nullptr;
+  double *d_part = nullptr, *d_f = nullptr;
   int zcap = 0;
 
   ~OmegaDev();

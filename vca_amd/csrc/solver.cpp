@@ -17,6 +17,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <map>
@@ -112,8 +113,14 @@ Solver::~Solver() {
 
 const uint32_t *solver_binom(const Solver *s) { return s->d_binom; }
 
+void clear_graphs(Sector &sec) {
+  for (auto &g : sec.graphs) (void)hipGraphExecDestroy(g.exec);
+  sec.graphs.clear();
+}
+
 Sector::~Sector() {
   if (stream) (void)hipStreamSynchronize(stream);
+  clear_graphs(*this);
   for (DevSpin *sp : {&up, &dw})
     for (void *q : {(void *)sp->map, (void *)sp->energy, (void *)sp->cnt, (void *)sp->ent,
                     (void *)sp->amp64, (void *)sp->amp32})
@@ -255,6 +262,7 @@ static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec
 // Lanczos workspace for nb chains of up to cap steps: three vector buffers
 // [nb][dim], one LanczosDev per chain with its alpha/beta/nrm/y arrays.
 static void ensure_lanczos(Sector &sec, int cap, int nb = 1) {
+  if (sec.nb_cap < nb || sec.lz_cap < cap) clear_graphs(sec);
   if (sec.nb_cap < nb) {
     VCA_HIP(hipStreamSynchronize(sec.stream));
     for (int i = 0; i < 3; ++i)
@@ -290,8 +298,41 @@ static void ensure_lanczos(Sector &sec, int cap, int nb = 1) {
   }
 }
 
+// A chunk of Lanczos steps recorded once into a HIP graph and replayed:
+// the kernels take the step index and the buffer parity from device
+// memory, so the same graph serves every chunk of the run.  The environment
+// variable VCA_NO_GRAPHS=1 launches every step eagerly instead.
+static hipGraphExec_t step_graph(Sector &sec, int n, int nb, bool accx) {
+  for (auto &g : sec.graphs)
+    if (g.nsteps == n && g.nb == nb && g.accx == accx) return g.exec;
+  hipGraph_t graph;
+  VCA_HIP(hipStreamBeginCapture(sec.stream, hipStreamCaptureModeThreadLocal));
+  for (int i = 0; i < n; ++i) {
+    launch_hxv(sec, sec.buf[0], sec.buf[1], sec.buf[2], true, nb);
+    launch_lanczos_update(sec, accx, nb);
+  }
+  VCA_HIP(hipStreamEndCapture(sec.stream, &graph));
+  hipGraphExec_t ex;
+  VCA_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+  VCA_HIP(hipGraphDestroy(graph));
+  sec.graphs.push_back(Sector::StepGraph{n, nb, accx, ex});
+  return ex;
+}
+
+static bool graphs_enabled() {
+  const char *e = std::getenv("VCA_NO_GRAPHS");
+  return !(e && e[0] == '1');
+}
+
 static void run_steps(Sector &sec, int nsteps, bool accx, Solver *s, int nb = 1) {
-  for (int i = 0; i < nsteps; ++i) {
+  const int chunk = 16;
+  ensure_step_partials(sec, nb);
+  int done = 0;
+  if (nsteps >= chunk && graphs_enabled()) {
+    hipGraphExec_t ex = step_graph(sec, chunk, nb, accx);
+    for (; done + chunk <= nsteps; done += chunk) VCA_HIP(hipGraphLaunch(ex, sec.stream));
+  }
+  for (; done < nsteps; ++done) {
     launch_hxv(sec, sec.buf[0], sec.buf[1], sec.buf[2], true, nb);
     launch_lanczos_update(sec, accx, nb);
   }

@@ -211,6 +211,13 @@ struct Sector {
   std::vector<LanczosDev> lzh_all;  // host copies for every chain
   ExcOp *d_ops = nullptr;    // [nb_cap]
   double *lz_arrays = nullptr;
+  // captured chunks of Lanczos steps, keyed by (steps, chains, accumulate)
+  struct StepGraph {
+    int nsteps, nb;
+    bool accx;
+    hipGraphExec_t exec;
+  };
+  std::vector<StepGraph> graphs;
   size_t vec_bytes() const {
     return (size_t)dim * (precision == VCA_FP64 ? sizeof(double2) : sizeof(float2));
   }
@@ -225,6 +232,8 @@ const uint32_t *solver_binom(const Solver *s);
 void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos,
                 int nb = 1);
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb = 1);
+void ensure_step_partials(Sector &sec, int nb);
+void clear_graphs(Sector &sec);
 void launch_start_vector(Sector &sec, uint64_t seed);
 void launch_norm0(Sector &sec);  // nrm[0] = ||buf[0]||, k = 0, done = 0
 void launch_scale_x(Sector &sec);  // xvec /= ||xvec||
