@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--full", action="store_true", help="also run the full vca_solve of the 3x4 cluster")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bath16", action="store_true",
+                    help="skip the 8-site + 8-bath-level sector (165,636,900 states)")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
     ap.add_argument("--only", choices=["all", "hxv", "omega"], default="all",
@@ -130,6 +132,45 @@ def main():
     log(f"Omega BHZ: {evals} (k,w) evals FP64 {t_om64:.3f}s, MIXED {t_om32:.3f}s; "
         f"integral rel err mixed {om_err:.2e}; full solve {t_solve_bhz:.2f}s, Omega={om64:.10f}")
 
+    # ------------------------------- 8 sites + 8 bath levels, 1.66e8 states
+    bath16 = None
+    if not args.no_bath16:
+        pb, hlb, hkb, bhb, bvb = configs.config_bath_2x4(nk=2, precision="mixed")
+        sb = VcaSolver(pb)
+        sb.vca_set_bath(bhb, bvb)
+        sb.vca_set_hcluster(hlb)
+        out16 = {}
+        for prec in ("mixed", "fp64"):
+            sec = sb.sector(8, 8, prec)
+            dt = torch.complex128 if prec == "fp64" else torch.complex64
+            g = torch.Generator(device=dev).manual_seed(2)
+            v = torch.randn(sec.dim, dtype=dt, device=dev, generator=g)
+            hv = torch.empty_like(v)
+            sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=2)
+            ms = sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=5)
+            del v, hv
+            torch.cuda.empty_cache()
+            out16[prec] = {"ms_per_hxv": ms, "gstates_per_s": sec.dim / (ms * 1e-3) / 1e9,
+                           "hbm_gbs_algorithmic": 2 * (16 if prec == "fp64" else 8) * sec.dim
+                           / (ms * 1e-3) / 1e9}
+            log(f"bath16 {prec}: {ms:.3f} ms/apply, dim {sec.dim}")
+            if prec == "mixed":
+                t0 = time.perf_counter()
+                e_mx, _, it_mx = sec.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
+                out16["gs_mixed"] = {"e0": e_mx, "seconds": time.perf_counter() - t0,
+                                     "refine_iters": it_mx}
+            else:
+                t0 = time.perf_counter()
+                e_64, _, it_64 = sec.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
+                out16["gs_fp64"] = {"e0": e_64, "seconds": time.perf_counter() - t0,
+                                    "iters": it_64}
+            sec.close()
+        out16["e0_rel_err_mixed"] = abs(out16["gs_mixed"]["e0"] - out16["gs_fp64"]["e0"]) / abs(
+            out16["gs_fp64"]["e0"])
+        out16["dim"] = 12870 * 12870
+        bath16 = out16
+        log(f"bath16 GS: mixed {out16['gs_mixed']} fp64 {out16['gs_fp64']}")
+
     full = None
     if args.full:
         pf, hlf, hkf, _, _ = configs.config_square_3x4(nk=64)
@@ -191,6 +232,8 @@ def main():
     }
     if full:
         line["full_solve"] = full
+    if bath16:
+        line["bath16"] = bath16
     print(json.dumps(line), flush=True)
 
 

@@ -12,7 +12,7 @@ import pytest
 
 from oracle import vca_ref as R
 from vca_amd import configs
-from vca_amd.api import VcaSolver
+from vca_amd.api import VcaParams, VcaSolver
 
 from . import _models as M
 
@@ -96,6 +96,33 @@ def test_hxv_square_3x4_full_sector():
     v = _rand(sec.dim, 77)
     err = np.linalg.norm(sec.hxv(v) - om.hxv(6, 6, v, workers=8)) / np.linalg.norm(v)
     assert err < 1e-12, err
+
+
+N_HXV_VARIANTS = 15  # launch shapes of the one-column kernels (kernels.hip kVariants)
+
+
+@pytest.mark.parametrize("variant", range(N_HXV_VARIANTS))
+def test_hxv_launch_variants(variant, monkeypatch):
+    """Every launch shape the autotuner can pick gives the oracle's H x v
+    (one-column kernels, plain and buffer-load forms: chain of 10 sites,
+    bath model, complex BHZ)."""
+    monkeypatch.setenv("VCA_HXV_VARIANT", str(variant))
+    bhz = VcaParams(nlat=5, norb=2, nspin=2, uloc=(2.0, 2.0, 0, 0, 0))
+    bhz_h, bhz_hk = configs.bhz(5, 1, 2.0, 0.3, nk=(2, 1))
+    for name, (p, hloc, hk, bh, bv), sector in (
+            ("chain10", M.chain(nlat=10, u=4.0), (5, 5)),
+            ("bath5+5", M.bath_model(nlat=5, nb=5), (5, 5)),
+            ("bhz5x1", (bhz, bhz_h, bhz_hk, None, None), (5, 4))):
+        om = M.oracle_model(p, hloc, bh, bv)
+        s = _solver(p, hloc, bh, bv)
+        for prec in ("fp64", "mixed"):
+            _say(f"variant {variant} {name} {sector} {prec}")
+            sec = s.sector(*sector, prec)
+            v = _rand(sec.dim, 31)
+            ref = om.hxv(*sector, v)
+            err = np.linalg.norm(sec.hxv(v) - ref) / np.linalg.norm(ref)
+            assert err < (1e-12 if prec == "fp64" else 2e-6), (name, prec, err)
+            sec.close()
 
 
 def test_lanczos_gs_against_dense():

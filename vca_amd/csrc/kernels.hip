@@ -99,6 +99,25 @@ __global__ void k_build_spin(int ns, int npart, int dim, int nterms, const int2 
   cnt[i] = row_hops(s, nterms, ab, binom, ent + i, dim);
 }
 
+// Row records and 4-hop packing of the ELL table for the buffer-load H x v.
+__global__ void k_pack_rows(int dim, int maxnnz, const uint32_t *map, const double *energy,
+                            const int32_t *cnt, const uint32_t *ent, RowMeta *meta,
+                            uint32_t *ent4) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= dim) return;
+  RowMeta m;
+  m.e = energy[i];
+  m.s = map[i];
+  m.cnt = cnt[i];
+  meta[i] = m;
+  const int nc = (maxnnz + 3) / 4;
+  for (int c = 0; c < nc; ++c)
+    for (int t = 0; t < 4; ++t) {
+      const int q = 4 * c + t;
+      ent4[((int64_t)c * dim + i) * 4 + t] = q < m.cnt ? ent[(int64_t)q * dim + i] : 0u;
+    }
+}
+
 // ------------------------------------------------------------ H x v -----
 struct HxvArgs {
   const void *v0, *v1;  // input (Lanczos: u_k in buffer k&1)
@@ -115,6 +134,9 @@ struct HxvArgs {
   double *partials;
   int64_t vstride;  // elements between the vectors of consecutive chains
   int dw_maxnnz;    // capacity of the staged down-spin hop list
+  const RowMeta *meta_up;
+  const uint32_t *ent4_up;
+  int up_nc;        // 4-hop chunks per row
 };
 
 __device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t sd) {
@@ -269,44 +291,36 @@ __device__ inline int xcd_column(int b, int ncols) {
   return x * q + min(x, r) + pos;
 }
 
-constexpr int kRows = 4;  // rows per thread processed together
 
 // One workgroup of BS lanes per output column idw (DimUp >= BS/2):
 //  * the down-spin hop list of the column (<= maxnnz entries) is staged in
 //    LDS as (column offset, signed amplitude);
 //  * the input column v(:, idw) is staged in LDS when it fits (STAGE), so
 //    the diagonal and the up-spin gathers read LDS;
-//  * each lane handles kRows rows at once, so the down-spin loop issues
-//    kRows independent coalesced loads of v(:, jdw) per hop.
-template <typename R, bool REAL_AMP, bool STAGE, int BS>
+//  * each lane handles KR rows at once, so the down-spin loop issues KR
+//    independent coalesced loads of v(:, jdw) per hop.
+// BS (lanes) and KR (rows per lane) are picked per sector shape by the
+// autotuner (hxv_tune).
+__device__ __host__ constexpr int align16(int n) { return (n + 15) & ~15; }
+
+template <typename R, bool REAL_AMP, bool STAGE, int BS, int KR, int U>
 __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
   using C = cplx<R>;
+  // U hops per batch of independent loads: KR*U loads in flight per lane
   extern __shared__ __align__(16) unsigned char smem[];
   const int maxnd = a.dw_maxnnz;
-  // every region starts on a 16-byte boundary (128-bit LDS accesses)
-  auto align16 = [](unsigned char *p) {
-    return reinterpret_cast<unsigned char *>((reinterpret_cast<uintptr_t>(p) + 15) & ~uintptr_t(15));
-  };
+  // every region starts on a 16-byte boundary (128-bit LDS accesses); the
+  // regions are byte offsets from smem so the compiler keeps LDS accesses
+  const int o_dw_amp = align16(a.namp_up * (int)sizeof(C));
+  const int o_dw_off = o_dw_amp + align16(maxnd * (int)sizeof(C));
+  const int o_col = o_dw_off + align16(maxnd * (int)sizeof(int64_t));
   C *s_amp_up = reinterpret_cast<C *>(smem);
-  C *s_dw_amp = reinterpret_cast<C *>(align16(reinterpret_cast<unsigned char *>(s_amp_up + a.namp_up)));
-  int64_t *s_dw_off = reinterpret_cast<int64_t *>(align16(reinterpret_cast<unsigned char *>(s_dw_amp + maxnd)));
-  C *s_col = reinterpret_cast<C *>(align16(reinterpret_cast<unsigned char *>(s_dw_off + maxnd)));
-  __shared__ int s_nd;
+  C *s_dw_amp = reinterpret_cast<C *>(smem + o_dw_amp);
+  int64_t *s_dw_off = reinterpret_cast<int64_t *>(smem + o_dw_off);
+  C *s_col = reinterpret_cast<C *>(smem + o_col);
 
   const int dimup = a.dimup;
   const int idw = xcd_column(blockIdx.x, a.dimdw);
-  const C *g_amp_up = static_cast<const C *>(a.amp_up);
-  const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
-  for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
-  if (threadIdx.x == 0) s_nd = a.cnt_dw[idw];
-  __syncthreads();
-  const int nd = s_nd;
-  for (int q = threadIdx.x; q < nd; q += BS) {
-    const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
-    s_dw_off[q] = (int64_t)(e & kColMask) * dimup;
-    s_dw_amp[q] = g_amp_dw[e >> kColBits];
-  }
-
   const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
   LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
   int k = 0;
@@ -324,45 +338,216 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
   double dot = 0.0;
 
   if (!done) {
+    // one staging phase: amplitudes, the column's down-spin hop list and
+    // (STAGE) the input column, then one barrier
+    const C *g_amp_up = static_cast<const C *>(a.amp_up);
+    const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
+    const int nd = a.cnt_dw[idw];  // same address in every lane
+    for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
+    for (int q = threadIdx.x; q < nd; q += BS) {
+      const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
+      s_dw_off[q] = (int64_t)(e & kColMask) * dimup;
+      s_dw_amp[q] = g_amp_dw[e >> kColBits];
+    }
     if (STAGE)
       for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
     __syncthreads();
     const double edw = a.e_dw[idw];
     const uint32_t sdw = a.map_dw[idw];
-    for (int base = threadIdx.x; base < dimup; base += BS * kRows) {
-      int row[kRows];
-      R ar[kRows], ai[kRows];
-      C x[kRows];
+    for (int base = threadIdx.x; base < dimup; base += BS * KR) {
+      int row[KR];
+      R ar[KR], ai[KR];
+      C x[KR];
+      int nu[KR];
 #pragma unroll
-      for (int j = 0; j < kRows; ++j) {
+      for (int j = 0; j < KR; ++j) {
         row[j] = min(base + j * BS, dimup - 1);  // clamped: loads stay in bounds
+        nu[j] = (base + j * BS < dimup) ? a.cnt_up[row[j]] : 0;
         x[j] = STAGE ? s_col[row[j]] : vcol[row[j]];
         const double ed = a.e_up[row[j]] + edw + cross_energy(a.cross, a.map_up[row[j]], sdw);
         ar[j] = (R)ed * x[j].x;
         ai[j] = (R)ed * x[j].y;
       }
-      // up-spin hops: gathers inside the column
+      // up-spin hops, U hop entries of every row fetched before use
+      int nu_max = 0;
 #pragma unroll
-      for (int j = 0; j < kRows; ++j) {
-        const int nu = (base + j * BS < dimup) ? a.cnt_up[row[j]] : 0;
-        for (int q = 0; q < nu; ++q) {
-          const uint32_t e = a.ent_up[(int64_t)q * dimup + row[j]];
-          const C y = STAGE ? s_col[e & kColMask] : vcol[e & kColMask];
-          cmac<R, REAL_AMP>(ar[j], ai[j], s_amp_up[e >> kColBits], y);
+      for (int j = 0; j < KR; ++j) nu_max = max(nu_max, nu[j]);
+      for (int q0 = 0; q0 < nu_max; q0 += U) {
+        uint32_t e[U][KR];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < KR; ++j) {
+            const int q = min(q0 + u, max(nu[j] - 1, 0));
+            e[u][j] = a.ent_up[(int64_t)q * dimup + row[j]];
+          }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+          for (int j = 0; j < KR; ++j)
+            if (q0 + u < nu[j]) {
+              const uint32_t ee = e[u][j];
+              const C y = STAGE ? s_col[ee & kColMask] : vcol[ee & kColMask];
+              cmac<R, REAL_AMP>(ar[j], ai[j], s_amp_up[ee >> kColBits], y);
+            }
+      }
+      // down-spin hops: U hops x KR rows of independent coalesced loads
+      for (int q0 = 0; q0 < nd; q0 += U) {
+        C y[U][KR];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const int64_t off = s_dw_off[min(q0 + u, nd - 1)];
+#pragma unroll
+          for (int j = 0; j < KR; ++j) y[u][j] = vin[off + row[j]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (q0 + u < nd) {
+            const C h = s_dw_amp[q0 + u];
+#pragma unroll
+            for (int j = 0; j < KR; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h, y[u][j]);
+          }
+      }
+#pragma unroll
+      for (int j = 0; j < KR; ++j) {
+        if (base + j * BS < dimup) {
+          C out;
+          out.x = ar[j] * scale;
+          out.y = ai[j] * scale;
+          hv[row[j]] = out;
+          if (lz)
+            dot += (double)(x[j].x * scale) * (double)out.x +
+                   (double)(x[j].y * scale) * (double)out.y;
         }
       }
-      // down-spin hops: kRows independent coalesced loads per hop
-      for (int q = 0; q < nd; ++q) {
-        const int64_t off = s_dw_off[q];
-        const C h = s_dw_amp[q];
-        C y[kRows];
+    }
+  }
+  if (lz) {
+    double tot;
+    if (grid_sum_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[0],
+                       &tot))
+      lz->alpha[k] = done ? 0.0 : tot;
+  }
+}
+
+// Buffer-load form of the one-column kernel.  The same arithmetic as
+// k_hxv_col, with fewer and cheaper instructions:
+//  * neighbour-column loads are raw buffer loads with the row offset in a
+//    VGPR fixed per lane and the column offset in an SGPR (read first
+//    lane of the staged hop list), so no per-load address arithmetic;
+//  * each row's energy, string and hop count come in one 16-byte record;
+//  * up-spin hop entries come 4 per 16-byte load.
+// The input column is always staged in LDS (the caller checks it fits).
+template <typename R, bool REAL_AMP, int BS, int KR>
+__global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
+  using C = cplx<R>;
+  constexpr int ESZ = sizeof(C);
+  constexpr int U = KR >= 4 ? 4 : 8;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int maxnd = a.dw_maxnnz;
+  const int o_dw_amp = align16(a.namp_up * ESZ);
+  const int o_dw_soff = o_dw_amp + align16(maxnd * ESZ);
+  const int o_col = o_dw_soff + align16(maxnd * (int)sizeof(uint32_t));
+  C *s_amp_up = reinterpret_cast<C *>(smem);
+  C *s_dw_amp = reinterpret_cast<C *>(smem + o_dw_amp);
+  uint32_t *s_dw_soff = reinterpret_cast<uint32_t *>(smem + o_dw_soff);
+  C *s_col = reinterpret_cast<C *>(smem + o_col);
+
+  const int dimup = a.dimup;
+  const int idw = xcd_column(blockIdx.x, a.dimdw);
+  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
+  int k = 0;
+  bool done = false;
+  R scale = (R)1;
+  const C *vin = static_cast<const C *>(a.v0) + chain_off;
+  if (lz) {
+    k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const C *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  C *hv = static_cast<C *>(a.hv) + chain_off + (int64_t)idw * dimup;
+  const C *vcol = vin + (int64_t)idw * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<C *>(vin), (short)0, (int)(uint32_t)(a.vstride * ESZ), 0x00020000);
+    const C *g_amp_up = static_cast<const C *>(a.amp_up);
+    const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
+    const int nd = a.cnt_dw[idw];
+    for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
+    for (int q = threadIdx.x; q < nd; q += BS) {
+      const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
+      s_dw_soff[q] = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
+      s_dw_amp[q] = g_amp_dw[e >> kColBits];
+    }
+    for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
+    __syncthreads();
+    const double edw = a.e_dw[idw];
+    const uint32_t sdw = a.map_dw[idw];
+    for (int base = threadIdx.x; base < dimup; base += BS * KR) {
+      int row[KR];
+      R ar[KR], ai[KR];
+      C x[KR];
+      int nu[KR];
 #pragma unroll
-        for (int j = 0; j < kRows; ++j) y[j] = vin[off + row[j]];
+      for (int j = 0; j < KR; ++j) {
+        row[j] = min(base + j * BS, dimup - 1);
+        const RowMeta m = a.meta_up[row[j]];
+        nu[j] = (base + j * BS < dimup) ? m.cnt : 0;
+        x[j] = s_col[row[j]];
+        const double ed = m.e + edw + cross_energy(a.cross, m.s, sdw);
+        ar[j] = (R)ed * x[j].x;
+        ai[j] = (R)ed * x[j].y;
+      }
+      int nu_max = 0;
 #pragma unroll
-        for (int j = 0; j < kRows; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h, y[j]);
+      for (int j = 0; j < KR; ++j) nu_max = max(nu_max, nu[j]);
+      // up-spin hops: 4 entries per 16-byte load, gathers from LDS
+      for (int c = 0; 4 * c < nu_max; ++c) {
+        uint4 e4[KR];
+#pragma unroll
+        for (int j = 0; j < KR; ++j)
+          e4[j] = reinterpret_cast<const uint4 *>(a.ent4_up)[(int64_t)c * dimup + row[j]];
+#pragma unroll
+        for (int j = 0; j < KR; ++j) {
+          const uint32_t ee[4] = {e4[j].x, e4[j].y, e4[j].z, e4[j].w};
+#pragma unroll
+          for (int t = 0; t < 4; ++t)
+            if (4 * c + t < nu[j]) {
+              const C y = s_col[ee[t] & kColMask];
+              cmac<R, REAL_AMP>(ar[j], ai[j], s_amp_up[ee[t] >> kColBits], y);
+            }
+        }
+      }
+      // down-spin hops: buffer loads, column offset in an SGPR
+      for (int q0 = 0; q0 < nd; q0 += U) {
+        C y[U][KR];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t so = __builtin_amdgcn_readfirstlane(s_dw_soff[min(q0 + u, nd - 1)]);
+#pragma unroll
+          for (int j = 0; j < KR; ++j) {
+            if constexpr (ESZ == 16)
+              y[u][j] = __builtin_bit_cast(
+                  C, __builtin_amdgcn_raw_buffer_load_b128(rv, row[j] * ESZ, so, 0));
+            else
+              y[u][j] = __builtin_bit_cast(
+                  C, __builtin_amdgcn_raw_buffer_load_b64(rv, row[j] * ESZ, so, 0));
+          }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (q0 + u < nd) {
+            const C h = s_dw_amp[q0 + u];
+#pragma unroll
+            for (int j = 0; j < KR; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h, y[u][j]);
+          }
       }
 #pragma unroll
-      for (int j = 0; j < kRows; ++j) {
+      for (int j = 0; j < KR; ++j) {
         if (base + j * BS < dimup) {
           C out;
           out.x = ar[j] * scale;
@@ -665,6 +850,12 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
                      ns, npart, sp.dim, sp.nterms, d_ab, d_binom, d_eps, d_pair, e_add,
                      sp.map, sp.energy, sp.cnt, sp.ent);
   VCA_HIP(hipGetLastError());
+  const int nc = (sp.maxnnz + 3) / 4;
+  VCA_HIP(hipMalloc(&sp.meta, sizeof(RowMeta) * sp.dim));
+  VCA_HIP(hipMalloc(&sp.ent4, sizeof(uint32_t) * 4 * (size_t)nc * sp.dim));
+  hipLaunchKernelGGL(k_pack_rows, dim3((sp.dim + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                     sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4);
+  VCA_HIP(hipGetLastError());
   VCA_HIP(hipStreamSynchronize(st));
   VCA_HIP(hipFree(d_ab));
   VCA_HIP(hipFree(d_eps));
@@ -686,27 +877,48 @@ static void ensure_partials(Sector &sec, int nblocks) {
   sec.partials_cap = cap;
 }
 
-// Launch shape of H x v on a sector: one workgroup per column (k_hxv_col)
-// when columns are long, several columns per workgroup (k_hxv) otherwise.
+// Launch shape of H x v on a sector: one workgroup per column (k_hxv_col,
+// with BS lanes and KR rows per lane) when columns are long, several
+// columns per workgroup (k_hxv) otherwise.  The shape of a sector comes from
+// the autotuner (sec.hxv_variant) or, before tuning, from the default below.
 struct HxvShape {
   bool onecol_kernel;
-  int bs, cpb, grid;
+  bool buf;  // buffer-load form (k_hxv_buf) when the column can be staged
+  int bs, kr, u, cpb, grid;
 };
+
+// Variants of the one-column kernel the autotuner times: (BS, KR, U, form),
+// form 0 = k_hxv_col, 1 = k_hxv_buf (U fixed by KR there).
+static const int kVariants[][4] = {{256, 4, 4, 0},  {256, 2, 8, 0},  {1024, 1, 8, 0},
+                                   {1024, 2, 8, 0}, {1024, 4, 4, 0}, {512, 2, 8, 0},
+                                   {256, 1, 16, 0}, {512, 1, 16, 0}, {1024, 1, 16, 0},
+                                   {256, 4, 4, 1},  {256, 2, 8, 1},  {512, 2, 8, 1},
+                                   {512, 1, 8, 1},  {1024, 1, 8, 1}, {1024, 2, 8, 1}};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+
 static HxvShape hxv_shape(const Sector &sec) {
   HxvShape h{};
   if (sec.up.dim >= 128) {
     h.onecol_kernel = true;
-    h.bs = sec.up.dim >= 4096 ? 1024 : 256;
+    int v = sec.hxv_variant;
+    if (v < 0 || v >= kNumVariants) v = sec.up.dim >= 4096 ? 4 : 2;  // default
+    h.bs = kVariants[v][0];
+    h.kr = kVariants[v][1];
+    h.u = kVariants[v][2];
+    h.buf = kVariants[v][3] != 0;
     h.cpb = 1;
     h.grid = sec.dw.dim;
   } else {
     h.onecol_kernel = false;
     h.bs = kBlock;
+    h.kr = 1;
     h.cpb = std::max(1, kBlock / sec.up.dim);
     h.grid = (sec.dw.dim + h.cpb - 1) / h.cpb;
   }
   return h;
 }
+
+int hxv_num_variants(const Sector &sec) { return sec.up.dim >= 128 ? kNumVariants : 1; }
 
 // Allocates the partial-sum buffer for the largest grid of a Lanczos step
 // on nb chains, so that launches recorded into a graph never allocate.
@@ -714,12 +926,40 @@ void ensure_step_partials(Sector &sec, int nb) {
   ensure_partials(sec, std::max(hxv_shape(sec).grid, grid_for(sec.dim)) * nb);
 }
 
-constexpr size_t kLdsBig = 150 * 1024;  // one 1024-lane workgroup per CU
+constexpr size_t kLdsBig = 150 * 1024;
 
-template <typename R, bool REAL_AMP, bool STAGE, int BS>
+template <typename R, bool REAL_AMP, bool STAGE, int BS, int KR, int U>
 static void launch_col(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
   static bool attr_set = false;
-  auto fn = k_hxv_col<R, REAL_AMP, STAGE, BS>;
+  auto fn = k_hxv_col<R, REAL_AMP, STAGE, BS, KR, U>;
+  if (!attr_set) {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    attr_set = true;
+  }
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
+}
+
+template <typename R, bool REAL_AMP, bool STAGE>
+static void launch_col_s(Sector &sec, const HxvArgs &a, const HxvShape &h, dim3 grid,
+                         size_t smem) {
+  const int bs = h.bs, kr = h.kr, u = h.u;
+  if (bs == 256 && kr == 4) launch_col<R, REAL_AMP, STAGE, 256, 4, 4>(sec, a, grid, smem);
+  else if (bs == 256 && kr == 2) launch_col<R, REAL_AMP, STAGE, 256, 2, 8>(sec, a, grid, smem);
+  else if (bs == 1024 && kr == 1 && u == 8)
+    launch_col<R, REAL_AMP, STAGE, 1024, 1, 8>(sec, a, grid, smem);
+  else if (bs == 1024 && kr == 2) launch_col<R, REAL_AMP, STAGE, 1024, 2, 8>(sec, a, grid, smem);
+  else if (bs == 1024 && kr == 4) launch_col<R, REAL_AMP, STAGE, 1024, 4, 4>(sec, a, grid, smem);
+  else if (bs == 512 && kr == 2) launch_col<R, REAL_AMP, STAGE, 512, 2, 8>(sec, a, grid, smem);
+  else if (bs == 256 && kr == 1) launch_col<R, REAL_AMP, STAGE, 256, 1, 16>(sec, a, grid, smem);
+  else if (bs == 512 && kr == 1) launch_col<R, REAL_AMP, STAGE, 512, 1, 16>(sec, a, grid, smem);
+  else launch_col<R, REAL_AMP, STAGE, 1024, 1, 16>(sec, a, grid, smem);
+}
+
+template <typename R, bool REAL_AMP, int BS, int KR>
+static void launch_buf(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static bool attr_set = false;
+  auto fn = k_hxv_buf<R, REAL_AMP, BS, KR>;
   if (!attr_set) {
     VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
@@ -729,15 +969,23 @@ static void launch_col(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
 }
 
 template <typename R, bool REAL_AMP>
-static void launch_col_t(Sector &sec, const HxvArgs &a, bool stage, int bs, dim3 grid,
+static void launch_buf_s(Sector &sec, const HxvArgs &a, const HxvShape &h, dim3 grid,
                          size_t smem) {
-  if (bs == 1024) {
-    if (stage) launch_col<R, REAL_AMP, true, 1024>(sec, a, grid, smem);
-    else launch_col<R, REAL_AMP, false, 1024>(sec, a, grid, smem);
-  } else {
-    if (stage) launch_col<R, REAL_AMP, true, 256>(sec, a, grid, smem);
-    else launch_col<R, REAL_AMP, false, 256>(sec, a, grid, smem);
-  }
+  const int bs = h.bs, kr = h.kr;
+  if (bs == 256 && kr == 4) launch_buf<R, REAL_AMP, 256, 4>(sec, a, grid, smem);
+  else if (bs == 256 && kr == 2) launch_buf<R, REAL_AMP, 256, 2>(sec, a, grid, smem);
+  else if (bs == 512 && kr == 2) launch_buf<R, REAL_AMP, 512, 2>(sec, a, grid, smem);
+  else if (bs == 512 && kr == 1) launch_buf<R, REAL_AMP, 512, 1>(sec, a, grid, smem);
+  else if (bs == 1024 && kr == 1) launch_buf<R, REAL_AMP, 1024, 1>(sec, a, grid, smem);
+  else launch_buf<R, REAL_AMP, 1024, 2>(sec, a, grid, smem);
+}
+
+template <typename R, bool REAL_AMP>
+static void launch_col_t(Sector &sec, const HxvArgs &a, bool stage, bool buf,
+                         const HxvShape &h, dim3 grid, size_t smem) {
+  if (buf) launch_buf_s<R, REAL_AMP>(sec, a, h, grid, smem);
+  else if (stage) launch_col_s<R, REAL_AMP, true>(sec, a, h, grid, smem);
+  else launch_col_s<R, REAL_AMP, false>(sec, a, h, grid, smem);
 }
 
 template <typename R, bool REAL_AMP>
@@ -790,18 +1038,23 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   if (shp.onecol_kernel) {
     if (lanczos) ensure_partials(sec, grid * nb);
     a.partials = sec.partials;
+    a.meta_up = sec.up.meta;
+    a.ent4_up = sec.up.ent4;
+    a.up_nc = (sec.up.maxnnz + 3) / 4;
     const size_t fixed = (size_t)a.namp_up * esz + (size_t)a.dw_maxnnz * (esz + 8) + 48;
     const size_t col_bytes = (size_t)a.dimup * esz;
-    const size_t budget = shp.bs == 1024 ? kLdsBig : kLdsBudget;
+    const size_t budget = kLdsBig;
     const bool stage = fixed + col_bytes <= budget;
     const size_t smem = fixed + (stage ? col_bytes : 0);
+    // buffer loads address one chain's vector with 32-bit offsets
+    const bool buf = shp.buf && stage && (uint64_t)sec.dim * esz < (1ull << 31);
     const dim3 g3(grid, nb);
     if (fp64) {
-      if (real_amp) launch_col_t<double, true>(sec, a, stage, shp.bs, g3, smem);
-      else launch_col_t<double, false>(sec, a, stage, shp.bs, g3, smem);
+      if (real_amp) launch_col_t<double, true>(sec, a, stage, buf, shp, g3, smem);
+      else launch_col_t<double, false>(sec, a, stage, buf, shp, g3, smem);
     } else {
-      if (real_amp) launch_col_t<float, true>(sec, a, stage, shp.bs, g3, smem);
-      else launch_col_t<float, false>(sec, a, stage, shp.bs, g3, smem);
+      if (real_amp) launch_col_t<float, true>(sec, a, stage, buf, shp, g3, smem);
+      else launch_col_t<float, false>(sec, a, stage, buf, shp, g3, smem);
     }
     VCA_HIP(hipGetLastError());
     return;

@@ -249,25 +249,28 @@ __global__ __launch_bounds__(64) void k_prep(const PrepArgs a) {
 // active row of largest |M(r,k)| (ties to the lowest row), found with a
 // 16-lane butterfly, then broadcast to the group; rows already used as
 // pivots go inactive, so no swap is needed and log|det| = sum log|pivot|.
-template <typename R>
+// NBT = 16 or 8: lanes (rows) per matrix; H_k stored [k][block][NBT][NBT],
+// G'(z) and P(z) with row stride 16.
+template <typename R, int NBT>
 __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, int nspin_blk,
                                                     const ocplx<R> *hk, const ocplx<R> *gmat,
                                                     const ocplx<R> *pmat, double *partials) {
   using C = ocplx<R>;
+  constexpr int MPB = kOBlock / NBT;  // matrices per block
   const int lane = threadIdx.x & 63;
-  const int gl = lane & 15;
-  const int grp = threadIdx.x >> 4;  // 0..15 within the block
+  const int gl = lane & (NBT - 1);
+  const int grp = threadIdx.x / NBT;
   const int kb = blockIdx.x;
   const int sb = blockIdx.z;
-  const int k = kb * 16 + grp;
+  const int k = kb * MPB + grp;
   const bool kvalid = k < nk;
-  __shared__ double s_red[16];
-  __shared__ C s_gz[kNB * kNB], s_pz[kNB * kNB];
-  R hr[kNB], hi[kNB];
+  __shared__ double s_red[MPB];
+  __shared__ C s_gz[NBT * NBT], s_pz[NBT * NBT];
+  R hr[NBT], hi[NBT];
   {
-    const C *row = hk + (((int64_t)(kvalid ? k : 0) * nspin_blk + sb) * kNB + gl) * kNB;
+    const C *row = hk + (((int64_t)(kvalid ? k : 0) * nspin_blk + sb) * NBT + gl) * NBT;
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) {
+    for (int j = 0; j < NBT; ++j) {
       const C v = row[j];
       hr[j] = kvalid ? v.x : (R)0;
       hi[j] = kvalid ? v.y : (R)0;
@@ -278,26 +281,27 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
   for (int iz = zbeg; iz < zend; ++iz) {
     const int64_t zoff = ((int64_t)iz * nspin_blk + sb) * kNB * kNB;
     __syncthreads();
-    for (int e = threadIdx.x; e < kNB * kNB; e += kOBlock) {
-      s_gz[e] = gmat[zoff + e];
-      s_pz[e] = pmat[zoff + e];
+    for (int e = threadIdx.x; e < NBT * NBT; e += kOBlock) {
+      const int r = e / NBT, c = e % NBT;
+      s_gz[e] = gmat[zoff + r * kNB + c];
+      s_pz[e] = pmat[zoff + r * kNB + c];
     }
     __syncthreads();
-    R ar[kNB], ai[kNB];
+    R ar[NBT], ai[NBT];
 #pragma unroll
-    for (int j = 0; j < kNB; ++j) {
-      const C pz = s_pz[gl * kNB + j];
+    for (int j = 0; j < NBT; ++j) {
+      const C pz = s_pz[gl * NBT + j];
       ar[j] = (gl == j ? (R)1 : (R)0) + pz.x;
       ai[j] = pz.y;
     }
     // - H_k G' (m is uniform across the wavefront: the row element is
     // picked from registers with an indexed move, not spilled)
 #pragma unroll 1
-    for (int m = 0; m < kNB; ++m) {
+    for (int m = 0; m < NBT; ++m) {
       const R xr = hr[m], xi = hi[m];
 #pragma unroll
-      for (int j = 0; j < kNB; ++j) {
-        const C g = s_gz[m * kNB + j];
+      for (int j = 0; j < NBT; ++j) {
+        const C g = s_gz[m * NBT + j];
         ar[j] -= xr * g.x - xi * g.y;
         ai[j] -= xr * g.y + xi * g.x;
       }
@@ -305,22 +309,22 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
     bool active = true;
     double ld = 0.0;
 #pragma unroll
-    for (int kk = 0; kk < kNB; ++kk) {
+    for (int kk = 0; kk < NBT; ++kk) {
       float mag = active ? (float)(ar[kk] * ar[kk] + ai[kk] * ai[kk]) : -1.0f;
       int idx = gl;
 #pragma unroll
-      for (int off = 8; off > 0; off >>= 1) {
-        const float m2 = __shfl_xor(mag, off, 16);
-        const int i2 = __shfl_xor(idx, off, 16);
+      for (int off = NBT / 2; off > 0; off >>= 1) {
+        const float m2 = __shfl_xor(mag, off, NBT);
+        const int i2 = __shfl_xor(idx, off, NBT);
         if (m2 > mag || (m2 == mag && i2 < idx)) {
           mag = m2;
           idx = i2;
         }
       }
-      const int src = (lane & ~15) | idx;
-      R pr[kNB], pim[kNB];
+      const int src = (lane & ~(NBT - 1)) | idx;
+      R pr[NBT], pim[NBT];
 #pragma unroll
-      for (int j = kk; j < kNB; ++j) {
+      for (int j = kk; j < NBT; ++j) {
         pr[j] = __shfl(ar[j], src, 64);
         pim[j] = __shfl(ai[j], src, 64);
       }
@@ -332,18 +336,18 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
         const R lr = (ar[kk] * pr[kk] + ai[kk] * pim[kk]) * inv;
         const R li = (ai[kk] * pr[kk] - ar[kk] * pim[kk]) * inv;
 #pragma unroll
-        for (int j = kk + 1; j < kNB; ++j) {
+        for (int j = kk + 1; j < NBT; ++j) {
           ar[j] -= lr * pr[j] - li * pim[j];
           ai[j] -= lr * pim[j] + li * pr[j];
         }
       }
     }
-    // ld is identical in the 16 lanes of a group; invalid k contribute 0
+    // ld is identical in the lanes of a group; invalid k contribute 0
     if (gl == 0) s_red[grp] = kvalid ? ld : 0.0;
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0;
-      for (int g = 0; g < 16; ++g) t += s_red[g];
+      for (int g = 0; g < MPB; ++g) t += s_red[g];
       partials[((int64_t)iz * nspin_blk + sb) * gridDim.x + kb] = t;
     }
   }
@@ -407,13 +411,21 @@ void OmegaDev::upload(const OmegaHost &h, hipStream_t st) {
     VCA_HIP(hipMemcpyAsync(d_pole, h.pole.data(), sizeof(double) * np, hipMemcpyHostToDevice,
                            st));
   }
-  const size_t hk_n = (size_t)h.nk * h.nspin_blk * kNB * kNB;
+  // H_k blocks repacked with the kernel's row stride (8 when blocks fit 8)
+  nbt = h.nb <= 8 ? 8 : kNB;
+  const size_t hk_n = (size_t)h.nk * h.nspin_blk * nbt * nbt;
+  std::vector<double2> hk64(hk_n);
   std::vector<float2> hk32(hk_n);
-  for (size_t i = 0; i < hk_n; ++i)
-    hk32[i] = make_float2((float)h.hk[i].x, (float)h.hk[i].y);
+  for (size_t kb = 0; kb < (size_t)h.nk * h.nspin_blk; ++kb)
+    for (int r = 0; r < nbt; ++r)
+      for (int c = 0; c < nbt; ++c) {
+        const double2 v = h.hk[(kb * kNB + r) * kNB + c];
+        hk64[(kb * nbt + r) * nbt + c] = v;
+        hk32[(kb * nbt + r) * nbt + c] = make_float2((float)v.x, (float)v.y);
+      }
   VCA_HIP(hipMalloc(&d_hk64, sizeof(double2) * hk_n));
   VCA_HIP(hipMalloc(&d_hk32, sizeof(float2) * hk_n));
-  VCA_HIP(hipMemcpyAsync(d_hk64, h.hk.data(), sizeof(double2) * hk_n, hipMemcpyHostToDevice,
+  VCA_HIP(hipMemcpyAsync(d_hk64, hk64.data(), sizeof(double2) * hk_n, hipMemcpyHostToDevice,
                          st));
   VCA_HIP(hipMemcpyAsync(d_hk32, hk32.data(), sizeof(float2) * hk_n, hipMemcpyHostToDevice, st));
   VCA_HIP(hipMalloc(&d_hloc, sizeof(double2) * h.hloc.size()));
@@ -436,7 +448,8 @@ void OmegaDev::ensure_z(int nz) {
                   (void *)d_p32, (void *)d_part, (void *)d_f})
     if (p) VCA_HIP(hipFree(p));
   zcap = std::max(nz, 64);
-  const int nkb = (host.nk + 15) / 16;
+  const int mpb = kOBlock / nbt;
+  const int nkb = (host.nk + mpb - 1) / mpb;
   VCA_HIP(hipMalloc(&d_z, sizeof(double2) * zcap));
   VCA_HIP(hipMalloc(&d_graw, sizeof(double2) * (size_t)zcap * host.nelem));
   const size_t nm = (size_t)zcap * host.nspin_blk * kNB * kNB;
@@ -484,7 +497,8 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   pa.pmat = d_p64;
   hipLaunchKernelGGL(k_prep, dim3(nz, host.nspin_blk), dim3(64), 0, st, pa);
   VCA_HIP(hipGetLastError());
-  const int nkb = (host.nk + 15) / 16;
+  const int mpb = kOBlock / nbt;
+  const int nkb = (host.nk + mpb - 1) / mpb;
   const int zchunk = 16;
   const dim3 grid(nkb, (nz + zchunk - 1) / zchunk, host.nspin_blk);
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -494,8 +508,12 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   }
   if (precision == VCA_FP64) {
     if (e0) VCA_HIP(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_logdet<double>, grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
-                       host.nspin_blk, d_hk64, d_g64, d_p64, d_part);
+    if (nbt == 8)
+      hipLaunchKernelGGL((k_logdet<double, 8>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
+                         host.nspin_blk, d_hk64, d_g64, d_p64, d_part);
+    else
+      hipLaunchKernelGGL((k_logdet<double, 16>), grid, dim3(kOBlock), 0, st, host.nk, nz,
+                         zchunk, host.nspin_blk, d_hk64, d_g64, d_p64, d_part);
   } else {
     const int64_t nm = (int64_t)nz * host.nspin_blk * kNB * kNB;
     hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, nm,
@@ -503,8 +521,12 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
     hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, nm,
                        d_p64, d_p32);
     if (e0) VCA_HIP(hipEventRecord(e0, st));
-    hipLaunchKernelGGL(k_logdet<float>, grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
-                       host.nspin_blk, d_hk32, d_g32, d_p32, d_part);
+    if (nbt == 8)
+      hipLaunchKernelGGL((k_logdet<float, 8>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
+                         host.nspin_blk, d_hk32, d_g32, d_p32, d_part);
+    else
+      hipLaunchKernelGGL((k_logdet<float, 16>), grid, dim3(kOBlock), 0, st, host.nk, nz,
+                         zchunk, host.nspin_blk, d_hk32, d_g32, d_p32, d_part);
   }
   VCA_HIP(hipGetLastError());
   if (e1) VCA_HIP(hipEventRecord(e1, st));

@@ -40,6 +40,7 @@ struct OmegaDev {
   float2 *d_g32 = nullptr, *d_p32 = nullptr;
   double *d_part = nullptr, *d_f = nullptr;
   int zcap = 0;
+  int nbt = 16;  // rows per matrix in the log-det kernel (8 or 16)
 
   ~OmegaDev();
   void release();

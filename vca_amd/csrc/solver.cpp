@@ -22,6 +22,8 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <mutex>
+#include <tuple>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -123,7 +125,7 @@ Sector::~Sector() {
   clear_graphs(*this);
   for (DevSpin *sp : {&up, &dw})
     for (void *q : {(void *)sp->map, (void *)sp->energy, (void *)sp->cnt, (void *)sp->ent,
-                    (void *)sp->amp64, (void *)sp->amp32})
+                    (void *)sp->amp64, (void *)sp->amp32, (void *)sp->meta, (void *)sp->ent4})
       if (q) (void)hipFree(q);
   for (void *q : {buf[0], buf[1], buf[2], xvec, (void *)partials, (void *)lz, (void *)lz_arrays,
                   (void *)d_ops})
@@ -242,6 +244,80 @@ static void derive_model(Solver &s) {
   s.md_valid = true;
 }
 
+// ---------------------------------------------------------- autotuner ---
+// Picks the launch shape of H x v (lanes per workgroup, rows per lane) for
+// a sector shape by timing every variant on the sector itself (events, a
+// warm-up launch then the best of three), once per process and shape.
+// VCA_HXV_VARIANT=<i> forces variant i; VCA_TUNE=0 keeps the default.
+struct TuneKey {
+  int dimup, dimdw, prec, real_amp, nt_up, nt_dw;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(dimup, dimdw, prec, real_amp, nt_up, nt_dw) <
+           std::tie(o.dimup, o.dimdw, o.prec, o.real_amp, o.nt_up, o.nt_dw);
+  }
+};
+static std::map<TuneKey, int> g_tuned;
+static std::mutex g_tune_mu;
+
+static void hxv_tune(Sector &sec) {
+  const char *force = std::getenv("VCA_HXV_VARIANT");
+  if (force && *force) {
+    sec.hxv_variant = std::atoi(force);
+    return;
+  }
+  const char *t = std::getenv("VCA_TUNE");
+  const int nv = hxv_num_variants(sec);
+  if ((t && t[0] == '0') || nv <= 1) return;
+  const TuneKey key{sec.up.dim, sec.dw.dim, sec.precision,
+                    (int)(sec.up.real_amp && sec.dw.real_amp), sec.up.nterms, sec.dw.nterms};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(key);
+    if (it != g_tuned.end()) {
+      sec.hxv_variant = it->second;
+      return;
+    }
+  }
+  void *v = nullptr, *hv = nullptr;
+  VCA_HIP(hipMalloc(&v, sec.vec_bytes()));
+  VCA_HIP(hipMalloc(&hv, sec.vec_bytes()));
+  VCA_HIP(hipMemsetAsync(v, 0, sec.vec_bytes(), sec.stream));
+  hipEvent_t e0, e1;
+  VCA_HIP(hipEventCreate(&e0));
+  VCA_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = std::numeric_limits<float>::max();
+  for (int var = 0; var < nv; ++var) {
+    sec.hxv_variant = var;
+    launch_hxv(sec, v, v, hv, false);
+    float ms_min = std::numeric_limits<float>::max();
+    for (int r = 0; r < 3; ++r) {
+      VCA_HIP(hipEventRecord(e0, sec.stream));
+      launch_hxv(sec, v, v, hv, false);
+      VCA_HIP(hipEventRecord(e1, sec.stream));
+      VCA_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms_min = std::min(ms_min, ms);
+    }
+    if (ms_min < best_ms) {
+      best_ms = ms_min;
+      best = var;
+    }
+    const char *verbose = std::getenv("VCA_TUNE_VERBOSE");
+    if (verbose && verbose[0] == '1')
+      std::fprintf(stderr, "hxv tune dim %dx%d prec %d variant %d: %.2f us\n", sec.up.dim,
+                   sec.dw.dim, sec.precision, var, ms_min * 1e3);
+  }
+  VCA_HIP(hipEventDestroy(e0));
+  VCA_HIP(hipEventDestroy(e1));
+  VCA_HIP(hipFree(v));
+  VCA_HIP(hipFree(hv));
+  sec.hxv_variant = best;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned[key] = best;
+}
+
 // ------------------------------------------------------------ sectors ---
 static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec) {
   if (!s.md_valid) fail(VCA_ERR_STATE, "Hloc not set");
@@ -256,6 +332,7 @@ static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec
   build_spin_tables(*sec, sec->up, s.md.spin[0], s.ns, nup, s.d_binom, s.md.e_const);
   build_spin_tables(*sec, sec->dw, s.md.spin[1], s.ns, ndw, s.d_binom, 0.0);
   sec->dim = (int64_t)sec->up.dim * sec->dw.dim;
+  hxv_tune(*sec);
   return sec;
 }
 

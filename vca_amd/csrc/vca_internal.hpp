@@ -175,6 +175,13 @@ struct LanczosDev {
   double *alpha, *beta, *nrm, *y;  // device arrays
 };
 
+// Per-row diagonal energy, string and hop count in one 16-byte record.
+struct __align__(16) RowMeta {
+  double e;
+  uint32_t s;
+  int32_t cnt;
+};
+
 struct DevSpin {
   int npart = 0, dim = 0, maxnnz = 0, nterms = 0;
   bool real_amp = true;
@@ -182,6 +189,8 @@ struct DevSpin {
   double *energy = nullptr;
   int32_t *cnt = nullptr;
   uint32_t *ent = nullptr;
+  RowMeta *meta = nullptr;   // [dim]
+  uint32_t *ent4 = nullptr;  // [ceil(maxnnz/4)][dim][4]: 4 hops of a row per 16 bytes
   double2 *amp64 = nullptr;  // [2*nterms]: +amp, -amp
   float2 *amp32 = nullptr;
 };
@@ -202,6 +211,7 @@ struct Sector {
   hipStream_t stream = nullptr;
   // Lanczos workspace (lazily allocated) for nb_cap chains of lz_cap steps
   int lz_cap = 0, nb_cap = 0;
+  int hxv_variant = -1;  // launch shape of H x v picked by the autotuner
   void *buf[3] = {nullptr, nullptr, nullptr};  // u_even, u_odd, hv [nb][dim]
   void *xvec = nullptr;                        // eigenvector accumulator
   double *partials = nullptr;
@@ -233,6 +243,7 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
                 int nb = 1);
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb = 1);
 void ensure_step_partials(Sector &sec, int nb);
+int hxv_num_variants(const Sector &sec);
 void clear_graphs(Sector &sec);
 void launch_start_vector(Sector &sec, uint64_t seed);
 void launch_norm0(Sector &sec);  // nrm[0] = ||buf[0]||, k = 0, done = 0
