@@ -22,11 +22,14 @@
  *                        bound to directMatVec_main,
  *                        VCA_HAMILTONIAN_DIRECT_HxV.f90:42-78)
  *   vca_lanczos_gs       SciFortran sp_lanc_eigh as called at VCA_DIAG.f90:172-184
+ *   vca_lanczos_eigs     SciFortran sp_eigh (ARPACK, which='SA') as called at
+ *                        VCA_DIAG.f90:153-170
  *   vca_lanczos_tridiag  SciFortran sp_lanc_tridiag as called at
  *                        VCA_GF_NORMAL.f90:218-223
  *   vca_gf_cluster       VCA_IO/gf_cluster.f90:1-64 (vca_gf_cluster_scalar)
  *   vca_gf_poles         VCA_VARS_GLOBAL.f90:61-71 (GFmatrix weights/poles)
  *   vca_omega_integrand  VCA_OMEGA.f90:37-75     (sum_kmesh, batched in omega)
+ *   vca_omega_integrand_complex VCA_OMEGA.f90:78-116 (sum_kmesh_complex)
  */
 #ifndef VCA_GPU_H
 #define VCA_GPU_H
@@ -92,6 +95,15 @@ typedef struct vca_params {
                             equivalent of VCA_SECTORS=T with a
                             sectors_list.restart file (VCA_DIAG.f90:287-319) */
   const int32_t *sectors; /* nsectors pairs (nup, ndw)                     */
+  /* finite temperature (lanc_nstates_total > 1) and the ARPACK basis */
+  int32_t lanc_nstates_total;  /* LANC_NSTATES_TOTAL (1: T = 0)           */
+  int32_t lanc_nstates_step;   /* LANC_NSTATES_STEP                       */
+  int32_t lanc_niter_spectrum; /* LANC_NITER_SPECTRUM                     */
+  int32_t lanc_ncv_factor;     /* LANC_NCV_FACTOR                         */
+  int32_t lanc_ncv_add;        /* LANC_NCV_ADD                            */
+  double cutoff;               /* CUTOFF                                  */
+  double bandwidth;            /* bandwidth (set by the caller, as the
+                                  reference drivers do)                   */
 } vca_params;
 
 /* Per-solve results and timings (seconds) in addition to the SFT potential. */
@@ -108,6 +120,11 @@ typedef struct vca_results {
   double t_diag, t_gf, t_omega;
   int64_t omega_evals;   /* (k, omega) log-det evaluations                */
   double omega_seconds;  /* GPU time in the k x omega kernels             */
+  double zeta;           /* zeta_function (VCA_DIAG.f90:355-364)          */
+  double max_exc;        /* largest excitation energy of the GF chains    */
+  int32_t finite_t;      /* 1 when lanc_nstates_total > 1                 */
+  int32_t nmax;          /* finite T: Matsubara terms summed (0..nmax)    */
+  int32_t spectrum_iters;/* finite T: spectrum annealing iterations       */
 } vca_results;
 
 typedef struct vca_solver vca_solver;
@@ -161,6 +178,15 @@ int vca_sector_spin_coo(const vca_sector *sec, int32_t spin, int64_t *nnz,
  * complex128, length dim) may be NULL. */
 int vca_lanczos_gs(vca_sector *sec, double tol, int32_t niter, double *e0,
                    double *vec, int32_t *iters);
+/* nev lowest eigenpairs (ARPACK semantics: ncv basis vectors, maxiter
+ * restarts, a Ritz pair is converged when |beta| |y_ncv| <= tol *
+ * max(eps^(2/3), |theta|)) by thick-restart Lanczos with full
+ * reorthogonalisation.  FP64 (a MIXED sector is solved in FP64).
+ * evals[nev] ascending; evecs (host complex128 [nev][dim]) may be NULL;
+ * *nconv = number of converged pairs (nconv < nev is not an error here). */
+int vca_lanczos_eigs(vca_sector *sec, int32_t nev, int32_t ncv, double tol,
+                     int32_t maxiter, double *evals, double *evecs,
+                     int32_t *nconv);
 /* Lanczos tridiagonalisation from host start vector v0 (need not be
  * normalised), nlanc steps: alpha[nlanc], beta[nlanc] (beta[0] = 0). */
 int vca_lanczos_tridiag(vca_sector *sec, const double *v0, int32_t nlanc,
@@ -178,6 +204,11 @@ int vca_gf_cluster(const vca_solver *s, double zre, double zim, double *gf);
  * complex128 frequencies z (imaginary axis or any z off the real axis). */
 int vca_omega_integrand(vca_solver *s, int32_t nz, const double *z,
                         int32_t precision, double *f);
+/* VCA_OMEGA.f90:78-116 (sum_kmesh_complex): the complex mean
+ * (1/Nk) sum_k log(det(...)) with the principal logarithm of each
+ * determinant; f complex128[nz]. */
+int vca_omega_integrand_complex(vca_solver *s, int32_t nz, const double *z,
+                                int32_t precision, double *f);
 
 #ifdef __cplusplus
 }

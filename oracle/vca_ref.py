@@ -79,6 +79,17 @@ class Params:
     lanc_dim_threshold: int = 256
     gs_threshold: float = 1e-9
     sectors: list | None = None  # list of (nup, ndw); None = every sector
+    lanc_nstates_step: int = 2
+    lanc_niter_spectrum: int = 10
+    lanc_ncv_factor: int = 3
+    lanc_ncv_add: int = 5
+    cutoff: float = 1e-9
+    bandwidth: float = 0.0
+
+    @property
+    def finite_t(self):
+        """VCA_SETUP.f90:157-162: finite T unless one state is kept."""
+        return self.lanc_nstates_total > 1
 
     @property
     def ns(self):
@@ -361,50 +372,123 @@ class State:
     vec: np.ndarray
 
 
-def diagonalize(model: Model, matvec_factory=None):
-    """T=0 sector scan and ground-state selection, VCA_DIAG.f90:83-235.
-    Per sector: Neigen = min(dim, lanc_nstates_sector) ('arpack') or 1
-    ('lanczos'); dense eigh when dim <= lanc_dim_threshold or Neigen == dim,
-    otherwise ARPACK (scipy.sparse.linalg.eigsh) or plain Lanczos.  Keep
-    every state within gs_threshold of the running minimum (:224-235)."""
+def solve_sector(model: Model, nup, ndw, neigen, matvec_factory=None):
+    """Eigenpairs of one sector (VCA_DIAG.f90:98-212): dense eigh when
+    dim <= lanc_dim_threshold or Neigen == dim, otherwise ARPACK
+    (scipy.sparse.linalg.eigsh, which='SA', ncv = ncv_factor*max(Neigen,
+    nstates_sector) + ncv_add, :100-102) or the plain Lanczos
+    ('lanczos')."""
     p = model.p
+    du, dd = model.dims(nup, ndw)
+    dim = du * dd
+    lanc_solve = not (neigen == dim or dim <= p.lanc_dim_threshold)
+    if not lanc_solve:
+        h = model.dense_h(nup, ndw)
+        e, z = np.linalg.eigh(h)
+        return e[:neigen], z[:, :neigen]
+    mv = (matvec_factory(nup, ndw) if matvec_factory
+          else (lambda x, a=nup, b=ndw: model.hxv(a, b, x)))
+    if p.lanc_method == "lanczos":
+        e0, x = lanczos_gs(mv, dim, p.lanc_tolerance, p.lanc_niter)
+        return np.array([e0]), x[:, None]
+    from scipy.sparse.linalg import LinearOperator, eigsh
+    op = LinearOperator((dim, dim), matvec=mv, dtype=np.complex128)
+    ncv = min(dim, p.lanc_ncv_factor * max(neigen, p.lanc_nstates_sector) + p.lanc_ncv_add)
+    e, z = eigsh(op, k=neigen, which="SA", ncv=ncv, tol=p.lanc_tolerance,
+                 v0=start_vector(dim), maxiter=min(dim, p.lanc_niter))
+    order = np.argsort(e)
+    return e[order], z[:, order]
+
+
+def es_insert(states: list, st: State):
+    """es_insert_state (VCA_EIGENSPACE.f90:160-212): ascending energies, a
+    new state goes before the first state of equal or higher energy."""
+    pos = 0
+    while pos < len(states) and not (st.energy <= states[pos].energy):
+        pos += 1
+    states.insert(pos, st)
+
+
+def es_add_sized(states: list, st: State, size: int):
+    """es_add_state with size (VCA_EIGENSPACE.f90:133-155)."""
+    if len(states) < size:
+        es_insert(states, st)
+    elif st.energy < states[-1].energy:
+        states.pop()
+        es_insert(states, st)
+
+
+@dataclass
+class DiagResult:
+    states: list
+    zeta: float
+    iters: int
+
+
+def diagonalize_full(model: Model, matvec_factory=None) -> DiagResult:
+    """vca_diag_c (VCA_DIAG.f90:49-267) with vca_post_diag (:333-474): the
+    sector scan; at T = 0 the ground-state selection (:224-235); at finite T
+    the size-limited state list and the spectrum annealing loop, which
+    adapts the states per sector and the list size until
+    exp(-beta (Emax - Emin)) < cutoff, trimming the list at the end."""
+    p = model.p
+    ft = p.finite_t
+    nst_sector, nst_total = p.lanc_nstates_sector, p.lanc_nstates_total
+    if ft:  # even counts, VCA_SETUP.f90:165-176
+        nst_sector += nst_sector % 2
+        nst_total += nst_total % 2
+    neigen_sector = {}
+    for nup in range(p.ns + 1):
+        for ndw in range(p.ns + 1):
+            du, dd = model.dims(nup, ndw)
+            neigen_sector[(nup, ndw)] = min(du * dd, nst_sector)
+    converged, it, zeta = False, 0, 1.0
     states: list[State] = []
-    oldzero = 1000.0
-    for (nup, ndw) in sector_list(p):
-        du, dd = model.dims(nup, ndw)
-        dim = du * dd
-        if p.lanc_method == "lanczos":
-            neigen = 1
-        else:
-            neigen = min(dim, p.lanc_nstates_sector)
-        lanc_solve = not (neigen == dim or dim <= p.lanc_dim_threshold)
-        if not lanc_solve:
-            h = model.dense_h(nup, ndw)
-            e, z = np.linalg.eigh(h)
-            evals, evecs = e[:neigen], z[:, :neigen]
-        else:
-            mv = (matvec_factory(nup, ndw) if matvec_factory
-                  else (lambda x, a=nup, b=ndw: model.hxv(a, b, x)))
-            if p.lanc_method == "lanczos":
-                e0, x = lanczos_gs(mv, dim, p.lanc_tolerance, p.lanc_niter)
-                evals, evecs = np.array([e0]), x[:, None]
+    while not converged and it < max(1, p.lanc_niter_spectrum):
+        it += 1
+        states = []
+        oldzero = 1000.0
+        for (nup, ndw) in sector_list(p):
+            du, dd = model.dims(nup, ndw)
+            neigen = 1 if p.lanc_method == "lanczos" else min(du * dd, neigen_sector[(nup, ndw)])
+            evals, evecs = solve_sector(model, nup, ndw, neigen, matvec_factory)
+            for i in range(len(evals)):
+                st = State(float(evals[i]), (nup, ndw), evecs[:, i].copy())
+                if ft:
+                    es_add_sized(states, st, nst_total)
+                elif st.energy < oldzero - 10.0 * p.gs_threshold:
+                    oldzero = st.energy
+                    states = [st]
+                elif abs(st.energy - oldzero) <= p.gs_threshold:
+                    oldzero = min(oldzero, st.energy)
+                    es_insert(states, st)
+        egs, emax = states[0].energy, states[-1].energy
+        converged = (math.exp(-p.beta * (emax - egs)) < p.cutoff) if ft else True
+        if ft:
+            zeta = sum(math.exp(-p.beta * (st.energy - egs)) for st in states)
+            count = {}
+            for st in states:
+                count[st.sector] = count.get(st.sector, 0) + 1
+            for key in neigen_sector:
+                c = count.get(key, 0)
+                ne = neigen_sector[key] + (1 if c > 0 else -1)
+                if ne > c:
+                    ne = c + 1
+                neigen_sector[key] = max(ne, 1)
+            if math.exp(-p.beta * (emax - egs)) > p.cutoff:
+                nst_total += p.lanc_nstates_step
             else:
-                from scipy.sparse.linalg import LinearOperator, eigsh
-                op = LinearOperator((dim, dim), matvec=mv, dtype=np.complex128)
-                ncv = min(dim, 3 * max(neigen, p.lanc_nstates_sector) + 5)
-                e, z = eigsh(op, k=neigen, which="SA", ncv=ncv, tol=p.lanc_tolerance,
-                             v0=start_vector(dim), maxiter=p.lanc_niter)
-                order = np.argsort(e)
-                evals, evecs = e[order], z[:, order]
-        for i in range(len(evals)):
-            en = float(evals[i])
-            if en < oldzero - 10.0 * p.gs_threshold:
-                oldzero = en
-                states = [State(en, (nup, ndw), evecs[:, i].copy())]
-            elif abs(en - oldzero) <= p.gs_threshold:
-                oldzero = min(oldzero, en)
-                states.append(State(en, (nup, ndw), evecs[:, i].copy()))
-    return states
+                while len(states) > 1 and math.exp(-p.beta * (states[-1].energy - egs)) <= p.cutoff:
+                    states.pop()
+                nst_total = max(len(states), p.lanc_nstates_step) + p.lanc_nstates_step
+        else:
+            zeta = float(len(states))
+    return DiagResult(states, zeta, it)
+
+
+def diagonalize(model: Model, matvec_factory=None):
+    """The kept states of diagonalize_full."""
+    return diagonalize_full(model, matvec_factory).states
 
 
 # --------------------------------------------------------- GF chains -----
@@ -421,10 +505,15 @@ class GFData:
     max_exc: float = -np.inf
 
 
-def _chain_poles(vnorm2, ei, alpha, beta, isign, emin, zeta):
-    """add_to_lanczos_gf_normal, VCA_GF_NORMAL.f90:970-1022 (T=0 weights)."""
+def _chain_poles(vnorm2, ei, alpha, beta, isign, emin, zeta, temp_beta=None):
+    """add_to_lanczos_gf_normal, VCA_GF_NORMAL.f90:970-1022; at finite T
+    (temp_beta given) the weights carry exp(-beta (Ei - Egs)) when
+    beta (Ei - Egs) < 200, else 0 (:987-994)."""
     e, z = tridiag_eigh(alpha, beta)
     peso_bz = vnorm2 / zeta
+    if temp_beta is not None:
+        x = temp_beta * (ei - emin)
+        peso_bz = peso_bz * (math.exp(-x) if x < 200 else 0.0)
     w = peso_bz * z[0, :] * z[0, :]
     de = e - ei
     return w.astype(np.complex128), isign * de, de.max()
@@ -448,15 +537,17 @@ def excitation(model: Model, st: State, ops):
     return (jnup, jndw), out
 
 
-def build_gf(model: Model, states: list, matvec_factory=None):
+def build_gf(model: Model, states: list, matvec_factory=None, zeta=None):
     """build_gf_normal (VCA_GF_NORMAL.f90:37-104) at T=0: diagonal chains
     c^+_i, c_i and, for i != j, the 4 mixed channels (c^+_i + c^+_j),
     (c_i + c_j), (c^+_i + i c^+_j) with weight -i norm2, (c_i - i c_j) with
     weight -i norm2; 2 channels when gf_symmetric."""
     p = model.p
     emin = min(s.energy for s in states)
-    zeta = float(len(states))
+    if zeta is None:
+        zeta = float(len(states))
     gd = GFData(p, emin, zeta)
+    tb = p.beta if p.finite_t else None
 
     def run_chain(st, ops, vscale, isign):
         jsec, v = excitation(model, st, ops)
@@ -473,7 +564,7 @@ def build_gf(model: Model, states: list, matvec_factory=None):
         mv = (matvec_factory(*jsec) if matvec_factory
               else (lambda x, a=jsec[0], b=jsec[1]: model.hxv(a, b, x)))
         alpha, beta = lanczos_tridiag(mv, v, nlanc)
-        w, poles, mx = _chain_poles(vscale * norm2, st.energy, alpha, beta, isign, emin, zeta)
+        w, poles, mx = _chain_poles(vscale * norm2, st.energy, alpha, beta, isign, emin, zeta, tb)
         gd.max_exc = max(gd.max_exc, mx)
         return (w, poles)
 
@@ -629,6 +720,63 @@ def matsubara_integration(model, gd, hk_lso, beta, nfreq):
     return (3 - model.p.nspin) * 2.0 / beta * s
 
 
+def quad_finite(f_batch, a, b, epsabs=1e-10, epsrel=1e-10, limit=400):
+    """Adaptive G-K 7/15 on [a, b] (QUADPACK qag-style bisection of the
+    interval with the largest error estimate; the reference calls quad
+    with key=6 at VCA_OMEGA.f90:179, whose converged value this matches at
+    the tolerance)."""
+    def panel(lo, hi):
+        t, wk, wg = gk15_nodes(lo, hi)
+        fx = np.asarray(f_batch(t))
+        ik, ig = float(wk @ fx), float(wg @ fx)
+        return ik, abs(ik - ig)
+
+    panels = [(a, b) + panel(a, b)]
+    for _ in range(limit):
+        tot = sum(pn[2] for pn in panels)
+        err = sum(pn[3] for pn in panels)
+        if err <= max(epsabs, epsrel * abs(tot)):
+            break
+        k = int(np.argmax([pn[3] for pn in panels]))
+        lo, hi, _, _ = panels.pop(k)
+        m = 0.5 * (lo + hi)
+        panels.append((lo, m) + panel(lo, m))
+        panels.append((m, hi) + panel(m, hi))
+    return sum(pn[2] for pn in panels), sum(pn[3] for pn in panels)
+
+
+def finite_t_nmax(max_exc, bandwidth, beta):
+    """VCA_OMEGA.f90:157-164: Nmax and the contour radius R."""
+    nmax = int(2 * (abs(max_exc) + bandwidth) * beta / math.pi)
+    nmax = nmax // 2 if nmax % 2 == 0 else (nmax + 1) // 2
+    return nmax, 2 * (nmax + 1) * math.pi / beta
+
+
+def frequency_integration_finite_t(model, gd, hk_lso, epsabs=1e-10, epsrel=1e-12):
+    """VCA_OMEGA.f90:153-204: (3 - Nspin) [ (2/beta) sum_{n=0}^{Nmax}
+    Re F(i w_n) + int_0^pi Re((1/pi) w f(w) F(w)) dzeta ], w = R e^{i zeta},
+    f the Fermi function (0 where Re(beta (w - mu)) >= 100), F the complex
+    sum_kmesh."""
+    p = model.p
+    beta = p.beta
+    nmax, radius = finite_t_nmax(gd.max_exc, p.bandwidth, beta)
+    s = sum(sum_kmesh(model, gd, hk_lso, 1j * (2 * n + 1) * math.pi / beta, complex_out=True).real
+            for n in range(nmax + 1))
+    s *= 2.0 / beta
+
+    def g(zs):
+        out = []
+        for zeta in zs:
+            w = radius * np.exp(1j * zeta)
+            fermi = 0.0 if ((w - p.xmu) * beta).real >= 100 else 1.0 / (np.exp(beta * (w - p.xmu)) + 1.0)
+            out.append(((1.0 / math.pi) * w * fermi *
+                        sum_kmesh(model, gd, hk_lso, w, complex_out=True)).real)
+        return np.array(out)
+
+    contour, _ = quad_finite(g, 0.0, math.pi, epsabs, epsrel)
+    return (3 - p.nspin) * (s + contour), nmax
+
+
 def hk_to_lso(p: Params, hk):
     nk = hk.shape[6]
     return np.stack([nnn2lso(hk[..., k], p.nlat, p.nspin, p.norb) for k in range(nk)])
@@ -639,13 +787,22 @@ def vca_solve(p: Params, hloc, hk, bath_h=None, bath_v=None, omega_mode="quad",
     """vca_solve_serial at T=0 (VCA_MAIN.f90:124-203): diagonalise, build the
     GF, integrate, Omega = E0 - integral."""
     model = Model(p, hloc, bath_h, bath_v)
-    states = diagonalize(model, matvec_factory)
-    gd = build_gf(model, states, matvec_factory)
+    dr = diagonalize_full(model, matvec_factory)
+    states = dr.states
+    gd = build_gf(model, states, matvec_factory, zeta=dr.zeta)
     hkl = hk_to_lso(p, np.asarray(hk))
-    if omega_mode == "quad":
+    e0 = min(s.energy for s in states)
+    nmax = None
+    if p.finite_t:
+        integ, nmax = frequency_integration_finite_t(model, gd, hkl, epsabs, epsrel)
+        # VCA_MAIN.f90:186-190
+        omegaprime = e0 - math.log(sum(math.exp(-p.beta * (s.energy - e0)) for s in states)) / p.beta
+    elif omega_mode == "quad":
         integ = frequency_integration(model, gd, hkl, epsabs, epsrel)
+        omegaprime = e0
     else:
         integ = matsubara_integration(model, gd, hkl, p.beta, nfreq)
-    e0 = min(s.energy for s in states)
-    return dict(sft_potential=e0 - integ, e0=e0, omega_integral=integ,
-                states=states, gf=gd, model=model)
+        omegaprime = e0
+    return dict(sft_potential=omegaprime - integ, e0=e0, omegaprime=omegaprime,
+                omega_integral=integ, states=states, gf=gd, model=model, zeta=dr.zeta,
+                nmax=nmax, spectrum_iters=dr.iters)

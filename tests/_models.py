@@ -17,7 +17,11 @@ def to_oracle(p: VcaParams) -> R.Params:
                     lanc_nstates_sector=p.lanc_nstates_sector, lanc_niter=p.lanc_niter,
                     lanc_ngfiter=p.lanc_ngfiter, lanc_tolerance=p.lanc_tolerance,
                     lanc_dim_threshold=p.lanc_dim_threshold, gs_threshold=p.gs_threshold,
-                    sectors=p.sectors)
+                    sectors=p.sectors, lanc_nstates_total=p.lanc_nstates_total,
+                    lanc_nstates_step=p.lanc_nstates_step,
+                    lanc_niter_spectrum=p.lanc_niter_spectrum,
+                    lanc_ncv_factor=p.lanc_ncv_factor, lanc_ncv_add=p.lanc_ncv_add,
+                    cutoff=p.cutoff, bandwidth=p.bandwidth)
 
 
 def chain(nlat=6, u=4.0, nk=8, tprime=-1.0):

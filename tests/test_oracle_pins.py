@@ -298,3 +298,74 @@ def test_sft_exact_u0_bhz():
     hkl = R.hk_to_lso(po, hk)
     exact = np.mean([np.sum(np.minimum(np.linalg.eigvalsh(h), 0)) for h in hkl])
     assert abs(r["sft_potential"] - exact) < 1e-8 * abs(exact)
+
+
+# ------------------------------------------------------------ finite T --
+@pytest.mark.parametrize("beta,tprime,tol", [(50.0, -1.0, 1e-11), (20.0, -0.7, 2e-8)])
+def test_sft_exact_u0_finite_t(beta, tprime, tol):
+    """U=0 at finite T: the reference's Matsubara sum to Nmax plus the
+    contour integral (VCA_OMEGA.f90:153-204), with Omega' from the kept
+    states (VCA_MAIN.f90:186-190) and Boltzmann GF weights, equals
+    -(3-Nspin)/(beta Nk) sum_k sum_n ln(1 + exp(-beta e_kn)) for every t'.
+    At t' != t the kept spectrum is cut at exp(-beta dE) <= CUTOFF, which
+    bounds the deviation (tol)."""
+    p, hloc, hk, _, _ = M.chain(nlat=4, u=0.0, nk=16, tprime=tprime)
+    po = M.to_oracle(p)
+    po.beta, po.lanc_nstates_total, po.bandwidth = beta, 2, 4.0
+    r = R.vca_solve(po, hloc, hk, epsabs=1e-11, epsrel=1e-12)
+    hkl = R.hk_to_lso(po, hk)
+    eps = np.concatenate([np.linalg.eigvalsh(h) for h in hkl])
+    exact = -(3 - po.nspin) / beta * np.sum(np.log1p(np.exp(-beta * eps))) / len(hkl)
+    assert abs(r["sft_potential"] - exact) < tol, (r["sft_potential"], exact)
+    assert r["nmax"] > 0
+
+
+def test_finite_t_state_list_semantics():
+    """es_add_state with a size limit keeps the lowest states in ascending
+    order, new states before equal energies (VCA_EIGENSPACE.f90:133-212)."""
+    st = lambda e, tag: R.State(e, (tag, 0), np.zeros(1))
+    lst = []
+    for e, tag in ((3.0, 0), (1.0, 1), (2.0, 2), (1.0, 3), (0.5, 4), (5.0, 5)):
+        R.es_add_sized(lst, st(e, tag), 3)
+    assert [(s.energy, s.sector[0]) for s in lst] == [(0.5, 4), (1.0, 3), (1.0, 1)]
+
+
+def test_finite_t_low_temperature_limit():
+    """A gapped cluster at beta = 200: the finite-T Omega (state list,
+    annealing, Matsubara sum + contour) equals the T=0 Omega (one ground
+    state, adaptive quadrature) up to exp(-beta gap) terms."""
+    p, hloc, hk, _, _ = M.chain(nlat=4, u=4.0, nk=8, tprime=-1.0)
+    po = M.to_oracle(p)
+    r0 = R.vca_solve(po, hloc, hk, epsabs=1e-11, epsrel=1e-12)
+    po.beta, po.lanc_nstates_total, po.bandwidth = 200.0, 2, 4.0
+    rt = R.vca_solve(po, hloc, hk, epsabs=1e-11, epsrel=1e-12)
+    assert abs(rt["sft_potential"] - r0["sft_potential"]) < 1e-8
+
+
+def test_sum_kmesh_complex_is_principal_log_det():
+    """sum_kmesh_complex takes log(det) per k (principal branch), not the
+    sum of the logs of the LU pivots: compare with numpy's determinant."""
+    p, hloc, hk, _, _ = M.bhz_small(nk=4)
+    po = M.to_oracle(p)
+    po.lanc_method = "lanczos"
+    r = R.vca_solve(po, hloc, hk, epsabs=1e-8, epsrel=1e-8)
+    model, gd = r["model"], r["gf"]
+    hkl = R.hk_to_lso(po, hk)
+    for z in (0.3 + 0.2j, -1.5 + 0.05j, 2.0j, 4.0 * np.exp(0.3j)):
+        g = R.nnn2lso(R.gf_cluster(gd, z), po.nlat, po.nspin, po.norb)
+        hl = R.nnn2lso(model.hloc, po.nlat, po.nspin, po.norb)
+        brute = np.mean([np.log(np.linalg.det(np.eye(po.nlso) + (hl - h) @ g)) for h in hkl])
+        f = R.sum_kmesh(model, gd, hkl, z, complex_out=True)
+        assert abs(f - brute) < 1e-10, (z, f, brute)
+
+
+def test_arpack_sector_equals_dense():
+    """ARPACK path of the oracle (eigsh, which='SA', the reference's ncv
+    rule) on a sector above LANC_DIM_THRESHOLD equals dense eigh."""
+    p, hloc, hk, _, _ = M.chain(nlat=8, u=4.0)
+    po = M.to_oracle(p)
+    po.lanc_method = "arpack"
+    model = R.Model(po, hloc)
+    ev, vec = R.solve_sector(model, 4, 3, 6)
+    ed = np.linalg.eigvalsh(model.dense_h(4, 3))[:6]
+    assert np.allclose(ev, ed, atol=1e-10)

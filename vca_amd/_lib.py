@@ -36,6 +36,10 @@ class CParams(ctypes.Structure):
         ("omega_nfreq", ctypes.c_int32),
         ("quad_epsabs", ctypes.c_double), ("quad_epsrel", ctypes.c_double),
         ("nsectors", ctypes.c_int32), ("sectors", ctypes.POINTER(ctypes.c_int32)),
+        ("lanc_nstates_total", ctypes.c_int32), ("lanc_nstates_step", ctypes.c_int32),
+        ("lanc_niter_spectrum", ctypes.c_int32), ("lanc_ncv_factor", ctypes.c_int32),
+        ("lanc_ncv_add", ctypes.c_int32),
+        ("cutoff", ctypes.c_double), ("bandwidth", ctypes.c_double),
     ]
 
 
@@ -48,6 +52,9 @@ class CResults(ctypes.Structure):
         ("hxv_seconds", ctypes.c_double),
         ("t_diag", ctypes.c_double), ("t_gf", ctypes.c_double), ("t_omega", ctypes.c_double),
         ("omega_evals", ctypes.c_int64), ("omega_seconds", ctypes.c_double),
+        ("zeta", ctypes.c_double), ("max_exc", ctypes.c_double),
+        ("finite_t", ctypes.c_int32), ("nmax", ctypes.c_int32),
+        ("spectrum_iters", ctypes.c_int32),
     ]
 
 
@@ -73,11 +80,14 @@ _SIGS = {
     "vca_hxv_device": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.POINTER(ctypes.c_float)]),
     "vca_sector_spin_coo": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_int64), _P, _P, _P]),
     "vca_lanczos_gs": (ctypes.c_int, [_P, ctypes.c_double, _I32, _D, _P, ctypes.POINTER(_I32)]),
+    "vca_lanczos_eigs": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_double, _I32, _P, _P,
+                                        ctypes.POINTER(_I32)]),
     "vca_lanczos_tridiag": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
     "vca_gf_poles": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _I32,
                                     ctypes.POINTER(_I32), _P, _P]),
     "vca_gf_cluster": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, _P]),
     "vca_omega_integrand": (ctypes.c_int, [_P, _I32, _P, _I32, _P]),
+    "vca_omega_integrand_complex": (ctypes.c_int, [_P, _I32, _P, _I32, _P]),
 }
 EXPORTS = tuple(_SIGS)
 

@@ -55,6 +55,13 @@ class VcaParams:
     quad_epsabs: float = 1e-10
     quad_epsrel: float = 1e-12
     sectors: list | None = None      # [(nup, ndw), ...] or None for all
+    lanc_nstates_total: int = 1      # > 1: finite temperature
+    lanc_nstates_step: int = 2
+    lanc_niter_spectrum: int = 10
+    lanc_ncv_factor: int = 3
+    lanc_ncv_add: int = 5
+    cutoff: float = 1e-9
+    bandwidth: float = 0.0
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -85,6 +92,10 @@ class VcaParams:
         c.omega_mode = {"quad": VCA_OMEGA_QUAD, "matsubara": VCA_OMEGA_MATSUBARA}[self.omega_mode]
         c.omega_nfreq = self.omega_nfreq
         c.quad_epsabs, c.quad_epsrel = self.quad_epsabs, self.quad_epsrel
+        c.lanc_nstates_total, c.lanc_nstates_step = self.lanc_nstates_total, self.lanc_nstates_step
+        c.lanc_niter_spectrum = self.lanc_niter_spectrum
+        c.lanc_ncv_factor, c.lanc_ncv_add = self.lanc_ncv_factor, self.lanc_ncv_add
+        c.cutoff, c.bandwidth = self.cutoff, self.bandwidth
         if self.sectors:
             arr = (ctypes.c_int32 * (2 * len(self.sectors)))(
                 *[int(x) for s in self.sectors for x in s])
@@ -161,6 +172,21 @@ class Sector:
         check(self._lib.vca_lanczos_gs(self._h, tol, niter, ctypes.byref(e0),
                                        vec.ctypes.data if want_vec else None, ctypes.byref(it)))
         return e0.value, vec, it.value
+
+    def lanczos_eigs(self, nev, ncv=None, tol=1e-12, maxiter=512, want_vecs=True):
+        """nev lowest eigenpairs (SciFortran sp_eigh / ARPACK semantics,
+        VCA_DIAG.f90:153-170); ncv defaults to the reference's Nblock rule
+        3*nev + 5 (VCA_DIAG.f90:102).  Returns (evals, evecs[nev][dim] or
+        None, nconv)."""
+        if ncv is None:
+            ncv = 3 * nev + 5
+        evals = np.zeros(nev)
+        vecs = np.zeros((nev, self.dim), np.complex128) if want_vecs else None
+        nconv = ctypes.c_int32()
+        check(self._lib.vca_lanczos_eigs(self._h, nev, ncv, tol, maxiter, evals.ctypes.data,
+                                         vecs.ctypes.data if want_vecs else None,
+                                         ctypes.byref(nconv)))
+        return evals, vecs, nconv.value
 
     def lanczos_tridiag(self, v0, nlanc):
         v0 = np.ascontiguousarray(v0, dtype=np.complex128)
@@ -262,6 +288,16 @@ class VcaSolver:
         f = np.zeros(z.size)
         prec = {"fp64": VCA_FP64, "mixed": VCA_MIXED}[precision]
         check(self._lib.vca_omega_integrand(self._h, z.size, z.ctypes.data, prec, f.ctypes.data))
+        return f
+
+    def omega_integrand_complex(self, z, precision="fp64"):
+        """sum_kmesh_complex (VCA_OMEGA.f90:78-116): mean over k of the
+        principal complex log of each determinant."""
+        z = np.ascontiguousarray(np.atleast_1d(z), dtype=np.complex128)
+        f = np.zeros(z.size, np.complex128)
+        prec = {"fp64": VCA_FP64, "mixed": VCA_MIXED}[precision]
+        check(self._lib.vca_omega_integrand_complex(self._h, z.size, z.ctypes.data, prec,
+                                                    f.ctypes.data))
         return f
 
 

@@ -251,10 +251,15 @@ __global__ __launch_bounds__(64) void k_prep(const PrepArgs a) {
 // pivots go inactive, so no swap is needed and log|det| = sum log|pivot|.
 // NBT = 16 or 8: lanes (rows) per matrix; H_k stored [k][block][NBT][NBT],
 // G'(z) and P(z) with row stride 16.
+// With phase != null the kernel also writes the phase of det(M) of every
+// (z, block, k) as a unit complex: the product of pivot/|pivot| times the
+// sign of the row permutation (the pivot order; its inversions are counted
+// with a ballot of the active rows below the pivot row).
 template <typename R, int NBT>
 __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, int nspin_blk,
                                                     const ocplx<R> *hk, const ocplx<R> *gmat,
-                                                    const ocplx<R> *pmat, double *partials) {
+                                                    const ocplx<R> *pmat, double *partials,
+                                                    double2 *phase) {
   using C = ocplx<R>;
   constexpr int MPB = kOBlock / NBT;  // matrices per block
   const int lane = threadIdx.x & 63;
@@ -308,6 +313,8 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
     }
     bool active = true;
     double ld = 0.0;
+    double phr = 1.0, phi = 0.0;  // unit phase of det(M)
+    int inv_parity = 0;
 #pragma unroll
     for (int kk = 0; kk < NBT; ++kk) {
       float mag = active ? (float)(ar[kk] * ar[kk] + ai[kk] * ai[kk]) : -1.0f;
@@ -330,6 +337,19 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
       }
       const double pm = (double)pr[kk] * pr[kk] + (double)pim[kk] * pim[kk];
       ld += 0.5 * log(pm);
+      if (phase) {
+        // rows still active with a lower index than the pivot row are
+        // eliminated after it: one inversion each
+        const uint64_t below = __ballot(active && gl < idx);
+        const int gbase = lane & ~(NBT - 1);
+        const uint64_t gm = (below >> gbase) & ((NBT == 64) ? ~0ull : ((1ull << NBT) - 1ull));
+        inv_parity ^= __popcll(gm) & 1;
+        const double rm = 1.0 / sqrt(pm);
+        const double ur = (double)pr[kk] * rm, ui = (double)pim[kk] * rm;
+        const double nr = phr * ur - phi * ui;
+        phi = phr * ui + phi * ur;
+        phr = nr;
+      }
       if (gl == idx) active = false;
       if (active) {
         const R inv = (R)1 / (pr[kk] * pr[kk] + pim[kk] * pim[kk]);
@@ -344,6 +364,10 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
     }
     // ld is identical in the lanes of a group; invalid k contribute 0
     if (gl == 0) s_red[grp] = kvalid ? ld : 0.0;
+    if (phase && gl == 0 && kvalid) {
+      const double sg = inv_parity ? -1.0 : 1.0;
+      phase[((int64_t)iz * nspin_blk + sb) * nk + k] = make_double2(sg * phr, sg * phi);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
       double t = 0;
@@ -368,6 +392,28 @@ __global__ __launch_bounds__(64) void k_reduce_z(int nz, int nspin_blk, int nkb,
   if (threadIdx.x == 0) f[iz] = s;
 }
 
+// fim[z] = (1/Nk) sum_k arg(det M_k(z)), the principal argument of the
+// product of the block phases (the imaginary part of log(det) as the
+// principal complex logarithm of the full determinant).
+__global__ __launch_bounds__(64) void k_reduce_phase(int nz, int nspin_blk, int nk,
+                                                     const double2 *phase, double *fim) {
+  const int iz = blockIdx.x;
+  if (iz >= nz) return;
+  double t = 0;
+  for (int k = threadIdx.x; k < nk; k += 64) {
+    double pr = 1.0, pi = 0.0;
+    for (int sb = 0; sb < nspin_blk; ++sb) {
+      const double2 u = phase[((int64_t)iz * nspin_blk + sb) * nk + k];
+      const double nr = pr * u.x - pi * u.y;
+      pi = pr * u.y + pi * u.x;
+      pr = nr;
+    }
+    t += atan2(pi, pr);
+  }
+  for (int off = 32; off > 0; off >>= 1) t += __shfl_xor(t, off, 64);
+  if (threadIdx.x == 0) fim[iz] = t / nk;
+}
+
 // ================================================================ host ===
 OmegaDev::~OmegaDev() { release(); }
 
@@ -375,8 +421,10 @@ void OmegaDev::release() {
   for (void *p : {(void *)d_off, (void *)d_w, (void *)d_pole, (void *)d_hk64, (void *)d_hk32,
                   (void *)d_hloc, (void *)d_hb, (void *)d_vb, (void *)d_z, (void *)d_graw,
                   (void *)d_g64, (void *)d_g32, (void *)d_p64, (void *)d_p32, (void *)d_part,
-                  (void *)d_f})
+                  (void *)d_f, (void *)d_phase, (void *)d_fim})
     if (p) (void)hipFree(p);
+  d_phase = nullptr;
+  d_fim = nullptr;
   d_off = nullptr;
   d_w = nullptr;
   d_pole = nullptr;
@@ -445,8 +493,10 @@ void OmegaDev::upload(const OmegaHost &h, hipStream_t st) {
 void OmegaDev::ensure_z(int nz) {
   if (nz <= zcap) return;
   for (void *p : {(void *)d_z, (void *)d_graw, (void *)d_g64, (void *)d_g32, (void *)d_p64,
-                  (void *)d_p32, (void *)d_part, (void *)d_f})
+                  (void *)d_p32, (void *)d_part, (void *)d_f, (void *)d_phase, (void *)d_fim})
     if (p) VCA_HIP(hipFree(p));
+  d_phase = nullptr;
+  d_fim = nullptr;
   zcap = std::max(nz, 64);
   const int mpb = kOBlock / nbt;
   const int nkb = (host.nk + mpb - 1) / mpb;
@@ -466,9 +516,15 @@ __global__ void k_b_to_f32(int64_t n, const double2 *s, float2 *d) {
   if (i < n) d[i] = make_float2((float)s[i].x, (float)s[i].y);
 }
 
-void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, float *ms_logdet) {
+void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, float *ms_logdet,
+                         double *fim) {
   if (nz <= 0) return;
   ensure_z(nz);
+  if (fim && !d_phase) {
+    VCA_HIP(hipMalloc(&d_phase, sizeof(double2) * (size_t)zcap * host.nspin_blk * host.nk));
+    VCA_HIP(hipMalloc(&d_fim, sizeof(double) * zcap));
+  }
+  double2 *phase = fim ? d_phase : nullptr;
   const hipStream_t st = stream;
   VCA_HIP(hipMemcpyAsync(d_z, z, sizeof(double2) * nz, hipMemcpyHostToDevice, st));
   {
@@ -510,10 +566,10 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
     if (e0) VCA_HIP(hipEventRecord(e0, st));
     if (nbt == 8)
       hipLaunchKernelGGL((k_logdet<double, 8>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
-                         host.nspin_blk, d_hk64, d_g64, d_p64, d_part);
+                         host.nspin_blk, d_hk64, d_g64, d_p64, d_part, phase);
     else
       hipLaunchKernelGGL((k_logdet<double, 16>), grid, dim3(kOBlock), 0, st, host.nk, nz,
-                         zchunk, host.nspin_blk, d_hk64, d_g64, d_p64, d_part);
+                         zchunk, host.nspin_blk, d_hk64, d_g64, d_p64, d_part, phase);
   } else {
     const int64_t nm = (int64_t)nz * host.nspin_blk * kNB * kNB;
     hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, nm,
@@ -523,16 +579,22 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
     if (e0) VCA_HIP(hipEventRecord(e0, st));
     if (nbt == 8)
       hipLaunchKernelGGL((k_logdet<float, 8>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
-                         host.nspin_blk, d_hk32, d_g32, d_p32, d_part);
+                         host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
     else
       hipLaunchKernelGGL((k_logdet<float, 16>), grid, dim3(kOBlock), 0, st, host.nk, nz,
-                         zchunk, host.nspin_blk, d_hk32, d_g32, d_p32, d_part);
+                         zchunk, host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
   }
   VCA_HIP(hipGetLastError());
   if (e1) VCA_HIP(hipEventRecord(e1, st));
   hipLaunchKernelGGL(k_reduce_z, dim3(nz), dim3(64), 0, st, nz, host.nspin_blk, nkb, host.nk,
                      d_part, d_f);
   VCA_HIP(hipGetLastError());
+  if (fim) {
+    hipLaunchKernelGGL(k_reduce_phase, dim3(nz), dim3(64), 0, st, nz, host.nspin_blk, host.nk,
+                       d_phase, d_fim);
+    VCA_HIP(hipGetLastError());
+    VCA_HIP(hipMemcpyAsync(fim, d_fim, sizeof(double) * nz, hipMemcpyDeviceToHost, st));
+  }
   VCA_HIP(hipMemcpyAsync(f, d_f, sizeof(double) * nz, hipMemcpyDeviceToHost, st));
   VCA_HIP(hipStreamSynchronize(st));
   if (ms_logdet) {

@@ -39,6 +39,8 @@ struct OmegaDev {
   double2 *d_z = nullptr, *d_graw = nullptr, *d_g64 = nullptr, *d_p64 = nullptr;
   float2 *d_g32 = nullptr, *d_p32 = nullptr;
   double *d_part = nullptr, *d_f = nullptr;
+  double2 *d_phase = nullptr;  // [zcap][nspin_blk][nk] unit phases (complex log mode)
+  double *d_fim = nullptr;
   int zcap = 0;
   int nbt = 16;  // rows per matrix in the log-det kernel (8 or 16)
 
@@ -47,8 +49,11 @@ struct OmegaDev {
   void upload(const OmegaHost &h, hipStream_t st);
   void ensure_z(int nz);
   // f[m] for the nz frequencies z (host array); FP64 or MIXED k x z
-  // log-dets; optional time of the log-det kernel in ms.
-  void integrand(int nz, const double2 *z, int precision, double *f, float *ms_logdet);
+  // log-dets; optional time of the log-det kernel in ms.  With fim, also
+  // the mean principal argument of det (the imaginary part of the complex
+  // logarithm of sum_kmesh_complex).
+  void integrand(int nz, const double2 *z, int precision, double *f, float *ms_logdet,
+                 double *fim = nullptr);
 };
 
 }  // namespace vca

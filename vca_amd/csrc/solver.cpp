@@ -581,10 +581,106 @@ static std::vector<zc> dense_h(const Solver &s, int nup, int ndw) {
 }
 
 // --------------------------------------------------------- GS search ----
+static bool finite_t(const vca_params &p) { return p.lanc_nstates_total > 1; }
+
+// Eigenpairs of one sector (VCA_DIAG.f90:98-212): dense eigh when
+// dim <= lanc_dim_threshold or Neigen == dim; else the GPU plain Lanczos
+// (LANC_METHOD=lanczos, one state, FP64 or MIXED) or the GPU thick-restart
+// Lanczos (ARPACK, FP64).  Eigenvectors are device complex128 vectors.
+static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<double> &evals,
+                         std::vector<void *> &evecs) {
+  const vca_params &p = s.p;
+  Binom bn;
+  const int64_t dim = (int64_t)bn(s.ns, nup) * bn(s.ns, ndw);
+  evals.clear();
+  evecs.clear();
+  const bool lanc_solve = !(neigen == dim || dim <= p.lanc_dim_threshold);
+  if (!lanc_solve) {
+    std::vector<zc> h = dense_h(s, nup, ndw);
+    std::vector<double> w(dim);
+    herm_eig((int)dim, h.data(), w.data());
+    for (int i = 0; i < neigen; ++i) {
+      evals.push_back(w[i]);
+      void *d = nullptr;
+      VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
+      VCA_HIP(hipMemcpy(d, h.data() + (size_t)i * dim, sizeof(double2) * dim,
+                        hipMemcpyHostToDevice));
+      evecs.push_back(d);
+    }
+    return;
+  }
+  if (p.lanc_method != VCA_LANC_LANCZOS) {
+    auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+    const int ncv = (int)std::min<int64_t>(
+        dim, (int64_t)p.lanc_ncv_factor * std::max(neigen, p.lanc_nstates_sector) + p.lanc_ncv_add);
+    const int maxiter = (int)std::min<int64_t>(dim, p.lanc_niter);
+    const EigsOut o =
+        eigs_lanczos(*s64, neigen, ncv, p.lanc_tolerance, maxiter, nullptr, evals, evecs);
+    if (o.nconv < neigen) {
+      for (void *d : evecs) (void)hipFree(d);
+      evecs.clear();
+      fail(VCA_ERR_NOCONV, "sector (%d,%d): %d of %d eigenpairs converged in %d restarts", nup,
+           ndw, o.nconv, neigen, maxiter);
+    }
+    return;
+  }
+  GsOut g{};
+  void *d = nullptr;
+  VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
+  if (p.precision == VCA_MIXED) {
+    auto s32 = make_sector(s, nup, ndw, VCA_MIXED);
+    auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+    double e32;
+    g = gs_mixed(s, *s32, *s64, p.lanc_tolerance, p.lanc_niter, &e32);
+    s.res.e0_mixed = e32;
+    VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
+  } else {
+    auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+    g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s);
+    VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
+  }
+  evals.push_back(g.e0);
+  evecs.push_back(d);
+}
+
+// es_insert_state (VCA_EIGENSPACE.f90:160-212): ascending energies, a new
+// state goes before the first state of equal or higher energy.
+static void es_insert(std::vector<KeptState> &list, const KeptState &st) {
+  size_t pos = 0;
+  while (pos < list.size() && !(st.e <= list[pos].e)) ++pos;
+  list.insert(list.begin() + (long)pos, st);
+}
+
+static void es_pop(std::vector<KeptState> &list) {
+  if (list.back().d_vec) VCA_HIP(hipFree(list.back().d_vec));
+  list.pop_back();
+}
+
+// es_add_state with a size limit (VCA_EIGENSPACE.f90:133-155): below the
+// limit insert; at the limit replace the highest state if e is lower.
+// Returns false when the state was not kept (the caller frees its vector).
+static bool es_add_sized(std::vector<KeptState> &list, const KeptState &st, int size) {
+  if ((int)list.size() < size) {
+    es_insert(list, st);
+    return true;
+  }
+  if (!list.empty() && st.e < list.back().e) {
+    es_pop(list);
+    es_insert(list, st);
+    return true;
+  }
+  return false;
+}
+
+// vca_diag_c (VCA_DIAG.f90:49-267) with vca_post_diag (:333-474): the
+// sector scan, T = 0 ground-state selection or, at finite T, the
+// size-limited state list with the spectrum annealing loop that adapts
+// the states per sector and the list size until exp(-beta (Emax - Emin))
+// < cutoff.
 static void diagonalize(Solver &s) {
   const vca_params &p = s.p;
+  const bool ft = finite_t(p);
   free_states(s);
-  double oldzero = 1000.0;
   std::vector<std::pair<int, int>> list;
   if (s.sectors.empty()) {
     for (int nup = 0; nup <= s.ns; ++nup)
@@ -594,72 +690,98 @@ static void diagonalize(Solver &s) {
       list.push_back({s.sectors[i], s.sectors[i + 1]});
   }
   Binom bn;
-  for (auto [nup, ndw] : list) {
-    const int64_t dim = (int64_t)bn(s.ns, nup) * bn(s.ns, ndw);
-    const int neigen =
-        p.lanc_method == VCA_LANC_LANCZOS ? 1 : (int)std::min<int64_t>(dim, p.lanc_nstates_sector);
-    const bool lanc_solve = !(neigen == dim || dim <= p.lanc_dim_threshold);
-    std::vector<double> evals;
-    std::vector<void *> evecs;  // device complex128
-    if (!lanc_solve) {
-      std::vector<zc> h = dense_h(s, nup, ndw);
-      std::vector<double> w(dim);
-      herm_eig((int)dim, h.data(), w.data());
-      for (int i = 0; i < neigen; ++i) {
-        evals.push_back(w[i]);
-        void *d = nullptr;
-        VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
-        VCA_HIP(hipMemcpy(d, h.data() + (size_t)i * dim, sizeof(double2) * dim,
-                          hipMemcpyHostToDevice));
-        evecs.push_back(d);
+  // even state counts at finite T (VCA_SETUP.f90:165-176)
+  int nst_sector = p.lanc_nstates_sector, nst_total = p.lanc_nstates_total;
+  if (ft) {
+    if (nst_sector % 2) ++nst_sector;
+    if (nst_total % 2) ++nst_total;
+  }
+  std::map<std::pair<int, int>, int> neigen_sector;  // VCA_SETUP.f90:313
+  for (int nup = 0; nup <= s.ns; ++nup)
+    for (int ndw = 0; ndw <= s.ns; ++ndw) {
+      const int64_t dim = (int64_t)bn(s.ns, nup) * bn(s.ns, ndw);
+      neigen_sector[{nup, ndw}] = (int)std::min<int64_t>(dim, nst_sector);
+    }
+  bool converged = false;
+  int iter = 0;
+  double zeta = 1.0;
+  while (!converged && iter < std::max(1, p.lanc_niter_spectrum)) {
+    ++iter;
+    free_states(s);
+    double oldzero = 1000.0;
+    for (auto [nup, ndw] : list) {
+      const int64_t dim = (int64_t)bn(s.ns, nup) * bn(s.ns, ndw);
+      const int neigen = p.lanc_method == VCA_LANC_LANCZOS
+                             ? 1
+                             : (int)std::min<int64_t>(dim, neigen_sector[{nup, ndw}]);
+      std::vector<double> evals;
+      std::vector<void *> evecs;
+      solve_sector(s, nup, ndw, neigen, evals, evecs);
+      for (size_t i = 0; i < evals.size(); ++i) {
+        const KeptState st{evals[i], nup, ndw, evecs[i]};
+        bool kept = false;
+        if (ft) {
+          kept = es_add_sized(s.states, st, nst_total);
+        } else {
+          // T = 0 selection (VCA_DIAG.f90:224-235)
+          const double en = evals[i];
+          if (en < oldzero - 10.0 * p.gs_threshold) {
+            oldzero = en;
+            free_states(s);
+            es_insert(s.states, st);
+            kept = true;
+          } else if (std::fabs(en - oldzero) <= p.gs_threshold) {
+            oldzero = std::min(oldzero, en);
+            es_insert(s.states, st);
+            kept = true;
+          }
+        }
+        if (kept) evecs[i] = nullptr;
+      }
+      for (void *d : evecs)
+        if (d) VCA_HIP(hipFree(d));
+    }
+    if (s.states.empty()) fail(VCA_ERR_NOCONV, "no ground state found");
+    const double egs = s.states.front().e, emax = s.states.back().e;
+    converged = ft ? std::exp(-p.beta * (emax - egs)) < p.cutoff : true;
+    // vca_post_diag
+    if (ft) {
+      zeta = 0.0;
+      for (auto &st : s.states) zeta += std::exp(-p.beta * (st.e - egs));
+      std::map<std::pair<int, int>, int> count;
+      for (auto &st : s.states) count[{st.nup, st.ndw}] += 1;
+      for (auto &kv : neigen_sector) {
+        const int c = count.count(kv.first) ? count[kv.first] : 0;
+        int &ne = kv.second;
+        ne += c > 0 ? 1 : -1;
+        if (ne > c) ne = c + 1;
+        if (ne <= 0) ne = 1;
+      }
+      if (std::exp(-p.beta * (emax - egs)) > p.cutoff) {
+        nst_total += p.lanc_nstates_step;
+      } else {
+        while (s.states.size() > 1 &&
+               std::exp(-p.beta * (s.states.back().e - egs)) <= p.cutoff)
+          es_pop(s.states);
+        nst_total = std::max((int)s.states.size(), p.lanc_nstates_step) + p.lanc_nstates_step;
       }
     } else {
-      if (neigen > 1)
-        fail(VCA_ERR_ARG,
-             "sector (%d,%d): %d states per Lanczos sector requested; this build's GPU "
-             "ground-state search returns one state per sector (LANC_METHOD=lanczos)",
-             nup, ndw, neigen);
-      GsOut g{};
-      void *d = nullptr;
-      VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
-      if (p.precision == VCA_MIXED) {
-        auto s32 = make_sector(s, nup, ndw, VCA_MIXED);
-        auto s64 = make_sector(s, nup, ndw, VCA_FP64);
-        double e32;
-        g = gs_mixed(s, *s32, *s64, p.lanc_tolerance, p.lanc_niter, &e32);
-        s.res.e0_mixed = e32;
-        VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
-      } else {
-        auto s64 = make_sector(s, nup, ndw, VCA_FP64);
-        g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s);
-        VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
-      }
-      evals.push_back(g.e0);
-      evecs.push_back(d);
+      zeta = (double)s.states.size();
     }
-    // T=0 selection, VCA_DIAG.f90:224-235
-    for (size_t i = 0; i < evals.size(); ++i) {
-      const double en = evals[i];
-      if (en < oldzero - 10.0 * p.gs_threshold) {
-        oldzero = en;
-        free_states(s);
-        s.states.push_back(KeptState{en, nup, ndw, evecs[i]});
-        evecs[i] = nullptr;
-      } else if (std::fabs(en - oldzero) <= p.gs_threshold) {
-        oldzero = std::min(oldzero, en);
-        s.states.push_back(KeptState{en, nup, ndw, evecs[i]});
-        evecs[i] = nullptr;
-      }
-    }
-    for (void *d : evecs)
-      if (d) VCA_HIP(hipFree(d));
   }
-  if (s.states.empty()) fail(VCA_ERR_NOCONV, "no ground state found");
-  s.emin = s.states[0].e;
-  for (auto &st : s.states) s.emin = std::min(s.emin, st.e);
-  s.zeta = (double)s.states.size();  // vca_post_diag at T=0 (VCA_DIAG.f90:363)
+  s.emin = s.states.front().e;
+  s.zeta = zeta;
   s.res.nstates = (int)s.states.size();
-  s.res.omegaprime = s.emin;
+  s.res.zeta = zeta;
+  s.res.finite_t = ft ? 1 : 0;
+  s.res.spectrum_iters = iter;
+  if (ft) {
+    double z = 0;
+    for (auto &st : s.states) z += std::exp(-p.beta * (st.e - s.emin));
+    s.res.omegaprime = s.emin - std::log(z) / p.beta;  // VCA_MAIN.f90:186-190
+  } else {
+    s.res.omegaprime = s.emin;
+  }
 }
 
 // ----------------------------------------------------------- GF chains --
@@ -800,7 +922,14 @@ static void build_gf(Solver &s) {
           std::vector<double> w(nlanc), z((size_t)nlanc * nlanc);
           tridiag_eig(nlanc, al.data() + (size_t)c * nlanc, be.data() + (size_t)c * nlanc,
                       w.data(), z.data());
-          const zc peso_bz = zc(chan_re[cs.ichan - 1], chan_im[cs.ichan - 1]) * norm2 / s.zeta;
+          // pesoBZ (VCA_GF_NORMAL.f90:987-994): Boltzmann factor at finite T
+          double boltz = 1.0;
+          if (finite_t(p)) {
+            const double x = p.beta * (st.e - s.emin);
+            boltz = x < 200.0 ? std::exp(-x) : 0.0;
+          }
+          const zc peso_bz =
+              zc(chan_re[cs.ichan - 1], chan_im[cs.ichan - 1]) * norm2 * boltz / s.zeta;
           GfChan &out = res[idx[b0 + c]];
           out.w.resize(nlanc);
           out.p.resize(nlanc);
@@ -947,13 +1076,15 @@ static void prepare_omega(Solver &s) {
   s.om_ready = true;
 }
 
-static std::vector<double> integrand(Solver &s, const std::vector<double2> &z, int prec) {
+static std::vector<double> integrand(Solver &s, const std::vector<double2> &z, int prec,
+                                     std::vector<double> *fim = nullptr) {
   std::vector<double> f(z.size());
-  const int batch = 1024;
+  if (fim) fim->assign(z.size(), 0.0);
+  const int batch = fim ? 256 : 1024;
   for (size_t i = 0; i < z.size(); i += batch) {
     const int n = (int)std::min<size_t>(batch, z.size() - i);
     float ms = 0;
-    s.om->integrand(n, z.data() + i, prec, f.data() + i, &ms);
+    s.om->integrand(n, z.data() + i, prec, f.data() + i, &ms, fim ? fim->data() + i : nullptr);
     s.res.omega_seconds += ms * 1e-3;
     s.res.omega_evals += (int64_t)n * s.nk;
   }
@@ -977,27 +1108,18 @@ struct Panel {
   double a, b, val, err;
 };
 
-// Nodes of the panel [a, b] in t, mapped to x = (1-t)/t (the QAGI map of
-// [0, inf) onto (0, 1]); fills 15 x values.
-static void panel_nodes(double a, double b, double *x) {
+// The 15 Kronrod nodes of [a, b].
+static void panel_nodes(double a, double b, double *t) {
   const double c = 0.5 * (a + b), hl = 0.5 * (b - a);
   for (int j = 0; j < 7; ++j) {
-    const double t1 = c - hl * kXgk[j], t2 = c + hl * kXgk[j];
-    x[j] = (1.0 - t1) / t1;
-    x[14 - j] = (1.0 - t2) / t2;
+    t[j] = c - hl * kXgk[j];
+    t[14 - j] = c + hl * kXgk[j];
   }
-  x[7] = (1.0 - c) / c;
+  t[7] = c;
 }
 
-static void panel_eval(Panel &pn, const double *f /*15 values at the nodes*/) {
-  const double c = 0.5 * (pn.a + pn.b), hl = 0.5 * (pn.b - pn.a);
-  double g[15];
-  for (int j = 0; j < 7; ++j) {
-    const double t1 = c - hl * kXgk[j], t2 = c + hl * kXgk[j];
-    g[j] = f[j] / (t1 * t1);
-    g[14 - j] = f[14 - j] / (t2 * t2);
-  }
-  g[7] = f[7] / (c * c);
+static void panel_eval(Panel &pn, const double *g /*15 values at the nodes*/) {
+  const double hl = 0.5 * (pn.b - pn.a);
   double rk = kWgk[7] * g[7], rg = kWg[3] * g[7];
   for (int j = 0; j < 7; ++j) {
     rk += kWgk[j] * (g[j] + g[14 - j]);
@@ -1007,22 +1129,20 @@ static void panel_eval(Panel &pn, const double *f /*15 values at the nodes*/) {
   pn.err = std::fabs((rk - rg) * hl);
 }
 
-// int_0^inf f(x) dx with f(x) = integrand at z = i x, adaptive bisection in
-// t with every panel of above-average error split in each round.
-static double quad_imag_axis(Solver &s, int prec) {
-  const double epsabs = s.p.quad_epsabs, epsrel = s.p.quad_epsrel;
+// Adaptive Gauss-Kronrod 7/15 of g on [a, b]; eval(t) returns g at a batch
+// of nodes.  Each round splits the largest-error panel and every panel
+// above the average share of the tolerance, until the summed error
+// estimate is below max(epsabs, epsrel |I|).
+template <typename Eval>
+static double adapt_gk15(double a, double b, double epsabs, double epsrel, Eval &&eval) {
   std::vector<Panel> panels;
   auto evaluate = [&](std::vector<Panel> &ps) {
-    std::vector<double2> z(ps.size() * 15);
-    std::vector<double> x(15);
-    for (size_t i = 0; i < ps.size(); ++i) {
-      panel_nodes(ps[i].a, ps[i].b, x.data());
-      for (int j = 0; j < 15; ++j) z[i * 15 + j] = make_double2(0.0, x[j]);
-    }
-    std::vector<double> f = integrand(s, z, prec);
-    for (size_t i = 0; i < ps.size(); ++i) panel_eval(ps[i], f.data() + i * 15);
+    std::vector<double> t(ps.size() * 15);
+    for (size_t i = 0; i < ps.size(); ++i) panel_nodes(ps[i].a, ps[i].b, t.data() + i * 15);
+    const std::vector<double> g = eval(t);
+    for (size_t i = 0; i < ps.size(); ++i) panel_eval(ps[i], g.data() + i * 15);
   };
-  std::vector<Panel> init = {{0.0, 1.0, 0, 0}};
+  std::vector<Panel> init = {{a, b, 0, 0}};
   evaluate(init);
   panels = init;
   for (int round = 0; round < 200; ++round) {
@@ -1035,7 +1155,6 @@ static double quad_imag_axis(Solver &s, int prec) {
     if (err <= tol || panels.size() > 20000) break;
     const double thr = tol / (double)panels.size();
     std::vector<Panel> keep, fresh;
-    // split the largest-error panel and all panels above the average share
     size_t imax = 0;
     for (size_t i = 1; i < panels.size(); ++i)
       if (panels[i].err > panels[imax].err) imax = i;
@@ -1058,11 +1177,62 @@ static double quad_imag_axis(Solver &s, int prec) {
   return tot;
 }
 
+// int_0^inf f(i x) dx through the QAGI map x = (1-t)/t, t in (0, 1]
+// (QUADPACK qagi, called through SciFortran quad at VCA_OMEGA.f90:132).
+static double quad_imag_axis(Solver &s, int prec) {
+  return adapt_gk15(0.0, 1.0, s.p.quad_epsabs, s.p.quad_epsrel, [&](const std::vector<double> &t) {
+    std::vector<double2> z(t.size());
+    for (size_t i = 0; i < t.size(); ++i) z[i] = make_double2(0.0, (1.0 - t[i]) / t[i]);
+    std::vector<double> g = integrand(s, z, prec);
+    for (size_t i = 0; i < t.size(); ++i) g[i] /= t[i] * t[i];
+    return g;
+  });
+}
+
+// frequency_integration_finite_t (VCA_OMEGA.f90:153-188): the Matsubara
+// sum (2/beta) sum_{n=0}^{nmax} Re F(i w_n) plus the contour integral
+// int_0^pi Re[(1/pi) w f(w) F(w)] dzeta on w = R e^{i zeta} (R = 2(nmax+1)
+// pi/beta, f the Fermi function, integral_contour :191-204), times
+// (3 - Nspin); F = sum_kmesh_complex.
+static double omega_finite_t(Solver &s, int prec) {
+  const vca_params &p = s.p;
+  const double beta = p.beta;
+  int nmax = (int)(2.0 * (std::fabs(s.max_exc) + p.bandwidth) * beta / M_PI);
+  nmax = (nmax % 2 == 0) ? nmax / 2 : (nmax + 1) / 2;
+  const double radius = 2.0 * (nmax + 1) * M_PI / beta;
+  s.res.nmax = nmax;
+  std::vector<double2> zm(nmax + 1);
+  for (int n = 0; n <= nmax; ++n) zm[n] = make_double2(0.0, (2.0 * n + 1.0) * M_PI / beta);
+  const std::vector<double> fm = integrand(s, zm, prec);
+  double sum = 0;
+  for (double v : fm) sum += v;
+  sum *= 2.0 / beta;
+  const double contour =
+      adapt_gk15(0.0, M_PI, p.quad_epsabs, p.quad_epsrel, [&](const std::vector<double> &t) {
+        std::vector<double2> z(t.size());
+        for (size_t i = 0; i < t.size(); ++i)
+          z[i] = make_double2(radius * std::cos(t[i]), radius * std::sin(t[i]));
+        std::vector<double> fim;
+        const std::vector<double> fre = integrand(s, z, prec, &fim);
+        std::vector<double> g(t.size());
+        for (size_t i = 0; i < t.size(); ++i) {
+          const zc w(z[i].x, z[i].y);
+          zc fermi(0.0, 0.0);
+          if ((w.real() - p.xmu) * beta < 100.0) fermi = 1.0 / (std::exp(beta * (w - p.xmu)) + 1.0);
+          g[i] = ((1.0 / M_PI) * w * fermi * zc(fre[i], fim[i])).real();
+        }
+        return g;
+      });
+  return (3.0 - p.nspin) * (sum + contour);
+}
+
 static void omega_stage(Solver &s) {
   const vca_params &p = s.p;
   prepare_omega(s);
   double integral;
-  if (p.omega_mode == VCA_OMEGA_MATSUBARA) {
+  if (finite_t(p)) {
+    integral = omega_finite_t(s, p.precision);
+  } else if (p.omega_mode == VCA_OMEGA_MATSUBARA) {
     std::vector<double2> z(p.omega_nfreq);
     for (int n = 0; n < p.omega_nfreq; ++n)
       z[n] = make_double2(0.0, (2.0 * n + 1.0) * M_PI / p.beta);
@@ -1074,7 +1244,8 @@ static void omega_stage(Solver &s) {
     integral = (3.0 - p.nspin) * quad_imag_axis(s, p.precision) / M_PI;
   }
   s.res.omega_integral = integral;
-  s.res.sft_potential = s.emin - integral;
+  s.res.sft_potential = s.res.omegaprime - integral;
+  s.res.max_exc = s.max_exc;
 }
 
 static double now() {
@@ -1127,6 +1298,13 @@ void vca_default_params(vca_params *p) {
   p->omega_nfreq = 4096;
   p->quad_epsabs = 1e-10;
   p->quad_epsrel = 1e-12;
+  p->lanc_nstates_total = 1;
+  p->lanc_nstates_step = 2;
+  p->lanc_niter_spectrum = 10;
+  p->lanc_ncv_factor = 3;
+  p->lanc_ncv_add = 5;
+  p->cutoff = 1e-9;
+  p->bandwidth = 0.0;
 }
 
 const char *vca_last_error(void) { return g_err.c_str(); }
@@ -1369,6 +1547,31 @@ int vca_lanczos_gs(vca_sector *sv, double tol, int32_t niter, double *e0, double
   VCA_GUARD_END
 }
 
+int vca_lanczos_eigs(vca_sector *sv, int32_t nev, int32_t ncv, double tol, int32_t maxiter,
+                     double *evals, double *evecs, int32_t *nconv) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !evals) fail(VCA_ERR_ARG, "null argument");
+  std::unique_ptr<Sector> twin;
+  Sector *s64 = sec;
+  if (sec->precision != VCA_FP64) {
+    twin = make_sector(*sec->solver, sec->nup, sec->ndw, VCA_FP64);
+    s64 = twin.get();
+  }
+  std::vector<double> ev;
+  std::vector<void *> vecs;
+  const EigsOut o = eigs_lanczos(*s64, nev, ncv, tol, maxiter, nullptr, ev, vecs);
+  for (int i = 0; i < nev; ++i) evals[i] = ev[i];
+  for (int i = 0; i < nev; ++i) {
+    if (evecs)
+      VCA_HIP(hipMemcpy(evecs + (size_t)2 * i * sec->dim, vecs[i], sizeof(double2) * sec->dim,
+                        hipMemcpyDeviceToHost));
+    VCA_HIP(hipFree(vecs[i]));
+  }
+  if (nconv) *nconv = o.nconv;
+  VCA_GUARD_END
+}
+
 int vca_lanczos_tridiag(vca_sector *sv, const double *v0, int32_t nlanc, double *alpha,
                         double *beta) {
   VCA_GUARD_BEGIN
@@ -1456,6 +1659,24 @@ int vca_omega_integrand(vca_solver *sv, int32_t nz, const double *z, int32_t pre
   for (int i = 0; i < nz; ++i) zz[i] = make_double2(z[2 * i], z[2 * i + 1]);
   std::vector<double> out = integrand(*s, zz, precision);
   std::memcpy(f, out.data(), sizeof(double) * nz);
+  VCA_GUARD_END
+}
+
+int vca_omega_integrand_complex(vca_solver *sv, int32_t nz, const double *z, int32_t precision,
+                                double *f) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s || !z || !f || nz < 1) fail(VCA_ERR_ARG, "bad argument");
+  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  if (!s->om_ready) prepare_omega(*s);
+  std::vector<double2> zz(nz);
+  for (int i = 0; i < nz; ++i) zz[i] = make_double2(z[2 * i], z[2 * i + 1]);
+  std::vector<double> fim;
+  std::vector<double> out = integrand(*s, zz, precision, &fim);
+  for (int i = 0; i < nz; ++i) {
+    f[2 * i] = out[i];
+    f[2 * i + 1] = fim[i];
+  }
   VCA_GUARD_END
 }
 

@@ -254,6 +254,15 @@ void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int 
                    const ExcOp *d_ops, int nb);
 CrossTerm make_cross(const ModelData &md);
 
+// Lowest eigenpairs by thick-restart Lanczos (eigs.hip).
+struct EigsOut {
+  int nconv;     // converged among the nev requested
+  int restarts;  // restarts used
+  int64_t hxv;   // H x v applications
+};
+EigsOut eigs_lanczos(Sector &sec, int nev, int ncv, double tol, int maxiter, const void *start,
+                     std::vector<double> &evals, std::vector<void *> &evecs);
+
 // Host linear algebra (linalg.cpp)
 // Symmetric tridiagonal eigenproblem, diag d[n], offdiag e[n] with e[i]
 // coupling i-1 and i (e[0] unused).  Eigenvalues ascending in w; if z is

@@ -24,6 +24,9 @@ module vca_gpu
   integer, public :: lanc_nstates_sector = 6, lanc_niter = 512, lanc_ngfiter = 200
   integer, public :: lanc_dim_threshold = 256
   real(8), public :: lanc_tolerance = 1d-12, gs_threshold = 1d-9
+  integer, public :: lanc_nstates_total = 1, lanc_nstates_step = 2, lanc_niter_spectrum = 10
+  integer, public :: lanc_ncv_factor = 3, lanc_ncv_add = 5
+  real(8), public :: cutoff = 1d-9, bandwidth = 0d0
   ! additions of this build
   integer, public :: vca_precision = 0        ! 0 FP64, 1 MIXED
   integer, public :: vca_omega_mode = 0       ! 0 adaptive quadrature, 1 Matsubara grid
@@ -42,6 +45,9 @@ module vca_gpu
      real(c_double)     :: quad_epsabs, quad_epsrel
      integer(c_int32_t) :: nsectors
      type(c_ptr)        :: sectors
+     integer(c_int32_t) :: lanc_nstates_total, lanc_nstates_step, lanc_niter_spectrum
+     integer(c_int32_t) :: lanc_ncv_factor, lanc_ncv_add
+     real(c_double)     :: cutoff, bandwidth
   end type vca_params_t
 
   interface
@@ -130,6 +136,11 @@ contains
     cparams%precision = vca_precision; cparams%omega_mode = vca_omega_mode
     cparams%omega_nfreq = vca_omega_nfreq
     cparams%quad_epsabs = vca_quad_epsabs; cparams%quad_epsrel = vca_quad_epsrel
+    cparams%lanc_nstates_total = lanc_nstates_total
+    cparams%lanc_nstates_step = lanc_nstates_step
+    cparams%lanc_niter_spectrum = lanc_niter_spectrum
+    cparams%lanc_ncv_factor = lanc_ncv_factor; cparams%lanc_ncv_add = lanc_ncv_add
+    cparams%cutoff = cutoff; cparams%bandwidth = bandwidth
     cparams%nsectors = 0; cparams%sectors = c_null_ptr
     if (allocated(vca_sectors_list)) then
        if (allocated(csectors)) deallocate(csectors)
