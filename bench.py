@@ -1,21 +1,36 @@
 """Benchmark of the VCA solver-core hot path on one MI355X.
 
-Workload (default): the 12-site 3x4 square Hubbard cluster, U=8, half-filled
-sector Nup=Ndw=6 (853,776 states), FP64 -- the plain FP64 GPU
-configuration.  A "step" is one H x v application on that sector with the
-vector resident in HBM.  Reported beside it in the same JSON line:
-  * the MIXED form (complex64 H x v) on the same sector;
-  * the ground-state Lanczos (FP64, and MIXED with FP64 refinement) with
-    its E0 error against FP64;
-  * Omega evals/s: (k, omega) log-det evaluations per second of the Omega
-    integrand on the 2x2 BHZ cluster (n=16 in two spin blocks of 8), 128x128
-    k-grid, 8192 Matsubara frequencies at beta=200, FP64 and MIXED;
-  * the roofline of the H x v kernel (algorithmic bytes 2 x 16 B per state
-    per application over its event-timed duration, against 8 TB/s);
-  * the CPU baseline: the oracle's restatement of the reference H x v loops
-    (C, -O2 -funroll-loops, one core), timed on a bounded sample.
+Headline workload: the 12-site 3x4 square Hubbard cluster, U=8, half-filled
+sector Nup=Ndw=6 (853,776 states), FP64.  A "step" is one H x v application
+on that sector with the vector resident in HBM; `value` is Gstates/s =
+dim / (average kernel time of one application, HIP events on the sector's
+stream).  Reported beside it in the same JSON line:
+  * mixed: the complex64 H x v on the same sector;
+  * ground_state: the Lanczos ground state, FP64 and MIXED (FP32 Lanczos +
+    FP64 refinement), with the E0 error of MIXED against FP64;
+  * omega: (k, omega) log-det evaluations per second of the Omega integrand
+    on the 2x2 BHZ cluster (Norb=2, Nspin=2: two spin blocks of 8),
+    128x128 k-grid x 8192 Matsubara frequencies at beta=200, FP64 and MIXED,
+    the MIXED error of the integral, and the roofline of the log-det
+    kernel (flops it performs per evaluation over the measured time);
+  * full_solve: vca_solve of the 3x4 cluster end to end (ground state, the
+    excitation chains of 300 steps on the N+-1 sectors, Omega on the 64x64
+    k-grid);
+  * batched_2x5: vca_solve of the 10-site 2x5 cluster, every (Nup, Ndw)
+    sector, ARPACK-equivalent eigenpairs per sector, all chains batched;
+  * bath16: the 8-site + 8-bath-level sector (165,636,900 states), H x v
+    FP64 and MIXED and the ground state both ways;
+  * roofline: of the FP64 H x v kernel, algorithmic bytes (one read of v and
+    one write of Hv, 2 x 16 B per state) over the event-timed duration,
+    against 8 TB/s; traffic = HBM bytes per launch measured with rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/hxv_traffic.json);
+  * cpu_baseline: the oracle's C restatement of the reference H x v loops
+    (-O2 -funroll-loops) on one core on a bounded sample, plus all host
+    cores with the reference's DW-slice split, and the Omega integrand
+    (numpy) on a sample of frequencies.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--full]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--only hxv|omega]
+                    [--no-full] [--no-c3] [--no-bath16] [--no-cpu]
 """
 from __future__ import annotations
 
@@ -29,10 +44,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hxv_traffic.json")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def logdet_flops(nb: int, nblk: int) -> float:
+    """Real flops of one (k, omega) evaluation of the log-det kernel: per
+    spin block of size nb the contraction H_k G' (nb^3 complex MACs, 8 flops
+    each) and the LU elimination (nb^3/3 complex MACs)."""
+    return nblk * (8.0 * nb ** 3 + 8.0 * nb ** 3 / 3.0)
 
 
 def main():
@@ -40,19 +65,20 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--full", action="store_true", help="also run the full vca_solve of the 3x4 cluster")
+    ap.add_argument("--no-full", action="store_true", help="skip the end-to-end 3x4 solve")
+    ap.add_argument("--no-c3", action="store_true", help="skip the every-sector 2x5 solve")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-bath16", action="store_true",
                     help="skip the 8-site + 8-bath-level sector (165,636,900 states)")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
     ap.add_argument("--only", choices=["all", "hxv", "omega"], default="all",
-                    help="hxv: only the H x v timing plus a 256 MiB copy kernel that "
-                         "calibrates HBM byte counters (for rocprofv3 runs)")
+                    help="hxv: the H x v timing plus a 256 MiB copy that calibrates the HBM "
+                         "byte counters; omega: the Omega integrand timing (rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU "
-              "sharding of this path is a design note, not a measured configuration.",
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would "
+              "shard over several GPUs is a design note, not a measured configuration.",
               file=sys.stderr)
         sys.exit(2)
 
@@ -63,10 +89,62 @@ def main():
     from vca_amd.api import VcaSolver
 
     dev = torch.device("cuda:0")
+
+    # --------------------------------------------------------- Omega -----
+    def omega_section():
+        po, hlo, hko, _, _ = configs.config_bhz_2x2(nk=args.omega_nk, nfreq=args.omega_nfreq,
+                                                   precision="fp64")
+        so = VcaSolver(po)
+        t0 = time.perf_counter()
+        om64 = so.vca_solve(hlo, hko)
+        t_solve = time.perf_counter() - t0
+        zs = 1j * (2 * np.arange(args.omega_nfreq) + 1) * np.pi / po.beta
+        nk = hko.shape[6]
+        so.omega_integrand(zs[:64], "fp64")
+        so.omega_integrand(zs[:64], "mixed")
+        r0 = so.results()
+        t0 = time.perf_counter()
+        f64 = so.omega_integrand(zs, "fp64")
+        t64 = time.perf_counter() - t0
+        r1 = so.results()
+        t0 = time.perf_counter()
+        f32 = so.omega_integrand(zs, "mixed")
+        t32 = time.perf_counter() - t0
+        r2 = so.results()
+        k64 = r1["omega_seconds"] - r0["omega_seconds"]  # log-det kernel time (events)
+        k32 = r2["omega_seconds"] - r1["omega_seconds"]
+        s64 = (3 - po.nspin) * 2.0 / po.beta * f64.sum()
+        s32 = (3 - po.nspin) * 2.0 / po.beta * f32.sum()
+        err = abs(s32 - s64) / abs(s64)
+        evals = nk * args.omega_nfreq
+        fl = logdet_flops(8, 2)  # two spin blocks of 8 (Nlat*Norb = 8)
+        log(f"Omega BHZ: {evals} (k,w) evals FP64 {t64:.3f}s (kernel {k64:.3f}s), MIXED "
+            f"{t32:.3f}s (kernel {k32:.3f}s); integral rel err mixed {err:.2e}; "
+            f"full solve {t_solve:.2f}s Omega={om64:.10f}")
+        a64 = evals * fl / k64 / 1e12
+        a32 = evals * fl / k32 / 1e12
+        return {
+            "workload": f"BHZ 2x2 Norb=2 Nspin=2 (2 spin blocks of 8), {nk} k x "
+                        f"{args.omega_nfreq} Matsubara frequencies, beta=200",
+            "evals": evals, "evals_per_s_fp64": evals / t64, "evals_per_s_mixed": evals / t32,
+            "kernel_seconds_fp64": k64, "kernel_seconds_mixed": k32,
+            "integral_rel_err_mixed": err, "omega_full_solve": om64,
+            "full_solve_seconds": t_solve,
+            "roofline": {"bound": "valu", "flops_per_eval": fl,
+                         "fp64": {"achieved": a64, "peak": FP64_PEAK_TFLOPS,
+                                  "unit": "TFLOP/s", "frac": a64 / FP64_PEAK_TFLOPS},
+                         "mixed": {"achieved": a32, "peak": FP32_PEAK_TFLOPS,
+                                   "unit": "TFLOP/s", "frac": a32 / FP32_PEAK_TFLOPS}},
+        }, (po, hlo, hko, so)
+
+    if args.only == "omega":
+        om, _ = omega_section()
+        print(json.dumps({"omega": om}), flush=True)
+        return
+
     p, hloc, hk, _, _ = configs.config_square_3x4(nk=2)
     s = VcaSolver(p)
     s.vca_set_hcluster(hloc)
-    out = {}
 
     # ---------------------------------------------------------- H x v ----
     def time_hxv(prec):
@@ -78,16 +156,14 @@ def main():
         torch.cuda.synchronize()
         sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=max(1, args.warmup))
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
         ms = sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=args.steps)
         torch.cuda.synchronize()
-        wall = time.perf_counter() - t0
-        return sec, ms, wall
+        return sec, ms
 
-    sec64, ms64, wall64 = time_hxv("fp64")
-    sec32, ms32, wall32 = time_hxv("mixed")
+    sec64, ms64 = time_hxv("fp64")
+    sec32, ms32 = time_hxv("mixed")
     dim = sec64.dim
-    log(f"H x v FP64: {ms64 * 1e3:.1f} us/apply; MIXED: {ms32 * 1e3:.1f} us/apply; dim {dim}")
+    log(f"H x v FP64: {ms64 * 1e3:.2f} us/apply; MIXED: {ms32 * 1e3:.2f} us/apply; dim {dim}")
     if args.only == "hxv":
         # calibration kernel for the HBM byte counters: a copy of 256 MiB
         a = torch.ones(1 << 25, dtype=torch.float64, device=dev)
@@ -108,33 +184,55 @@ def main():
     log(f"GS FP64 E0={e64:.14f} ({it64} it, {t_gs64:.2f}s); MIXED E0={emx:.14f} "
         f"(refine {itmx} it, {t_gsmx:.2f}s); dE0 rel {abs(emx - e64) / abs(e64):.2e}")
 
-    # --------------------------------------------------------- Omega -----
-    po, hlo, hko, _, _ = configs.config_bhz_2x2(nk=args.omega_nk, nfreq=args.omega_nfreq,
-                                               precision="fp64")
-    so = VcaSolver(po)
-    t0 = time.perf_counter()
-    om64 = so.vca_solve(hlo, hko)
-    t_solve_bhz = time.perf_counter() - t0
-    zs = 1j * (2 * np.arange(args.omega_nfreq) + 1) * np.pi / po.beta
-    nk = hko.shape[6]
-    so.omega_integrand(zs[:64], "fp64")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    f64 = so.omega_integrand(zs, "fp64")
-    t_om64 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    f32 = so.omega_integrand(zs, "mixed")
-    t_om32 = time.perf_counter() - t0
-    s64 = (3 - po.nspin) * 2.0 / po.beta * f64.sum()
-    s32 = (3 - po.nspin) * 2.0 / po.beta * f32.sum()
-    om_err = abs(s32 - s64) / abs(s64)
-    evals = nk * args.omega_nfreq
-    log(f"Omega BHZ: {evals} (k,w) evals FP64 {t_om64:.3f}s, MIXED {t_om32:.3f}s; "
-        f"integral rel err mixed {om_err:.2e}; full solve {t_solve_bhz:.2f}s, Omega={om64:.10f}")
+    omega, (po, hlo, hko, so) = omega_section()
+
+    # --------------------------------------------- end-to-end solves -----
+    def guarded(name, fn):
+        """Optional sections never cost the headline line: a failure is
+        logged and recorded instead."""
+        try:
+            return fn()
+        except Exception as exc:
+            log(f"{name} failed: {exc}")
+            return {"error": str(exc)}
+
+    def run_full():
+        pf, hlf, hkf, _, _ = configs.config_square_3x4(nk=64)
+        sf = VcaSolver(pf)
+        t0 = time.perf_counter()
+        omf = sf.vca_solve(hlf, hkf)
+        r = sf.results()
+        full = {"workload": "3x4 square cluster U=8, sector (6,6) ground state, GF chains "
+                            "on the N+-1 sectors (300 steps), Omega on the 64x64 k-grid",
+                "seconds": time.perf_counter() - t0, "omega": omf,
+                "hxv_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9,
+                **{k: r[k] for k in ("omegaprime", "omega_integral", "nstates", "nchains",
+                                     "hxv_count", "t_diag", "t_gf", "t_omega",
+                                     "omega_evals")}}
+        log(f"full 3x4 solve: {full['seconds']:.1f}s Omega={omf:.10f} "
+            f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, Omega {r['t_omega']:.2f}s, "
+            f"{r['nchains']} chains)")
+        return full
+
+    def run_c3():
+        pc, hlc, hkc, _, _ = configs.config_batched_2x5(nk=8)
+        sc = VcaSolver(pc)
+        t0 = time.perf_counter()
+        omc = sc.vca_solve(hlc, hkc)
+        r = sc.results()
+        c3 = {"workload": "2x5 cluster U=6, Nspin=2, every (Nup,Ndw) sector (121, dims "
+                          "1..63,504), up to 6 eigenpairs per sector (ARPACK-equivalent), "
+                          "all chains batched per target sector",
+              "seconds": time.perf_counter() - t0, "omega": omc,
+              "hxv_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9,
+              **{k: r[k] for k in ("omegaprime", "nstates", "nchains", "hxv_count",
+                                   "t_diag", "t_gf", "t_omega")}}
+        log(f"2x5 every-sector solve: {c3['seconds']:.1f}s Omega={omc:.10f} "
+            f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, {r['nchains']} chains)")
+        return c3
 
     # ------------------------------- 8 sites + 8 bath levels, 1.66e8 states
-    bath16 = None
-    if not args.no_bath16:
+    def run_bath16():
         pb, hlb, hkb, bhb, bvb = configs.config_bath_2x4(nk=2, precision="mixed")
         sb = VcaSolver(pb)
         sb.vca_set_bath(bhb, bvb)
@@ -154,31 +252,20 @@ def main():
                            "hbm_gbs_algorithmic": 2 * (16 if prec == "fp64" else 8) * sec.dim
                            / (ms * 1e-3) / 1e9}
             log(f"bath16 {prec}: {ms:.3f} ms/apply, dim {sec.dim}")
-            if prec == "mixed":
-                t0 = time.perf_counter()
-                e_mx, _, it_mx = sec.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
-                out16["gs_mixed"] = {"e0": e_mx, "seconds": time.perf_counter() - t0,
-                                     "refine_iters": it_mx}
-            else:
-                t0 = time.perf_counter()
-                e_64, _, it_64 = sec.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
-                out16["gs_fp64"] = {"e0": e_64, "seconds": time.perf_counter() - t0,
-                                    "iters": it_64}
+            t0 = time.perf_counter()
+            e_, _, it_ = sec.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
+            out16["gs_" + prec] = {"e0": e_, "seconds": time.perf_counter() - t0, "iters": it_}
             sec.close()
         out16["e0_rel_err_mixed"] = abs(out16["gs_mixed"]["e0"] - out16["gs_fp64"]["e0"]) / abs(
             out16["gs_fp64"]["e0"])
+        out16["gs_speedup_mixed"] = out16["gs_fp64"]["seconds"] / out16["gs_mixed"]["seconds"]
         out16["dim"] = 12870 * 12870
-        bath16 = out16
         log(f"bath16 GS: mixed {out16['gs_mixed']} fp64 {out16['gs_fp64']}")
+        return out16
 
-    full = None
-    if args.full:
-        pf, hlf, hkf, _, _ = configs.config_square_3x4(nk=64)
-        sf = VcaSolver(pf)
-        t0 = time.perf_counter()
-        omf = sf.vca_solve(hlf, hkf)
-        full = dict(seconds=time.perf_counter() - t0, omega=omf, **sf.results())
-        log(f"full 3x4 solve: {full['seconds']:.1f}s Omega={omf:.10f}")
+    full = None if args.no_full else guarded("full 3x4 solve", run_full)
+    c3 = None if args.no_c3 else guarded("2x5 every-sector solve", run_c3)
+    bath16 = None if args.no_bath16 else guarded("bath16", run_bath16)
 
     # ---------------------------------------------------- CPU baseline ---
     cpu = None
@@ -187,22 +274,58 @@ def main():
         from tests._models import to_oracle
         om = R.Model(to_oracle(p), hloc)
         v = np.random.default_rng(3).normal(size=dim).astype(np.complex128)
-        n = 0
-        t0 = time.perf_counter()
-        while True:
-            om.hxv(6, 6, v)
-            n += 1
-            if time.perf_counter() - t0 > 10.0 or n >= 20:
-                break
-        tcpu = (time.perf_counter() - t0) / n
-        cpu = {"value": dim / tcpu / 1e9, "unit": "Gstates/s", "cores": 1, "kind": "port",
-               "sample": f"{n} H x v applications on the 853,776-state sector (6,6), "
-                         f"oracle C restatement of the reference loops, -O2 -funroll-loops"}
-        log(f"CPU 1-core: {tcpu:.3f} s/apply")
+
+        def timed(workers, budget):
+            n, t0 = 0, time.perf_counter()
+            while True:
+                om.hxv(6, 6, v, workers=workers)
+                n += 1
+                if time.perf_counter() - t0 > budget or n >= 20:
+                    break
+            return n, (time.perf_counter() - t0) / n
+
+        n1, t1 = timed(0, 10.0)
+        ncores = len(os.sched_getaffinity(0))
+        threads = int(os.environ.get("OMP_NUM_THREADS", ncores))
+        threads = max(1, min(threads, ncores))
+        nn, tn = timed(threads, 5.0)
+        cpu_om = None
+        try:
+            # the per-k work of the Omega integrand on the CPU: for each
+            # sampled frequency M_k = 1 + (Hloc - H_k) G'(z) and log|det M_k|
+            # over the 128x128 k-grid (numpy batched matmul + slogdet), with
+            # G'(z) from the solved BHZ cluster
+            po_o = to_oracle(po)
+            hkl = R.hk_to_lso(po_o, hko)
+            hll = R.nnn2lso(hlo, po.nlat, po.nspin, po.norb)
+            zsamp = 1j * (2 * np.arange(4) + 1) * np.pi / po.beta
+            gz = [R.nnn2lso(so.vca_gf_cluster(z), po.nlat, po.nspin, po.norb) for z in zsamp]
+            t0 = time.perf_counter()
+            for g in gz:
+                m = np.eye(po.nlso)[None] + (hll[None] - hkl) @ g[None]
+                np.mean(np.linalg.slogdet(m)[1])
+            tz = (time.perf_counter() - t0) / len(zsamp)
+            cpu_om = {"evals_per_s": hkl.shape[0] / tz, "threads": "numpy default",
+                      "sample": f"{len(zsamp)} frequencies x {hkl.shape[0]} k-points of the "
+                                "BHZ Omega integrand (16x16 contraction + slogdet per k)"}
+        except Exception as exc:  # the CPU Omega sample is informative only
+            log(f"CPU Omega sample skipped: {exc}")
+        cpu = {"value": dim / t1 / 1e9, "unit": "Gstates/s", "cores": 1, "kind": "port",
+               "sample": f"{n1} H x v applications on the 853,776-state sector (6,6), oracle C "
+                         f"restatement of the reference loops (-O2 -funroll-loops), one core",
+               "all_cores": {"value": dim / tn / 1e9, "cores": threads,
+                             "sample": f"{nn} applications, DW-slice split over {threads} "
+                                       "OpenMP threads (the reference's MPI decomposition)"},
+               "omega": cpu_om}
+        log(f"CPU 1-core: {t1:.3f} s/apply; {threads} threads: {tn:.3f} s/apply")
 
     gst64 = dim / (ms64 * 1e-3) / 1e9
     gst32 = dim / (ms32 * 1e-3) / 1e9
     bytes64 = 2 * 16 * dim
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("traffic_bytes")
     line = {
         "metric": "H×v Gstates/s + Ω evals/s, mixed vs FP64 on 1 GPU at ΔE0<1e-10",
         "value": gst64,
@@ -215,23 +338,25 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded random vectors; Hubbard model inputs generated in-repo)",
+        "data": "synthetic (seeded random vectors; Hubbard/BHZ/bath model inputs generated "
+                "in-repo)",
         "config": {"workload": "hubbard 3x4 square cluster U=8 half-filled sector (6,6), "
                                "853,776 states, H x v FP64", "sector_dim": dim},
         "mixed": {"value": gst32, "ms_per_step": ms32, "dtype": "f32 (complex64)"},
         "ground_state": {"e0_fp64": e64, "iters_fp64": it64, "seconds_fp64": t_gs64,
                          "e0_mixed": emx, "seconds_mixed": t_gsmx,
                          "e0_rel_err_mixed": abs(emx - e64) / abs(e64)},
-        "omega": {"workload": f"BHZ 2x2 Norb=2 Nspin=2, {nk} k x {args.omega_nfreq} Matsubara",
-                  "evals": evals, "evals_per_s_fp64": evals / t_om64,
-                  "evals_per_s_mixed": evals / t_om32, "integral_rel_err_mixed": om_err},
+        "omega": omega,
         "roofline": {"bound": "hbm", "achieved": bytes64 / (ms64 * 1e-3) / 1e9,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": bytes64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None},
+                     "frac": bytes64 / (ms64 * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic},
         "cpu_baseline": cpu,
     }
     if full:
         line["full_solve"] = full
+    if c3:
+        line["batched_2x5"] = c3
     if bath16:
         line["bath16"] = bath16
     print(json.dumps(line), flush=True)

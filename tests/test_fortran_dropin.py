@@ -48,3 +48,29 @@ def test_fortran_driver_matches_python_mirror(tmp_path):
     sft_py = VcaSolver(p).vca_solve(hloc, hk)
     assert abs(sft_f - sft_py) < 1e-12 * abs(sft_py)
     assert (tmp_path / "SFT_potential.vca").exists()
+
+
+@needs_fc
+def test_fortran_struct_layouts_match_the_c_abi(tmp_path):
+    """The bind(C) mirrors of vca_params and vca_results have the C sizes
+    (the ctypes mirrors are the same structs)."""
+    import ctypes
+
+    from vca_amd import _lib
+
+    mod = tmp_path / "mod"
+    mod.mkdir()
+    obj = tmp_path / "vca_gpu.o"
+    subprocess.check_call([FC, "-c", "-fPIC", f"-J{mod}", os.path.join(FDIR, "vca_gpu.f90"),
+                           "-o", str(obj)])
+    src = tmp_path / "sizes.f90"
+    src.write_text("program sizes\n  use iso_c_binding\n  use vca_gpu\n"
+                   "  type(vca_params_t) :: p\n  type(vca_results_t) :: r\n"
+                   "  print '(I0,1X,I0)', c_sizeof(p), c_sizeof(r)\nend program sizes\n")
+    exe = tmp_path / "sizes"
+    libdir = os.path.join(ROOT, "vca_amd")
+    subprocess.check_call([FC, f"-I{mod}", str(src), str(obj), f"-L{libdir}", "-lvca_gpu",
+                           f"-Wl,-rpath,{libdir}", "-o", str(exe)])
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60).stdout.split()
+    assert int(out[0]) == ctypes.sizeof(_lib.CParams)
+    assert int(out[1]) == ctypes.sizeof(_lib.CResults)

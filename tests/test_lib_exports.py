@@ -51,3 +51,24 @@ def test_errors_are_reported_not_aborted():
     assert b"nspin" in lib.vca_last_error()
     rc = lib.vca_get_sft_potential(None, None)
     assert rc == -1
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """sizeof(vca_params) / sizeof(vca_results) of include/vca_gpu.h equal
+    the ctypes mirrors (compiled with the host C compiler)."""
+    import ctypes
+    import os
+    import subprocess
+
+    from vca_amd import _lib
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include "vca_gpu.h"\n'
+                   'int main(void){printf("%zu %zu\\n", sizeof(vca_params), '
+                   'sizeof(vca_results));return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.check_call(["gcc", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)])
+    a, b = subprocess.run([str(exe)], capture_output=True, text=True).stdout.split()
+    assert int(a) == ctypes.sizeof(_lib.CParams)
+    assert int(b) == ctypes.sizeof(_lib.CResults)

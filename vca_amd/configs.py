@@ -176,7 +176,8 @@ def config_square_3x4(nk=64, ngfiter=300):
 
 def config_batched_2x5(nk=16, seed_eps=3001, seed_h=3002):
     """10-site 2x5 cluster, U=6, Nspin=2, random on-site field and Weiss AF
-    field in Hloc only; every sector."""
+    field in Hloc only; every sector, LANC_METHOD=arpack (the reference
+    default: up to LANC_NSTATES_SECTOR=6 eigenpairs per sector)."""
     eps = np.random.default_rng(seed_eps).uniform(-0.2, 0.2, 10)
     h = np.random.default_rng(seed_h).uniform(0.0, 0.3)
     onsite = np.zeros((10, 2))
@@ -186,7 +187,7 @@ def config_batched_2x5(nk=16, seed_eps=3001, seed_h=3002):
             stag = 1.0 if (ix + iy) % 2 == 0 else -1.0
             onsite[i, 0] = eps[i] - stag * h
             onsite[i, 1] = eps[i] + stag * h
-    p = VcaParams(nlat=10, nspin=2, uloc=(6.0, 0, 0, 0, 0), lanc_method="lanczos")
+    p = VcaParams(nlat=10, nspin=2, uloc=(6.0, 0, 0, 0, 0), lanc_method="arpack")
     hloc, _ = square_lattice(2, 5, 2, -1.0, onsite, nk=(1, 1))
     _, hk = square_lattice(2, 5, 2, -1.0, np.zeros((10, 2)), nk=(nk, nk))
     return p, hloc, hk, None, None

@@ -50,6 +50,17 @@ module vca_gpu
      real(c_double)     :: cutoff, bandwidth
   end type vca_params_t
 
+  ! vca_results of include/vca_gpu.h (same field order and kinds)
+  type, bind(C) :: vca_results_t
+     real(c_double)     :: sft_potential, omegaprime, omega_integral, e0_mixed
+     integer(c_int32_t) :: nstates, nchains
+     integer(c_int64_t) :: hxv_count
+     real(c_double)     :: hxv_states, hxv_seconds, t_diag, t_gf, t_omega
+     integer(c_int64_t) :: omega_evals
+     real(c_double)     :: omega_seconds, zeta, max_exc
+     integer(c_int32_t) :: finite_t, nmax, spectrum_iters
+  end type vca_results_t
+
   interface
      subroutine c_default_params(p) bind(C, name="vca_default_params")
        import :: vca_params_t
@@ -81,6 +92,12 @@ module vca_gpu
        real(c_double), intent(out) :: out
        integer(c_int) :: rc
      end function c_get_sft
+     function c_get_results(s, out) bind(C, name="vca_get_results") result(rc)
+       import :: c_ptr, c_int, vca_results_t
+       type(c_ptr), value :: s
+       type(vca_results_t), intent(out) :: out
+       integer(c_int) :: rc
+     end function c_get_results
   end interface
 
   type(c_ptr), save :: handle = c_null_ptr
@@ -99,6 +116,7 @@ module vca_gpu
   end interface vca_solve
 
   public :: vca_init_solver, vca_solve, vca_get_sft_potential, vca_finalize_solver
+  public :: vca_params_t, vca_results_t
 
 contains
 
@@ -179,6 +197,7 @@ contains
     type(c_ptr) :: pbh, pbv
     integer :: unit
     real(8) :: sft
+    type(vca_results_t) :: res
     if (.not. c_associated(handle)) call vca_init_solver_serial()
     if (any(shape(Hloc) /= [Nlat, Nlat, Nspin, Nspin, Norb, Norb])) &
          stop "STOP: wrong cluster matrix dimensions"
@@ -197,10 +216,12 @@ contains
     call check(c_solve(handle, c_loc(Hloc), c_loc(Hk), int(size(Hk, 7), c_int64_t), pbh, pbv), &
          "vca_solve")
     call check(c_get_sft(handle, sft), "vca_get_sft_potential")
+    call check(c_get_results(handle, res), "vca_get_results")
     last_sft = sft
+    last_omegaprime = res%omegaprime
     ! SFT_potential.vca, VCA_MAIN.f90:199-203 (sft, omegaprime, -integral)
     open(newunit=unit, file="SFT_potential.vca", position="append")
-    write(unit, *) sft
+    write(unit, *) sft, res%omegaprime, -res%omega_integral
     close(unit)
   end subroutine vca_solve_serial
 

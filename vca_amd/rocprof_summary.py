@@ -1,0 +1,92 @@
+"""Summaries of rocprofv3 result databases (kernel trace and PMC passes).
+
+    python -m vca_amd.rocprof_summary trace  TRACE.db
+    python -m vca_amd.rocprof_summary pmc    FETCH.db WRITE.db KERNEL_SUBSTRING COPY_BYTES OUT.json
+
+`trace` prints the per-kernel table (calls, average and total duration).
+`pmc` reads a FETCH_SIZE and a WRITE_SIZE pass (KiB per dispatch), finds
+the calibration copy kernel of COPY_BYTES bytes each way (the largest
+__amd_rocclr_copyBuffer / elementwise copy dispatch) to turn the counters
+into bytes, and writes the HBM traffic per dispatch of the kernel whose
+name contains KERNEL_SUBSTRING (the most-dispatched match) to OUT.json.
+"""
+from __future__ import annotations
+
+import json
+import re
+import sqlite3
+import sys
+
+
+def _short(name: str) -> str:
+    name = re.sub(r"\(vca::\w+\)$", "", name)
+    return name if len(name) < 80 else name[:77] + "..."
+
+
+def trace_table(db: str, top: int = 14) -> str:
+    c = sqlite3.connect(db)
+    rows = c.execute("select name, total_calls, average, total_duration, percentage "
+                     "from top_kernels order by total_duration desc limit ?", (top,)).fetchall()
+    out = ["| kernel | calls | avg us | total us | % |", "|---|---:|---:|---:|---:|"]
+    for r in rows:
+        out.append(f"| `{_short(r[0])}` | {r[1]} | {r[2]:.2f} | {r[3]:.1f} | {r[4]:.1f} |")
+    return "\n".join(out)
+
+
+def _per_kernel(db: str, counter: str) -> dict:
+    c = sqlite3.connect(db)
+    res = {}
+    for name, n, avg, mx in c.execute(
+            "select kernel_name, count(*), avg(value), max(value) from counters_collection "
+            "where counter_name=? group by kernel_name", (counter,)):
+        res[name] = (n, avg, mx)
+    return res
+
+
+def pmc_traffic(fetch_db: str, write_db: str, kernel: str, copy_bytes: int) -> dict:
+    fetch = _per_kernel(fetch_db, "FETCH_SIZE")
+    write = _per_kernel(write_db, "WRITE_SIZE")
+    # calibration: the copy dispatch with the largest counts
+    copies = [k for k in fetch if "copy" in k.lower()]
+    if not copies:
+        raise SystemExit("no copy kernel in the FETCH_SIZE pass")
+    ck = max(copies, key=lambda k: fetch[k][2])
+    cf_kib = fetch[ck][2]
+    cw_kib = write.get(ck, (0, 0, 0))[2]
+    f_scale = copy_bytes / (cf_kib * 1024.0)
+    w_scale = copy_bytes / (cw_kib * 1024.0) if cw_kib else None
+    cand = [k for k in fetch if kernel in k]
+    if not cand:
+        raise SystemExit(f"no kernel matching {kernel!r}")
+    kn = max(cand, key=lambda k: fetch[k][0])
+    f_kib = fetch[kn][1]
+    w_kib = write[kn][1]
+    return {
+        "kernel": kn,
+        "dispatches": fetch[kn][0],
+        "fetch_kib_per_dispatch": f_kib,
+        "write_kib_per_dispatch": w_kib,
+        "calibration_kernel": ck,
+        "calibration_bytes_each_way": copy_bytes,
+        "fetch_scale": f_scale,
+        "write_scale": w_scale,
+        "traffic_bytes": f_kib * 1024.0 * f_scale + w_kib * 1024.0 * (w_scale or 1.0),
+    }
+
+
+def main(argv):
+    if len(argv) >= 2 and argv[0] == "trace":
+        print(trace_table(argv[1]))
+    elif len(argv) >= 6 and argv[0] == "pmc":
+        d = pmc_traffic(argv[1], argv[2], argv[3], int(argv[4]))
+        with open(argv[5], "w") as f:
+            json.dump(d, f, indent=1)
+        print(json.dumps(d, indent=1))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv[1:]))
