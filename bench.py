@@ -60,6 +60,30 @@ def logdet_flops(nb: int, nblk: int) -> float:
     return nblk * (8.0 * nb ** 3 + 8.0 * nb ** 3 / 3.0)
 
 
+def run_full():
+    """vca_solve of the 3x4 cluster end to end."""
+    from vca_amd import configs
+    from vca_amd.api import VcaSolver
+
+    pf, hlf, hkf, _, _ = configs.config_square_3x4(nk=64)
+    sf = VcaSolver(pf)
+    t0 = time.perf_counter()
+    omf = sf.vca_solve(hlf, hkf)
+    r = sf.results()
+    full = {"workload": "3x4 square cluster U=8, sector (6,6) ground state, GF chains "
+                        "on the N+-1 sectors (300 steps), Omega on the 64x64 k-grid",
+            "seconds": time.perf_counter() - t0, "omega": omf,
+            "lanczos_step_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30)
+            / 1e9,
+            **{k: r[k] for k in ("omegaprime", "omega_integral", "nstates", "nchains",
+                                 "hxv_count", "t_diag", "t_gf", "t_omega",
+                                 "omega_evals")}}
+    log(f"full 3x4 solve: {full['seconds']:.1f}s Omega={omf:.10f} "
+        f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, Omega {r['t_omega']:.2f}s, "
+        f"{r['nchains']} chains)")
+    return full
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,9 +96,10 @@ def main():
                     help="skip the 8-site + 8-bath-level sector (165,636,900 states)")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
-    ap.add_argument("--only", choices=["all", "hxv", "omega"], default="all",
+    ap.add_argument("--only", choices=["all", "hxv", "omega", "full"], default="all",
                     help="hxv: the H x v timing plus a 256 MiB copy that calibrates the HBM "
-                         "byte counters; omega: the Omega integrand timing (rocprofv3 runs)")
+                         "byte counters; omega: the Omega integrand timing; full: the "
+                         "end-to-end 3x4 solve (rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would "
@@ -141,6 +166,9 @@ def main():
         om, _ = omega_section()
         print(json.dumps({"omega": om}), flush=True)
         return
+    if args.only == "full":
+        print(json.dumps({"full_solve": run_full()}), flush=True)
+        return
 
     p, hloc, hk, _, _ = configs.config_square_3x4(nk=2)
     s = VcaSolver(p)
@@ -196,24 +224,6 @@ def main():
             log(f"{name} failed: {exc}")
             return {"error": str(exc)}
 
-    def run_full():
-        pf, hlf, hkf, _, _ = configs.config_square_3x4(nk=64)
-        sf = VcaSolver(pf)
-        t0 = time.perf_counter()
-        omf = sf.vca_solve(hlf, hkf)
-        r = sf.results()
-        full = {"workload": "3x4 square cluster U=8, sector (6,6) ground state, GF chains "
-                            "on the N+-1 sectors (300 steps), Omega on the 64x64 k-grid",
-                "seconds": time.perf_counter() - t0, "omega": omf,
-                "hxv_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9,
-                **{k: r[k] for k in ("omegaprime", "omega_integral", "nstates", "nchains",
-                                     "hxv_count", "t_diag", "t_gf", "t_omega",
-                                     "omega_evals")}}
-        log(f"full 3x4 solve: {full['seconds']:.1f}s Omega={omf:.10f} "
-            f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, Omega {r['t_omega']:.2f}s, "
-            f"{r['nchains']} chains)")
-        return full
-
     def run_c3():
         pc, hlc, hkc, _, _ = configs.config_batched_2x5(nk=8)
         sc = VcaSolver(pc)
@@ -224,7 +234,8 @@ def main():
                           "1..63,504), up to 6 eigenpairs per sector (ARPACK-equivalent), "
                           "all chains batched per target sector",
               "seconds": time.perf_counter() - t0, "omega": omc,
-              "hxv_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9,
+              "lanczos_step_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30)
+              / 1e9,
               **{k: r[k] for k in ("omegaprime", "nstates", "nchains", "hxv_count",
                                    "t_diag", "t_gf", "t_omega")}}
         log(f"2x5 every-sector solve: {c3['seconds']:.1f}s Omega={omc:.10f} "
@@ -238,7 +249,9 @@ def main():
         sb.vca_set_bath(bhb, bvb)
         sb.vca_set_hcluster(hlb)
         out16 = {}
-        for prec in ("mixed", "fp64"):
+        # FP64 first: the MIXED ground state builds an FP64 twin sector for
+        # its refinement, whose launch shape is then already tuned
+        for prec in ("fp64", "mixed"):
             sec = sb.sector(8, 8, prec)
             dt = torch.complex128 if prec == "fp64" else torch.complex64
             g = torch.Generator(device=dev).manual_seed(2)

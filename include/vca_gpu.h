@@ -114,9 +114,10 @@ typedef struct vca_results {
   double e0_mixed;       /* MIXED: E0 before FP64 refinement              */
   int32_t nstates;       /* ground states kept (zeta function)            */
   int32_t nchains;       /* excitation chains run                         */
-  int64_t hxv_count;     /* H x v applications (all precisions)           */
-  double hxv_states;     /* sum over applications of the sector dimension */
-  double hxv_seconds;    /* GPU time in H x v kernels (events)            */
+  int64_t hxv_count;     /* Lanczos steps (H x v applications), all chains */
+  double hxv_states;     /* sum over those steps of the sector dimension  */
+  double hxv_seconds;    /* GPU time of those steps (H x v + vector update
+                            kernels, events)                              */
   double t_diag, t_gf, t_omega;
   int64_t omega_evals;   /* (k, omega) log-det evaluations                */
   double omega_seconds;  /* GPU time in the k x omega kernels             */

@@ -58,32 +58,22 @@ __device__ inline double block_sum(double v) {
   return r;
 }
 
-// Grid-wide deterministic sum: every block publishes its partial, the last
-// block to arrive sums all partials in block order.  Returns true in thread
-// 0 of the last block, with the total in *total.
-__device__ inline bool grid_sum(double v, double *partials, uint32_t *counter,
-                                double *total) {
-  __shared__ bool s_last;
-  double b = block_sum(v);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = b;
-    __threadfence();
-    uint32_t prev = atomicAdd(counter, 1u);
-    s_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
+// Grid-wide sums are two-stage: every workgroup stores its partial with a
+// plain store, and a kernel that runs after it (same stream) sums the
+// partials in block order.  No device-scope fence inside a kernel: on
+// MI355X the XCD L2s are not coherent with each other, so such a fence
+// writes the L2 back, and one per workgroup stalls large grids.
+__device__ inline void block_partial(double v, double *partials) {
+  const double b = block_sum(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+}
+
+// Sum of nblk partials in block order (deterministic), valid in thread 0
+// of a kBlock-lane workgroup.
+__device__ inline double sum_partials(const double *p, int nblk) {
   double s = 0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += kBlock)
-    s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  s = block_sum(s);
-  if (threadIdx.x == 0) {
-    *total = s;
-    *counter = 0u;
-    return true;
-  }
-  return false;
+  for (int i = threadIdx.x; i < nblk; i += kBlock) s += p[i];
+  return block_sum(s);
 }
 
 // ------------------------------------------------------------ tables ----
@@ -234,11 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_hxv(const HxvArgs a) {
       if (lz) dot += (double)(x.x * scale) * (double)out.x + (double)(x.y * scale) * (double)out.y;
     }
   }
-  if (lz) {
-    double tot;
-    if (grid_sum(dot, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[0], &tot))
-      lz->alpha[k] = done ? 0.0 : tot;
-  }
+  if (lz) block_partial(dot, a.partials + (int64_t)blockIdx.y * gridDim.x);
 }
 
 // Block sum for a block of BS threads, result valid in thread 0.
@@ -257,29 +243,9 @@ __device__ inline double block_sum_t(double v) {
 }
 
 template <int BS>
-__device__ inline bool grid_sum_t(double v, double *partials, uint32_t *counter,
-                                  double *total) {
-  __shared__ bool s_last;
-  double b = block_sum_t<BS>(v);
-  if (threadIdx.x == 0) {
-    partials[blockIdx.x] = b;
-    __threadfence();
-    uint32_t prev = atomicAdd(counter, 1u);
-    s_last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
-  double s = 0;
-  for (int i = threadIdx.x; i < (int)gridDim.x; i += BS)
-    s += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  s = block_sum_t<BS>(s);
-  if (threadIdx.x == 0) {
-    *total = s;
-    *counter = 0u;
-    return true;
-  }
-  return false;
+__device__ inline void block_partial_t(double v, double *partials) {
+  const double b = block_sum_t<BS>(v);
+  if (threadIdx.x == 0) partials[blockIdx.x] = b;
 }
 
 // Workgroup b -> output column.  Workgroups are dealt round-robin to the 8
@@ -422,12 +388,7 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
       }
     }
   }
-  if (lz) {
-    double tot;
-    if (grid_sum_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[0],
-                       &tot))
-      lz->alpha[k] = done ? 0.0 : tot;
-  }
+  if (lz) block_partial_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x);
 }
 
 // Buffer-load form of the one-column kernel.  The same arithmetic as
@@ -560,12 +521,7 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
       }
     }
   }
-  if (lz) {
-    double tot;
-    if (grid_sum_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[0],
-                       &tot))
-      lz->alpha[k] = done ? 0.0 : tot;
-  }
+  if (lz) block_partial_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x);
 }
 
 // ------------------------------------------------------ Lanczos update --
@@ -617,24 +573,57 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_update(const UpdArgs a) {
       }
     }
   }
-  double tot;
-  if (grid_sum(ss, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[1], &tot)) {
-    if (!done) {
-      const double b = sqrt(tot);
-      if (!(b > 1e-300)) {  // exact breakdown (or NaN)
-        lz->beta[k + 1] = 0.0;
-        lz->nrm[k + 1] = 1.0;
-        lz->done = 1;
-      } else {
-        lz->beta[k + 1] = b;
-        lz->nrm[k + 1] = b;
-      }
+  block_partial(ss, a.partials + (int64_t)blockIdx.y * gridDim.x);
+}
+
+// Per-chain finishers of the two-stage sums (one workgroup per chain).
+// alpha[k] = <v_k|H v_k> from the H x v partials.
+__global__ __launch_bounds__(kBlock) void k_lz_alpha(LanczosDev *lz, const double *partials,
+                                                     int nblk) {
+  LanczosDev *c = lz + blockIdx.x;
+  const double s = sum_partials(partials + (int64_t)blockIdx.x * nblk, nblk);
+  if (threadIdx.x == 0) c->alpha[c->k] = c->done ? 0.0 : s;
+}
+
+// beta_{k+1} = |u_{k+1}| from the update partials, then k -> k+1; an exact
+// breakdown (or NaN) ends the chain.
+__global__ __launch_bounds__(kBlock) void k_lz_beta(LanczosDev *lz, const double *partials,
+                                                    int nblk) {
+  LanczosDev *c = lz + blockIdx.x;
+  const double s = sum_partials(partials + (int64_t)blockIdx.x * nblk, nblk);
+  if (threadIdx.x == 0) {
+    const int k = c->k;
+    const double b = sqrt(s);
+    if (c->done || !(b > 1e-300)) {
+      c->beta[k + 1] = 0.0;
+      c->nrm[k + 1] = 1.0;
+      c->done = 1;
     } else {
-      lz->beta[k + 1] = 0.0;
-      lz->nrm[k + 1] = 1.0;
+      c->beta[k + 1] = b;
+      c->nrm[k + 1] = b;
     }
-    lz->k = k + 1;
+    c->k = k + 1;
   }
+}
+
+// Start of a chain: nrm[0] = |u_0|, beta[0] = 0, k = 0, not done.
+__global__ __launch_bounds__(kBlock) void k_lz_start(LanczosDev *lz, const double *partials,
+                                                     int nblk) {
+  LanczosDev *c = lz + blockIdx.x;
+  const double s = sum_partials(partials + (int64_t)blockIdx.x * nblk, nblk);
+  if (threadIdx.x == 0) {
+    c->nrm[0] = sqrt(s);
+    c->beta[0] = 0.0;
+    c->k = 0;
+    c->done = 0;
+  }
+}
+
+// y[0] = |x| (the eigenvector norm, y is free after the second pass).
+__global__ __launch_bounds__(kBlock) void k_lz_xnorm(LanczosDev *lz, const double *partials,
+                                                     int nblk) {
+  const double s = sum_partials(partials, nblk);
+  if (threadIdx.x == 0) lz->y[0] = sqrt(s);
 }
 
 // ------------------------------------------------------ vector helpers --
@@ -674,13 +663,8 @@ __global__ __launch_bounds__(kBlock) void k_norm0(const void *u, int64_t n, Lanc
     const C c = p[i];
     ss += (double)c.x * c.x + (double)c.y * c.y;
   }
-  double tot;
-  if (grid_sum(ss, partials, &lz->counter[2], &tot)) {
-    lz->nrm[0] = sqrt(tot);
-    lz->beta[0] = 0.0;
-    lz->k = 0;
-    lz->done = 0;
-  }
+  (void)lz;
+  block_partial(ss, partials);
 }
 
 // x /= |x|
@@ -695,9 +679,9 @@ __global__ __launch_bounds__(kBlock) void k_normalize(void *u, int64_t n, Lanczo
     const C c = p[i];
     ss += (double)c.x * c.x + (double)c.y * c.y;
   }
-  // the last block publishes the norm in y[0]; k_scale_by then divides
-  double tot;
-  if (grid_sum(ss, partials, &lz->counter[3], &tot)) lz->y[0] = sqrt(tot);
+  // k_lz_xnorm puts the norm in y[0]; k_scale_by then divides
+  (void)lz;
+  block_partial(ss, partials);
 }
 
 template <typename R>
@@ -787,13 +771,8 @@ __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
     out[j] = c;
     ss += (double)c.x * c.x + (double)c.y * c.y;
   }
-  double tot;
-  if (grid_sum(ss, a.partials + (int64_t)blockIdx.y * gridDim.x, &lz->counter[2], &tot)) {
-    lz->nrm[0] = sqrt(tot);
-    lz->beta[0] = 0.0;
-    lz->k = 0;
-    lz->done = 0;
-  }
+  (void)lz;
+  block_partial(ss, a.partials + (int64_t)blockIdx.y * gridDim.x);
 }
 
 // ================================================== host launch wrappers
@@ -1057,6 +1036,11 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
       else launch_col_t<float, false>(sec, a, stage, buf, shp, g3, smem);
     }
     VCA_HIP(hipGetLastError());
+    if (lanczos) {
+      hipLaunchKernelGGL(k_lz_alpha, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz,
+                         sec.partials, grid);
+      VCA_HIP(hipGetLastError());
+    }
     return;
   }
   const size_t amp_bytes = (size_t)(a.namp_up + a.namp_dw) * esz;
@@ -1075,6 +1059,11 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
     else launch_hxv_t<float, false>(sec, a, stage, onecol, g3, smem);
   }
   VCA_HIP(hipGetLastError());
+  if (lanczos) {
+    hipLaunchKernelGGL(k_lz_alpha, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                       grid);
+    VCA_HIP(hipGetLastError());
+  }
 }
 
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
@@ -1093,6 +1082,9 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
     hipLaunchKernelGGL(k_lanczos_update<double>, dim3(grid, nb), dim3(kBlock), 0, sec.stream, a);
   else
     hipLaunchKernelGGL(k_lanczos_update<float>, dim3(grid, nb), dim3(kBlock), 0, sec.stream, a);
+  VCA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_lz_beta, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                     grid);
   VCA_HIP(hipGetLastError());
 }
 
@@ -1117,6 +1109,9 @@ void launch_norm0(Sector &sec) {
     hipLaunchKernelGGL(k_norm0<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
                        sec.dim, sec.lz, sec.partials);
   VCA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_lz_start, dim3(1), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                     grid);
+  VCA_HIP(hipGetLastError());
 }
 
 void launch_scale_x(Sector &sec) {
@@ -1127,11 +1122,15 @@ void launch_scale_x(Sector &sec) {
   if (sec.precision == VCA_FP64) {
     hipLaunchKernelGGL(k_normalize<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
                        sec.dim, sec.lz, sec.partials);
+    hipLaunchKernelGGL(k_lz_xnorm, dim3(1), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                       grid);
     hipLaunchKernelGGL(k_scale_by<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
                        sec.dim, den);
   } else {
     hipLaunchKernelGGL(k_normalize<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
                        sec.dim, sec.lz, sec.partials);
+    hipLaunchKernelGGL(k_lz_xnorm, dim3(1), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                       grid);
     hipLaunchKernelGGL(k_scale_by<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
                        sec.dim, den);
   }
@@ -1174,6 +1173,9 @@ void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int 
     hipLaunchKernelGGL(k_excite<double>, dim3(grid, nb), dim3(kBlock), 0, jsec.stream, a);
   else
     hipLaunchKernelGGL(k_excite<float>, dim3(grid, nb), dim3(kBlock), 0, jsec.stream, a);
+  VCA_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_lz_start, dim3(nb), dim3(kBlock), 0, jsec.stream, jsec.lz, jsec.partials,
+                     grid);
   VCA_HIP(hipGetLastError());
 }
 

@@ -16,6 +16,10 @@
 
 namespace vca {
 
+// sqrt(a^2 + b^2) without hypot's overflow guards: the tridiagonal
+// matrices here hold Hamiltonian-scale numbers (far from 1e+-150).
+static inline double pyth(double a, double b) { return std::sqrt(a * a + b * b); }
+
 // QL with implicit Wilkinson-type shifts on d[0..n), off-diagonal f[0..n)
 // with f[i] coupling i and i+1 (f[n-1] = 0).  If z != null it is updated
 // with the rotations (column-major, ldz rows, nz rows to update).
@@ -32,7 +36,7 @@ static void ql_implicit(int n, double *d, double *f, double *z, int nz, int ldz)
       if (m != l) {
         if (++iter > 60) break;  // no convergence: leave as is
         double g = (d[l + 1] - d[l]) / (2.0 * f[l]);
-        double r = std::hypot(g, 1.0);
+        double r = pyth(g, 1.0);
         g = d[m] - d[l] + f[l] / (g + (g >= 0 ? std::fabs(r) : -std::fabs(r)));
         double s = 1.0, c = 1.0, p = 0.0;
         int i;
@@ -40,7 +44,7 @@ static void ql_implicit(int n, double *d, double *f, double *z, int nz, int ldz)
         for (i = m - 1; i >= l; --i) {
           double ff = s * f[i];
           double b = c * f[i];
-          r = std::hypot(ff, g);
+          r = pyth(ff, g);
           f[i + 1] = r;
           if (r == 0.0) {
             d[i + 1] -= p;
@@ -89,6 +93,23 @@ void tridiag_eig(int n, const double *d, const double *e, double *w, double *z) 
     w[i] = dd[idx[i]];
     if (z)
       for (int k = 0; k < n; ++k) z[(size_t)i * n + k] = zz[(size_t)idx[i] * n + k];
+  }
+}
+
+void tridiag_eig_first(int n, const double *d, const double *e, double *w, double *z0) {
+  std::vector<double> dd(d, d + n), ff(n, 0.0);
+  for (int i = 0; i + 1 < n; ++i) ff[i] = e[i + 1];
+  // row 0 of the identity, carried through the same rotations: O(n) per
+  // sweep instead of O(n^2) for the whole eigenvector matrix
+  std::vector<double> zr(n, 0.0);
+  zr[0] = 1.0;
+  ql_implicit(n, dd.data(), ff.data(), zr.data(), 1, 1);
+  std::vector<int> idx(n);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return dd[a] < dd[b]; });
+  for (int i = 0; i < n; ++i) {
+    w[i] = dd[idx[i]];
+    z0[i] = zr[idx[i]];
   }
 }
 

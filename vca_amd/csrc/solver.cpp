@@ -26,6 +26,7 @@
 #include <tuple>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "omega.hpp"
@@ -405,6 +406,12 @@ static void run_steps(Sector &sec, int nsteps, bool accx, Solver *s, int nb = 1)
   const int chunk = 16;
   ensure_step_partials(sec, nb);
   int done = 0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (s) {
+    VCA_HIP(hipEventCreate(&e0));
+    VCA_HIP(hipEventCreate(&e1));
+    VCA_HIP(hipEventRecord(e0, sec.stream));
+  }
   if (nsteps >= chunk && graphs_enabled()) {
     hipGraphExec_t ex = step_graph(sec, chunk, nb, accx);
     for (; done + chunk <= nsteps; done += chunk) VCA_HIP(hipGraphLaunch(ex, sec.stream));
@@ -414,6 +421,13 @@ static void run_steps(Sector &sec, int nsteps, bool accx, Solver *s, int nb = 1)
     launch_lanczos_update(sec, accx, nb);
   }
   if (s) {
+    VCA_HIP(hipEventRecord(e1, sec.stream));
+    VCA_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    VCA_HIP(hipEventDestroy(e0));
+    VCA_HIP(hipEventDestroy(e1));
+    s->res.hxv_seconds += ms * 1e-3;
     s->res.hxv_count += (int64_t)nsteps * nb;
     s->res.hxv_states += (double)nsteps * (double)sec.dim * nb;
   }
@@ -434,9 +448,11 @@ struct GsOut {
 // Plain Lanczos ground state on sec (its precision), start = device vector
 // in that precision or null for the hash start vector.  The converged
 // eigenvector is left normalised in sec.xvec.  rel_tol: stop when
-// |dE| < tol*max(1,|E|) instead of |dE| < tol.
+// |dE| < tol*max(1,|E|) instead of |dE| < tol.  stall > 0: also stop once
+// |dE| has not reached a new minimum for `stall` steps (after 20 steps),
+// the noise floor of an FP32 run.
 static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, bool rel_tol,
-                        Solver *s) {
+                        Solver *s, int stall = 0) {
   niter = (int)std::min<int64_t>(niter, sec.dim);
   ensure_lanczos(sec, niter + 2);
   auto init = [&]() {
@@ -452,6 +468,8 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
   const int chunk = 16;
   int k0 = 0, m = 0;
   double e_old = std::numeric_limits<double>::infinity(), e_m = 0;
+  double best_de = std::numeric_limits<double>::infinity();
+  int since_best = 0;
   bool conv = false;
   while (k0 < niter && !conv) {
     const int n = std::min(chunk, niter - k0);
@@ -463,7 +481,14 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
       const int mm = j + 1;
       const double e = tridiag_lowest(mm, al.data(), be.data());
       const double thr = rel_tol ? tol * std::max(1.0, std::fabs(e)) : tol;
-      if (std::fabs(e - e_old) < thr || be[j + 1] == 0.0) {
+      const double de = std::fabs(e - e_old);
+      if (de < best_de) {
+        best_de = de;
+        since_best = 0;
+      } else {
+        ++since_best;
+      }
+      if (de < thr || be[j + 1] == 0.0 || (stall > 0 && mm >= 20 && since_best >= stall)) {
         m = mm;
         e_m = e;
         conv = true;
@@ -494,7 +519,9 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
 // state.  Returns the FP64 result; the eigenvector ends in sec64.xvec.
 static GsOut gs_mixed(Solver &s, Sector &sec32, Sector &sec64, double tol, int niter,
                       double *e32) {
-  GsOut g32 = gs_lanczos(sec32, 1e-7, niter, nullptr, true, &s);
+  // FP32 to its noise floor (a near-FP32-accurate vector), so that the FP64
+  // restart starts within ~1e-6 of the ground state and needs few steps
+  GsOut g32 = gs_lanczos(sec32, 1e-12, niter, nullptr, true, &s, 8);
   if (e32) *e32 = g32.e0;
   void *start64 = nullptr;
   VCA_HIP(hipMalloc(&start64, sec64.vec_bytes()));
@@ -848,6 +875,24 @@ static std::vector<ChainSpec> chain_list(const Solver &s, const KeptState &st) {
   return out;
 }
 
+// Runs fn(0..n-1) on up to 16 host threads (static slices); fn must only
+// write per-index state.
+template <typename Fn>
+static void parallel_for(int n, Fn &&fn) {
+  const int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  const int nth = std::min(nt, n);
+  if (nth <= 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t)
+    th.emplace_back([&, t]() {
+      for (int i = t; i < n; i += nth) fn(i);
+    });
+  for (auto &x : th) x.join();
+}
+
 // Chains of the same target sector run together: one launch per Lanczos
 // step covers a batch of chains (grid.y = chain), their vectors stacked
 // [chain][dim] in HBM, each chain with its own alpha/beta in device memory.
@@ -913,21 +958,22 @@ static void build_gf(Solver &s) {
                                  hipMemcpyDeviceToHost, js.stream));
         }
         VCA_HIP(hipStreamSynchronize(js.stream));
-        // add_to_lanczos_gf_normal (VCA_GF_NORMAL.f90:970-1022), T=0 weights
-        for (int c = 0; c < nb; ++c) {
+        // add_to_lanczos_gf_normal (VCA_GF_NORMAL.f90:970-1022): the chains'
+        // tridiagonal eigensolves run on host threads, one slice each
+        // pesoBZ (VCA_GF_NORMAL.f90:987-994): Boltzmann factor at finite T
+        double boltz = 1.0;
+        if (finite_t(p)) {
+          const double x = p.beta * (st.e - s.emin);
+          boltz = x < 200.0 ? std::exp(-x) : 0.0;
+        }
+        std::vector<double> cmax(nb, -std::numeric_limits<double>::infinity());
+        auto post = [&](int c) {
           const ChainSpec &cs = specs[idx[b0 + c]];
           const double norm2 = nrm0[c] * nrm0[c];
-          s.res.nchains += 1;
-          if (!(norm2 > 0.0)) continue;  // O|psi> = 0: empty channel
-          std::vector<double> w(nlanc), z((size_t)nlanc * nlanc);
-          tridiag_eig(nlanc, al.data() + (size_t)c * nlanc, be.data() + (size_t)c * nlanc,
-                      w.data(), z.data());
-          // pesoBZ (VCA_GF_NORMAL.f90:987-994): Boltzmann factor at finite T
-          double boltz = 1.0;
-          if (finite_t(p)) {
-            const double x = p.beta * (st.e - s.emin);
-            boltz = x < 200.0 ? std::exp(-x) : 0.0;
-          }
+          if (!(norm2 > 0.0)) return;  // O|psi> = 0: empty channel
+          std::vector<double> w(nlanc), z(nlanc);
+          tridiag_eig_first(nlanc, al.data() + (size_t)c * nlanc,
+                            be.data() + (size_t)c * nlanc, w.data(), z.data());
           const zc peso_bz =
               zc(chan_re[cs.ichan - 1], chan_im[cs.ichan - 1]) * norm2 * boltz / s.zeta;
           GfChan &out = res[idx[b0 + c]];
@@ -935,12 +981,14 @@ static void build_gf(Solver &s) {
           out.p.resize(nlanc);
           for (int j = 0; j < nlanc; ++j) {
             const double de = w[j] - st.e;
-            s.max_exc = std::max(s.max_exc, de);
-            const double z1 = z[(size_t)j * nlanc];
-            out.w[j] = peso_bz * z1 * z1;
+            cmax[c] = std::max(cmax[c], de);
+            out.w[j] = peso_bz * z[j] * z[j];
             out.p[j] = cs.isign * de;
           }
-        }
+        };
+        parallel_for(nb, post);
+        for (int c = 0; c < nb; ++c) s.max_exc = std::max(s.max_exc, cmax[c]);
+        s.res.nchains += nb;
       }
     }
     for (size_t c = 0; c < specs.size(); ++c) {

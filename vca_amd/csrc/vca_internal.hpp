@@ -268,6 +268,9 @@ EigsOut eigs_lanczos(Sector &sec, int nev, int ncv, double tol, int maxiter, con
 // coupling i-1 and i (e[0] unused).  Eigenvalues ascending in w; if z is
 // not null, z is n x n column-major eigenvectors.
 void tridiag_eig(int n, const double *d, const double *e, double *w, double *z);
+// Eigenvalues (ascending) and the first component of each eigenvector,
+// z0[j] = Z(0, j): what the continued-fraction weights need, in O(n^2).
+void tridiag_eig_first(int n, const double *d, const double *e, double *w, double *z0);
 // Lowest eigenvalue of the leading m x m block.
 double tridiag_lowest(int m, const double *d, const double *e);
 // Complex Hermitian eigenproblem (column-major a, n x n, overwritten by the
