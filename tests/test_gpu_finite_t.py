@@ -153,3 +153,18 @@ def test_finite_t_exact_at_u0(beta, tprime):
     eps = np.concatenate([np.linalg.eigvalsh(h) for h in hkl])
     exact = -(3 - p.nspin) / beta * np.sum(np.log1p(np.exp(-beta * eps))) / len(hkl)
     assert abs(omega - exact) < 2e-8, (omega, exact)
+
+
+def test_logdet_qr_equals_pivoted_lu(monkeypatch):
+    """The FP64 log-det by Householder QR (8-row blocks) and by pivoted LU
+    give the same complex integrand (log|det| and the principal phase)."""
+    p, hloc, hk, _, _ = M.bhz_small(nk=8)
+    _say("QR vs LU log-det, BHZ 2 sites")
+    s = VcaSolver(p)
+    s.vca_solve(hloc, hk)
+    zs = np.array([0.05j, 0.7j, 3.0j, 40.0j, 0.4 + 0.3j, -2.0 + 0.02j, 5.0 * np.exp(2.0j)])
+    monkeypatch.setenv("VCA_OMEGA_LU", "0")
+    f_qr = s.omega_integrand_complex(zs)
+    monkeypatch.setenv("VCA_OMEGA_LU", "1")
+    f_lu = s.omega_integrand_complex(zs)
+    assert np.allclose(f_qr, f_lu, rtol=1e-12, atol=1e-13), (f_qr, f_lu)

@@ -22,12 +22,15 @@
 //   k_logdet     per (k, z): M_k = 1 + P - H_k G' with H_k's rows held in
 //                registers across a chunk of frequencies and G'(z), P(z)
 //                staged in LDS; LU with partial pivoting, one matrix per
-//                16-lane group (row per lane); FP64 log accumulation
+//                16-lane group (row per lane); the pivot product is kept
+//                in FP64 as mantissa and exponent, one log per matrix
 //   k_reduce_z   deterministic sum of the k-block partials per z
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "omega.hpp"
@@ -312,7 +315,10 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
       }
     }
     bool active = true;
-    double ld = 0.0;
+    // prod |pivot|^2 kept as mantissa * 2^e (frexp after every factor, so
+    // no overflow), one log per matrix at the end
+    double pm_acc = 1.0;
+    int pm_exp = 0;
     double phr = 1.0, phi = 0.0;  // unit phase of det(M)
     int inv_parity = 0;
 #pragma unroll
@@ -336,7 +342,11 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
         pim[j] = __shfl(ai[j], src, 64);
       }
       const double pm = (double)pr[kk] * pr[kk] + (double)pim[kk] * pim[kk];
-      ld += 0.5 * log(pm);
+      {
+        int ex;
+        pm_acc = frexp(pm_acc * pm, &ex);
+        pm_exp += ex;
+      }
       if (phase) {
         // rows still active with a lower index than the pivot row are
         // eliminated after it: one inversion each
@@ -363,6 +373,7 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
       }
     }
     // ld is identical in the lanes of a group; invalid k contribute 0
+    const double ld = 0.5 * (log(pm_acc) + (double)pm_exp * 0.69314718055994530942);
     if (gl == 0) s_red[grp] = kvalid ? ld : 0.0;
     if (phase && gl == 0 && kvalid) {
       const double sg = inv_parity ? -1.0 : 1.0;
@@ -390,6 +401,204 @@ __global__ __launch_bounds__(64) void k_reduce_z(int nz, int nspin_blk, int nkb,
     s += t / nk;
   }
   if (threadIdx.x == 0) f[iz] = s;
+}
+
+// ----------------------------------------------- k x z log-det, QR form --
+// Cross-lane moves inside a quad with DPP (no LDS round trip): quad_perm
+// controls are p0 | p1 << 2 | p2 << 4 | p3 << 6.
+template <int CTRL>
+__device__ inline float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ inline double dpp_mov(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(unsigned)(b & 0xffffffffull), CTRL, 0xF, 0xF,
+                                          false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) |
+                                          (unsigned long long)(unsigned)lo));
+}
+
+// Sum over the L lanes of a group (L = 1, 2, 4; groups are aligned inside a
+// quad).  Every lane of the group gets bit-identical sums.
+template <int L, typename T>
+__device__ inline T group_sum(T v) {
+  if constexpr (L >= 2) v += dpp_mov<0xB1>(v);  // lane ^ 1
+  if constexpr (L >= 4) v += dpp_mov<0x4E>(v);  // lane ^ 2
+  return v;
+}
+
+// Value of lane O of the group in every lane of the group.
+template <int L, int O, typename T>
+__device__ inline T group_bcast(T v) {
+  if constexpr (L == 1) return v;
+  else if constexpr (L == 2) return dpp_mov<(O | (O << 2) | ((2 + O) << 4) | ((2 + O) << 6))>(v);
+  else return dpp_mov<O * 0x55>(v);
+}
+
+// Compile-time loop: f(integral_constant<int, I>) for I = B .. E-1.
+template <int B, int E>
+struct StaticFor {
+  template <typename F>
+  __device__ static void run(F &&f) {
+    if constexpr (B < E) {
+      f(std::integral_constant<int, B>{});
+      StaticFor<B + 1, E>::run(f);
+    }
+  }
+};
+
+// Same integrand as k_logdet, by Householder QR instead of pivoted LU:
+// |det M| = prod_k ||x_k|| (x_k = column k from the diagonal down, after
+// the previous reflections) and the phase of det M = prod_k x_k0/|x_k0|
+// (each reflector I - tau v v^H, tau = 1/(||x||(||x|| + |x_0|)), has
+// determinant -1, and R_kk = -e^{i arg x_0} ||x||).  QR needs no pivot
+// search and no row exchange, so every cross-lane step is a DPP sum or a
+// broadcast from a fixed lane.  L lanes per matrix, RL = NB/L rows per lane
+// distributed cyclically (row = lane + L*q) so that the shrinking active
+// part stays spread over the lanes; H_k rows in registers across the
+// frequency chunk, G'(z) and P(z) staged in LDS per frequency.
+template <typename R, int NB, int L>
+__global__ __launch_bounds__(kOBlock) void k_logdet_qr(int nk, int nz, int zchunk,
+                                                       int nspin_blk, const ocplx<R> *hk,
+                                                       const ocplx<R> *gmat,
+                                                       const ocplx<R> *pmat, double *partials,
+                                                       double2 *phase) {
+  using C = ocplx<R>;
+  constexpr int RL = NB / L;
+  constexpr int MPB = kOBlock / L;  // matrices per workgroup
+  const int l = threadIdx.x % L;
+  const int grp = threadIdx.x / L;
+  const int kb = blockIdx.x;
+  const int sb = blockIdx.z;
+  const int k = kb * MPB + grp;
+  const bool kvalid = k < nk;
+  __shared__ double s_red[MPB];
+  __shared__ C s_gz[NB * NB], s_pz[NB * NB];
+  R hr[RL][NB], hi[RL][NB];
+  {
+    const C *hb = hk + ((int64_t)(kvalid ? k : 0) * nspin_blk + sb) * NB * NB;
+#pragma unroll
+    for (int q = 0; q < RL; ++q)
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const C v = hb[(l + L * q) * NB + j];
+        hr[q][j] = kvalid ? v.x : (R)0;
+        hi[q][j] = kvalid ? v.y : (R)0;
+      }
+  }
+  const int zbeg = blockIdx.y * zchunk;
+  const int zend = min(zbeg + zchunk, nz);
+  for (int iz = zbeg; iz < zend; ++iz) {
+    const int64_t zoff = ((int64_t)iz * nspin_blk + sb) * kNB * kNB;
+    __syncthreads();
+    for (int e = threadIdx.x; e < NB * NB; e += kOBlock) {
+      const int r = e / NB, c = e % NB;
+      s_gz[e] = gmat[zoff + r * kNB + c];
+      s_pz[e] = pmat[zoff + r * kNB + c];
+    }
+    __syncthreads();
+    // M = 1 + P - H_k G'
+    R mr[RL][NB], mi[RL][NB];
+#pragma unroll
+    for (int q = 0; q < RL; ++q) {
+      const int r = l + L * q;
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const C pz = s_pz[r * NB + j];
+        mr[q][j] = (r == j ? (R)1 : (R)0) + pz.x;
+        mi[q][j] = pz.y;
+      }
+    }
+    // one row of G' live at a time (m is uniform: H_k(:, m) is picked
+    // from registers with an indexed move)
+#pragma unroll 1
+    for (int m = 0; m < NB; ++m) {
+      R hmr[RL], hmi[RL];
+#pragma unroll
+      for (int q = 0; q < RL; ++q) {
+        hmr[q] = hr[q][m];
+        hmi[q] = hi[q][m];
+      }
+#pragma unroll
+      for (int j = 0; j < NB; ++j) {
+        const C g = s_gz[m * NB + j];
+#pragma unroll
+        for (int q = 0; q < RL; ++q) {
+          mr[q][j] -= hmr[q] * g.x - hmi[q] * g.y;
+          mi[q][j] -= hmr[q] * g.y + hmi[q] * g.x;
+        }
+      }
+    }
+    double pm_acc = 1.0;
+    int pm_exp = 0;
+    double phr = 1.0, phi = 0.0;
+    StaticFor<0, NB>::run([&](auto kkc) {
+      constexpr int kk = decltype(kkc)::value;
+      constexpr int ow = kk % L, q0 = kk / L;  // lane and slot holding row kk
+      R vr[RL], vi[RL];
+      double n2 = 0.0;
+#pragma unroll
+      for (int q = q0; q < RL; ++q) {
+        R xr = mr[q][kk], xi = mi[q][kk];
+        if (q == q0 && l < ow) xr = xi = (R)0;  // rows above kk are done
+        vr[q] = xr;
+        vi[q] = xi;
+        n2 += (double)xr * xr + (double)xi * xi;
+      }
+      n2 = group_sum<L>(n2);
+      const R x0r = group_bcast<L, ow>(mr[q0][kk]);
+      const R x0i = group_bcast<L, ow>(mi[q0][kk]);
+      const double ax0 = sqrt((double)x0r * x0r + (double)x0i * x0i);
+      const double nx = sqrt(n2);
+      {
+        int ex;
+        pm_acc = frexp(pm_acc * n2, &ex);
+        pm_exp += ex;
+      }
+      const double er = ax0 > 0.0 ? (double)x0r / ax0 : 1.0;
+      const double ei = ax0 > 0.0 ? (double)x0i / ax0 : 0.0;
+      if (phase) {
+        const double t = phr * er - phi * ei;
+        phi = phr * ei + phi * er;
+        phr = t;
+      }
+      if constexpr (kk + 1 < NB) {
+        const R tau = nx > 0.0 ? (R)(1.0 / (nx * (nx + ax0))) : (R)0;
+        if (l == ow) {
+          vr[q0] = x0r + (R)(er * nx);
+          vi[q0] = x0i + (R)(ei * nx);
+        }
+#pragma unroll
+        for (int j = kk + 1; j < NB; ++j) {
+          R wr = 0, wi = 0;  // w = v^H M(:, j)
+#pragma unroll
+          for (int q = q0; q < RL; ++q) {
+            wr += vr[q] * mr[q][j] + vi[q] * mi[q][j];
+            wi += vr[q] * mi[q][j] - vi[q] * mr[q][j];
+          }
+          wr = group_sum<L>(wr) * tau;
+          wi = group_sum<L>(wi) * tau;
+#pragma unroll
+          for (int q = q0; q < RL; ++q) {
+            mr[q][j] -= vr[q] * wr - vi[q] * wi;
+            mi[q][j] -= vr[q] * wi + vi[q] * wr;
+          }
+        }
+      }
+    });
+    const double ld = 0.5 * (log(pm_acc) + (double)pm_exp * 0.69314718055994530942);
+    if (l == 0) s_red[grp] = kvalid ? ld : 0.0;
+    if (phase && l == 0 && kvalid)
+      phase[((int64_t)iz * nspin_blk + sb) * nk + k] = make_double2(phr, phi);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0;
+      for (int g = 0; g < MPB; ++g) t += s_red[g];
+      partials[((int64_t)iz * nspin_blk + sb) * gridDim.x + kb] = t;
+    }
+  }
 }
 
 // fim[z] = (1/Nk) sum_k arg(det M_k(z)), the principal argument of the
@@ -553,7 +762,16 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   pa.pmat = d_p64;
   hipLaunchKernelGGL(k_prep, dim3(nz, host.nspin_blk), dim3(64), 0, st, pa);
   VCA_HIP(hipGetLastError());
-  const int mpb = kOBlock / nbt;
+  // FP64 8-row blocks: the QR kernel (4 lanes per matrix).  FP32, 16-row
+  // blocks, or VCA_OMEGA_LU=1: the pivoted-LU kernel (a row per lane).  In
+  // FP32 the LU is the more accurate of the two: M is close to the
+  // identity, so its multipliers are small, while a Householder reflection
+  // always makes O(1) updates (3x the rounding error of the Omega integral
+  // in FP32 on the 2x2 BHZ grid).
+  const char *lu_env = std::getenv("VCA_OMEGA_LU");
+  const bool use_qr = nbt == 8 && precision == VCA_FP64 && !(lu_env && lu_env[0] == '1');
+  const int lanes = use_qr ? 4 : nbt;
+  const int mpb = kOBlock / lanes;
   const int nkb = (host.nk + mpb - 1) / mpb;
   const int zchunk = 16;
   const dim3 grid(nkb, (nz + zchunk - 1) / zchunk, host.nspin_blk);
@@ -564,7 +782,10 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   }
   if (precision == VCA_FP64) {
     if (e0) VCA_HIP(hipEventRecord(e0, st));
-    if (nbt == 8)
+    if (use_qr)
+      hipLaunchKernelGGL((k_logdet_qr<double, 8, 4>), grid, dim3(kOBlock), 0, st, host.nk, nz,
+                         zchunk, host.nspin_blk, d_hk64, d_g64, d_p64, d_part, phase);
+    else if (nbt == 8)
       hipLaunchKernelGGL((k_logdet<double, 8>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
                          host.nspin_blk, d_hk64, d_g64, d_p64, d_part, phase);
     else
