@@ -295,13 +295,17 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
       s_pz[e] = pmat[zoff + r * kNB + c];
     }
     __syncthreads();
+    // The row is held as X(r, :) with M(r, :) = X(r, :) + e_cu: the unit
+    // of the identity stays exact (cu = its column, -1 once eliminated), so
+    // the small X of high frequencies is not rounded away against 1.
     R ar[NBT], ai[NBT];
 #pragma unroll
     for (int j = 0; j < NBT; ++j) {
       const C pz = s_pz[gl * NBT + j];
-      ar[j] = (gl == j ? (R)1 : (R)0) + pz.x;
+      ar[j] = pz.x;
       ai[j] = pz.y;
     }
+    int cu = gl;
     // - H_k G' (m is uniform across the wavefront: the row element is
     // picked from registers with an indexed move, not spilled)
 #pragma unroll 1
@@ -323,7 +327,8 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
     int inv_parity = 0;
 #pragma unroll
     for (int kk = 0; kk < NBT; ++kk) {
-      float mag = active ? (float)(ar[kk] * ar[kk] + ai[kk] * ai[kk]) : -1.0f;
+      const R mkr = ar[kk] + (cu == kk ? (R)1 : (R)0);  // M(r, kk)
+      float mag = active ? (float)(mkr * mkr + ai[kk] * ai[kk]) : -1.0f;
       int idx = gl;
 #pragma unroll
       for (int off = NBT / 2; off > 0; off >>= 1) {
@@ -341,7 +346,11 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
         pr[j] = __shfl(ar[j], src, 64);
         pim[j] = __shfl(ai[j], src, 64);
       }
-      const double pm = (double)pr[kk] * pr[kk] + (double)pim[kk] * pim[kk];
+      const int pcu = __shfl(cu, src, 64);
+      // pivot = X(p, kk) + [its unit is in column kk], formed in FP64
+      const double pvr = (double)pr[kk] + (pcu == kk ? 1.0 : 0.0);
+      const double pvi = (double)pim[kk];
+      const double pm = pvr * pvr + pvi * pvi;
       {
         int ex;
         pm_acc = frexp(pm_acc * pm, &ex);
@@ -355,21 +364,24 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
         const uint64_t gm = (below >> gbase) & ((NBT == 64) ? ~0ull : ((1ull << NBT) - 1ull));
         inv_parity ^= __popcll(gm) & 1;
         const double rm = 1.0 / sqrt(pm);
-        const double ur = (double)pr[kk] * rm, ui = (double)pim[kk] * rm;
+        const double ur = pvr * rm, ui = pvi * rm;
         const double nr = phr * ur - phi * ui;
         phi = phr * ui + phi * ur;
         phr = nr;
       }
       if (gl == idx) active = false;
       if (active) {
-        const R inv = (R)1 / (pr[kk] * pr[kk] + pim[kk] * pim[kk]);
-        const R lr = (ar[kk] * pr[kk] + ai[kk] * pim[kk]) * inv;
-        const R li = (ai[kk] * pr[kk] - ar[kk] * pim[kk]) * inv;
+        const R pfr = (R)pvr, pfi = pim[kk];
+        const R inv = (R)1 / (pfr * pfr + pfi * pfi);
+        const R lr = (mkr * pfr + ai[kk] * pfi) * inv;  // l = M(r, kk) / pivot
+        const R li = (ai[kk] * pfr - mkr * pfi) * inv;
 #pragma unroll
         for (int j = kk + 1; j < NBT; ++j) {
-          ar[j] -= lr * pr[j] - li * pim[j];
-          ai[j] -= lr * pim[j] + li * pr[j];
+          const R prj = pr[j] + (pcu == j ? (R)1 : (R)0);  // pivot row's unit
+          ar[j] -= lr * prj - li * pim[j];
+          ai[j] -= lr * pim[j] + li * prj;
         }
+        if (cu == kk) cu = -1;  // this row's unit sat in the eliminated column
       }
     }
     // ld is identical in the lanes of a group; invalid k contribute 0
