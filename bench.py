@@ -6,6 +6,8 @@ on that sector with the vector resident in HBM; `value` is Gstates/s =
 dim / (average kernel time of one application, HIP events on the sector's
 stream).  Reported beside it in the same JSON line:
   * mixed: the complex64 H x v on the same sector;
+  * batched: 64 vectors per launch on the same sector (the form the
+    excitation chains run in), FP64 and MIXED;
   * ground_state: the Lanczos ground state, FP64 and MIXED (FP32 Lanczos +
     FP64 refinement), with the E0 error of MIXED against FP64;
   * omega: (k, omega) log-det evaluations per second of the Omega integrand
@@ -108,12 +110,15 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    import torch
 
     from vca_amd import configs
-    from vca_amd.api import VcaSolver
+    from vca_amd.api import DeviceArray, VcaSolver, device_synchronize
 
-    dev = torch.device("cuda:0")
+    def rand_vec(n, prec, seed):
+        rng = np.random.default_rng(seed)
+        dt = np.complex128 if prec == "fp64" else np.complex64
+        return (rng.standard_normal(n, dtype=np.float32)
+                + 1j * rng.standard_normal(n, dtype=np.float32)).astype(dt)
 
     # --------------------------------------------------------- Omega -----
     def omega_section():
@@ -177,30 +182,56 @@ def main():
     # ---------------------------------------------------------- H x v ----
     def time_hxv(prec):
         sec = s.sector(6, 6, prec)
-        dt = torch.complex128 if prec == "fp64" else torch.complex64
-        g = torch.Generator(device=dev).manual_seed(1)
-        v = torch.randn(sec.dim, dtype=dt, device=dev, generator=g)
-        hv = torch.empty_like(v)
-        torch.cuda.synchronize()
-        sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=max(1, args.warmup))
-        torch.cuda.synchronize()
-        ms = sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=args.steps)
-        torch.cuda.synchronize()
+        v = rand_vec(sec.dim, prec, 1)
+        dv = DeviceArray(v.shape, v.dtype).upload(v)
+        dh = DeviceArray(v.shape, v.dtype)
+        sec.hxv_device(dv.ptr, dh.ptr, repeat=max(1, args.warmup))
+        device_synchronize()
+        ms = sec.hxv_device(dv.ptr, dh.ptr, repeat=args.steps)  # event-timed on its stream
+        device_synchronize()
+        dv.free()
+        dh.free()
         return sec, ms
 
     sec64, ms64 = time_hxv("fp64")
     sec32, ms32 = time_hxv("mixed")
     dim = sec64.dim
+
+    def time_hxv_batch(sec, prec, nb=64, steps=20):
+        """nb stacked vectors per launch (the form the GF chains run in)."""
+        v = np.tile(rand_vec(sec.dim, prec, 5), nb)
+        dv = DeviceArray(v.shape, v.dtype).upload(v)
+        dh = DeviceArray(v.shape, v.dtype)
+        del v
+        sec.hxv_device_batch(nb, dv.ptr, dh.ptr, repeat=2)
+        device_synchronize()
+        ms = sec.hxv_device_batch(nb, dv.ptr, dh.ptr, repeat=steps)
+        device_synchronize()
+        dv.free()
+        dh.free()
+        return {"nb": nb, "ms_per_launch": ms, "gstates_per_s": nb * sec.dim / (ms * 1e-3) / 1e9,
+                "hbm_gbs_algorithmic": nb * sec.dim * 2 * (16 if prec == "fp64" else 8)
+                / (ms * 1e-3) / 1e9}
+
     log(f"H x v FP64: {ms64 * 1e3:.2f} us/apply; MIXED: {ms32 * 1e3:.2f} us/apply; dim {dim}")
     if args.only == "hxv":
-        # calibration kernel for the HBM byte counters: a copy of 256 MiB
-        a = torch.ones(1 << 25, dtype=torch.float64, device=dev)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
+        # calibration kernel for the HBM byte counters: a device copy of
+        # 256 MiB (the runtime's copy kernel)
+        nbytes = 1 << 28
+        a = DeviceArray((nbytes // 8,), np.float64).upload(np.ones(nbytes // 8))
+        b = DeviceArray((nbytes // 8,), np.float64)
+        from vca_amd import _lib as L
+        import ctypes
+        L.check(L.load().vca_device_copy(ctypes.c_void_p(b.ptr), ctypes.c_void_p(a.ptr),
+                                         nbytes, 2))
+        device_synchronize()
         print(json.dumps({"hxv_fp64_us": ms64 * 1e3, "hxv_mixed_us": ms32 * 1e3, "dim": dim,
-                          "copy_bytes_each_way": a.numel() * 8}), flush=True)
+                          "copy_bytes_each_way": nbytes}), flush=True)
         return
+
+    batched = {"fp64": time_hxv_batch(sec64, "fp64"), "mixed": time_hxv_batch(sec32, "mixed")}
+    log(f"H x v batched (64 vectors): FP64 {batched['fp64']['gstates_per_s']:.1f} Gstates/s, "
+        f"MIXED {batched['mixed']['gstates_per_s']:.1f} Gstates/s")
 
     # --------------------------------------------------- ground state ----
     t0 = time.perf_counter()
@@ -253,14 +284,16 @@ def main():
         # its refinement, whose launch shape is then already tuned
         for prec in ("fp64", "mixed"):
             sec = sb.sector(8, 8, prec)
-            dt = torch.complex128 if prec == "fp64" else torch.complex64
-            g = torch.Generator(device=dev).manual_seed(2)
-            v = torch.randn(sec.dim, dtype=dt, device=dev, generator=g)
-            hv = torch.empty_like(v)
-            sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=2)
-            ms = sec.hxv_device(v.data_ptr(), hv.data_ptr(), repeat=5)
-            del v, hv
-            torch.cuda.empty_cache()
+            v = rand_vec(sec.dim, prec, 2)
+            dv = DeviceArray(v.shape, v.dtype).upload(v)
+            dh = DeviceArray(v.shape, v.dtype)
+            del v
+            sec.hxv_device(dv.ptr, dh.ptr, repeat=2)
+            device_synchronize()
+            ms = sec.hxv_device(dv.ptr, dh.ptr, repeat=5)
+            device_synchronize()
+            dv.free()
+            dh.free()
             out16[prec] = {"ms_per_hxv": ms, "gstates_per_s": sec.dim / (ms * 1e-3) / 1e9,
                            "hbm_gbs_algorithmic": 2 * (16 if prec == "fp64" else 8) * sec.dim
                            / (ms * 1e-3) / 1e9}
@@ -356,6 +389,7 @@ def main():
         "config": {"workload": "hubbard 3x4 square cluster U=8 half-filled sector (6,6), "
                                "853,776 states, H x v FP64", "sector_dim": dim},
         "mixed": {"value": gst32, "ms_per_step": ms32, "dtype": "f32 (complex64)"},
+        "batched": batched,
         "ground_state": {"e0_fp64": e64, "iters_fp64": it64, "seconds_fp64": t_gs64,
                          "e0_mixed": emx, "seconds_mixed": t_gsmx,
                          "e0_rel_err_mixed": abs(emx - e64) / abs(e64)},

@@ -20,7 +20,8 @@
  *                        delete_Hv_sector)
  *   vca_hxv              VCA_VARS_GLOBAL.f90:83-89 (cc_sparse_HxV operator,
  *                        bound to directMatVec_main,
- *                        VCA_HAMILTONIAN_DIRECT_HxV.f90:42-78)
+ *                        VCA_HAMILTONIAN_DIRECT_HxV.f90:42-78); the device
+ *                        forms vca_hxv_device / vca_hxv_device_batch too
  *   vca_lanczos_gs       SciFortran sp_lanc_eigh as called at VCA_DIAG.f90:172-184
  *   vca_lanczos_eigs     SciFortran sp_eigh (ARPACK, which='SA') as called at
  *                        VCA_DIAG.f90:153-170
@@ -169,6 +170,21 @@ int vca_hxv(vca_sector *sec, int64_t n, const double *v, double *hv);
  * average kernel time in *ms when ms is not NULL. */
 int vca_hxv_device(vca_sector *sec, const void *v, void *hv, int32_t repeat,
                    float *ms);
+/* Device buffers for the device-resident forms (for callers without a HIP
+ * runtime of their own, e.g. Fortran).  Copies are synchronous. */
+#define VCA_COPY_H2D 0
+#define VCA_COPY_D2H 1
+#define VCA_COPY_D2D 2
+int vca_device_alloc(int64_t bytes, void **ptr);
+int vca_device_free(void *ptr);
+int vca_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind);
+/* Waits for all work the library queued on the device. */
+int vca_device_synchronize(void);
+/* The same for nb vectors stacked [nb][dim] in v and hv: one launch per
+ * application covers all nb products (the batched form the excitation
+ * chains use); *ms is the average time of one application to all nb. */
+int vca_hxv_device_batch(vca_sector *sec, int32_t nb, const void *v, void *hv,
+                         int32_t repeat, float *ms);
 /* One-spin hop table in coordinate form (rows, cols: 0-based string
  * indices; amps complex128), spin 1 = up, 2 = down.  With rows == NULL only
  * the count is returned in *nnz.  Connectivity/sign check of the device

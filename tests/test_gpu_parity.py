@@ -219,3 +219,32 @@ def test_graph_replays_match_eager_launches(monkeypatch):
         a_g, b_g = sec.lanczos_tridiag(v0, 50)
         assert np.array_equal(a_g, a_e) and np.array_equal(b_g, b_e)
         sec.hxv(v0)  # eager work between the replays
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_hxv_device_batch_equals_single(prec):
+    """vca_hxv_device_batch on nb stacked vectors gives, bit for bit, the nb
+    single products (and the oracle's within the usual bar)."""
+    from vca_amd.api import DeviceArray
+    p, hloc, hk, bh, bv = M.bath_model(nlat=5, nb=5)
+    om = M.oracle_model(p, hloc, bh, bv)
+    s = _solver(p, hloc, bh, bv)
+    sec = s.sector(5, 5, prec)
+    nb = 3
+    _say(f"batched hxv bath5+5 (5,5) nb={nb} {prec}")
+    dt = np.complex128 if prec == "fp64" else np.complex64
+    rng = np.random.default_rng(4)
+    v = (rng.normal(size=(nb, sec.dim)) + 1j * rng.normal(size=(nb, sec.dim))).astype(dt)
+    dv = DeviceArray(v.shape, dt).upload(v)
+    dh = DeviceArray(v.shape, dt)
+    sec.hxv_device_batch(nb, dv.ptr, dh.ptr)
+    hv = dh.download()
+    d1 = DeviceArray((sec.dim,), dt)
+    h1 = DeviceArray((sec.dim,), dt)
+    for c in range(nb):
+        d1.upload(v[c])
+        sec.hxv_device(d1.ptr, h1.ptr)
+        assert np.array_equal(h1.download(), hv[c])
+    ref = om.hxv(5, 5, v[1].astype(np.complex128))
+    err = np.linalg.norm(hv[1] - ref) / np.linalg.norm(ref)
+    assert err < (1e-12 if prec == "fp64" else 2e-6), err

@@ -78,6 +78,12 @@ _SIGS = {
                                        ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "vca_hxv": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P]),
     "vca_hxv_device": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.POINTER(ctypes.c_float)]),
+    "vca_hxv_device_batch": (ctypes.c_int, [_P, _I32, _P, _P, _I32,
+                                            ctypes.POINTER(ctypes.c_float)]),
+    "vca_device_alloc": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_P)]),
+    "vca_device_free": (ctypes.c_int, [_P]),
+    "vca_device_synchronize": (ctypes.c_int, []),
+    "vca_device_copy": (ctypes.c_int, [_P, _P, ctypes.c_int64, _I32]),
     "vca_sector_spin_coo": (ctypes.c_int, [_P, _I32, ctypes.POINTER(ctypes.c_int64), _P, _P, _P]),
     "vca_lanczos_gs": (ctypes.c_int, [_P, ctypes.c_double, _I32, _D, _P, ctypes.POINTER(_I32)]),
     "vca_lanczos_eigs": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_double, _I32, _P, _P,

@@ -154,6 +154,13 @@ class Sector:
                                        repeat, ctypes.byref(ms)))
         return ms.value
 
+    def hxv_device_batch(self, nb, v_ptr, hv_ptr, repeat=1):
+        """nb products at once on device buffers [nb][dim] (sector precision)."""
+        ms = ctypes.c_float()
+        check(self._lib.vca_hxv_device_batch(self._h, nb, ctypes.c_void_p(v_ptr),
+                                             ctypes.c_void_p(hv_ptr), repeat, ctypes.byref(ms)))
+        return ms.value
+
     def spin_coo(self, spin):
         nnz = ctypes.c_int64()
         check(self._lib.vca_sector_spin_coo(self._h, spin, ctypes.byref(nnz), None, None, None))
@@ -195,6 +202,49 @@ class Sector:
         check(self._lib.vca_lanczos_tridiag(self._h, v0.ctypes.data, nlanc, alpha.ctypes.data,
                                             beta.ctypes.data))
         return alpha, beta
+
+
+def device_synchronize():
+    """Waits for all work the library queued on the device."""
+    check(_lib.load().vca_device_synchronize())
+
+
+class DeviceArray:
+    """A device buffer of the library (vca_device_alloc), with numpy
+    upload/download; `ptr` is what the device-resident entry points take."""
+
+    def __init__(self, shape, dtype):
+        self._lib = _lib.load()
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = ctypes.c_void_p()
+        check(self._lib.vca_device_alloc(self.nbytes, ctypes.byref(p)))
+        self.ptr = p.value
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes == self.nbytes
+        check(self._lib.vca_device_copy(ctypes.c_void_p(self.ptr), a.ctypes.data_as(ctypes.c_void_p),
+                                        self.nbytes, 0))
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, self.dtype)
+        check(self._lib.vca_device_copy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr),
+                                        self.nbytes, 1))
+        return out
+
+    def free(self):
+        if self.ptr:
+            self._lib.vca_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 class VcaSolver:

@@ -1543,6 +1543,63 @@ int vca_hxv_device(vca_sector *sv, const void *v, void *hv, int32_t repeat, floa
   VCA_GUARD_END
 }
 
+int vca_device_alloc(int64_t bytes, void **ptr) {
+  VCA_GUARD_BEGIN
+  if (!ptr || bytes < 0) fail(VCA_ERR_ARG, "bad argument");
+  *ptr = nullptr;
+  if (bytes > 0) VCA_HIP(hipMalloc(ptr, (size_t)bytes));
+  VCA_GUARD_END
+}
+
+int vca_device_synchronize(void) {
+  VCA_GUARD_BEGIN
+  VCA_HIP(hipDeviceSynchronize());
+  VCA_GUARD_END
+}
+
+int vca_device_free(void *ptr) {
+  VCA_GUARD_BEGIN
+  if (ptr) VCA_HIP(hipFree(ptr));
+  VCA_GUARD_END
+}
+
+int vca_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind) {
+  VCA_GUARD_BEGIN
+  if (!dst || !src || bytes < 0) fail(VCA_ERR_ARG, "bad argument");
+  hipMemcpyKind k;
+  switch (kind) {
+    case VCA_COPY_H2D: k = hipMemcpyHostToDevice; break;
+    case VCA_COPY_D2H: k = hipMemcpyDeviceToHost; break;
+    case VCA_COPY_D2D: k = hipMemcpyDeviceToDevice; break;
+    default: fail(VCA_ERR_ARG, "unknown copy kind %d", kind);
+  }
+  VCA_HIP(hipMemcpy(dst, src, (size_t)bytes, k));
+  VCA_GUARD_END
+}
+
+int vca_hxv_device_batch(vca_sector *sv, int32_t nb, const void *v, void *hv, int32_t repeat,
+                         float *ms) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !v || !hv || nb < 1) fail(VCA_ERR_ARG, "bad argument");
+  if (repeat < 1) repeat = 1;
+  hipEvent_t e0, e1;
+  VCA_HIP(hipEventCreate(&e0));
+  VCA_HIP(hipEventCreate(&e1));
+  VCA_HIP(hipEventRecord(e0, sec->stream));
+  for (int r = 0; r < repeat; ++r) launch_hxv(*sec, v, v, hv, false, nb);
+  VCA_HIP(hipEventRecord(e1, sec->stream));
+  VCA_HIP(hipEventSynchronize(e1));
+  if (ms) {
+    float t = 0;
+    VCA_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeat;
+  }
+  VCA_HIP(hipEventDestroy(e0));
+  VCA_HIP(hipEventDestroy(e1));
+  VCA_GUARD_END
+}
+
 int vca_sector_spin_coo(const vca_sector *sv, int32_t spin, int64_t *nnz, int32_t *rows,
                         int32_t *cols, double *amps) {
   VCA_GUARD_BEGIN
