@@ -83,6 +83,22 @@ def run_full():
     log(f"full 3x4 solve: {full['seconds']:.1f}s Omega={omf:.10f} "
         f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, Omega {r['t_omega']:.2f}s, "
         f"{r['nchains']} chains)")
+    # G, Sigma, G0 on the default grids (Lmats = Lreal = 5000): the pole sums
+    # and the per-frequency inversions of build_gf_normal / build_sigma_normal
+    grids = {}
+    for axis in ("matsubara", "realaxis"):
+        t1 = time.perf_counter()
+        sf.gf_grid(axis)
+        grids[axis + "_seconds"] = time.perf_counter() - t1
+    r2 = sf.results()
+    grids.update(npoles=r2["npoles"], pole_evals=r2["grid_pole_evals"],
+                 pole_kernel_seconds=r2["grid_seconds"],
+                 pole_evals_per_s=r2["grid_pole_evals"] / max(r2["grid_seconds"], 1e-30))
+    full["grids"] = grids
+    log(f"grids 3x4 ({r2['npoles']} poles, 2 x 5000 points): Matsubara "
+        f"{grids['matsubara_seconds'] * 1e3:.1f} ms, real {grids['realaxis_seconds'] * 1e3:.1f} ms "
+        f"(pole-sum kernels {r2['grid_seconds'] * 1e3:.2f} ms, "
+        f"{grids['pole_evals_per_s'] / 1e9:.1f} G pole terms/s)")
     return full
 
 

@@ -31,6 +31,11 @@
  *   vca_gf_poles         VCA_VARS_GLOBAL.f90:61-71 (GFmatrix weights/poles)
  *   vca_omega_integrand  VCA_OMEGA.f90:37-75     (sum_kmesh, batched in omega)
  *   vca_omega_integrand_complex VCA_OMEGA.f90:78-116 (sum_kmesh_complex)
+ *   vca_get_gf_grid      VCA_IO/get_gimp_matsubara.f90, get_gimp_realaxis.f90,
+ *                        get_sigma_matsubara.f90, get_sigma_realaxis.f90
+ *                        (impGmats/impGreal of VCA_GF_NORMAL.f90:89-101,
+ *                        1024-1031; impSmats/impSreal/impG0* of
+ *                        build_sigma_normal, VCA_GF_NORMAL.f90:1045-1111)
  */
 #ifndef VCA_GPU_H
 #define VCA_GPU_H
@@ -63,6 +68,11 @@ extern "C" {
 #define VCA_OMEGA_QUAD 0      /* adaptive Gauss-Kronrod on [0,inf), T=0     */
 #define VCA_OMEGA_MATSUBARA 1 /* (2/beta) sum over omega_nfreq Matsubara
                                  frequencies                               */
+
+/* frequency axis of the grid outputs (VCA_GF_SHARED.f90:43-56) */
+#define VCA_AXIS_MATSUBARA 0 /* z_n = i (2n - 1) pi / beta, n = 1..lmats     */
+#define VCA_AXIS_REAL 1      /* z_n = wini + (n-1)(wfin - wini)/(lreal - 1)
+                                + i eps, n = 1..lreal                      */
 
 /* Input variables of the path (VCA_INPUT_VARS.f90:112-172 names in
  * comments).  vca_default_params fills the reference defaults. */
@@ -105,6 +115,11 @@ typedef struct vca_params {
   double cutoff;               /* CUTOFF                                  */
   double bandwidth;            /* bandwidth (set by the caller, as the
                                   reference drivers do)                   */
+  /* frequency grids of the G / Sigma outputs (VCA_GF_SHARED.f90:43-56) */
+  int32_t lmats;               /* LMATS: w_n = (2n-1) pi / beta, n=1..   */
+  int32_t lreal;               /* LREAL: linspace(WINI, WFIN, LREAL)     */
+  double wini, wfin;           /* WINI, WFIN                             */
+  double eps;                  /* EPS: real-axis broadening              */
 } vca_params;
 
 /* Per-solve results and timings (seconds) in addition to the SFT potential. */
@@ -127,6 +142,10 @@ typedef struct vca_results {
   int32_t finite_t;      /* 1 when lanc_nstates_total > 1                 */
   int32_t nmax;          /* finite T: Matsubara terms summed (0..nmax)    */
   int32_t spectrum_iters;/* finite T: spectrum annealing iterations       */
+  int32_t reserved;      /* alignment                                     */
+  int64_t npoles;        /* Lehmann poles stored, all elements            */
+  int64_t grid_pole_evals; /* pole x frequency terms of vca_get_gf_grid   */
+  double grid_seconds;   /* GPU time of those pole sums (events)          */
 } vca_results;
 
 typedef struct vca_solver vca_solver;
@@ -226,6 +245,14 @@ int vca_omega_integrand(vca_solver *s, int32_t nz, const double *z,
  * determinant; f complex128[nz]. */
 int vca_omega_integrand_complex(vca_solver *s, int32_t nz, const double *z,
                                 int32_t precision, double *f);
+/* The cluster Green's function G, the self-energy Sigma = G0^-1 - G^-1 and
+ * G0 = ((z + xmu) - Hloc - Delta(z))^-1 on the Matsubara or real-axis grid
+ * (lmats or lreal points of vca_params), each complex128
+ * [nlat,nlat,nspin,nspin,norb,norb,L] with the first index fastest.  Any of
+ * gimp, sigma, g0 may be NULL.  The pole sums and the per-frequency
+ * inversions run on the GPU. */
+int vca_get_gf_grid(vca_solver *s, int32_t axis, double *gimp, double *sigma,
+                    double *g0);
 
 #ifdef __cplusplus
 }

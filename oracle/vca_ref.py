@@ -85,6 +85,11 @@ class Params:
     lanc_ncv_add: int = 5
     cutoff: float = 1e-9
     bandwidth: float = 0.0
+    lmats: int = 5000
+    lreal: int = 5000
+    wini: float = -5.0
+    wfin: float = 5.0
+    eps: float = 0.01
 
     @property
     def finite_t(self):
@@ -631,6 +636,80 @@ def delta_bath(model: Model, z):
     vv = rect_n2j(model.bath_v, p.nlat, p.nlat_bath, p.nspin, p.norb, p.norb_bath)
     m = (z + p.xmu) * np.eye(hb.shape[0]) - hb
     return vv @ np.linalg.inv(m) @ vv.conj().T
+
+
+def frequency_grids(p: Params):
+    """allocate_grids, VCA_GF_SHARED.f90:43-56: wm = pi/beta (2n - 1),
+    n = 1..Lmats; wr = linspace(wini, wfin, Lreal), SciFortran's form
+    start + (i - 1) (stop - start)/(num - 1) (SciFortran is not shipped with
+    the reference; its linspace with both end points is restated)."""
+    wm = (math.pi / p.beta) * (2 * np.arange(1, p.lmats + 1) - 1).astype(np.float64)
+    step = (p.wfin - p.wini) / float(p.lreal - 1)
+    wr = p.wini + np.arange(p.lreal, dtype=np.float64) * step
+    return wm, wr
+
+
+def gf_cluster_grid(gd: GFData, zs):
+    """impGmats / impGreal of build_gf_normal: the pole sums of
+    add_to_lanczos_gf_normal (VCA_GF_NORMAL.f90:1024-1031) at every grid point,
+    then the off-diagonal recombination (:89-101).  Returns
+    [Nlat,Nlat,Nspin,Nspin,Norb,Norb,L]."""
+    p = gd.p
+    zs = np.asarray(zs, dtype=np.complex128)
+    chan4 = 0.0 if p.gf_symmetric else 1.0
+    g = zeros_nnn(p, (len(zs),))
+    for (il, jl, s, io, jo), per_state in gd.elems.items():
+        acc = np.zeros(len(zs), np.complex128)
+        for chans in per_state:
+            for (w, poles) in chans:
+                if len(w):
+                    acc += (w[None, :] / (zs[:, None] - poles[None, :])).sum(axis=1)
+        g[il - 1, jl - 1, s - 1, s - 1, io - 1, jo - 1] = acc
+    f = 1 - chan4 * 1j
+    for s in range(p.nspin):
+        for il in range(p.nlat):
+            for jl in range(p.nlat):
+                for io in range(p.norb):
+                    for jo in range(p.norb):
+                        if il == jl and io == jo:
+                            continue
+                        g[il, jl, s, s, io, jo] = 0.5 * (g[il, jl, s, s, io, jo]
+                                                         - f * g[il, il, s, s, io, io]
+                                                         - f * g[jl, jl, s, s, jo, jo])
+    return g
+
+
+def _nnn2lso_many(a, p: Params):
+    """[Nlat,Nlat,Nspin,Nspin,Norb,Norb,L] -> [L, nlso, nlso] (lso index
+    iorb + ilat Norb + ispin Norb Nlat, VCA_AUX_FUNX.f90:255-279)."""
+    n = p.nlso
+    # axes (il, jl, s, t, io, jo, z) -> (z, s, il, io, t, jl, jo)
+    return np.ascontiguousarray(a.transpose(6, 2, 0, 4, 3, 1, 5)).reshape(a.shape[6], n, n)
+
+
+def _lso2nnn_many(m, p: Params):
+    L = m.shape[0]
+    t = m.reshape(L, p.nspin, p.nlat, p.norb, p.nspin, p.nlat, p.norb)
+    return np.asfortranarray(t.transpose(2, 5, 1, 4, 3, 6, 0))
+
+
+def sigma_grid(model: Model, gd: GFData, axis):
+    """build_gf_normal's grid G and build_sigma_normal
+    (VCA_GF_NORMAL.f90:1045-1111) on the Matsubara (axis 0, z = i wm) or real
+    (axis 1, z = wr + i eps) grid: G0^-1 = (z + xmu) 1 - Hloc - Delta(z),
+    Sigma = G0^-1 - G^-1, G0 = (G0^-1)^-1.  Returns (G, Sigma, G0), each
+    [Nlat,Nlat,Nspin,Nspin,Norb,Norb,L]."""
+    p = model.p
+    wm, wr = frequency_grids(p)
+    zs = 1j * wm if axis == 0 else wr + 1j * p.eps
+    g = gf_cluster_grid(gd, zs)
+    n = p.nlso
+    hl = nnn2lso(model.hloc, p.nlat, p.nspin, p.norb)
+    g0inv = np.empty((len(zs), n, n), np.complex128)
+    for i, z in enumerate(zs):
+        g0inv[i] = (z + p.xmu) * np.eye(n) - hl - delta_bath(model, z)
+    ginv = np.linalg.inv(_nnn2lso_many(g, p))
+    return g, _lso2nnn_many(g0inv - ginv, p), _lso2nnn_many(np.linalg.inv(g0inv), p)
 
 
 def sum_kmesh(model: Model, gd: GFData, hk_lso, omega, complex_out=False):

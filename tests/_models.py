@@ -21,7 +21,8 @@ def to_oracle(p: VcaParams) -> R.Params:
                     lanc_nstates_step=p.lanc_nstates_step,
                     lanc_niter_spectrum=p.lanc_niter_spectrum,
                     lanc_ncv_factor=p.lanc_ncv_factor, lanc_ncv_add=p.lanc_ncv_add,
-                    cutoff=p.cutoff, bandwidth=p.bandwidth)
+                    cutoff=p.cutoff, bandwidth=p.bandwidth, lmats=p.lmats, lreal=p.lreal,
+                    wini=p.wini, wfin=p.wfin, eps=p.eps)
 
 
 def chain(nlat=6, u=4.0, nk=8, tprime=-1.0):

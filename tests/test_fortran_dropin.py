@@ -45,9 +45,22 @@ def test_fortran_driver_matches_python_mirror(tmp_path):
     line = [x for x in r.stdout.splitlines() if x.startswith("SFT_POTENTIAL")][0]
     sft_f = float(line.split()[1])
     p, hloc, hk, _, _ = configs.config_chain4(nk=16)
-    sft_py = VcaSolver(p).vca_solve(hloc, hk)
+    s = VcaSolver(p)
+    sft_py = s.vca_solve(hloc, hk)
     assert abs(sft_f - sft_py) < 1e-12 * abs(sft_py)
     assert (tmp_path / "SFT_potential.vca").exists()
+    # grid getters: full Matsubara Sigma and the (1,2) slice of real-axis G
+
+    def val(tag):
+        x = [ln for ln in r.stdout.splitlines() if ln.startswith(tag + " ")][0].split()
+        return complex(float(x[1]), float(x[2]))
+
+    sm = s.vca_get_sigma_matsubara()
+    gr = s.vca_get_gimp_realaxis(1, 2)
+    for tag, ref in (("SIGMA_MATS_11_1", sm[0, 0, 0, 0, 0, 0, 0]),
+                     ("SIGMA_MATS_23_7", sm[1, 2, 0, 0, 0, 0, 6]),
+                     ("GREAL_12_2500", gr[0, 0, 0, 0, 2499])):
+        assert abs(val(tag) - ref) < 1e-12 * max(1.0, abs(ref)), (tag, val(tag), ref)
 
 
 @needs_fc

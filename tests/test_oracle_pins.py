@@ -272,6 +272,73 @@ def test_offdiagonal_recombination_exact():
             assert np.abs(g - ge).max() < 1e-10, sym
 
 
+# ------------------------------------------ grid G, Sigma = G0^-1 - G^-1 --
+def test_frequency_grids():
+    p = R.Params(beta=37.0, lmats=11, lreal=9, wini=-3.0, wfin=2.0)
+    wm, wr = R.frequency_grids(p)
+    assert np.abs(wm - np.array([(2 * n + 1) * math.pi / 37.0 for n in range(11)])).max() < 1e-15
+    assert wr[0] == -3.0 and abs(wr[-1] - 2.0) < 1e-15
+    assert np.abs(np.diff(wr) - 5.0 / 8.0).max() < 1e-15
+
+
+def _u0(model):
+    if model == "chain":
+        p, hloc, hk, bh, bv = M.chain(nlat=4, u=0.0, nk=2)
+    elif model == "bath":
+        p, hloc, hk, bh, bv = M.bath_model(nlat=2, nb=2)
+        p.uloc = (0.0, 0, 0, 0, 0)
+    else:
+        p, hloc, hk, bh, bv = M.bhz_small(nk=2)
+        p.uloc = (0.0, 0.0, 0, 0, 0)
+    p.lmats, p.lreal, p.beta, p.eps = 24, 24, 30.0, 0.05
+    return p, hloc, hk, bh, bv
+
+
+@pytest.mark.parametrize("model", ["chain", "bath", "bhz"])
+def test_sigma_vanishes_at_u0(model):
+    """At U=0 the cluster G is the resolvent of Hloc + Delta, so
+    Sigma = G0^-1 - G^-1 = 0 and G = G0 on both grids: pins the grid pole
+    sums, the recombination, Delta and the lso reshapes together."""
+    p, hloc, hk, bh, bv = _u0(model)
+    r = R.vca_solve(M.to_oracle(p), hloc, hk, bh, bv, epsabs=1e-8, epsrel=1e-8)
+    for axis in (0, 1):
+        g, s, g0 = R.sigma_grid(r["model"], r["gf"], axis)
+        assert g.shape == (p.nlat, p.nlat, p.nspin, p.nspin, p.norb, p.norb, 24)
+        assert np.abs(s).max() < 1e-9, (model, axis, np.abs(s).max())
+        assert np.abs(g - g0).max() < 1e-10
+
+
+def test_sigma_hubbard_atom():
+    """Half-filled Hubbard atom (HFMODE, xmu=0): G = (1/(z-U/2) + 1/(z+U/2))/2,
+    so Sigma(z) = U^2/(4z) exactly, on the Matsubara and real grids."""
+    u = 3.0
+    p = R.Params(nlat=1, uloc=(u, 0, 0, 0, 0), beta=25.0, lmats=40, lreal=31, eps=0.02)
+    r = R.vca_solve(p, np.zeros((1, 1, 1, 1, 1, 1), complex),
+                    np.zeros((1, 1, 1, 1, 1, 1, 1), complex))
+    wm, wr = R.frequency_grids(p)
+    for axis, z in ((0, 1j * wm), (1, wr + 1j * p.eps)):
+        _, s, g0 = R.sigma_grid(r["model"], r["gf"], axis)
+        assert np.abs(s[0, 0, 0, 0, 0, 0] - u * u / (4 * z)).max() < 1e-11
+        assert np.abs(g0[0, 0, 0, 0, 0, 0] - 1 / z).max() < 1e-13
+
+
+def test_sigma_equals_exact_lehmann():
+    """U=4 chain: Sigma from the oracle's grid equals (z + mu) - Hloc -
+    G_exact^-1 with G_exact from full Fock-space diagonalisation."""
+    p, hloc, hk, _, _ = M.chain(nlat=4, u=4.0, nk=2)
+    p.lmats, p.lreal, p.beta = 6, 5, 10.0
+    po = M.to_oracle(p)
+    r = R.vca_solve(po, hloc, hk)
+    wm, wr = R.frequency_grids(po)
+    hl = R.nnn2lso(hloc, 4, 1, 1)
+    for axis, zs in ((0, 1j * wm), (1, wr + 1j * po.eps)):
+        _, s, _ = R.sigma_grid(r["model"], r["gf"], axis)
+        for i in (0, len(zs) - 1):
+            z = zs[i]
+            se = (z + po.xmu) * np.eye(4) - hl - np.linalg.inv(exact_lehmann(p, hloc, z))
+            assert np.abs(R.nnn2lso(s[..., i], 4, 1, 1) - se).max() < 1e-9
+
+
 # --------------------------------------------------- Omega, quadrature --
 def test_quadrature_closed_form():
     for a in (0.3, 1.0, 4.0):

@@ -18,6 +18,8 @@ VCA_LANC_LANCZOS = 0
 VCA_LANC_ARPACK = 1
 VCA_OMEGA_QUAD = 0
 VCA_OMEGA_MATSUBARA = 1
+VCA_AXIS_MATSUBARA = 0
+VCA_AXIS_REAL = 1
 
 
 class CParams(ctypes.Structure):
@@ -40,6 +42,8 @@ class CParams(ctypes.Structure):
         ("lanc_niter_spectrum", ctypes.c_int32), ("lanc_ncv_factor", ctypes.c_int32),
         ("lanc_ncv_add", ctypes.c_int32),
         ("cutoff", ctypes.c_double), ("bandwidth", ctypes.c_double),
+        ("lmats", ctypes.c_int32), ("lreal", ctypes.c_int32),
+        ("wini", ctypes.c_double), ("wfin", ctypes.c_double), ("eps", ctypes.c_double),
     ]
 
 
@@ -54,7 +58,9 @@ class CResults(ctypes.Structure):
         ("omega_evals", ctypes.c_int64), ("omega_seconds", ctypes.c_double),
         ("zeta", ctypes.c_double), ("max_exc", ctypes.c_double),
         ("finite_t", ctypes.c_int32), ("nmax", ctypes.c_int32),
-        ("spectrum_iters", ctypes.c_int32),
+        ("spectrum_iters", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("npoles", ctypes.c_int64), ("grid_pole_evals", ctypes.c_int64),
+        ("grid_seconds", ctypes.c_double),
     ]
 
 
@@ -94,6 +100,7 @@ _SIGS = {
     "vca_gf_cluster": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, _P]),
     "vca_omega_integrand": (ctypes.c_int, [_P, _I32, _P, _I32, _P]),
     "vca_omega_integrand_complex": (ctypes.c_int, [_P, _I32, _P, _I32, _P]),
+    "vca_get_gf_grid": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
 }
 EXPORTS = tuple(_SIGS)
 

@@ -22,7 +22,8 @@ import numpy as np
 
 from . import _lib
 from ._lib import (VCA_FP64, VCA_MIXED, VCA_LANC_LANCZOS, VCA_LANC_ARPACK,  # noqa: F401
-                   VCA_OMEGA_QUAD, VCA_OMEGA_MATSUBARA, VcaError, check)
+                   VCA_OMEGA_QUAD, VCA_OMEGA_MATSUBARA, VCA_AXIS_MATSUBARA, VCA_AXIS_REAL,
+                   VcaError, check)
 
 
 @dataclass
@@ -62,6 +63,11 @@ class VcaParams:
     lanc_ncv_add: int = 5
     cutoff: float = 1e-9
     bandwidth: float = 0.0
+    lmats: int = 5000
+    lreal: int = 5000
+    wini: float = -5.0
+    wfin: float = 5.0
+    eps: float = 0.01
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -96,6 +102,8 @@ class VcaParams:
         c.lanc_niter_spectrum = self.lanc_niter_spectrum
         c.lanc_ncv_factor, c.lanc_ncv_add = self.lanc_ncv_factor, self.lanc_ncv_add
         c.cutoff, c.bandwidth = self.cutoff, self.bandwidth
+        c.lmats, c.lreal = self.lmats, self.lreal
+        c.wini, c.wfin, c.eps = self.wini, self.wfin, self.eps
         if self.sectors:
             arr = (ctypes.c_int32 * (2 * len(self.sectors)))(
                 *[int(x) for s in self.sectors for x in s])
@@ -349,6 +357,49 @@ class VcaSolver:
         check(self._lib.vca_omega_integrand_complex(self._h, z.size, z.ctypes.data, prec,
                                                     f.ctypes.data))
         return f
+
+    # --- G, Sigma, G0 on the frequency grids -----------------------------
+    def frequency_grid(self, axis):
+        """wm (axis "matsubara") or wr (axis "realaxis") of allocate_grids,
+        VCA_GF_SHARED.f90:43-56."""
+        p = self.params
+        if axis == "matsubara":
+            return (np.pi / p.beta) * (2 * np.arange(1, p.lmats + 1) - 1).astype(np.float64)
+        step = (p.wfin - p.wini) / float(p.lreal - 1)
+        return p.wini + np.arange(p.lreal, dtype=np.float64) * step
+
+    def gf_grid(self, axis, which=("g", "sigma", "g0")):
+        """The arrays named in `which` ("g", "sigma", "g0") on the Matsubara
+        or real-axis grid, each [Nlat,Nlat,Nspin,Nspin,Norb,Norb,L]."""
+        p = self.params
+        ax = {"matsubara": VCA_AXIS_MATSUBARA, "realaxis": VCA_AXIS_REAL}[axis]
+        L = p.lmats if ax == VCA_AXIS_MATSUBARA else p.lreal
+        out = {k: np.zeros(self._shape_hloc() + (L,), np.complex128, order="F") for k in which}
+        ptr = [out[k].ctypes.data if k in out else None for k in ("g", "sigma", "g0")]
+        check(self._lib.vca_get_gf_grid(self._h, ax, *ptr))
+        return out
+
+    def _grid_get(self, axis, which, ilat, jlat):
+        a = self.gf_grid(axis, (which,))[which]
+        if ilat is None:
+            return a
+        return np.asfortranarray(a[ilat - 1, jlat - 1])
+
+    def vca_get_gimp_matsubara(self, ilat=None, jlat=None):
+        """VCA_IO/get_gimp_matsubara.f90 (full, or the (ilat, jlat) slice)."""
+        return self._grid_get("matsubara", "g", ilat, jlat)
+
+    def vca_get_gimp_realaxis(self, ilat=None, jlat=None):
+        """VCA_IO/get_gimp_realaxis.f90."""
+        return self._grid_get("realaxis", "g", ilat, jlat)
+
+    def vca_get_sigma_matsubara(self, ilat=None, jlat=None):
+        """VCA_IO/get_sigma_matsubara.f90."""
+        return self._grid_get("matsubara", "sigma", ilat, jlat)
+
+    def vca_get_sigma_realaxis(self, ilat=None, jlat=None):
+        """VCA_IO/get_sigma_realaxis.f90."""
+        return self._grid_get("realaxis", "sigma", ilat, jlat)
 
 
 def vca_init_solver(params: VcaParams, bath_h=None, bath_v=None) -> VcaSolver:

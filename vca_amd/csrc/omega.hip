@@ -25,6 +25,12 @@
 //                16-lane group (row per lane); the pivot product is kept
 //                in FP64 as mantissa and exponent, one log per matrix
 //   k_reduce_z   deterministic sum of the k-block partials per z
+//   k_pole_grid  G_raw on the Matsubara or real-axis grid (grid points made
+//                in the kernel), VCA_GF_NORMAL.f90:1024-1031
+//   k_grid_sigma per (z, spin block): recombined G, G0^-1 = (z + mu) - Hloc
+//                - Delta(z), Sigma = G0^-1 - G^-1, G0, in the 7-index layout
+//                of impGmats / impSmats / impG0mats (VCA_GF_NORMAL.f90:
+//                1045-1111)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -635,7 +641,241 @@ __global__ __launch_bounds__(64) void k_reduce_phase(int nz, int nspin_blk, int 
   if (threadIdx.x == 0) fim[iz] = t / nk;
 }
 
+// ------------------------------------------- G, Sigma, G0 on a grid ----
+// Grid point iz of the Matsubara axis (i (2 iz + 1) pi / beta) or of the
+// real axis (wini + iz (wfin - wini)/(L - 1) + i eps).
+struct GridSpec {
+  int axis;      // 0 Matsubara, 1 real
+  double pib;    // pi / beta
+  double w0, dw, eps;
+  int iz0;       // first grid index of this pass
+};
+
+__device__ __forceinline__ double2 grid_z(const GridSpec &g, int iz) {
+  const int n = g.iz0 + iz;
+  return g.axis == 0 ? make_double2(0.0, g.pib * (double)(2 * n + 1))
+                     : make_double2(g.w0 + (double)n * g.dw, g.eps);
+}
+
+// graw[e][iz] = sum_p w_p / (z_iz - pole_p) over the poles of element e.
+// A workgroup covers kOBlock frequencies of one element, so every lane reads
+// the same pole at the same time (uniform loads) and the stores coalesce.
+__global__ __launch_bounds__(kOBlock) void k_pole_grid(int nz, GridSpec gs, const int64_t *off,
+                                                       const double2 *w, const double *pole,
+                                                       double2 *graw) {
+  const int e = blockIdx.y;
+  const int iz = blockIdx.x * kOBlock + threadIdx.x;
+  const double2 zz = grid_z(gs, iz < nz ? iz : 0);
+  double gr = 0, gi = 0;
+  for (int64_t p = off[e]; p < off[e + 1]; ++p) {
+    const double dr = zz.x - pole[p], di = zz.y;
+    const double den = 1.0 / (dr * dr + di * di);
+    const double2 ww = w[p];
+    gr += (ww.x * dr + ww.y * di) * den;
+    gi += (ww.y * dr - ww.x * di) * den;
+  }
+  if (iz < nz) graw[(int64_t)e * nz + iz] = make_double2(gr, gi);
+}
+
+struct SigmaArgs {
+  int nz, nspin_blk, nb, nlat, norb, nspin, nelem;
+  double chan4, xmu;
+  GridSpec gs;
+  const double2 *graw;      // [nelem][nz]
+  const double2 *hloc_blk;  // [nspin_blk][nb][nb]
+  int nbb;
+  const double2 *hb_blk, *vb_blk;
+  // outputs [Nlat][Nlat][Nspin][Nspin][Norb][Norb][nz], first index fastest
+  double2 *g, *sigma, *g0;
+};
+
+// One workgroup per (frequency, spin block): the block of G (off-diagonal
+// elements recombined from the mixed channels), G0^-1 = (z + mu) - Hloc -
+// Delta(z) with Delta = V (z + mu - h)^-1 V^+, Sigma = G0^-1 - G^-1 and
+// G0 = (G0^-1)^-1, all in LDS, written in the element order of the 7-index
+// arrays.
+__global__ __launch_bounds__(64) void k_grid_sigma(const SigmaArgs a) {
+  __shared__ double2 s_g[kNB * kNB], s_a[kNB * kNB], s_t[kNB * kNB], s_inv[kNB * kNB];
+  __shared__ double2 s_hb[kNB * kNB], s_hbinv[kNB * kNB];
+  __shared__ double s_ld;
+  __shared__ int s_piv;
+  const int iz = blockIdx.x, sb = blockIdx.y, tid = threadIdx.x;
+  const int nb = a.nb, no = a.norb, nl = a.nlat, nsp = a.nspin;
+  const int ls = nl * no;
+  const bool mixed = a.nspin_blk == 1 && nsp == 2;
+  const double2 zz = grid_z(a.gs, iz);
+  auto gel = [&](int sp, int i1, int j1, int o1, int o2) {
+    return a.graw[(int64_t)((((sp * nl + i1) * nl + j1) * no + o1) * no + o2) * a.nz + iz];
+  };
+  for (int e = tid; e < nb * nb; e += blockDim.x) {
+    const int r = e / nb, c = e % nb;
+    const int sr = mixed ? r / ls : sb, sc = mixed ? c / ls : sb;
+    double2 v = make_double2(0.0, 0.0);
+    if (sr == sc) {
+      const int rr = r % ls, cc = c % ls;
+      const int il = rr / no, io = rr % no, jl = cc / no, jo = cc % no;
+      v = gel(sr, il, jl, io, jo);
+      if (!(il == jl && io == jo)) {
+        const double2 gii = gel(sr, il, il, io, io), gjj = gel(sr, jl, jl, jo, jo);
+        const double sx = gii.x + gjj.x, sy = gii.y + gjj.y;
+        const double tx = sx + a.chan4 * sy, ty = sy - a.chan4 * sx;
+        v = make_double2(0.5 * (v.x - tx), 0.5 * (v.y - ty));
+      }
+    }
+    s_g[r * kNB + c] = v;
+    s_t[r * kNB + c] = v;
+    const double2 h = a.hloc_blk[((int64_t)sb * nb + r) * nb + c];
+    double2 g0i = make_double2(-h.x, -h.y);
+    if (r == c) {
+      g0i.x += zz.x + a.xmu;
+      g0i.y += zz.y;
+    }
+    s_a[r * kNB + c] = g0i;
+  }
+  if (a.nbb > 0) {
+    const int nbb = a.nbb;
+    const double2 *hb = a.hb_blk + (int64_t)sb * nbb * nbb;
+    for (int e = tid; e < nbb * nbb; e += blockDim.x) {
+      const int r = e / nbb, c = e % nbb;
+      const double2 h = hb[r * nbb + c];
+      double2 v = make_double2(-h.x, -h.y);
+      if (r == c) {
+        v.x += zz.x + a.xmu;
+        v.y += zz.y;
+      }
+      s_hb[r * kNB + c] = v;
+    }
+    __syncthreads();
+    gj_inverse(nbb, s_hb, s_hbinv, kNB, &s_ld, &s_piv);
+    const double2 *vb = a.vb_blk + (int64_t)sb * nb * nbb;
+    for (int e = tid; e < nb * nb; e += blockDim.x) {
+      const int r = e / nb, c = e % nb;
+      double dr = 0, di = 0;
+      for (int p = 0; p < nbb; ++p) {
+        const double2 vr = vb[r * nbb + p];
+        for (int q = 0; q < nbb; ++q) {
+          const double2 m = s_hbinv[p * kNB + q];
+          const double2 vc = vb[c * nbb + q];
+          const double tx = vr.x * m.x - vr.y * m.y, ty = vr.x * m.y + vr.y * m.x;
+          dr += tx * vc.x + ty * vc.y;
+          di += ty * vc.x - tx * vc.y;
+        }
+      }
+      s_a[r * kNB + c].x -= dr;
+      s_a[r * kNB + c].y -= di;
+    }
+  }
+  __syncthreads();
+  // position of block entry (r, c) in the 7-index output
+  auto oidx = [&](int r, int c) {
+    const int sr = mixed ? r / ls : sb, sc = mixed ? c / ls : sb;
+    const int rr = r % ls, cc = c % ls;
+    const int il = rr / no, io = rr % no, jl = cc / no, jo = cc % no;
+    return (int64_t)il +
+           (int64_t)nl * (jl + (int64_t)nl * (sr + (int64_t)nsp *
+                                                       (sc + (int64_t)nsp *
+                                                                 (io + (int64_t)no *
+                                                                           (jo + (int64_t)no * iz)))));
+  };
+  if (a.sigma) {
+    gj_inverse(nb, s_t, s_inv, kNB, &s_ld, &s_piv);  // G^-1
+    for (int e = tid; e < nb * nb; e += blockDim.x) {
+      const int r = e / nb, c = e % nb;
+      const double2 x = s_a[r * kNB + c], y = s_inv[r * kNB + c];
+      a.sigma[oidx(r, c)] = make_double2(x.x - y.x, x.y - y.y);
+    }
+    __syncthreads();
+  }
+  if (a.g)
+    for (int e = tid; e < nb * nb; e += blockDim.x) {
+      const int r = e / nb, c = e % nb;
+      a.g[oidx(r, c)] = s_g[r * kNB + c];
+    }
+  if (a.g0) {
+    for (int e = tid; e < nb * nb; e += blockDim.x) {
+      const int r = e / nb, c = e % nb;
+      s_t[r * kNB + c] = s_a[r * kNB + c];
+    }
+    __syncthreads();
+    gj_inverse(nb, s_t, s_inv, kNB, &s_ld, &s_piv);
+    for (int e = tid; e < nb * nb; e += blockDim.x) {
+      const int r = e / nb, c = e % nb;
+      a.g0[oidx(r, c)] = s_inv[r * kNB + c];
+    }
+  }
+}
+
 // ================================================================ host ===
+float OmegaDev::grid_gf(int axis, int L, double beta, double wini, double wfin, double eps,
+                        double2 *g, double2 *sigma, double2 *g0) {
+  if (L <= 0 || !(g || sigma || g0)) return 0.0f;
+  const hipStream_t st = stream;
+  const int nlso = host.nlat * host.nspin * host.norb;
+  const int64_t per_z = (int64_t)nlso * nlso;
+  const int chunk = std::min(L, 4096);
+  double2 *d_raw = nullptr, *d_out[3] = {nullptr, nullptr, nullptr};
+  double2 *h_out[3] = {g, sigma, g0};
+  VCA_HIP(hipMalloc(&d_raw, sizeof(double2) * (size_t)chunk * std::max(host.nelem, 1)));
+  for (int i = 0; i < 3; ++i)
+    if (h_out[i]) VCA_HIP(hipMalloc(&d_out[i], sizeof(double2) * (size_t)chunk * per_z));
+  GridSpec gs{};
+  gs.axis = axis;
+  gs.pib = M_PI / beta;
+  gs.w0 = wini;
+  gs.dw = L > 1 ? (wfin - wini) / (double)(L - 1) : 0.0;
+  gs.eps = eps;
+  hipEvent_t e0, e1;
+  VCA_HIP(hipEventCreate(&e0));
+  VCA_HIP(hipEventCreate(&e1));
+  float ms_total = 0.0f;
+  for (int z0 = 0; z0 < L; z0 += chunk) {
+    const int nz = std::min(chunk, L - z0);
+    gs.iz0 = z0;
+    VCA_HIP(hipEventRecord(e0, st));
+    hipLaunchKernelGGL(k_pole_grid, dim3((nz + kOBlock - 1) / kOBlock, host.nelem),
+                       dim3(kOBlock), 0, st, nz, gs, d_off, d_w, d_pole, d_raw);
+    VCA_HIP(hipGetLastError());
+    VCA_HIP(hipEventRecord(e1, st));
+    for (int i = 0; i < 3; ++i)
+      if (d_out[i])
+        VCA_HIP(hipMemsetAsync(d_out[i], 0, sizeof(double2) * (size_t)nz * per_z, st));
+    SigmaArgs sa{};
+    sa.nz = nz;
+    sa.nspin_blk = host.nspin_blk;
+    sa.nb = host.nb;
+    sa.nlat = host.nlat;
+    sa.norb = host.norb;
+    sa.nspin = host.nspin;
+    sa.nelem = host.nelem;
+    sa.chan4 = host.chan4;
+    sa.xmu = host.xmu;
+    sa.gs = gs;
+    sa.graw = d_raw;
+    sa.hloc_blk = d_hloc;
+    sa.nbb = host.nbb;
+    sa.hb_blk = d_hb;
+    sa.vb_blk = d_vb;
+    sa.g = d_out[0];
+    sa.sigma = d_out[1];
+    sa.g0 = d_out[2];
+    hipLaunchKernelGGL(k_grid_sigma, dim3(nz, host.nspin_blk), dim3(64), 0, st, sa);
+    VCA_HIP(hipGetLastError());
+    for (int i = 0; i < 3; ++i)
+      if (d_out[i])
+        VCA_HIP(hipMemcpyAsync(h_out[i] + (int64_t)z0 * per_z, d_out[i],
+                               sizeof(double2) * (size_t)nz * per_z, hipMemcpyDeviceToHost, st));
+    VCA_HIP(hipStreamSynchronize(st));
+    float ms = 0.0f;
+    VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
+    ms_total += ms;
+  }
+  VCA_HIP(hipEventDestroy(e0));
+  VCA_HIP(hipEventDestroy(e1));
+  for (void *p : {(void *)d_raw, (void *)d_out[0], (void *)d_out[1], (void *)d_out[2]})
+    if (p) VCA_HIP(hipFree(p));
+  return ms_total;
+}
+
 OmegaDev::~OmegaDev() { release(); }
 
 void OmegaDev::release() {

@@ -54,6 +54,12 @@ struct OmegaDev {
   // logarithm of sum_kmesh_complex).
   void integrand(int nz, const double2 *z, int precision, double *f, float *ms_logdet,
                  double *fim = nullptr);
+  // G, Sigma and G0 on L points of the Matsubara (axis 0) or real (axis 1)
+  // grid, written to the host arrays that are not null, each
+  // [Nlat][Nlat][Nspin][Nspin][Norb][Norb][L] with the first index fastest.
+  // Returns the GPU time (ms) of the pole-sum kernel.
+  float grid_gf(int axis, int L, double beta, double wini, double wfin, double eps, double2 *g,
+                double2 *sigma, double2 *g0);
 };
 
 }  // namespace vca

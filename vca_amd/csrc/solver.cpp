@@ -1119,6 +1119,7 @@ static void prepare_omega(Solver &s) {
               }
       }
     }
+  s.res.npoles = (int64_t)h.w.size();
   if (!s.om) s.om = std::make_unique<OmegaDev>();
   s.om->upload(h, s.stream);
   s.om_ready = true;
@@ -1353,6 +1354,11 @@ void vca_default_params(vca_params *p) {
   p->lanc_ncv_add = 5;
   p->cutoff = 1e-9;
   p->bandwidth = 0.0;
+  p->lmats = 5000;
+  p->lreal = 5000;
+  p->wini = -5.0;
+  p->wfin = 5.0;
+  p->eps = 0.01;
 }
 
 const char *vca_last_error(void) { return g_err.c_str(); }
@@ -1750,6 +1756,27 @@ int vca_gf_cluster(const vca_solver *sv, double zre, double zim, double *gfo) {
                 0.5 * (at(il, jl, sp, io, jo) - f * at(il, il, sp, io, io) - f * at(jl, jl, sp, jo, jo));
           }
   std::memcpy(gfo, g.data(), sizeof(zc) * g.size());
+  VCA_GUARD_END
+}
+
+int vca_get_gf_grid(vca_solver *sv, int32_t axis, double *gimp, double *sigma, double *g0) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s) fail(VCA_ERR_ARG, "null solver");
+  if (axis != VCA_AXIS_MATSUBARA && axis != VCA_AXIS_REAL)
+    fail(VCA_ERR_ARG, "axis must be VCA_AXIS_MATSUBARA or VCA_AXIS_REAL");
+  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  const vca_params &p = s->p;
+  const int L = axis == VCA_AXIS_MATSUBARA ? p.lmats : p.lreal;
+  if (axis == VCA_AXIS_MATSUBARA && (L < 1 || !(p.beta > 0)))
+    fail(VCA_ERR_ARG, "Matsubara grid needs LMATS >= 1 and BETA > 0");
+  if (axis == VCA_AXIS_REAL && L < 2) fail(VCA_ERR_ARG, "real-axis grid needs LREAL >= 2");
+  if (!s->om_ready) prepare_omega(*s);
+  const float ms =
+      s->om->grid_gf(axis, L, p.beta, p.wini, p.wfin, p.eps, reinterpret_cast<double2 *>(gimp),
+                     reinterpret_cast<double2 *>(sigma), reinterpret_cast<double2 *>(g0));
+  s->res.grid_seconds += ms * 1e-3;
+  s->res.grid_pole_evals += (int64_t)s->om->host.w.size() * L;
   VCA_GUARD_END
 }
 

@@ -27,6 +27,8 @@ module vca_gpu
   integer, public :: lanc_nstates_total = 1, lanc_nstates_step = 2, lanc_niter_spectrum = 10
   integer, public :: lanc_ncv_factor = 3, lanc_ncv_add = 5
   real(8), public :: cutoff = 1d-9, bandwidth = 0d0
+  integer, public :: Lmats = 5000, Lreal = 5000
+  real(8), public :: wini = -5d0, wfin = 5d0, eps = 0.01d0
   ! additions of this build
   integer, public :: vca_precision = 0        ! 0 FP64, 1 MIXED
   integer, public :: vca_omega_mode = 0       ! 0 adaptive quadrature, 1 Matsubara grid
@@ -48,6 +50,8 @@ module vca_gpu
      integer(c_int32_t) :: lanc_nstates_total, lanc_nstates_step, lanc_niter_spectrum
      integer(c_int32_t) :: lanc_ncv_factor, lanc_ncv_add
      real(c_double)     :: cutoff, bandwidth
+     integer(c_int32_t) :: lmats, lreal
+     real(c_double)     :: wini, wfin, eps
   end type vca_params_t
 
   ! vca_results of include/vca_gpu.h (same field order and kinds)
@@ -58,7 +62,9 @@ module vca_gpu
      real(c_double)     :: hxv_states, hxv_seconds, t_diag, t_gf, t_omega
      integer(c_int64_t) :: omega_evals
      real(c_double)     :: omega_seconds, zeta, max_exc
-     integer(c_int32_t) :: finite_t, nmax, spectrum_iters
+     integer(c_int32_t) :: finite_t, nmax, spectrum_iters, reserved
+     integer(c_int64_t) :: npoles, grid_pole_evals
+     real(c_double)     :: grid_seconds
   end type vca_results_t
 
   interface
@@ -98,6 +104,12 @@ module vca_gpu
        type(vca_results_t), intent(out) :: out
        integer(c_int) :: rc
      end function c_get_results
+     function c_get_gf_grid(s, axis, g, sg, g0) bind(C, name="vca_get_gf_grid") result(rc)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: s, g, sg, g0
+       integer(c_int32_t), value :: axis
+       integer(c_int) :: rc
+     end function c_get_gf_grid
   end interface
 
   type(c_ptr), save :: handle = c_null_ptr
@@ -115,7 +127,27 @@ module vca_gpu
      module procedure vca_solve_comm
   end interface vca_solve
 
+  ! VCA_IO.f90:15-36
+  interface vca_get_gimp_matsubara
+     module procedure vca_get_gimp_matsubara_full
+     module procedure vca_get_gimp_matsubara_ij
+  end interface vca_get_gimp_matsubara
+  interface vca_get_gimp_realaxis
+     module procedure vca_get_gimp_real_full
+     module procedure vca_get_gimp_real_ij
+  end interface vca_get_gimp_realaxis
+  interface vca_get_sigma_matsubara
+     module procedure vca_get_sigma_matsubara_full
+     module procedure vca_get_sigma_matsubara_ij
+  end interface vca_get_sigma_matsubara
+  interface vca_get_sigma_realaxis
+     module procedure vca_get_sigma_real_full
+     module procedure vca_get_sigma_real_ij
+  end interface vca_get_sigma_realaxis
+
   public :: vca_init_solver, vca_solve, vca_get_sft_potential, vca_finalize_solver
+  public :: vca_get_gimp_matsubara, vca_get_gimp_realaxis
+  public :: vca_get_sigma_matsubara, vca_get_sigma_realaxis
   public :: vca_params_t, vca_results_t
 
 contains
@@ -159,6 +191,8 @@ contains
     cparams%lanc_niter_spectrum = lanc_niter_spectrum
     cparams%lanc_ncv_factor = lanc_ncv_factor; cparams%lanc_ncv_add = lanc_ncv_add
     cparams%cutoff = cutoff; cparams%bandwidth = bandwidth
+    cparams%lmats = Lmats; cparams%lreal = Lreal
+    cparams%wini = wini; cparams%wfin = wfin; cparams%eps = eps
     cparams%nsectors = 0; cparams%sectors = c_null_ptr
     if (allocated(vca_sectors_list)) then
        if (allocated(csectors)) deallocate(csectors)
@@ -240,6 +274,80 @@ contains
     real(8), intent(out) :: sft
     call check(c_get_sft(handle, sft), "vca_get_sft_potential")
   end subroutine vca_get_sft_potential
+
+  ! One array (which: 1 G, 2 Sigma, 3 G0) on the Matsubara (axis 0) or real
+  ! (axis 1) grid of the solver, [Nlat,Nlat,Nspin,Nspin,Norb,Norb,L].
+  subroutine grid_full(axis, which, A, name)
+    integer, intent(in) :: axis, which
+    complex(8), intent(inout), target, contiguous :: A(:,:,:,:,:,:,:)
+    character(len=*), intent(in) :: name
+    type(c_ptr) :: p(3)
+    integer :: L
+    if (.not. c_associated(handle)) stop "vca_get: solver not initialised"
+    L = cparams%lmats
+    if (axis == 1) L = cparams%lreal
+    if (any(shape(A) /= [Nlat, Nlat, Nspin, Nspin, Norb, Norb, L])) then
+       write(*, "(A)") name // ": wrong array shape"
+       stop 1
+    end if
+    p = c_null_ptr
+    p(which) = c_loc(A)
+    call check(c_get_gf_grid(handle, int(axis, c_int32_t), p(1), p(2), p(3)), name)
+  end subroutine grid_full
+
+  subroutine grid_ij(axis, which, A, ilat, jlat, name)
+    integer, intent(in) :: axis, which, ilat, jlat
+    complex(8), intent(inout) :: A(:,:,:,:,:)
+    character(len=*), intent(in) :: name
+    complex(8), allocatable, target :: full(:,:,:,:,:,:,:)
+    integer :: L
+    L = cparams%lmats
+    if (axis == 1) L = cparams%lreal
+    allocate(full(Nlat, Nlat, Nspin, Nspin, Norb, Norb, L))
+    call grid_full(axis, which, full, name)
+    A = full(ilat, jlat, :, :, :, :, :)
+  end subroutine grid_ij
+
+  ! VCA_IO/get_gimp_matsubara.f90
+  subroutine vca_get_gimp_matsubara_full(Gmats)
+    complex(8), intent(inout), target, contiguous :: Gmats(:,:,:,:,:,:,:)
+    call grid_full(0, 1, Gmats, "vca_get_gimp_matsubara")
+  end subroutine vca_get_gimp_matsubara_full
+  subroutine vca_get_gimp_matsubara_ij(Gmats, ilat, jlat)
+    complex(8), intent(inout) :: Gmats(:,:,:,:,:)
+    integer, intent(in) :: ilat, jlat
+    call grid_ij(0, 1, Gmats, ilat, jlat, "vca_get_gimp_matsubara")
+  end subroutine vca_get_gimp_matsubara_ij
+  ! VCA_IO/get_gimp_realaxis.f90
+  subroutine vca_get_gimp_real_full(Greal)
+    complex(8), intent(inout), target, contiguous :: Greal(:,:,:,:,:,:,:)
+    call grid_full(1, 1, Greal, "vca_get_gimp_realaxis")
+  end subroutine vca_get_gimp_real_full
+  subroutine vca_get_gimp_real_ij(Greal, ilat, jlat)
+    complex(8), intent(inout) :: Greal(:,:,:,:,:)
+    integer, intent(in) :: ilat, jlat
+    call grid_ij(1, 1, Greal, ilat, jlat, "vca_get_gimp_realaxis")
+  end subroutine vca_get_gimp_real_ij
+  ! VCA_IO/get_sigma_matsubara.f90
+  subroutine vca_get_sigma_matsubara_full(Smats)
+    complex(8), intent(inout), target, contiguous :: Smats(:,:,:,:,:,:,:)
+    call grid_full(0, 2, Smats, "vca_get_sigma_matsubara")
+  end subroutine vca_get_sigma_matsubara_full
+  subroutine vca_get_sigma_matsubara_ij(Smats, ilat, jlat)
+    complex(8), intent(inout) :: Smats(:,:,:,:,:)
+    integer, intent(in) :: ilat, jlat
+    call grid_ij(0, 2, Smats, ilat, jlat, "vca_get_sigma_matsubara")
+  end subroutine vca_get_sigma_matsubara_ij
+  ! VCA_IO/get_sigma_realaxis.f90
+  subroutine vca_get_sigma_real_full(Sreal)
+    complex(8), intent(inout), target, contiguous :: Sreal(:,:,:,:,:,:,:)
+    call grid_full(1, 2, Sreal, "vca_get_sigma_realaxis")
+  end subroutine vca_get_sigma_real_full
+  subroutine vca_get_sigma_real_ij(Sreal, ilat, jlat)
+    complex(8), intent(inout) :: Sreal(:,:,:,:,:)
+    integer, intent(in) :: ilat, jlat
+    call grid_ij(1, 2, Sreal, ilat, jlat, "vca_get_sigma_realaxis")
+  end subroutine vca_get_sigma_real_ij
 
   subroutine vca_finalize_solver()
     if (c_associated(handle)) call c_finalize_solver(handle)
