@@ -36,6 +36,12 @@
  *                        (impGmats/impGreal of VCA_GF_NORMAL.f90:89-101,
  *                        1024-1031; impSmats/impSreal/impG0* of
  *                        build_sigma_normal, VCA_GF_NORMAL.f90:1045-1111)
+ *   vca_save_gfprime     VCA_AUX_FUNX.f90:608-652 (save_gfprime) with the
+ *                        GFmatrix defined I/O at :484-604
+ *   vca_read_gfprime     VCA_AUX_FUNX.f90:654-697 (read_gfprime)
+ *   vca_omega_restart    VCA_MAIN.f90:181-197 (the Omega stage of vca_solve)
+ *                        on a pole store read back, the restart that
+ *                        VCA_MAIN.f90:280-281 sketches
  */
 #ifndef VCA_GPU_H
 #define VCA_GPU_H
@@ -253,6 +259,25 @@ int vca_omega_integrand_complex(vca_solver *s, int32_t nz, const double *z,
  * inversions run on the GPU. */
 int vca_get_gf_grid(vca_solver *s, int32_t axis, double *gimp, double *sigma,
                     double *g0);
+/* Writes the pole store to file + ".restart" (".used" when used != 0) in
+ * the reference's layout: one entry per (ilat, jlat, ispin, iorb, jorb),
+ * jorb fastest, each Nstates, per state Nchan, per channel Nexc, poles,
+ * weights (complex128).  use_formatted = 0: Fortran sequential unformatted,
+ * one record per entry with 4-byte markers; 1: list-directed text. */
+int vca_save_gfprime(const vca_solver *s, const char *file, int32_t used,
+                     int32_t use_formatted);
+/* Reads such a file (also those the reference writes) into the pole store:
+ * vca_gf_cluster, vca_gf_poles and vca_save_gfprime then use it; the Omega
+ * integrands and the grid outputs need vca_omega_restart (or vca_solve). */
+int vca_read_gfprime(vca_solver *s, const char *file, int32_t used,
+                     int32_t use_formatted);
+/* Omega = omegaprime - integral from the current pole store without
+ * diagonalising: sets Hloc, H(k) and the bath as vca_solve does and runs its
+ * frequency integration; results as after vca_solve.  After a file read the
+ * finite-T Matsubara cutoff takes the largest |pole| as the largest
+ * excitation (the file keeps no excitation energies). */
+int vca_omega_restart(vca_solver *s, const double *hloc, const double *hk, int64_t nk,
+                      const double *bath_h, const double *bath_v, double omegaprime);
 
 #ifdef __cplusplus
 }

@@ -712,6 +712,135 @@ def sigma_grid(model: Model, gd: GFData, axis):
     return g, _lso2nnn_many(g0inv - ginv, p), _lso2nnn_many(np.linalg.inv(g0inv), p)
 
 
+# ------------------------------------------------------------ pole file
+def _gf_file_order(p: Params):
+    """Loop order of save_gfprime / read_gfprime (VCA_AUX_FUNX.f90:633-647):
+    ilat, jlat, ispin, iorb, jorb with jorb innermost (1-based keys)."""
+    for il in range(1, p.nlat + 1):
+        for jl in range(1, p.nlat + 1):
+            for s in range(1, p.nspin + 1):
+                for io in range(1, p.norb + 1):
+                    for jo in range(1, p.norb + 1):
+                        yield (il, jl, s, io, jo)
+
+
+def save_gfprime(gd: GFData, path, formatted=False):
+    """save_gfprime with the GFmatrix defined I/O (VCA_AUX_FUNX.f90:484-604,
+    608-652).  Unformatted: one Fortran sequential record per element (the
+    child transfers of a defined WRITE do not start records), 4-byte length
+    markers, int32 counts, complex128 poles and weights.  Formatted:
+    list-directed, one line per element, (re,im) complex items and the
+    literal \\n item after each state."""
+    import struct
+    with open(path, "w" if formatted else "wb") as f:
+        for key in _gf_file_order(gd.p):
+            states = gd.elems.get(key, [])
+            if formatted:
+                items = [f"{len(states):12d}"]
+                for chans in states:
+                    items.append(f"{len(chans):12d}")
+                    for (w, poles) in chans:
+                        items.append(f"{len(poles):12d}")
+                        items += [f"({complex(x).real:.17E},{complex(x).imag:.17E})" for x in poles]
+                        items += [f"({complex(x).real:.17E},{complex(x).imag:.17E})" for x in w]
+                    items.append("\\n")
+                f.write(" ".join(items) + "\n")
+            else:
+                rec = [struct.pack("<i", len(states))]
+                for chans in states:
+                    rec.append(struct.pack("<i", len(chans)))
+                    for (w, poles) in chans:
+                        rec.append(struct.pack("<i", len(poles)))
+                        rec.append(np.asarray(poles, dtype="<c16").tobytes())
+                        rec.append(np.asarray(w, dtype="<c16").tobytes())
+                body = b"".join(rec)
+                f.write(struct.pack("<i", len(body)) + body + struct.pack("<i", len(body)))
+
+
+def _fortran_list_tokens(text):
+    """List-directed items of a text: blanks, commas and line ends separate;
+    (re,im) is one item; r*item repeats; the \\n character items are
+    dropped."""
+    import re
+    tok = re.compile(r"\s*(?:(\d+)\*)?(\([^)]*\)|[^\s,()]+)\s*,?")
+    out = []
+    for m in tok.finditer(text):
+        item = m.group(2)
+        if item == "\\n":
+            continue
+        out += [item] * (int(m.group(1)) if m.group(1) else 1)
+    return out
+
+
+def _fortran_complex(item):
+    re_, im_ = item.strip("()").split(",")
+    return complex(float(re_.replace("D", "E").replace("d", "e")),
+                   float(im_.replace("D", "E").replace("d", "e")))
+
+
+def read_gfprime(p: Params, path, formatted=False) -> GFData:
+    """read_gfprime (VCA_AUX_FUNX.f90:654-697) into a GFData (emin and zeta
+    are not in the file).  Poles keep their complex values."""
+    import struct
+    gd = GFData(p, 0.0, 1.0)
+    if formatted:
+        with open(path) as f:
+            items = iter(_fortran_list_tokens(f.read()))
+        for key in _gf_file_order(p):
+            states = []
+            for _ in range(int(next(items))):
+                chans = []
+                for _ in range(int(next(items))):
+                    n = int(next(items))
+                    poles = np.array([_fortran_complex(next(items)) for _ in range(n)])
+                    w = np.array([_fortran_complex(next(items)) for _ in range(n)])
+                    chans.append((w.astype(np.complex128), poles.astype(np.complex128)))
+                states.append(chans)
+            gd.elems[key] = states
+        if next(items, None) is not None:
+            raise ValueError("data after the last element")
+        return gd
+    with open(path, "rb") as f:
+        data = f.read()
+    pos = 0
+    for key in _gf_file_order(p):
+        (n1,) = struct.unpack_from("<i", data, pos)
+        rec = data[pos + 4:pos + 4 + n1]
+        (n2,) = struct.unpack_from("<i", data, pos + 4 + n1)
+        if n1 != n2:
+            raise ValueError("record markers differ")
+        pos += n1 + 8
+        q = 0
+
+        def i32():
+            nonlocal q
+            (v,) = struct.unpack_from("<i", rec, q)
+            q += 4
+            return v
+
+        def c16(n):
+            nonlocal q
+            a = np.frombuffer(rec, dtype="<c16", count=n, offset=q).astype(np.complex128)
+            q += 16 * n
+            return a
+
+        states = []
+        for _ in range(i32()):
+            chans = []
+            for _ in range(i32()):
+                n = i32()
+                poles = c16(n)
+                w = c16(n)
+                chans.append((w, poles))
+            states.append(chans)
+        if q != len(rec):
+            raise ValueError("record longer than its counts")
+        gd.elems[key] = states
+    if pos != len(data):
+        raise ValueError("records beyond the last element")
+    return gd
+
+
 def sum_kmesh(model: Model, gd: GFData, hk_lso, omega, complex_out=False):
     """sum_kmesh / sum_kmesh_complex, VCA_OMEGA.f90:37-116:
     (1/Nk) sum_k log|det(1 + (Delta + Hloc - H_k) G')| in lso order."""

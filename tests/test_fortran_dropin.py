@@ -4,6 +4,7 @@ import os
 import shutil
 import subprocess
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -61,6 +62,18 @@ def test_fortran_driver_matches_python_mirror(tmp_path):
                      ("SIGMA_MATS_23_7", sm[1, 2, 0, 0, 0, 0, 6]),
                      ("GREAL_12_2500", gr[0, 0, 0, 0, 2499])):
         assert abs(val(tag) - ref) < 1e-12 * max(1.0, abs(ref)), (tag, val(tag), ref)
+    # save_gfprime from Fortran: the same records as the Python mirror's file
+    from oracle import vca_ref as R
+    from tests._models import to_oracle
+    s.save_gfprime(tmp_path / "gfprime_py")
+    gf_f = R.read_gfprime(to_oracle(p), tmp_path / "gfprime_f.restart")
+    gf_p = R.read_gfprime(to_oracle(p), str(tmp_path / "gfprime_py") + ".restart")
+    assert gf_f.elems.keys() == gf_p.elems.keys()
+    for k in gf_p.elems:
+        for a, b in zip(gf_f.elems[k], gf_p.elems[k]):
+            for (w1, p1), (w2, p2) in zip(a, b):
+                assert len(p1) == len(p2)
+                assert np.allclose(p1, p2, rtol=0, atol=1e-11) and np.allclose(w1, w2, atol=1e-11)
 
 
 @needs_fc

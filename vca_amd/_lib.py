@@ -101,6 +101,9 @@ _SIGS = {
     "vca_omega_integrand": (ctypes.c_int, [_P, _I32, _P, _I32, _P]),
     "vca_omega_integrand_complex": (ctypes.c_int, [_P, _I32, _P, _I32, _P]),
     "vca_get_gf_grid": (ctypes.c_int, [_P, _I32, _P, _P, _P]),
+    "vca_save_gfprime": (ctypes.c_int, [_P, ctypes.c_char_p, _I32, _I32]),
+    "vca_read_gfprime": (ctypes.c_int, [_P, ctypes.c_char_p, _I32, _I32]),
+    "vca_omega_restart": (ctypes.c_int, [_P, _P, _P, ctypes.c_int64, _P, _P, ctypes.c_double]),
 }
 EXPORTS = tuple(_SIGS)
 

@@ -358,6 +358,35 @@ class VcaSolver:
                                                     f.ctypes.data))
         return f
 
+    # --- pole file (checkpoint) -----------------------------------------
+    def save_gfprime(self, file="gfprime", used=False, use_formatted=False):
+        """save_gfprime, VCA_AUX_FUNX.f90:608-652: writes file.restart
+        (file.used when used)."""
+        check(self._lib.vca_save_gfprime(self._h, str(file).encode(), int(bool(used)),
+                                         int(bool(use_formatted))))
+
+    def read_gfprime(self, file="gfprime", used=False, use_formatted=False):
+        """read_gfprime, VCA_AUX_FUNX.f90:654-697: replaces the pole store."""
+        check(self._lib.vca_read_gfprime(self._h, str(file).encode(), int(bool(used)),
+                                         int(bool(use_formatted))))
+
+    def omega_restart(self, hloc, hk, omegaprime, bath_h=None, bath_v=None):
+        """Omega = omegaprime - integral from the current pole store, without
+        diagonalising (the Omega stage of vca_solve, VCA_MAIN.f90:181-197)."""
+        a = _carr(hloc, self._shape_hloc())
+        hk = np.asfortranarray(np.asarray(hk, dtype=np.complex128))
+        if hk.ndim != 7 or hk.shape[:6] != self._shape_hloc():
+            raise VcaError(-1, "STOP: wrong lattice matrix dimensions")
+        bh = bv = None
+        if bath_h is not None and bath_v is not None:
+            sh, sv = self._shape_bath()
+            bh, bv = _carr(bath_h, sh), _carr(bath_v, sv)
+        check(self._lib.vca_omega_restart(self._h, a.ctypes.data, hk.ctypes.data, hk.shape[6],
+                                          bh.ctypes.data if bh is not None else None,
+                                          bv.ctypes.data if bv is not None else None,
+                                          float(omegaprime)))
+        return self.vca_get_sft_potential()
+
     # --- G, Sigma, G0 on the frequency grids -----------------------------
     def frequency_grid(self, axis):
         """wm (axis "matsubara") or wr (axis "realaxis") of allocate_grids,

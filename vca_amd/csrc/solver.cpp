@@ -97,6 +97,7 @@ struct Solver {
   double emin = 0, zeta = 1, max_exc = 0;
   std::unique_ptr<OmegaDev> om;
   bool om_ready = false;
+  bool gf_loaded = false;  // pole store read from a file (vca_read_gfprime)
   std::vector<KeptState> states;
   ~Solver();
 };
@@ -1707,7 +1708,7 @@ int vca_gf_poles(const vca_solver *sv, int32_t ilat, int32_t jlat, int32_t ispin
   VCA_GUARD_BEGIN
   const Solver *s = reinterpret_cast<const Solver *>(sv);
   if (!s || !n) fail(VCA_ERR_ARG, "null argument");
-  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  if (!s->solved && !s->gf_loaded) fail(VCA_ERR_STATE, "no Green's function: vca_solve or vca_read_gfprime first");
   auto it = s->gf.find({ilat, jlat, ispin, iorb, jorb});
   if (it == s->gf.end()) fail(VCA_ERR_ARG, "no such element");
   if (istate < 1 || istate > (int)it->second.size()) fail(VCA_ERR_ARG, "bad state index");
@@ -1729,7 +1730,7 @@ int vca_gf_cluster(const vca_solver *sv, double zre, double zim, double *gfo) {
   VCA_GUARD_BEGIN
   const Solver *s = reinterpret_cast<const Solver *>(sv);
   if (!s || !gfo) fail(VCA_ERR_ARG, "null argument");
-  if (!s->solved) fail(VCA_ERR_STATE, "vca_solve has not completed");
+  if (!s->solved && !s->gf_loaded) fail(VCA_ERR_STATE, "no Green's function: vca_solve or vca_read_gfprime first");
   const vca_params &p = s->p;
   const int nl = p.nlat, no = p.norb, nsp = p.nspin;
   const zc z(zre, zim);
@@ -1756,6 +1757,123 @@ int vca_gf_cluster(const vca_solver *sv, double zre, double zim, double *gfo) {
                 0.5 * (at(il, jl, sp, io, jo) - f * at(il, il, sp, io, io) - f * at(jl, jl, sp, jo, jo));
           }
   std::memcpy(gfo, g.data(), sizeof(zc) * g.size());
+  VCA_GUARD_END
+}
+
+// save_gfprime / read_gfprime file name: file + ".restart" (".used" when
+// used), VCA_AUX_FUNX.f90:620-622 (the reference's file_suffix is empty by
+// default and is not an input of this path).
+static std::string gfprime_name(const char *file, int32_t used) {
+  if (!file || !*file) fail(VCA_ERR_ARG, "empty file name");
+  return std::string(file) + (used ? ".used" : ".restart");
+}
+
+int vca_save_gfprime(const vca_solver *sv, const char *file, int32_t used,
+                     int32_t use_formatted) {
+  VCA_GUARD_BEGIN
+  const Solver *s = reinterpret_cast<const Solver *>(sv);
+  if (!s) fail(VCA_ERR_ARG, "null solver");
+  if (!s->solved && !s->gf_loaded)
+    fail(VCA_ERR_STATE, "vca_gf_cluster ERROR: impGmatrix not allocated!");
+  const vca_params &p = s->p;
+  std::vector<GfEntry> el;
+  el.reserve((size_t)p.nlat * p.nlat * p.nspin * p.norb * p.norb);
+  for (int il = 1; il <= p.nlat; ++il)
+    for (int jl = 1; jl <= p.nlat; ++jl)
+      for (int sp = 1; sp <= p.nspin; ++sp)
+        for (int io = 1; io <= p.norb; ++io)
+          for (int jo = 1; jo <= p.norb; ++jo) {
+            GfEntry e;
+            auto it = s->gf.find({il, jl, sp, io, jo});
+            if (it != s->gf.end())
+              for (const auto &chans : it->second) {
+                e.emplace_back();
+                for (const GfChan &c : chans) {
+                  GfChanIO io_c;
+                  io_c.w = c.w;
+                  io_c.p.assign(c.p.begin(), c.p.end());
+                  e.back().push_back(std::move(io_c));
+                }
+              }
+            el.push_back(std::move(e));
+          }
+  gfprime_write(gfprime_name(file, used), use_formatted != 0, el);
+  VCA_GUARD_END
+}
+
+int vca_read_gfprime(vca_solver *sv, const char *file, int32_t used, int32_t use_formatted) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s) fail(VCA_ERR_ARG, "null solver");
+  const vca_params &p = s->p;
+  const std::string path = gfprime_name(file, used);
+  std::vector<GfEntry> el = gfprime_read(
+      path, use_formatted != 0, (size_t)p.nlat * p.nlat * p.nspin * p.norb * p.norb);
+  std::map<ElemKey, std::vector<std::vector<GfChan>>> gf;
+  double max_pole = 0.0;
+  size_t k = 0;
+  for (int il = 1; il <= p.nlat; ++il)
+    for (int jl = 1; jl <= p.nlat; ++jl)
+      for (int sp = 1; sp <= p.nspin; ++sp)
+        for (int io = 1; io <= p.norb; ++io)
+          for (int jo = 1; jo <= p.norb; ++jo) {
+            auto &dst = gf[{il, jl, sp, io, jo}];
+            for (const auto &st : el[k]) {
+              dst.emplace_back();
+              for (const GfChanIO &c : st) {
+                GfChan g;
+                g.w = c.w;
+                g.p.resize(c.p.size());
+                for (size_t j = 0; j < c.p.size(); ++j) {
+                  if (c.p[j].imag() != 0.0)
+                    fail(VCA_ERR_ARG, "%s: complex pole (%g,%g); the cluster poles are real",
+                         path.c_str(), c.p[j].real(), c.p[j].imag());
+                  g.p[j] = c.p[j].real();
+                  max_pole = std::max(max_pole, std::fabs(g.p[j]));
+                }
+                dst.back().push_back(std::move(g));
+              }
+            }
+            ++k;
+          }
+  s->gf = std::move(gf);
+  s->gf_loaded = true;
+  s->solved = false;
+  s->om_ready = false;
+  // The file keeps no excitation energies: the largest |pole| bounds the
+  // largest excitation, which sets the finite-T Matsubara cutoff.
+  s->max_exc = max_pole;
+  VCA_GUARD_END
+}
+
+int vca_omega_restart(vca_solver *sv, const double *hloc, const double *hk, int64_t nk,
+                      const double *bath_h, const double *bath_v, double omegaprime) {
+  VCA_GUARD_BEGIN
+  Solver *s = reinterpret_cast<Solver *>(sv);
+  if (!s || !hloc || !hk || nk < 1) fail(VCA_ERR_ARG, "null argument or nk < 1");
+  if (!s->solved && !s->gf_loaded)
+    fail(VCA_ERR_STATE, "no Green's function: vca_solve or vca_read_gfprime first");
+  const bool bath = s->p.nlat_bath * s->p.norb_bath > 0;
+  if (bath && (!bath_h || !bath_v)) fail(VCA_ERR_ARG, "model has a bath: bath_h/bath_v needed");
+  if (bath) {
+    int rc = vca_set_bath(sv, bath_h, bath_v);
+    if (rc) return rc;
+  }
+  int rc = vca_set_hcluster(sv, hloc);
+  if (rc) return rc;
+  const size_t per_k = (size_t)s->nlso * s->nlso;
+  s->hk.assign(reinterpret_cast<const zc *>(hk), reinterpret_cast<const zc *>(hk) + per_k * nk);
+  s->nk = nk;
+  s->solved = false;
+  s->om_ready = false;
+  const double max_exc = s->max_exc;
+  s->res = vca_results{};
+  s->res.omegaprime = omegaprime;
+  s->max_exc = max_exc;
+  const double t0 = now();
+  omega_stage(*s);
+  s->res.t_omega = now() - t0;
+  s->solved = true;
   VCA_GUARD_END
 }
 

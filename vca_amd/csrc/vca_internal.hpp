@@ -277,4 +277,13 @@ double tridiag_lowest(int m, const double *d, const double *e);
 // eigenvectors), eigenvalues ascending in w.
 void herm_eig(int n, zc *a, double *w);
 
+// Pole file of the cluster Green's function (gfprime.cpp).  One entry per
+// element: [state][channel] -> poles and weights, complex as in the file.
+struct GfChanIO {
+  std::vector<zc> w, p;
+};
+using GfEntry = std::vector<std::vector<GfChanIO>>;
+void gfprime_write(const std::string &path, bool formatted, const std::vector<GfEntry> &el);
+std::vector<GfEntry> gfprime_read(const std::string &path, bool formatted, size_t nelem);
+
 }  // namespace vca

@@ -4,7 +4,8 @@
 ! with HFMODE=T and XMU=0, t = t' = -1, Nk k-points on the reduced zone),
 ! written against module vca_gpu instead of module VCA: the calls are the
 ! reference's vca_init_solver / vca_solve / vca_get_sft_potential and the
-! grid getters vca_get_sigma_matsubara / vca_get_gimp_realaxis.
+! grid getters vca_get_sigma_matsubara / vca_get_gimp_realaxis, and the
+! pole file of save_gfprime (gfprime_f.restart, unformatted).
 program example_chain4
   use vca_gpu
   implicit none
@@ -44,5 +45,6 @@ program example_chain4
   write(*, "(A,2ES25.17)") "SIGMA_MATS_11_1 ", smats(1, 1, 1, 1, 1, 1, 1)
   write(*, "(A,2ES25.17)") "SIGMA_MATS_23_7 ", smats(2, 3, 1, 1, 1, 1, 7)
   write(*, "(A,2ES25.17)") "GREAL_12_2500 ", greal(1, 1, 1, 1, 2500)
+  call save_gfprime("gfprime_f")
   call vca_finalize_solver()
 end program example_chain4

@@ -104,6 +104,20 @@ module vca_gpu
        type(vca_results_t), intent(out) :: out
        integer(c_int) :: rc
      end function c_get_results
+     function c_save_gfprime(s, file, used, fmt) bind(C, name="vca_save_gfprime") result(rc)
+       import :: c_ptr, c_int, c_int32_t, c_char
+       type(c_ptr), value :: s
+       character(kind=c_char), dimension(*), intent(in) :: file
+       integer(c_int32_t), value :: used, fmt
+       integer(c_int) :: rc
+     end function c_save_gfprime
+     function c_read_gfprime(s, file, used, fmt) bind(C, name="vca_read_gfprime") result(rc)
+       import :: c_ptr, c_int, c_int32_t, c_char
+       type(c_ptr), value :: s
+       character(kind=c_char), dimension(*), intent(in) :: file
+       integer(c_int32_t), value :: used, fmt
+       integer(c_int) :: rc
+     end function c_read_gfprime
      function c_get_gf_grid(s, axis, g, sg, g0) bind(C, name="vca_get_gf_grid") result(rc)
        import :: c_ptr, c_int, c_int32_t
        type(c_ptr), value :: s, g, sg, g0
@@ -146,6 +160,7 @@ module vca_gpu
   end interface vca_get_sigma_realaxis
 
   public :: vca_init_solver, vca_solve, vca_get_sft_potential, vca_finalize_solver
+  public :: save_gfprime, read_gfprime
   public :: vca_get_gimp_matsubara, vca_get_gimp_realaxis
   public :: vca_get_sigma_matsubara, vca_get_sigma_realaxis
   public :: vca_params_t, vca_results_t
@@ -274,6 +289,40 @@ contains
     real(8), intent(out) :: sft
     call check(c_get_sft(handle, sft), "vca_get_sft_potential")
   end subroutine vca_get_sft_potential
+
+  ! VCA_AUX_FUNX.f90:608-652 and 654-697: file//".restart" (".used")
+  subroutine save_gfprime(file, used, use_formatted)
+    character(len=*), intent(in), optional :: file
+    logical, intent(in), optional :: used, use_formatted
+    call check(c_save_gfprime(handle, gf_file(file), gf_flag(used), gf_flag(use_formatted)), &
+         "save_gfprime")
+  end subroutine save_gfprime
+
+  subroutine read_gfprime(file, used, use_formatted)
+    character(len=*), intent(in), optional :: file
+    logical, intent(in), optional :: used, use_formatted
+    if (.not. c_associated(handle)) call vca_init_solver_serial()
+    call check(c_read_gfprime(handle, gf_file(file), gf_flag(used), gf_flag(use_formatted)), &
+         "read_gfprime")
+  end subroutine read_gfprime
+
+  function gf_file(file) result(c)
+    character(len=*), intent(in), optional :: file
+    character(kind=c_char, len=:), allocatable :: c
+    if (present(file)) then
+       c = trim(file) // c_null_char
+    else
+       c = "gfprime" // c_null_char
+    end if
+  end function gf_file
+
+  integer(c_int32_t) function gf_flag(x)
+    logical, intent(in), optional :: x
+    gf_flag = 0
+    if (present(x)) then
+       if (x) gf_flag = 1
+    end if
+  end function gf_flag
 
   ! One array (which: 1 G, 2 Sigma, 3 G0) on the Matsubara (axis 0) or real
   ! (axis 1) grid of the solver, [Nlat,Nlat,Nspin,Nspin,Norb,Norb,L].
