@@ -17,7 +17,9 @@ stream).  Reported beside it in the same JSON line:
     kernel (flops it performs per evaluation over the measured time);
   * full_solve: vca_solve of the 3x4 cluster end to end (ground state, the
     excitation chains of 300 steps on the N+-1 sectors, Omega on the 64x64
-    k-grid);
+    k-grid), then G, Sigma and G0 on the default 5000-point Matsubara and
+    real-axis grids (pole terms per second of the grid kernel, its FP64
+    roofline, and the oracle's pole sums on one core beside it);
   * batched_2x5: vca_solve of the 10-site 2x5 cluster, every (Nup, Ndw)
     sector, ARPACK-equivalent eigenpairs per sector, all chains batched;
   * bath16: the 8-site + 8-bath-level sector (165,636,900 states), H x v
@@ -62,8 +64,15 @@ def logdet_flops(nb: int, nblk: int) -> float:
     return nblk * (8.0 * nb ** 3 + 8.0 * nb ** 3 / 3.0)
 
 
-def run_full():
-    """vca_solve of the 3x4 cluster end to end."""
+# real flops of one pole term w / (z - p) accumulated into G: z - p (1),
+# |z - p|^2 (3), reciprocal (1), w * conj(z - p) (6), scale (2), add (2)
+POLE_TERM_FLOPS = 15.0
+
+
+def run_full(cpu=False):
+    """vca_solve of the 3x4 cluster end to end, then G, Sigma, G0 on the
+    default grids; with cpu, the oracle's pole sums on a sample of the same
+    grid for the CPU figure."""
     from vca_amd import configs
     from vca_amd.api import VcaSolver
 
@@ -91,9 +100,35 @@ def run_full():
         sf.gf_grid(axis)
         grids[axis + "_seconds"] = time.perf_counter() - t1
     r2 = sf.results()
+    rate = r2["grid_pole_evals"] / max(r2["grid_seconds"], 1e-30)
     grids.update(npoles=r2["npoles"], pole_evals=r2["grid_pole_evals"],
-                 pole_kernel_seconds=r2["grid_seconds"],
-                 pole_evals_per_s=r2["grid_pole_evals"] / max(r2["grid_seconds"], 1e-30))
+                 pole_kernel_seconds=r2["grid_seconds"], pole_evals_per_s=rate,
+                 roofline={"bound": "valu", "flops_per_pole_term": POLE_TERM_FLOPS,
+                           "achieved": rate * POLE_TERM_FLOPS / 1e12, "peak": FP64_PEAK_TFLOPS,
+                           "unit": "TFLOP/s",
+                           "frac": rate * POLE_TERM_FLOPS / 1e12 / FP64_PEAK_TFLOPS})
+    if cpu:
+        # the same pole sums on one core (the oracle's numpy restatement of
+        # the loops of VCA_GF_NORMAL.f90:1024-1031) over 64 Matsubara points
+        import numpy as np
+        from oracle import vca_ref as R
+        from tests._models import to_oracle
+        po = to_oracle(pf)
+        gd = R.GFData(po, 0.0, 1.0)
+        for il in range(1, pf.nlat + 1):
+            for jl in range(1, pf.nlat + 1):
+                nch = 2 if il == jl else 4
+                gd.elems[(il, jl, 1, 1, 1)] = [[sf.gf_poles(il, jl, 1, 1, 1, 1, c)
+                                                for c in range(1, nch + 1)]]
+        wm = np.pi / pf.beta * (2 * np.arange(1, 65) - 1)
+        t1 = time.perf_counter()
+        R.gf_cluster_grid(gd, 1j * wm)
+        tc = time.perf_counter() - t1
+        grids["cpu_baseline"] = {
+            "pole_evals_per_s": r2["npoles"] * len(wm) / tc, "cores": 1, "kind": "port",
+            "sample": f"{r2['npoles']} poles x {len(wm)} Matsubara points, the oracle's "
+                      "numpy pole sums (one core)"}
+        log(f"grids CPU: {r2['npoles'] * len(wm) / tc / 1e9:.3f} G pole terms/s")
     full["grids"] = grids
     log(f"grids 3x4 ({r2['npoles']} poles, 2 x 5000 points): Matsubara "
         f"{grids['matsubara_seconds'] * 1e3:.1f} ms, real {grids['realaxis_seconds'] * 1e3:.1f} ms "
@@ -188,7 +223,7 @@ def main():
         print(json.dumps({"omega": om}), flush=True)
         return
     if args.only == "full":
-        print(json.dumps({"full_solve": run_full()}), flush=True)
+        print(json.dumps({"full_solve": run_full(cpu=not args.no_cpu)}), flush=True)
         return
 
     p, hloc, hk, _, _ = configs.config_square_3x4(nk=2)
@@ -325,7 +360,8 @@ def main():
         log(f"bath16 GS: mixed {out16['gs_mixed']} fp64 {out16['gs_fp64']}")
         return out16
 
-    full = None if args.no_full else guarded("full 3x4 solve", run_full)
+    full = None if args.no_full else guarded("full 3x4 solve",
+                                             lambda: run_full(cpu=not args.no_cpu))
     c3 = None if args.no_c3 else guarded("2x5 every-sector solve", run_c3)
     bath16 = None if args.no_bath16 else guarded("bath16", run_bath16)
 
