@@ -137,6 +137,29 @@ def run_full(cpu=False):
     return full
 
 
+def run_c3():
+    """vca_solve of the 2x5 cluster over every sector."""
+    from vca_amd import configs
+    from vca_amd.api import VcaSolver
+
+    pc, hlc, hkc, _, _ = configs.config_batched_2x5(nk=8)
+    sc = VcaSolver(pc)
+    t0 = time.perf_counter()
+    omc = sc.vca_solve(hlc, hkc)
+    r = sc.results()
+    c3 = {"workload": "2x5 cluster U=6, Nspin=2, every (Nup,Ndw) sector (121, dims "
+                      "1..63,504), up to 6 eigenpairs per sector (ARPACK-equivalent), "
+                      "all chains batched per target sector",
+          "seconds": time.perf_counter() - t0, "omega": omc,
+          "sector_threads": int(os.environ.get("VCA_SECTOR_THREADS", "8")),
+          "lanczos_step_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9,
+          **{k: r[k] for k in ("omegaprime", "nstates", "nchains", "hxv_count",
+                               "t_diag", "t_gf", "t_omega")}}
+    log(f"2x5 every-sector solve: {c3['seconds']:.1f}s Omega={omc:.10f} "
+        f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, {r['nchains']} chains)")
+    return c3
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,10 +172,11 @@ def main():
                     help="skip the 8-site + 8-bath-level sector (165,636,900 states)")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
-    ap.add_argument("--only", choices=["all", "hxv", "omega", "full"], default="all",
+    ap.add_argument("--only", choices=["all", "hxv", "omega", "full", "c3"], default="all",
                     help="hxv: the H x v timing plus a 256 MiB copy that calibrates the HBM "
                          "byte counters; omega: the Omega integrand timing; full: the "
-                         "end-to-end 3x4 solve (rocprofv3 runs)")
+                         "end-to-end 3x4 solve; c3: the every-sector 2x5 solve (rocprofv3 "
+                         "runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would "
@@ -224,6 +248,9 @@ def main():
         return
     if args.only == "full":
         print(json.dumps({"full_solve": run_full(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.only == "c3":
+        print(json.dumps({"batched_2x5": run_c3()}), flush=True)
         return
 
     p, hloc, hk, _, _ = configs.config_square_3x4(nk=2)
@@ -305,24 +332,6 @@ def main():
         except Exception as exc:
             log(f"{name} failed: {exc}")
             return {"error": str(exc)}
-
-    def run_c3():
-        pc, hlc, hkc, _, _ = configs.config_batched_2x5(nk=8)
-        sc = VcaSolver(pc)
-        t0 = time.perf_counter()
-        omc = sc.vca_solve(hlc, hkc)
-        r = sc.results()
-        c3 = {"workload": "2x5 cluster U=6, Nspin=2, every (Nup,Ndw) sector (121, dims "
-                          "1..63,504), up to 6 eigenpairs per sector (ARPACK-equivalent), "
-                          "all chains batched per target sector",
-              "seconds": time.perf_counter() - t0, "omega": omc,
-              "lanczos_step_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30)
-              / 1e9,
-              **{k: r[k] for k in ("omegaprime", "nstates", "nchains", "hxv_count",
-                                   "t_diag", "t_gf", "t_omega")}}
-        log(f"2x5 every-sector solve: {c3['seconds']:.1f}s Omega={omc:.10f} "
-            f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, {r['nchains']} chains)")
-        return c3
 
     # ------------------------------- 8 sites + 8 bath levels, 1.66e8 states
     def run_bath16():

@@ -22,6 +22,8 @@
 #include <limits>
 #include <map>
 #include <memory>
+#include <condition_variable>
+#include <exception>
 #include <mutex>
 #include <tuple>
 #include <numeric>
@@ -97,7 +99,8 @@ struct Solver {
   double emin = 0, zeta = 1, max_exc = 0;
   std::unique_ptr<OmegaDev> om;
   bool om_ready = false;
-  bool gf_loaded = false;  // pole store read from a file (vca_read_gfprime)
+  bool gf_loaded = false;
+  std::mutex res_mu;  // res updates from the concurrent sector solves  // pole store read from a file (vca_read_gfprime)
   std::vector<KeptState> states;
   ~Solver();
 };
@@ -428,6 +431,7 @@ static void run_steps(Sector &sec, int nsteps, bool accx, Solver *s, int nb = 1)
     VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
     VCA_HIP(hipEventDestroy(e0));
     VCA_HIP(hipEventDestroy(e1));
+    std::lock_guard<std::mutex> lk(s->res_mu);
     s->res.hxv_seconds += ms * 1e-3;
     s->res.hxv_count += (int64_t)nsteps * nb;
     s->res.hxv_states += (double)nsteps * (double)sec.dim * nb;
@@ -627,14 +631,20 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     std::vector<zc> h = dense_h(s, nup, ndw);
     std::vector<double> w(dim);
     herm_eig((int)dim, h.data(), w.data());
+    // copies on a stream of their own: a copy on the null stream would wait
+    // for the sector solves running concurrently on other streams
+    hipStream_t st;
+    VCA_HIP(hipStreamCreate(&st));
     for (int i = 0; i < neigen; ++i) {
       evals.push_back(w[i]);
       void *d = nullptr;
       VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
-      VCA_HIP(hipMemcpy(d, h.data() + (size_t)i * dim, sizeof(double2) * dim,
-                        hipMemcpyHostToDevice));
+      VCA_HIP(hipMemcpyAsync(d, h.data() + (size_t)i * dim, sizeof(double2) * dim,
+                             hipMemcpyHostToDevice, st));
       evecs.push_back(d);
     }
+    VCA_HIP(hipStreamSynchronize(st));
+    VCA_HIP(hipStreamDestroy(st));
     return;
   }
   if (p.lanc_method != VCA_LANC_LANCZOS) {
@@ -660,16 +670,140 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     auto s64 = make_sector(s, nup, ndw, VCA_FP64);
     double e32;
     g = gs_mixed(s, *s32, *s64, p.lanc_tolerance, p.lanc_niter, &e32);
-    s.res.e0_mixed = e32;
-    VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
+    {
+      std::lock_guard<std::mutex> lk(s.res_mu);
+      s.res.e0_mixed = e32;
+    }
+    VCA_HIP(hipMemcpyAsync(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice,
+                           s64->stream));
+    VCA_HIP(hipStreamSynchronize(s64->stream));
   } else {
     auto s64 = make_sector(s, nup, ndw, VCA_FP64);
     g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s);
-    VCA_HIP(hipMemcpy(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice));
+    VCA_HIP(hipMemcpyAsync(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice,
+                           s64->stream));
+    VCA_HIP(hipStreamSynchronize(s64->stream));
   }
   evals.push_back(g.e0);
   evecs.push_back(d);
 }
+
+// The sector solves of one sweep of the diagonalisation are independent
+// (VCA_DIAG.f90:83-256 fixes each sector's neigen before the sweep): worker
+// threads solve them concurrently, each sector on streams of its own, while
+// the caller takes the results in list order -- the order the state list is
+// built in.  Small sectors are bound by the host round trips of their
+// Lanczos steps, so overlapping them fills the GPU.  Workers claim sectors
+// in list order, at most `nthreads` ahead of the caller, and within a
+// device-memory budget (half the free HBM at the start of the sweep, the
+// sector at the head of the list always admitted), so large sectors do not
+// run out of memory together.  VCA_SECTOR_THREADS sets the worker count
+// (1: the sectors one after another on the calling thread).
+class SectorSweep {
+ public:
+  SectorSweep(Solver &s, const std::vector<std::pair<int, int>> &list,
+              const std::vector<int> &neig)
+      : s_(s), list_(list), neig_(neig), slots_(list.size()) {
+    const char *env = std::getenv("VCA_SECTOR_THREADS");
+    int nt = env && *env ? std::atoi(env) : 8;
+    nt = std::max(1, std::min<int>(nt, (int)list.size()));
+    nthreads_ = nt;
+    if (nt <= 1) return;
+    size_t free_b = 0, total_b = 0;
+    VCA_HIP(hipMemGetInfo(&free_b, &total_b));
+    budget_ = (int64_t)(free_b / 2);
+    for (int t = 0; t < nt; ++t) workers_.emplace_back([this]() { work(); });
+  }
+  ~SectorSweep() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      abort_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : workers_) t.join();
+    for (auto &sl : slots_)
+      for (void *d : sl.evecs)
+        if (d) (void)hipFree(d);
+  }
+  // Results of list entry k (k = 0, 1, ... in order); rethrows its error.
+  void take(size_t k, std::vector<double> &evals, std::vector<void *> &evecs) {
+    if (nthreads_ <= 1) {
+      solve_sector(s_, list_[k].first, list_[k].second, neig_[k], evals, evecs);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(mu_);
+    cv_.wait(lk, [&]() { return slots_[k].done; });
+    Slot &sl = slots_[k];
+    if (sl.err) std::rethrow_exception(sl.err);
+    evals = std::move(sl.evals);
+    evecs = std::move(sl.evecs);
+    sl.evecs.clear();
+    used_ -= sl.bytes;
+    taken_ = k + 1;
+    lk.unlock();
+    cv_.notify_all();
+  }
+
+ private:
+  struct Slot {
+    std::vector<double> evals;
+    std::vector<void *> evecs;
+    std::exception_ptr err;
+    bool done = false;
+    int64_t bytes = 0;
+  };
+  // device bytes a sector solve holds: the thick-restart basis, rotation
+  // target and eigenvectors, or the Lanczos buffers
+  int64_t need(size_t k) const {
+    Binom bn;
+    const int64_t dim = (int64_t)bn(s_.ns, list_[k].first) * bn(s_.ns, list_[k].second);
+    const int64_t m = (int64_t)s_.p.lanc_ncv_factor *
+                          std::max(neig_[k], s_.p.lanc_nstates_sector) + s_.p.lanc_ncv_add;
+    const int64_t vecs = s_.p.lanc_method == VCA_LANC_LANCZOS ? 8 : 2 * m + 1 + neig_[k];
+    return vecs * dim * (int64_t)sizeof(double2);
+  }
+  void work() {
+    for (;;) {
+      size_t k;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&]() {
+          return abort_ || next_ >= list_.size() || next_ < taken_ + (size_t)nthreads_;
+        });
+        if (abort_ || next_ >= list_.size()) return;
+        k = next_++;
+        const int64_t b = need(k);
+        cv_.wait(lk, [&]() { return abort_ || used_ + b <= budget_ || k == taken_; });
+        if (abort_) return;
+        used_ += b;
+        slots_[k].bytes = b;
+      }
+      Slot &sl = slots_[k];
+      try {
+        solve_sector(s_, list_[k].first, list_[k].second, neig_[k], sl.evals, sl.evecs);
+      } catch (...) {
+        sl.err = std::current_exception();
+      }
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        sl.done = true;
+      }
+      cv_.notify_all();
+    }
+  }
+
+  Solver &s_;
+  const std::vector<std::pair<int, int>> &list_;
+  const std::vector<int> &neig_;
+  std::vector<Slot> slots_;
+  std::vector<std::thread> workers_;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  int nthreads_ = 1;
+  size_t next_ = 0, taken_ = 0;
+  int64_t used_ = 0, budget_ = 0;
+  bool abort_ = false;
+};
 
 // es_insert_state (VCA_EIGENSPACE.f90:160-212): ascending energies, a new
 // state goes before the first state of equal or higher energy.
@@ -737,14 +871,20 @@ static void diagonalize(Solver &s) {
     ++iter;
     free_states(s);
     double oldzero = 1000.0;
-    for (auto [nup, ndw] : list) {
+    std::vector<int> neig(list.size());
+    for (size_t k = 0; k < list.size(); ++k) {
+      const auto [nup, ndw] = list[k];
       const int64_t dim = (int64_t)bn(s.ns, nup) * bn(s.ns, ndw);
-      const int neigen = p.lanc_method == VCA_LANC_LANCZOS
-                             ? 1
-                             : (int)std::min<int64_t>(dim, neigen_sector[{nup, ndw}]);
+      neig[k] = p.lanc_method == VCA_LANC_LANCZOS
+                    ? 1
+                    : (int)std::min<int64_t>(dim, neigen_sector[{nup, ndw}]);
+    }
+    SectorSweep sweep(s, list, neig);
+    for (size_t k = 0; k < list.size(); ++k) {
+      const auto [nup, ndw] = list[k];
       std::vector<double> evals;
       std::vector<void *> evecs;
-      solve_sector(s, nup, ndw, neigen, evals, evecs);
+      sweep.take(k, evals, evecs);
       for (size_t i = 0; i < evals.size(); ++i) {
         const KeptState st{evals[i], nup, ndw, evecs[i]};
         bool kept = false;
