@@ -151,7 +151,7 @@ def run_c3():
                       "1..63,504), up to 6 eigenpairs per sector (ARPACK-equivalent), "
                       "all chains batched per target sector",
           "seconds": time.perf_counter() - t0, "omega": omc,
-          "sector_threads": int(os.environ.get("VCA_SECTOR_THREADS", "8")),
+          "sector_threads": int(os.environ.get("VCA_SECTOR_THREADS", "16")),
           "lanczos_step_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9,
           **{k: r[k] for k in ("omegaprime", "nstates", "nchains", "hxv_count",
                                "t_diag", "t_gf", "t_omega")}}

@@ -7,8 +7,13 @@
 // Device side: the basis V[0..m] (complex128, [m+1][dim]) and four
 // memory-bound kernels -- a multi-dot that reads every basis vector once
 // per group of G vectors, a fused w -= V h with |w|^2, a basis rotation
-// V Y and a scale.  Host side: the m x m projected matrix T = V^H H V,
-// filled column by column from the Gram-Schmidt coefficients, its
+// V Y and a scale.  The Lanczos steps of one restart cycle run without a
+// host round trip: every step orthogonalises twice (classical Gram-Schmidt
+// twice, so no decision on the host), keeps its coefficients and norm in
+// device memory and normalises by the norm read on the device.  Host side,
+// once per cycle: the m x m projected matrix T = V^H H V from those
+// coefficients, the breakdown check (a step whose norm fell to rounding is
+// redone from a fresh direction, and the steps after it rerun), its
 // eigenpairs (herm_eig) and the restart logic.
 //
 // Convergence test of a Ritz pair (theta, y): |beta_m| |y_m| <= tol *
@@ -26,7 +31,8 @@ namespace vca {
 namespace {
 
 constexpr int kEBS = 256;  // lanes per workgroup
-constexpr int kEG = 8;     // basis vectors per pass of the multi-dot / rotation
+constexpr int kEG = 8;     // basis vectors per pass of the rotation
+constexpr int kED = 16;    // basis vectors per pass of the multi-dot
 constexpr int kEMaxBlocks = 1024;
 
 __device__ inline double e_wave_sum(double v) {
@@ -47,39 +53,44 @@ __device__ inline double e_block_sum(double v, double *s_part) {
   return r;
 }
 
-// Per-workgroup partial dots <V_j, w> for j in [j0, j0+nj), nj <= kEG:
+// Per-workgroup partial dots <V_j, w> for j in [j0, j0+nj), nj <= kED:
 // pr[(2 j + c) * nblk + b], c = 0 real, 1 imaginary part.  With self != 0
-// also |w|^2 into column self_col.
+// also |w|^2 into column self_col.  All 2 nj + 1 sums are reduced together:
+// wavefront butterflies, then one LDS exchange across the 4 wavefronts
+// (one barrier for the whole set).
 __global__ __launch_bounds__(kEBS) void k_mdot(const double2 *V, int64_t dim, int j0, int nj,
                                                const double2 *w, int self_col, double *pr) {
-  __shared__ double s_part[kEBS / 64];
-  double ar[kEG], ai[kEG], ss = 0;
+  constexpr int NV = 2 * kED + 1;
+  __shared__ double s_part[kEBS / 64][NV];
+  double acc[NV];
 #pragma unroll
-  for (int g = 0; g < kEG; ++g) ar[g] = ai[g] = 0;
+  for (int g = 0; g < NV; ++g) acc[g] = 0;
   const int64_t stride = (int64_t)gridDim.x * kEBS;
   for (int64_t x = (int64_t)blockIdx.x * kEBS + threadIdx.x; x < dim; x += stride) {
     const double2 wx = w[x];
-    ss += wx.x * wx.x + wx.y * wx.y;
+    acc[2 * kED] += wx.x * wx.x + wx.y * wx.y;
 #pragma unroll
-    for (int g = 0; g < kEG; ++g)
+    for (int g = 0; g < kED; ++g)
       if (g < nj) {
         const double2 v = V[(int64_t)(j0 + g) * dim + x];
-        ar[g] += v.x * wx.x + v.y * wx.y;  // conj(v) * w
-        ai[g] += v.x * wx.y - v.y * wx.x;
+        acc[2 * g] += v.x * wx.x + v.y * wx.y;  // conj(v) * w
+        acc[2 * g + 1] += v.x * wx.y - v.y * wx.x;
       }
   }
+#pragma unroll
+  for (int g = 0; g < NV; ++g) acc[g] = e_wave_sum(acc[g]);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0)
+#pragma unroll
+    for (int g = 0; g < NV; ++g) s_part[wid][g] = acc[g];
+  __syncthreads();
   const int nblk = gridDim.x;
-  for (int g = 0; g < nj; ++g) {
-    const double r = e_block_sum(ar[g], s_part);
-    const double i = e_block_sum(ai[g], s_part);
-    if (threadIdx.x == 0) {
-      pr[(int64_t)(2 * (j0 + g)) * nblk + blockIdx.x] = r;
-      pr[(int64_t)(2 * (j0 + g) + 1) * nblk + blockIdx.x] = i;
-    }
-  }
-  if (self_col >= 0) {
-    const double s = e_block_sum(ss, s_part);
-    if (threadIdx.x == 0) pr[(int64_t)self_col * nblk + blockIdx.x] = s;
+  const int t = threadIdx.x;
+  if (t < 2 * nj || (t == 2 * kED && self_col >= 0)) {
+    double s = 0;
+    for (int wv = 0; wv < kEBS / 64; ++wv) s += s_part[wv][t];
+    const int64_t col = t == 2 * kED ? self_col : 2 * j0 + t;
+    pr[col * nblk + blockIdx.x] = s;
   }
 }
 
@@ -158,6 +169,21 @@ __global__ __launch_bounds__(kEBS) void k_scale2(double2 *v, int64_t dim, double
   }
 }
 
+// v *= 1/sqrt(*nrm2) with the squared norm read from device memory (left
+// as is when it is not positive: the host redoes such a step).
+__global__ __launch_bounds__(kEBS) void k_scale_nrm(double2 *v, int64_t dim, const double *nrm2) {
+  const double n2 = *nrm2;
+  if (!(n2 > 0.0)) return;
+  const double s = 1.0 / sqrt(n2);
+  const int64_t stride = (int64_t)gridDim.x * kEBS;
+  for (int64_t x = (int64_t)blockIdx.x * kEBS + threadIdx.x; x < dim; x += stride) {
+    double2 a = v[x];
+    a.x *= s;
+    a.y *= s;
+    v[x] = a;
+  }
+}
+
 // Deterministic pseudo-random vector: splitmix64 of (seed, i) in [-1, 1).
 __global__ __launch_bounds__(kEBS) void k_hash_vec(double2 *v, int64_t dim, uint64_t seed) {
   const int64_t stride = (int64_t)gridDim.x * kEBS;
@@ -187,39 +213,66 @@ struct Work {
   double *pr = nullptr;  // [(2(m+1)+1)][nblk]
   double *red = nullptr; // [2(m+1)+2]
   double2 *Y = nullptr;  // [m][m]
+  // per step j of a cycle: the coefficients of both Gram-Schmidt passes,
+  // [j][pass][2(m+1)], and the squared norm after them, [j]
+  double *coef = nullptr, *nrm2 = nullptr;
+  int ldc() const { return 2 * (m + 1); }
   explicit Work(Sector &s, int m_) : sec(s), dim(s.dim), m(m_), nblk(eig_grid(s.dim)) {
-    VCA_HIP(hipMalloc(&V, sizeof(double2) * (size_t)(m + 1) * dim));
-    VCA_HIP(hipMalloc(&W, sizeof(double2) * (size_t)m * dim));
-    VCA_HIP(hipMalloc(&pr, sizeof(double) * (size_t)(2 * (m + 1) + 1) * nblk));
-    VCA_HIP(hipMalloc(&red, sizeof(double) * (2 * (m + 1) + 2)));
-    VCA_HIP(hipMalloc(&Y, sizeof(double2) * (size_t)m * m));
+    const hipStream_t st = s.stream;
+    V = static_cast<double2 *>(dev_alloc_async(sizeof(double2) * (size_t)(m + 1) * dim, st));
+    W = static_cast<double2 *>(dev_alloc_async(sizeof(double2) * (size_t)m * dim, st));
+    pr = static_cast<double *>(
+        dev_alloc_async(sizeof(double) * (size_t)(2 * (m + 1) + 1) * nblk, st));
+    red = static_cast<double *>(dev_alloc_async(sizeof(double) * (2 * (m + 1) + 2), st));
+    Y = static_cast<double2 *>(dev_alloc_async(sizeof(double2) * (size_t)m * m, st));
+    coef = static_cast<double *>(dev_alloc_async(sizeof(double) * (size_t)m * 2 * ldc(), st));
+    nrm2 = static_cast<double *>(dev_alloc_async(sizeof(double) * (size_t)m, st));
   }
+  // stream-ordered frees: every use of the buffers is on sec.stream
   ~Work() {
-    if (sec.stream) (void)hipStreamSynchronize(sec.stream);
-    for (void *q : {(void *)V, (void *)W, (void *)pr, (void *)red, (void *)Y})
-      if (q) (void)hipFree(q);
+    for (void *q : {(void *)V, (void *)W, (void *)pr, (void *)red, (void *)Y, (void *)coef,
+                    (void *)nrm2})
+      if (q) (void)hipFreeAsync(q, sec.stream);
   }
   double2 *vec(int j) const { return V + (size_t)j * dim; }
 
-  // h[j] = <V_j, w> for j < n into red[0..2n), |w|^2 into red[2n] if self
-  void mdot(int n, const double2 *w, bool self) {
+  // h[j] = <V_j, w> for j < n into dst[0..2n), |w|^2 into dst[2n] if self
+  void mdot(int n, const double2 *w, bool self, double *dst = nullptr) {
     const hipStream_t st = sec.stream;
-    for (int j0 = 0; j0 < n; j0 += kEG) {
-      const int nj = std::min(kEG, n - j0);
+    if (!dst) dst = red;
+    for (int j0 = 0; j0 < n; j0 += kED) {
+      const int nj = std::min(kED, n - j0);
       hipLaunchKernelGGL(k_mdot, dim3(nblk), dim3(kEBS), 0, st, V, dim, j0, nj, w,
                          (self && j0 == 0) ? 2 * n : -1, pr);
     }
     if (n == 0 && self)
       hipLaunchKernelGGL(k_mdot, dim3(nblk), dim3(kEBS), 0, st, V, dim, 0, 0, w, 0, pr);
-    hipLaunchKernelGGL(k_colsum, dim3(2 * n + (self ? 1 : 0)), dim3(kEBS), 0, st, pr, nblk, red);
+    hipLaunchKernelGGL(k_colsum, dim3(2 * n + (self ? 1 : 0)), dim3(kEBS), 0, st, pr, nblk, dst);
     VCA_HIP(hipGetLastError());
   }
-  // w -= V[0..n) h (h = red[0..2n)), |w|^2 into red[2n+1]
-  void sub(int n, double2 *w) {
+  // w -= V[0..n) h with h = coefs[0..2n) (default red), |w|^2 into *nrm_dst
+  // (default red[2n+1])
+  void sub(int n, double2 *w, const double *coefs = nullptr, double *nrm_dst = nullptr) {
     const hipStream_t st = sec.stream;
+    if (!coefs) coefs = red;
+    if (!nrm_dst) nrm_dst = red + 2 * n + 1;
     hipLaunchKernelGGL(k_gs_sub, dim3(nblk), dim3(kEBS), sizeof(double2) * std::max(1, n), st,
-                       V, dim, n, red, w, pr);
-    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(kEBS), 0, st, pr, nblk, red + 2 * n + 1);
+                       V, dim, n, coefs, w, pr);
+    hipLaunchKernelGGL(k_colsum, dim3(1), dim3(kEBS), 0, st, pr, nblk, nrm_dst);
+    VCA_HIP(hipGetLastError());
+  }
+  // One Lanczos step without a host round trip: w = H V_j, two classical
+  // Gram-Schmidt passes against V[0..j] (coefficients to coef[j]), |w|^2 to
+  // nrm2[j], then w /= |w| on the device.
+  void device_step(int j) {
+    double2 *w = vec(j + 1);
+    launch_hxv(sec, vec(j), vec(j), w, false);
+    for (int pass = 0; pass < 2; ++pass) {
+      double *c = coef + ((size_t)j * 2 + pass) * ldc();
+      mdot(j + 1, w, false, c);
+      sub(j + 1, w, c, nrm2 + j);
+    }
+    hipLaunchKernelGGL(k_scale_nrm, dim3(nblk), dim3(kEBS), 0, sec.stream, w, dim, nrm2 + j);
     VCA_HIP(hipGetLastError());
   }
   // out[i] = V[0..mm) Y[:, i0 + i] for i < ni (Y already on the device)
@@ -301,41 +354,58 @@ EigsOut eigs_lanczos(Sector &sec, int nev, int ncv, double tol, int maxiter, con
   std::vector<zc> Yh((size_t)m * m);
   EigsOut out{};
   uint64_t reseed = 5002;
+  const int ldc = wk.ldc();
+  std::vector<double> hc((size_t)m * 2 * ldc), hn(m);
   for (int it = 0; it < std::max(1, maxiter); ++it) {
     out.restarts = it;
-    for (int j = k; j < m; ++j) {
-      double2 *w = wk.vec(j + 1);
-      launch_hxv(sec, wk.vec(j), wk.vec(j), w, false);
-      std::vector<zc> h(j + 1, zc(0.0));
-      double b = wk.orthogonalise(j + 1, w, h.data());
-      ++out.hxv;
-      for (int i = 0; i <= j; ++i) {
-        t_at(i, j) = h[i];
-        t_at(j, i) = std::conj(h[i]);
-      }
-      t_at(j, j) = zc(h[j].real(), 0.0);
-      double tnorm = 0;
-      for (int i = 0; i <= j; ++i) tnorm = std::max(tnorm, std::abs(h[i]));
-      if (b <= 1e-14 * std::max(1.0, tnorm)) {
-        // invariant subspace: continue from a fresh direction orthogonal to V
-        b = 0;
-        if (j + 1 < m) {
-          hipLaunchKernelGGL(k_hash_vec, dim3(wk.nblk), dim3(kEBS), 0, st, w, dim, reseed++);
-          std::vector<zc> hz(j + 1, zc(0.0));
-          const double nz = wk.orthogonalise(j + 1, w, hz.data());
-          if (!(nz > 0)) fail(VCA_ERR_NOCONV, "eigs: could not extend the basis");
-          wk.scale(w, 1.0 / nz);
+    int j0 = k;
+    while (j0 < m) {
+      // steps j0..m-1 on the device, then their coefficients in one copy
+      for (int j = j0; j < m; ++j) wk.device_step(j);
+      out.hxv += m - j0;
+      VCA_HIP(hipMemcpyAsync(hc.data() + (size_t)j0 * 2 * ldc, wk.coef + (size_t)j0 * 2 * ldc,
+                             sizeof(double) * (size_t)(m - j0) * 2 * ldc,
+                             hipMemcpyDeviceToHost, st));
+      VCA_HIP(hipMemcpyAsync(hn.data() + j0, wk.nrm2 + j0, sizeof(double) * (m - j0),
+                             hipMemcpyDeviceToHost, st));
+      VCA_HIP(hipStreamSynchronize(st));
+      int next = m;
+      for (int j = j0; j < m; ++j) {
+        const double *c1 = hc.data() + ((size_t)j * 2) * ldc;
+        const double *c2 = c1 + ldc;
+        double tnorm = 0;
+        for (int i = 0; i <= j; ++i) {
+          const zc h(c1[2 * i] + c2[2 * i], c1[2 * i + 1] + c2[2 * i + 1]);
+          t_at(i, j) = h;
+          t_at(j, i) = std::conj(h);
+          tnorm = std::max(tnorm, std::abs(h));
         }
-      } else if (j + 1 < m) {
-        wk.scale(w, 1.0 / b);
+        t_at(j, j) = zc(t_at(j, j).real(), 0.0);
+        double b = std::sqrt(std::max(0.0, hn[j]));
+        const bool breakdown = b <= 1e-14 * std::max(1.0, tnorm);
+        if (breakdown) b = 0;
+        if (j + 1 < m) {
+          t_at(j + 1, j) = zc(b, 0.0);
+          t_at(j, j + 1) = zc(b, 0.0);
+        } else {
+          beta_m = b;
+        }
+        if (breakdown) {
+          // invariant subspace: V_{j+1} from a fresh direction orthogonal to
+          // V, and the steps after it again
+          if (j + 1 < m) {
+            double2 *w = wk.vec(j + 1);
+            hipLaunchKernelGGL(k_hash_vec, dim3(wk.nblk), dim3(kEBS), 0, st, w, dim, reseed++);
+            std::vector<zc> hz(j + 1, zc(0.0));
+            const double nz = wk.orthogonalise(j + 1, w, hz.data());
+            if (!(nz > 0)) fail(VCA_ERR_NOCONV, "eigs: could not extend the basis");
+            wk.scale(w, 1.0 / nz);
+            next = j + 1;
+          }
+          break;
+        }
       }
-      if (j + 1 < m) {
-        t_at(j + 1, j) = zc(b, 0.0);
-        t_at(j, j + 1) = zc(b, 0.0);
-      } else {
-        beta_m = b;
-        if (b > 0) wk.scale(w, 1.0 / b);
-      }
+      j0 = next;
     }
     Yh = T;
     herm_eig(m, Yh.data(), theta.data());
@@ -375,8 +445,7 @@ EigsOut eigs_lanczos(Sector &sec, int nev, int ncv, double tol, int maxiter, con
   evals.assign(theta.begin(), theta.begin() + nev);
   evecs.clear();
   for (int i = 0; i < nev; ++i) {
-    void *d = nullptr;
-    VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
+    void *d = dev_alloc_async(sizeof(double2) * dim, st);
     VCA_HIP(hipMemcpyAsync(d, wk.W + (size_t)i * dim, sizeof(double2) * dim,
                            hipMemcpyDeviceToDevice, st));
     evecs.push_back(d);

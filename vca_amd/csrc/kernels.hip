@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <mutex>
 
 #include "vca_internal.hpp"
 
@@ -909,13 +910,12 @@ constexpr size_t kLdsBig = 150 * 1024;
 
 template <typename R, bool REAL_AMP, bool STAGE, int BS, int KR, int U>
 static void launch_col(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
-  static bool attr_set = false;
+  static std::once_flag attr_once;
   auto fn = k_hxv_col<R, REAL_AMP, STAGE, BS, KR, U>;
-  if (!attr_set) {
+  std::call_once(attr_once, [&]() {
     VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
-    attr_set = true;
-  }
+  });
   hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
 }
 
@@ -937,13 +937,12 @@ static void launch_col_s(Sector &sec, const HxvArgs &a, const HxvShape &h, dim3 
 
 template <typename R, bool REAL_AMP, int BS, int KR>
 static void launch_buf(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
-  static bool attr_set = false;
+  static std::once_flag attr_once;
   auto fn = k_hxv_buf<R, REAL_AMP, BS, KR>;
-  if (!attr_set) {
+  std::call_once(attr_once, [&]() {
     VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
-    attr_set = true;
-  }
+  });
   hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
 }
 

@@ -105,9 +105,19 @@ struct Solver {
   ~Solver();
 };
 
+void *dev_alloc_async(size_t bytes, hipStream_t st) {
+  void *p = nullptr;
+  VCA_HIP(hipMallocAsync(&p, std::max<size_t>(bytes, 1), st));
+  return p;
+}
+
+void dev_free_async(void *p, hipStream_t st) {
+  if (p) VCA_HIP(hipFreeAsync(p, st));
+}
+
 static void free_states(Solver &s) {
   for (auto &st : s.states)
-    if (st.d_vec) (void)hipFree(st.d_vec);
+    if (st.d_vec) (void)hipFreeAsync(st.d_vec, s.stream);
   s.states.clear();
 }
 
@@ -273,6 +283,10 @@ static void hxv_tune(Sector &sec) {
   const char *t = std::getenv("VCA_TUNE");
   const int nv = hxv_num_variants(sec);
   if ((t && t[0] == '0') || nv <= 1) return;
+  // below ~1e5 states every launch shape is bound by launch latency: the
+  // default shape, no timing runs (they would be noise next to concurrent
+  // sector solves anyway)
+  if (sec.dim < (int64_t)1 << 17) return;
   const TuneKey key{sec.up.dim, sec.dw.dim, sec.precision,
                     (int)(sec.up.real_amp && sec.dw.real_amp), sec.up.nterms, sec.dw.nterms};
   {
@@ -283,9 +297,8 @@ static void hxv_tune(Sector &sec) {
       return;
     }
   }
-  void *v = nullptr, *hv = nullptr;
-  VCA_HIP(hipMalloc(&v, sec.vec_bytes()));
-  VCA_HIP(hipMalloc(&hv, sec.vec_bytes()));
+  void *v = dev_alloc_async(sec.vec_bytes(), sec.stream);
+  void *hv = dev_alloc_async(sec.vec_bytes(), sec.stream);
   VCA_HIP(hipMemsetAsync(v, 0, sec.vec_bytes(), sec.stream));
   hipEvent_t e0, e1;
   VCA_HIP(hipEventCreate(&e0));
@@ -316,8 +329,8 @@ static void hxv_tune(Sector &sec) {
   }
   VCA_HIP(hipEventDestroy(e0));
   VCA_HIP(hipEventDestroy(e1));
-  VCA_HIP(hipFree(v));
-  VCA_HIP(hipFree(hv));
+  dev_free_async(v, sec.stream);
+  dev_free_async(hv, sec.stream);
   sec.hxv_variant = best;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[key] = best;
@@ -637,8 +650,7 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     VCA_HIP(hipStreamCreate(&st));
     for (int i = 0; i < neigen; ++i) {
       evals.push_back(w[i]);
-      void *d = nullptr;
-      VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
+      void *d = dev_alloc_async(sizeof(double2) * dim, st);
       VCA_HIP(hipMemcpyAsync(d, h.data() + (size_t)i * dim, sizeof(double2) * dim,
                              hipMemcpyHostToDevice, st));
       evecs.push_back(d);
@@ -655,7 +667,7 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     const EigsOut o =
         eigs_lanczos(*s64, neigen, ncv, p.lanc_tolerance, maxiter, nullptr, evals, evecs);
     if (o.nconv < neigen) {
-      for (void *d : evecs) (void)hipFree(d);
+      for (void *d : evecs) dev_free_async(d, s64->stream);
       evecs.clear();
       fail(VCA_ERR_NOCONV, "sector (%d,%d): %d of %d eigenpairs converged in %d restarts", nup,
            ndw, o.nconv, neigen, maxiter);
@@ -663,8 +675,8 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     return;
   }
   GsOut g{};
-  void *d = nullptr;
-  VCA_HIP(hipMalloc(&d, sizeof(double2) * dim));
+  void *d = dev_alloc_async(sizeof(double2) * dim, s.stream);
+  VCA_HIP(hipStreamSynchronize(s.stream));
   if (p.precision == VCA_MIXED) {
     auto s32 = make_sector(s, nup, ndw, VCA_MIXED);
     auto s64 = make_sector(s, nup, ndw, VCA_FP64);
@@ -698,14 +710,14 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
 // device-memory budget (half the free HBM at the start of the sweep, the
 // sector at the head of the list always admitted), so large sectors do not
 // run out of memory together.  VCA_SECTOR_THREADS sets the worker count
-// (1: the sectors one after another on the calling thread).
+// (default 16; 1: the sectors one after another on the calling thread).
 class SectorSweep {
  public:
   SectorSweep(Solver &s, const std::vector<std::pair<int, int>> &list,
               const std::vector<int> &neig)
       : s_(s), list_(list), neig_(neig), slots_(list.size()) {
     const char *env = std::getenv("VCA_SECTOR_THREADS");
-    int nt = env && *env ? std::atoi(env) : 8;
+    int nt = env && *env ? std::atoi(env) : 16;
     nt = std::max(1, std::min<int>(nt, (int)list.size()));
     nthreads_ = nt;
     if (nt <= 1) return;
@@ -723,7 +735,7 @@ class SectorSweep {
     for (auto &t : workers_) t.join();
     for (auto &sl : slots_)
       for (void *d : sl.evecs)
-        if (d) (void)hipFree(d);
+        if (d) (void)hipFreeAsync(d, s_.stream);
   }
   // Results of list entry k (k = 0, 1, ... in order); rethrows its error.
   void take(size_t k, std::vector<double> &evals, std::vector<void *> &evecs) {
@@ -734,7 +746,10 @@ class SectorSweep {
     std::unique_lock<std::mutex> lk(mu_);
     cv_.wait(lk, [&]() { return slots_[k].done; });
     Slot &sl = slots_[k];
-    if (sl.err) std::rethrow_exception(sl.err);
+    if (sl.err) {
+      set_error("%s", sl.msg.c_str());  // the message of the worker's thread
+      std::rethrow_exception(sl.err);
+    }
     evals = std::move(sl.evals);
     evecs = std::move(sl.evecs);
     sl.evecs.clear();
@@ -749,6 +764,7 @@ class SectorSweep {
     std::vector<double> evals;
     std::vector<void *> evecs;
     std::exception_ptr err;
+    std::string msg;
     bool done = false;
     int64_t bytes = 0;
   };
@@ -783,6 +799,7 @@ class SectorSweep {
         solve_sector(s_, list_[k].first, list_[k].second, neig_[k], sl.evals, sl.evecs);
       } catch (...) {
         sl.err = std::current_exception();
+        sl.msg = g_err;
       }
       {
         std::lock_guard<std::mutex> lk(mu_);
@@ -813,21 +830,22 @@ static void es_insert(std::vector<KeptState> &list, const KeptState &st) {
   list.insert(list.begin() + (long)pos, st);
 }
 
-static void es_pop(std::vector<KeptState> &list) {
-  if (list.back().d_vec) VCA_HIP(hipFree(list.back().d_vec));
+static void es_pop(std::vector<KeptState> &list, hipStream_t st) {
+  dev_free_async(list.back().d_vec, st);
   list.pop_back();
 }
 
 // es_add_state with a size limit (VCA_EIGENSPACE.f90:133-155): below the
 // limit insert; at the limit replace the highest state if e is lower.
 // Returns false when the state was not kept (the caller frees its vector).
-static bool es_add_sized(std::vector<KeptState> &list, const KeptState &st, int size) {
+static bool es_add_sized(std::vector<KeptState> &list, const KeptState &st, int size,
+                         hipStream_t st_free) {
   if ((int)list.size() < size) {
     es_insert(list, st);
     return true;
   }
   if (!list.empty() && st.e < list.back().e) {
-    es_pop(list);
+    es_pop(list, st_free);
     es_insert(list, st);
     return true;
   }
@@ -889,7 +907,7 @@ static void diagonalize(Solver &s) {
         const KeptState st{evals[i], nup, ndw, evecs[i]};
         bool kept = false;
         if (ft) {
-          kept = es_add_sized(s.states, st, nst_total);
+          kept = es_add_sized(s.states, st, nst_total, s.stream);
         } else {
           // T = 0 selection (VCA_DIAG.f90:224-235)
           const double en = evals[i];
@@ -906,8 +924,7 @@ static void diagonalize(Solver &s) {
         }
         if (kept) evecs[i] = nullptr;
       }
-      for (void *d : evecs)
-        if (d) VCA_HIP(hipFree(d));
+      for (void *d : evecs) dev_free_async(d, s.stream);
     }
     if (s.states.empty()) fail(VCA_ERR_NOCONV, "no ground state found");
     const double egs = s.states.front().e, emax = s.states.back().e;
@@ -930,7 +947,7 @@ static void diagonalize(Solver &s) {
       } else {
         while (s.states.size() > 1 &&
                std::exp(-p.beta * (s.states.back().e - egs)) <= p.cutoff)
-          es_pop(s.states);
+          es_pop(s.states, s.stream);
         nst_total = std::max((int)s.states.size(), p.lanc_nstates_step) + p.lanc_nstates_step;
       }
     } else {
@@ -1818,7 +1835,7 @@ int vca_lanczos_eigs(vca_sector *sv, int32_t nev, int32_t ncv, double tol, int32
     if (evecs)
       VCA_HIP(hipMemcpy(evecs + (size_t)2 * i * sec->dim, vecs[i], sizeof(double2) * sec->dim,
                         hipMemcpyDeviceToHost));
-    VCA_HIP(hipFree(vecs[i]));
+    dev_free_async(vecs[i], s64->stream);
   }
   if (nconv) *nconv = o.nconv;
   VCA_GUARD_END

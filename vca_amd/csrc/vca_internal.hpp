@@ -263,6 +263,13 @@ struct EigsOut {
 EigsOut eigs_lanczos(Sector &sec, int nev, int ncv, double tol, int maxiter, const void *start,
                      std::vector<double> &evals, std::vector<void *> &evecs);
 
+// Stream-ordered device memory (hipMallocAsync / hipFreeAsync) for the
+// eigenvectors and the eigensolver workspaces: a plain hipFree waits for the
+// whole device, which stalls sector solves running concurrently on other
+// streams.  Free only what the host has already synchronised with.
+void *dev_alloc_async(size_t bytes, hipStream_t st);
+void dev_free_async(void *p, hipStream_t st);
+
 // Host linear algebra (linalg.cpp)
 // Symmetric tridiagonal eigenproblem, diag d[n], offdiag e[n] with e[i]
 // coupling i-1 and i (e[0] unused).  Eigenvalues ascending in w; if z is
