@@ -273,6 +273,7 @@ struct TuneKey {
 };
 static std::map<TuneKey, int> g_tuned;
 static std::mutex g_tune_mu;
+constexpr int64_t kTuneMinDim = (int64_t)1 << 17;  // one vector is timed from here up
 
 static void hxv_tune(Sector &sec) {
   const char *force = std::getenv("VCA_HXV_VARIANT");
@@ -286,7 +287,7 @@ static void hxv_tune(Sector &sec) {
   // below ~1e5 states every launch shape is bound by launch latency: the
   // default shape, no timing runs (they would be noise next to concurrent
   // sector solves anyway)
-  if (sec.dim < (int64_t)1 << 17) return;
+  if (sec.dim < kTuneMinDim) return;
   const TuneKey key{sec.up.dim, sec.dw.dim, sec.precision,
                     (int)(sec.up.real_amp && sec.dw.real_amp), sec.up.nterms, sec.dw.nterms};
   {
@@ -334,6 +335,72 @@ static void hxv_tune(Sector &sec) {
   sec.hxv_variant = best;
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[key] = best;
+}
+
+// Launch shape for nb stacked vectors (the batched steps of the excitation
+// chains), timed with the batch itself, for sectors below the size where
+// one vector is timed (hxv_tune): there one vector is launch-bound while a
+// batch fills the chip, so the choices differ (the 2x5 cluster's 190-chain
+// batches: 212 us against 373 us with the default shape).  Above that size
+// the single-vector choice carries over (on the 3x4 cluster's 276-chain
+// batches both pick the same shape).  Once per (sector shape, nb);
+// sec.buf[0] (zeroed) is the input, sec.buf[2] the output.
+static std::map<std::pair<TuneKey, int>, int> g_tuned_batch;
+
+static void hxv_tune_batch(Sector &sec, int nb) {
+  const char *force = std::getenv("VCA_HXV_VARIANT");
+  const char *t = std::getenv("VCA_TUNE");
+  const int nv = hxv_num_variants(sec);
+  if ((force && *force) || (t && t[0] == '0') || nv <= 1 || nb <= 1) return;
+  if (sec.dim >= kTuneMinDim) return;
+  const TuneKey tk{sec.up.dim, sec.dw.dim, sec.precision,
+                   (int)(sec.up.real_amp && sec.dw.real_amp), sec.up.nterms, sec.dw.nterms};
+  const auto key = std::make_pair(tk, nb);
+  const int orig = sec.hxv_variant;
+  int best = -1;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned_batch.find(key);
+    if (it != g_tuned_batch.end()) best = it->second;
+  }
+  if (best < 0) {
+    VCA_HIP(hipMemsetAsync(sec.buf[0], 0, sec.vec_bytes() * nb, sec.stream));
+    hipEvent_t e0, e1;
+    VCA_HIP(hipEventCreate(&e0));
+    VCA_HIP(hipEventCreate(&e1));
+    float best_ms = std::numeric_limits<float>::max();
+    for (int var = 0; var < nv; ++var) {
+      sec.hxv_variant = var;
+      launch_hxv(sec, sec.buf[0], sec.buf[0], sec.buf[2], false, nb);
+      float ms_min = std::numeric_limits<float>::max();
+      for (int r = 0; r < 2; ++r) {
+        VCA_HIP(hipEventRecord(e0, sec.stream));
+        launch_hxv(sec, sec.buf[0], sec.buf[0], sec.buf[2], false, nb);
+        VCA_HIP(hipEventRecord(e1, sec.stream));
+        VCA_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
+        ms_min = std::min(ms_min, ms);
+      }
+      if (ms_min < best_ms) {
+        best_ms = ms_min;
+        best = var;
+      }
+      const char *verbose = std::getenv("VCA_TUNE_VERBOSE");
+      if (verbose && verbose[0] == '1')
+        std::fprintf(stderr, "hxv batch tune dim %dx%d x %d prec %d variant %d: %.2f us\n",
+                     sec.up.dim, sec.dw.dim, nb, sec.precision, var, ms_min * 1e3);
+    }
+    VCA_HIP(hipEventDestroy(e0));
+    VCA_HIP(hipEventDestroy(e1));
+    sec.hxv_variant = orig;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tuned_batch[key] = best;
+  }
+  if (best != sec.hxv_variant) {
+    clear_graphs(sec);
+    sec.hxv_variant = best;
+  }
 }
 
 // ------------------------------------------------------------ sectors ---
@@ -1100,6 +1167,7 @@ static void build_gf(Solver &s) {
       for (size_t b0 = 0; b0 < idx.size(); b0 += nb_max) {
         const int nb = (int)std::min<size_t>(nb_max, idx.size() - b0);
         ensure_lanczos(js, nlanc + 2, std::max(nb, js.nb_cap));
+        hxv_tune_batch(js, nb);
         std::vector<ExcOp> ops(nb);
         for (int c = 0; c < nb; ++c) ops[c] = specs[idx[b0 + c]].op;
         VCA_HIP(hipMemcpyAsync(js.d_ops, ops.data(), sizeof(ExcOp) * nb, hipMemcpyHostToDevice,
