@@ -191,6 +191,24 @@ def test_vca_solve_pipeline(name):
     assert abs(omega - ref["sft_potential"]) < 1e-8 * max(1, abs(ref["sft_potential"]))
 
 
+def test_chain_affine_batches_in_a_solve(monkeypatch):
+    """The batched chain steps with each chain on one XCD (forced) give the
+    same solve as with the columns spread over the XCDs: only the order of
+    the alpha partial sums differs."""
+    p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
+    p.lanc_ngfiter = 80
+    out = {}
+    for xcd in ("0", "1"):
+        monkeypatch.setenv("VCA_XCD_CHAINS", xcd)
+        _say(f"10-site chain solve, xcd_chains={xcd}")
+        s = VcaSolver(p)
+        om = s.vca_solve(hloc, hk)
+        out[xcd] = (om, s.vca_gf_cluster(0.3j), s.vca_gf_cluster(1.0 + 0.5j))
+    assert abs(out["0"][0] - out["1"][0]) <= 1e-12 * abs(out["0"][0])
+    for a, b in zip(out["0"][1:], out["1"][1:]):
+        assert np.abs(a - b).max() <= 1e-10 * np.abs(a).max()
+
+
 def test_sft_exact_at_u0():
     """U=0: Omega(t') = (3-Nspin)/Nk sum_k sum_{e_kn<0} e_kn for every t'."""
     for tprime in (-1.0, -0.7):
@@ -222,16 +240,20 @@ def test_graph_replays_match_eager_launches(monkeypatch):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_hxv_device_batch_equals_single(prec):
+@pytest.mark.parametrize("xcd", ["0", "1"])
+def test_hxv_device_batch_equals_single(prec, xcd, monkeypatch):
     """vca_hxv_device_batch on nb stacked vectors gives, bit for bit, the nb
-    single products (and the oracle's within the usual bar)."""
+    single products (and the oracle's within the usual bar), with the
+    columns of each chain spread over the XCDs or (xcd=1) each chain on one
+    XCD, for a chain count that is not a multiple of 8."""
     from vca_amd.api import DeviceArray
+    monkeypatch.setenv("VCA_XCD_CHAINS", xcd)
     p, hloc, hk, bh, bv = M.bath_model(nlat=5, nb=5)
     om = M.oracle_model(p, hloc, bh, bv)
     s = _solver(p, hloc, bh, bv)
     sec = s.sector(5, 5, prec)
-    nb = 3
-    _say(f"batched hxv bath5+5 (5,5) nb={nb} {prec}")
+    nb = 13
+    _say(f"batched hxv bath5+5 (5,5) nb={nb} {prec} xcd_chains={xcd}")
     dt = np.complex128 if prec == "fp64" else np.complex64
     rng = np.random.default_rng(4)
     v = (rng.normal(size=(nb, sec.dim)) + 1j * rng.normal(size=(nb, sec.dim))).astype(dt)

@@ -28,8 +28,8 @@ def _cluster_2x4(**kw):
             i = configs.site(ix, iy, 2)
             stag = 0.1 if (ix + iy) % 2 == 0 else -0.1
             onsite[i, 0], onsite[i, 1] = -stag, stag
-    p = VcaParams(nlat=8, nspin=2, uloc=(4.0, 0, 0, 0, 0), lanc_method="arpack",
-                  lanc_ngfiter=60, **kw)
+    kw.setdefault("lanc_method", "arpack")
+    p = VcaParams(nlat=8, nspin=2, uloc=(4.0, 0, 0, 0, 0), lanc_ngfiter=60, **kw)
     hloc, _ = configs.square_lattice(2, 4, 2, -1.0, onsite, nk=(1, 1))
     _, hk = configs.square_lattice(2, 4, 2, -1.0, np.zeros((8, 2)), nk=(4, 4))
     return p, hloc, hk
@@ -43,12 +43,16 @@ def _solve(monkeypatch, threads, **kw):
     return s, om, s.results()
 
 
-@pytest.mark.parametrize("finite_t", [False, True])
-def test_concurrent_sweep_equals_sequential(monkeypatch, finite_t):
-    kw = dict(beta=20.0, lanc_nstates_total=4, bandwidth=6.0) if finite_t else {}
-    _say(f"sweep 2x4 every sector finite_t={finite_t}: 1 worker")
+@pytest.mark.parametrize("mode", ["arpack", "arpack_finite_t", "lanczos"])
+def test_concurrent_sweep_equals_sequential(monkeypatch, mode):
+    """lanczos: one state per sector by the plain Lanczos, whose steps run
+    from captured graphs on the calling thread and eagerly on the workers
+    (graph capture on one thread would make allocations on the others fail)."""
+    kw = {"arpack": {}, "arpack_finite_t": dict(beta=20.0, lanc_nstates_total=4, bandwidth=6.0),
+          "lanczos": dict(lanc_method="lanczos")}[mode]
+    _say(f"sweep 2x4 every sector {mode}: 1 worker")
     s1, om1, r1 = _solve(monkeypatch, 1, **kw)
-    _say(f"sweep 2x4 every sector finite_t={finite_t}: 16 workers")
+    _say(f"sweep 2x4 every sector {mode}: 16 workers")
     s16, om16, r16 = _solve(monkeypatch, 16, **kw)
     assert r1["nstates"] == r16["nstates"] and r1["spectrum_iters"] == r16["spectrum_iters"]
     assert abs(r1["omegaprime"] - r16["omegaprime"]) <= 1e-12 * abs(r1["omegaprime"])

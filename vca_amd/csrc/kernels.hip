@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <cstdlib>
 #include <mutex>
 
 #include "vca_internal.hpp"
@@ -128,6 +129,9 @@ struct HxvArgs {
   const RowMeta *meta_up;
   const uint32_t *ent4_up;
   int up_nc;        // 4-hop chunks per row
+  // chain-affine batches (one-column kernels): a 1-D grid of ncb x nb8
+  // workgroups, nb8 = nchains rounded up to a multiple of 8
+  int xcd_chains, ncb, nchains;
 };
 
 __device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t sd) {
@@ -244,9 +248,9 @@ __device__ inline double block_sum_t(double v) {
 }
 
 template <int BS>
-__device__ inline void block_partial_t(double v, double *partials) {
+__device__ inline void block_partial_t(double v, double *partials, int64_t slot) {
   const double b = block_sum_t<BS>(v);
-  if (threadIdx.x == 0) partials[blockIdx.x] = b;
+  if (threadIdx.x == 0) partials[slot] = b;
 }
 
 // Workgroup b -> output column.  Workgroups are dealt round-robin to the 8
@@ -256,6 +260,37 @@ __device__ inline int xcd_column(int b, int ncols) {
   const int x = b & 7, pos = b >> 3;
   const int q = ncols >> 3, r = ncols & 7;
   return x * q + min(x, r) + pos;
+}
+
+// Workgroup -> (output column, chain, slot of its partial sum) of the
+// one-column kernels.  Default grid (columns, chains): the columns of each
+// chain are spread over the 8 XCDs by xcd_column.  Chain-affine batches
+// (xcd_chains): workgroup b runs on XCD b & 7 and takes, in order, the
+// columns of the chains (b & 7) + 8q, so each chain's vector is read
+// through one XCD's L2 with its columns in rank order, instead of through
+// all eight L2s at once.  Workgroups past the last chain have nothing to do.
+struct WgPos {
+  int idw, chain;
+  int64_t slot;
+  bool valid;
+};
+__device__ inline WgPos wg_pos(const HxvArgs &a) {
+  WgPos w;
+  if (!a.xcd_chains) {
+    w.idw = xcd_column(blockIdx.x, a.dimdw);
+    w.chain = blockIdx.y;
+    w.slot = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    w.valid = true;
+  } else {
+    const int x = blockIdx.x & 7;
+    const int p = blockIdx.x >> 3;
+    const int q = p / a.ncb;
+    w.idw = p - q * a.ncb;
+    w.chain = x + 8 * q;
+    w.slot = (int64_t)w.chain * a.ncb + w.idw;
+    w.valid = w.chain < a.nchains;
+  }
+  return w;
 }
 
 
@@ -287,9 +322,11 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
   C *s_col = reinterpret_cast<C *>(smem + o_col);
 
   const int dimup = a.dimup;
-  const int idw = xcd_column(blockIdx.x, a.dimdw);
-  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
-  LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
+  const WgPos wp = wg_pos(a);
+  if (!wp.valid) return;
+  const int idw = wp.idw;
+  const int64_t chain_off = (int64_t)wp.chain * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + wp.chain : nullptr;
   int k = 0;
   bool done = false;
   R scale = (R)1;
@@ -389,7 +426,7 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
       }
     }
   }
-  if (lz) block_partial_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x);
+  if (lz) block_partial_t<BS>(dot, a.partials, wp.slot);
 }
 
 // Buffer-load form of the one-column kernel.  The same arithmetic as
@@ -416,9 +453,11 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
   C *s_col = reinterpret_cast<C *>(smem + o_col);
 
   const int dimup = a.dimup;
-  const int idw = xcd_column(blockIdx.x, a.dimdw);
-  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
-  LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
+  const WgPos wp = wg_pos(a);
+  if (!wp.valid) return;
+  const int idw = wp.idw;
+  const int64_t chain_off = (int64_t)wp.chain * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + wp.chain : nullptr;
   int k = 0;
   bool done = false;
   R scale = (R)1;
@@ -522,7 +561,7 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
       }
     }
   }
-  if (lz) block_partial_t<BS>(dot, a.partials + (int64_t)blockIdx.y * gridDim.x);
+  if (lz) block_partial_t<BS>(dot, a.partials, wp.slot);
 }
 
 // ------------------------------------------------------ Lanczos update --
@@ -907,6 +946,7 @@ void ensure_step_partials(Sector &sec, int nb) {
 }
 
 constexpr size_t kLdsBig = 150 * 1024;
+constexpr int kXcdChainsMin = 64;
 
 template <typename R, bool REAL_AMP, bool STAGE, int BS, int KR, int U>
 static void launch_col(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
@@ -1026,7 +1066,14 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
     const size_t smem = fixed + (stage ? col_bytes : 0);
     // buffer loads address one chain's vector with 32-bit offsets
     const bool buf = shp.buf && stage && (uint64_t)sec.dim * esz < (1ull << 31);
-    const dim3 g3(grid, nb);
+    // chain-affine batches from kXcdChainsMin chains up (fewer would leave
+    // XCDs idle); VCA_XCD_CHAINS=0/1 forces the choice
+    const char *xc = std::getenv("VCA_XCD_CHAINS");
+    const bool xcd = xc && *xc ? (xc[0] == '1' && nb > 1) : nb >= kXcdChainsMin;
+    a.xcd_chains = xcd ? 1 : 0;
+    a.ncb = grid;
+    a.nchains = nb;
+    const dim3 g3 = xcd ? dim3((unsigned)(grid * ((nb + 7) / 8 * 8)), 1) : dim3(grid, nb);
     if (fp64) {
       if (real_amp) launch_col_t<double, true>(sec, a, stage, buf, shp, g3, smem);
       else launch_col_t<double, false>(sec, a, stage, buf, shp, g3, smem);

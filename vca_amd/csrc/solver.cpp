@@ -481,7 +481,16 @@ static hipGraphExec_t step_graph(Sector &sec, int n, int nb, bool accx) {
   return ex;
 }
 
+// Set on the worker threads of a concurrent sector sweep: while one thread
+// captures a graph, the HIP runtime refuses device allocations on the
+// other threads (hipMallocAsync returns "operation not permitted when
+// stream is capturing" even with thread-local capture), so sweep workers
+// launch their Lanczos steps eagerly; their launch latency is hidden by the
+// other sectors running meanwhile.
+static thread_local bool t_no_graphs = false;
+
 static bool graphs_enabled() {
+  if (t_no_graphs) return false;
   const char *e = std::getenv("VCA_NO_GRAPHS");
   return !(e && e[0] == '1');
 }
@@ -846,6 +855,7 @@ class SectorSweep {
     return vecs * dim * (int64_t)sizeof(double2);
   }
   void work() {
+    t_no_graphs = true;
     for (;;) {
       size_t k;
       {
