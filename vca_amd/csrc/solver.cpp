@@ -1462,10 +1462,24 @@ static double adapt_gk15(double a, double b, double epsabs, double epsrel, Eval 
   return tot;
 }
 
+// Quadrature tolerances for an integrand of precision prec.  FP32 log-dets
+// carry ~1e-8 relative noise (the exact-identity LU), so an adaptive rule
+// asked for 1e-12 would bisect to its panel limit chasing the noise (the
+// 3x4 solve: 7.5 s of Omega instead of 0.04 s): in MIXED the relative
+// target is floored at 5e-8 of the integral, which costs at most ~1e-9 of
+// Omega itself.
+static void quad_tol(const vca_params &p, int prec, double *epsabs, double *epsrel) {
+  *epsabs = p.quad_epsabs;
+  *epsrel = p.quad_epsrel;
+  if (prec == VCA_MIXED) *epsrel = std::max(*epsrel, 5e-8);
+}
+
 // int_0^inf f(i x) dx through the QAGI map x = (1-t)/t, t in (0, 1]
 // (QUADPACK qagi, called through SciFortran quad at VCA_OMEGA.f90:132).
 static double quad_imag_axis(Solver &s, int prec) {
-  return adapt_gk15(0.0, 1.0, s.p.quad_epsabs, s.p.quad_epsrel, [&](const std::vector<double> &t) {
+  double ea, er;
+  quad_tol(s.p, prec, &ea, &er);
+  return adapt_gk15(0.0, 1.0, ea, er, [&](const std::vector<double> &t) {
     std::vector<double2> z(t.size());
     for (size_t i = 0; i < t.size(); ++i) z[i] = make_double2(0.0, (1.0 - t[i]) / t[i]);
     std::vector<double> g = integrand(s, z, prec);
@@ -1492,8 +1506,10 @@ static double omega_finite_t(Solver &s, int prec) {
   double sum = 0;
   for (double v : fm) sum += v;
   sum *= 2.0 / beta;
+  double ea, er;
+  quad_tol(p, prec, &ea, &er);
   const double contour =
-      adapt_gk15(0.0, M_PI, p.quad_epsabs, p.quad_epsrel, [&](const std::vector<double> &t) {
+      adapt_gk15(0.0, M_PI, ea, er, [&](const std::vector<double> &t) {
         std::vector<double2> z(t.size());
         for (size_t i = 0; i < t.size(); ++i)
           z[i] = make_double2(radius * std::cos(t[i]), radius * std::sin(t[i]));
