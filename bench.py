@@ -22,6 +22,9 @@ stream).  Reported beside it in the same JSON line:
     roofline, and the oracle's pole sums on one core beside it);
   * batched_2x5: vca_solve of the 10-site 2x5 cluster, every (Nup, Ndw)
     sector, ARPACK-equivalent eigenpairs per sector, all chains batched;
+  * chain4: the 4-site chain end to end (every sector, Matsubara-sum Omega
+    at beta=100 over 4096 frequencies, G on 512 real-axis points), on the
+    GPU and by the oracle on one host core, with the differences;
   * bath16: the 8-site + 8-bath-level sector (165,636,900 states), H x v
     FP64 and MIXED and the ground state both ways;
   * roofline: of the FP64 H x v kernel, algorithmic bytes (one read of v and
@@ -164,6 +167,45 @@ def run_full(cpu=False):
     return full
 
 
+def run_c1(cpu=True):
+    """The reference's CPU-runnable case: the 4-site chain (U=4, half
+    filling, Nk=32), every sector, Omega by the Matsubara sum at beta=100 over
+    4096 frequencies, G on 512 real-axis points with eta=0.05; on the GPU and
+    (cpu) on the host by the oracle, with the differences."""
+    import numpy as np
+    from vca_amd import configs
+    from vca_amd.api import VcaSolver
+
+    p, hloc, hk, _, _ = configs.config_chain4(nk=32)
+    p.omega_mode, p.omega_nfreq, p.beta = "matsubara", 4096, 100.0
+    p.lreal, p.eps = 512, 0.05
+    t0 = time.perf_counter()
+    s = VcaSolver(p)
+    om = s.vca_solve(hloc, hk)
+    greal = s.gf_grid("realaxis", ("g",))["g"]
+    tg = time.perf_counter() - t0
+    out = {"workload": "4-site Hubbard chain U=4, half filling, Nk=32, every sector; Omega by "
+                       "the Matsubara sum (beta=100, 4096 frequencies); G on 512 real-axis "
+                       "points, eta=0.05",
+           "gpu_seconds": tg, "omega": om}
+    if cpu:
+        from oracle import vca_ref as R
+        from tests._models import to_oracle
+        po = to_oracle(p)
+        t0 = time.perf_counter()
+        ref = R.vca_solve(po, hloc, hk, omega_mode="matsubara", nfreq=4096)
+        g_cpu, _, _ = R.sigma_grid(ref["model"], ref["gf"], 1)
+        tc = time.perf_counter() - t0
+        out["cpu_baseline"] = {"seconds": tc, "cores": 1, "kind": "port",
+                               "sample": "the whole case: the oracle's numpy/C restatement"}
+        out["omega_rel_diff"] = abs(om - ref["sft_potential"]) / abs(ref["sft_potential"])
+        out["g_real_max_diff"] = float(np.abs(greal - g_cpu).max())
+    log(f"C1 chain4: GPU {tg:.2f}s Omega={om:.10f}" +
+        (f"; CPU {out['cpu_baseline']['seconds']:.2f}s, dOmega {out['omega_rel_diff']:.1e}, "
+         f"max dG(w) {out['g_real_max_diff']:.1e}" if cpu else ""))
+    return out
+
+
 def run_c3():
     """vca_solve of the 2x5 cluster over every sector."""
     from vca_amd import configs
@@ -199,7 +241,7 @@ def main():
                     help="skip the 8-site + 8-bath-level sector (165,636,900 states)")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
-    ap.add_argument("--only", choices=["all", "hxv", "omega", "full", "c3"], default="all",
+    ap.add_argument("--only", choices=["all", "hxv", "omega", "full", "c3", "c1"], default="all",
                     help="hxv: the H x v timing plus a 256 MiB copy that calibrates the HBM "
                          "byte counters; omega: the Omega integrand timing; full: the "
                          "end-to-end 3x4 solve; c3: the every-sector 2x5 solve (rocprofv3 "
@@ -278,6 +320,9 @@ def main():
         return
     if args.only == "c3":
         print(json.dumps({"batched_2x5": run_c3()}), flush=True)
+        return
+    if args.only == "c1":
+        print(json.dumps({"chain4": run_c1(cpu=not args.no_cpu)}), flush=True)
         return
 
     p, hloc, hk, _, _ = configs.config_square_3x4(nk=2)
@@ -399,6 +444,7 @@ def main():
     full = None if args.no_full else guarded("full 3x4 solve",
                                              lambda: run_full(cpu=not args.no_cpu))
     c3 = None if args.no_c3 else guarded("2x5 every-sector solve", run_c3)
+    c1 = guarded("4-site chain", lambda: run_c1(cpu=not args.no_cpu))
     bath16 = None if args.no_bath16 else guarded("bath16", run_bath16)
 
     # ---------------------------------------------------- CPU baseline ---
@@ -492,6 +538,8 @@ def main():
         line["full_solve"] = full
     if c3:
         line["batched_2x5"] = c3
+    if c1:
+        line["chain4"] = c1
     if bath16:
         line["bath16"] = bath16
     print(json.dumps(line), flush=True)
