@@ -506,21 +506,45 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
       int nu_max = 0;
 #pragma unroll
       for (int j = 0; j < KR; ++j) nu_max = max(nu_max, nu[j]);
-      // up-spin hops: 4 entries per 16-byte load, gathers from LDS
-      for (int c = 0; 4 * c < nu_max; ++c) {
-        uint4 e4[KR];
+      // up-spin hops: 4 entries per 16-byte load, gathers from LDS.  No
+      // branch per entry: a row's 4 gathers (and amplitudes) of a chunk are
+      // issued together and waited for once; padding entries (0 in the
+      // table: row 0, amplitude 0) are read like the others and weighted 0.
+      // The next chunk's entries are loaded while this one is summed.
+      const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+      uint4 e4[KR];
+      if (nu_max > 0) {
 #pragma unroll
-        for (int j = 0; j < KR; ++j)
-          e4[j] = reinterpret_cast<const uint4 *>(a.ent4_up)[(int64_t)c * dimup + row[j]];
+        for (int j = 0; j < KR; ++j) e4[j] = ent4[row[j]];
+      }
+      for (int c = 0; 4 * c < nu_max; ++c) {
+        const bool more = 4 * (c + 1) < nu_max;
+        uint4 nx[KR];
+        if (more) {
+#pragma unroll
+          for (int j = 0; j < KR; ++j) nx[j] = ent4[(int64_t)(c + 1) * dimup + row[j]];
+        }
 #pragma unroll
         for (int j = 0; j < KR; ++j) {
           const uint32_t ee[4] = {e4[j].x, e4[j].y, e4[j].z, e4[j].w};
+          C y[4], h[4];
 #pragma unroll
-          for (int t = 0; t < 4; ++t)
-            if (4 * c + t < nu[j]) {
-              const C y = s_col[ee[t] & kColMask];
-              cmac<R, REAL_AMP>(ar[j], ai[j], s_amp_up[ee[t] >> kColBits], y);
-            }
+          for (int t = 0; t < 4; ++t) {
+            y[t] = s_col[ee[t] & kColMask];
+            h[t] = s_amp_up[ee[t] >> kColBits];
+          }
+#pragma unroll
+          for (int t = 0; t < 4; ++t) {
+            const bool valid = 4 * c + t < nu[j];
+            C hv;
+            hv.x = valid ? h[t].x : (R)0;
+            hv.y = valid ? h[t].y : (R)0;
+            cmac<R, REAL_AMP>(ar[j], ai[j], hv, y[t]);
+          }
+        }
+        if (more) {
+#pragma unroll
+          for (int j = 0; j < KR; ++j) e4[j] = nx[j];
         }
       }
       // down-spin hops: buffer loads, column offset in an SGPR
