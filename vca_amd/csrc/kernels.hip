@@ -547,29 +547,60 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
           for (int j = 0; j < KR; ++j) e4[j] = nx[j];
         }
       }
-      // down-spin hops: buffer loads, column offset in an SGPR
-      for (int q0 = 0; q0 < nd; q0 += U) {
-        C y[U][KR];
+      // down-spin hops: buffer loads, column offset in an SGPR.  complex64:
+      // two stages of U/2 hops, one stage's loads issued before the other
+      // stage is summed, so the loads never wait on the sums (16.5 against
+      // 17.5 us on the 3x4 sector).  complex128: one batch of U hops at a
+      // time; two stages there cost more registers than they hide latency
+      // (22.8 against 19.9 us).
+      if constexpr (ESZ == 16) {
+        for (int q0 = 0; q0 < nd; q0 += U) {
+          C y[U][KR];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-          const uint32_t so = __builtin_amdgcn_readfirstlane(s_dw_soff[min(q0 + u, nd - 1)]);
+          for (int u = 0; u < U; ++u) {
+            const uint32_t so = __builtin_amdgcn_readfirstlane(s_dw_soff[min(q0 + u, nd - 1)]);
 #pragma unroll
-          for (int j = 0; j < KR; ++j) {
-            if constexpr (ESZ == 16)
+            for (int j = 0; j < KR; ++j)
               y[u][j] = __builtin_bit_cast(
                   C, __builtin_amdgcn_raw_buffer_load_b128(rv, row[j] * ESZ, so, 0));
-            else
+          }
+#pragma unroll
+          for (int u = 0; u < U; ++u)
+            if (q0 + u < nd) {
+              const C h = s_dw_amp[q0 + u];
+#pragma unroll
+              for (int j = 0; j < KR; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h, y[u][j]);
+            }
+        }
+      } else {
+        constexpr int UH = U / 2;
+        C ya[UH][KR], yb[UH][KR];
+        auto load_stage = [&](C(&y)[UH][KR], int q0) {
+#pragma unroll
+          for (int u = 0; u < UH; ++u) {
+            const uint32_t so = __builtin_amdgcn_readfirstlane(s_dw_soff[min(q0 + u, nd - 1)]);
+#pragma unroll
+            for (int j = 0; j < KR; ++j)
               y[u][j] = __builtin_bit_cast(
                   C, __builtin_amdgcn_raw_buffer_load_b64(rv, row[j] * ESZ, so, 0));
           }
+        };
+        auto sum_stage = [&](const C(&y)[UH][KR], int q0) {
+#pragma unroll
+          for (int u = 0; u < UH; ++u)
+            if (q0 + u < nd) {
+              const C h = s_dw_amp[q0 + u];
+#pragma unroll
+              for (int j = 0; j < KR; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h, y[u][j]);
+            }
+        };
+        if (nd > 0) load_stage(ya, 0);
+        for (int q0 = 0; q0 < nd; q0 += 2 * UH) {
+          if (q0 + UH < nd) load_stage(yb, q0 + UH);
+          sum_stage(ya, q0);
+          if (q0 + 2 * UH < nd) load_stage(ya, q0 + 2 * UH);
+          if (q0 + UH < nd) sum_stage(yb, q0 + UH);
         }
-#pragma unroll
-        for (int u = 0; u < U; ++u)
-          if (q0 + u < nd) {
-            const C h = s_dw_amp[q0 + u];
-#pragma unroll
-            for (int j = 0; j < KR; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h, y[u][j]);
-          }
       }
 #pragma unroll
       for (int j = 0; j < KR; ++j) {
