@@ -161,7 +161,8 @@ def _oracle_solve(p, hloc, hk, bh=None, bv=None, **kw):
     return R.vca_solve(M.to_oracle(p), hloc, hk, bh, bv, **kw)
 
 
-@pytest.mark.parametrize("name", ["chain4", "chain4_u0_tp07", "bath", "bhz"])
+@pytest.mark.parametrize("name", ["chain4", "chain4_u0_tp07", "bath", "bath_hfshift_off",
+                                  "bhz", "multiorb", "chain4_symmetric_gf"])
 def test_vca_solve_pipeline(name):
     if name == "chain4":
         p, hloc, hk, bh, bv = configs.config_chain4(nk=16)
@@ -169,6 +170,17 @@ def test_vca_solve_pipeline(name):
         p, hloc, hk, bh, bv = M.chain(nlat=4, u=0.0, nk=16, tprime=-0.7)
     elif name == "bath":
         p, hloc, hk, bh, bv = M.bath_model(nlat=2, nb=2)
+    elif name == "bath_hfshift_off":
+        # HFMODE with HFSHIFT=F: the bath levels shifted by -U/2
+        # (VCA_BATH_SETUP.f90:127-136)
+        p, hloc, hk, bh, bv = M.bath_model(nlat=2, nb=2, hfshift=False)
+    elif name == "multiorb":
+        # Norb=2, Nspin=2: Ust, Jh, Hartree shifts, complex inter-orbital hops
+        p, hloc, hk, bh, bv = M.multiorb()
+    elif name == "chain4_symmetric_gf":
+        # VCA_GF_SYMMETRIC: 2 channels per off-diagonal pair (chan2)
+        p, hloc, hk, bh, bv = configs.config_chain4(nk=16)
+        p.gf_symmetric = True
     else:
         p, hloc, hk, bh, bv = M.bhz_small(nk=4)
     _say(f"vca_solve {name}")
