@@ -98,15 +98,19 @@ def test_hxv_square_3x4_full_sector():
     assert err < 1e-12, err
 
 
-N_HXV_VARIANTS = 15  # launch shapes of the one-column kernels (kernels.hip kVariants)
+N_HXV_VARIANTS = 19  # launch shapes of the one-column kernels (kernels.hip kVariants)
+N_HXV_DIRECT = 15    # the first 15: direct form; the rest: transposed form
 
 
 @pytest.mark.parametrize("variant", range(N_HXV_VARIANTS))
 def test_hxv_launch_variants(variant, monkeypatch):
     """Every launch shape the autotuner can pick gives the oracle's H x v
-    (one-column kernels, plain and buffer-load forms: chain of 10 sites,
-    bath model, complex BHZ)."""
+    (one-column kernels, plain and buffer-load forms, and the transposed
+    form, which the size threshold is lowered for: chain of 10 sites, bath
+    model, complex BHZ on a sector with DimUp != DimDw)."""
     monkeypatch.setenv("VCA_HXV_VARIANT", str(variant))
+    if variant >= N_HXV_DIRECT:
+        monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
     bhz = VcaParams(nlat=5, norb=2, nspin=2, uloc=(2.0, 2.0, 0, 0, 0))
     bhz_h, bhz_hk = configs.bhz(5, 1, 2.0, 0.3, nk=(2, 1))
     for name, (p, hloc, hk, bh, bv), sector in (
@@ -282,3 +286,57 @@ def test_hxv_device_batch_equals_single(prec, xcd, monkeypatch):
     ref = om.hxv(5, 5, v[1].astype(np.complex128))
     err = np.linalg.norm(hv[1] - ref) / np.linalg.norm(ref)
     assert err < (1e-12 if prec == "fp64" else 2e-6), err
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_transposed_form_in_lanczos_and_batches(prec, monkeypatch):
+    """The transposed H x v form (forced on small sectors) inside the
+    Lanczos steps: its two gather passes' partials of alpha, the buffer
+    parity and the skipping of finished chains.  Bars: the oracle's Lanczos
+    coefficients (12 steps, bath model); stacked products equal to the
+    single ones bit for bit; a whole solve (GS, batched chains) equal to the
+    direct form's within rounding."""
+    from vca_amd.api import DeviceArray
+    monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
+    monkeypatch.setenv("VCA_HXV_VARIANT", "16")
+    p, hloc, hk, bh, bv = M.bath_model(nlat=5, nb=5)
+    om = M.oracle_model(p, hloc, bh, bv)
+    s = _solver(p, hloc, bh, bv)
+    sec = s.sector(5, 5, prec)
+    _say(f"transposed form: tridiag bath5+5 (5,5) {prec}")
+    v0 = _rand(sec.dim, 5)
+    v0 /= np.linalg.norm(v0)
+    a, b = sec.lanczos_tridiag(v0, 12)
+    ao, bo = R.lanczos_tridiag(lambda x: om.hxv(5, 5, x), v0, 12)
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    assert np.allclose(a, ao, rtol=tol, atol=tol) and np.allclose(b, bo, rtol=tol, atol=tol)
+    _say(f"transposed form: batched products bath5+5 (5,5) nb=13 {prec}")
+    dt = np.complex128 if prec == "fp64" else np.complex64
+    nb = 13
+    rng = np.random.default_rng(8)
+    v = (rng.normal(size=(nb, sec.dim)) + 1j * rng.normal(size=(nb, sec.dim))).astype(dt)
+    dv = DeviceArray(v.shape, dt).upload(v)
+    dh = DeviceArray(v.shape, dt)
+    sec.hxv_device_batch(nb, dv.ptr, dh.ptr)
+    hv = dh.download()
+    d1 = DeviceArray((sec.dim,), dt)
+    h1 = DeviceArray((sec.dim,), dt)
+    for c in (0, 6, 12):
+        d1.upload(v[c])
+        sec.hxv_device(d1.ptr, h1.ptr)
+        assert np.array_equal(h1.download(), hv[c])
+    sec.close()
+    if prec != "fp64":
+        return
+    pc, hlc, hkc, _, _ = M.chain(nlat=10, u=4.0, nk=4)
+    pc.lanc_ngfiter = 80
+    out = {}
+    for form, var in (("transposed", "16"), ("direct", "13")):
+        monkeypatch.setenv("VCA_HXV_VARIANT", var)
+        _say(f"10-site chain solve, {form} H x v")
+        sc = VcaSolver(pc)
+        om_ = sc.vca_solve(hlc, hkc)
+        out[form] = (om_, sc.vca_gf_cluster(0.3j), sc.vca_gf_cluster(1.0 + 0.5j))
+    assert abs(out["transposed"][0] - out["direct"][0]) <= 1e-11 * abs(out["direct"][0])
+    for x, y in zip(out["transposed"][1:], out["direct"][1:]):
+        assert np.abs(x - y).max() <= 1e-9 * np.abs(y).max()

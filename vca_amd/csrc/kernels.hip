@@ -91,10 +91,32 @@ __global__ void k_build_spin(int ns, int npart, int dim, int nterms, const int2 
   cnt[i] = row_hops(s, nterms, ab, binom, ent + i, dim);
 }
 
-// Row records and 4-hop packing of the ELL table for the buffer-load H x v.
+__device__ __host__ constexpr int align16(int n) { return (n + 15) & ~15; }
+
+// LDS layout (byte offsets) of the staged one-column kernel k_hxv_buf: the
+// namp signed gather amplitudes and a zero at 0, the staged column of dimup
+// elements, then the column's down-spin hop list (maxnd amplitudes and
+// offsets).  Every region starts on a 16-byte boundary.
+struct BufLds {
+  int col, dw_amp, dw_soff, total;
+};
+__device__ __host__ inline BufLds buf_lds(int namp, int dimup, int maxnd, int esz) {
+  BufLds l;
+  l.col = align16((namp + 1) * esz);
+  l.dw_amp = l.col + align16(dimup * esz);
+  l.dw_soff = l.dw_amp + align16(maxnd * esz);
+  l.total = l.dw_soff + align16(maxnd * 4);
+  return l;
+}
+
+// Row records and 4-hop packing of the ELL table for the buffer-load H x v,
+// each hop as the two LDS byte addresses its gather reads (element of the
+// staged column, signed amplitude) for elements of esz bytes and namp
+// amplitudes; padding entries read row 0 and the zero amplitude.  lds_form
+// 0 (addresses would not fit the entry): entries left 0, never read.
 __global__ void k_pack_rows(int dim, int maxnnz, const uint32_t *map, const double *energy,
                             const int32_t *cnt, const uint32_t *ent, RowMeta *meta,
-                            uint32_t *ent4) {
+                            uint32_t *ent4, int esz, int namp, int lds_form) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= dim) return;
   RowMeta m;
@@ -102,11 +124,22 @@ __global__ void k_pack_rows(int dim, int maxnnz, const uint32_t *map, const doub
   m.s = map[i];
   m.cnt = cnt[i];
   meta[i] = m;
+  const uint32_t o_col = (uint32_t)buf_lds(namp, 0, 0, esz).col;
+  const uint32_t pad = o_col | ((uint32_t)(namp * esz) << kLdsAddrBits);
   const int nc = (maxnnz + 3) / 4;
   for (int c = 0; c < nc; ++c)
     for (int t = 0; t < 4; ++t) {
       const int q = 4 * c + t;
-      ent4[((int64_t)c * dim + i) * 4 + t] = q < m.cnt ? ent[(int64_t)q * dim + i] : 0u;
+      uint32_t w = 0u;
+      if (lds_form) {
+        w = pad;
+        if (q < m.cnt) {
+          const uint32_t e = ent[(int64_t)q * dim + i];
+          w = (o_col + (e & kColMask) * (uint32_t)esz) |
+              (((e >> kColBits) * (uint32_t)esz) << kLdsAddrBits);
+        }
+      }
+      ent4[((int64_t)c * dim + i) * 4 + t] = w;
     }
 }
 
@@ -132,6 +165,11 @@ struct HxvArgs {
   // chain-affine batches (one-column kernels): a 1-D grid of ncb x nb8
   // workgroups, nb8 = nchains rounded up to a multiple of 8
   int xcd_chains, ncb, nchains;
+  // one-column kernels: the workgroup of column c of chain b stores its
+  // partial sum at pbase + b * pstride + c; the passes of the transposed
+  // form (skip_dw, skip_diag) share one row of partials per chain
+  int64_t pbase, pstride;
+  int skip_dw, skip_diag;
 };
 
 __device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t sd) {
@@ -141,6 +179,30 @@ __device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t 
     for (int jo = 0; jo < c.norb; ++jo)
       e += c.wx[io][jo] * (double)__builtin_popcount((su >> io) & (sd >> jo) & c.site_mask);
   return e;
+}
+
+// Element of type T at LDS byte address addr.  Used by k_hxv_buf, which has
+// no static LDS, so that its dynamic LDS (smem) starts at address 0 and the
+// hop tables' LDS addresses are read with no base added (launch_buf checks
+// the kernel's static LDS size once).
+template <typename R> struct Vec2;
+template <> struct Vec2<double> { typedef double T __attribute__((ext_vector_type(2))); };
+template <> struct Vec2<float> { typedef float T __attribute__((ext_vector_type(2))); };
+
+template <typename R>
+__device__ inline cplx<R> lds_load(uint32_t addr) {
+  using V = typename Vec2<R>::T;
+  const V v = *(const __attribute__((address_space(3))) V *)(size_t)addr;
+  cplx<R> c;
+  c.x = v.x;
+  c.y = v.y;
+  return c;
+}
+
+// Real part only (real amplitudes).
+template <typename R>
+__device__ inline R lds_load_re(uint32_t addr) {
+  return *(const __attribute__((address_space(3))) R *)(size_t)addr;
 }
 
 template <typename R, bool REAL_AMP>
@@ -253,6 +315,24 @@ __device__ inline void block_partial_t(double v, double *partials, int64_t slot)
   if (threadIdx.x == 0) partials[slot] = b;
 }
 
+// The same with the per-wave sums in caller-provided LDS (BS/64 doubles,
+// free once every wave has arrived): a kernel with no static LDS has its
+// dynamic LDS at address 0, so LDS addresses held in tables need no base.
+template <int BS>
+__device__ inline void block_partial_lds(double v, double *partials, int64_t slot,
+                                         double *s_part) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_part[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int w = 0; w < BS / 64; ++w) r += s_part[w];
+    partials[slot] = r;
+  }
+}
+
 // Workgroup b -> output column.  Workgroups are dealt round-robin to the 8
 // XCDs, so column ranges are assigned per XCD: each XCD's L2 then holds
 // the neighbour columns of one contiguous range of columns.
@@ -279,7 +359,7 @@ __device__ inline WgPos wg_pos(const HxvArgs &a) {
   if (!a.xcd_chains) {
     w.idw = xcd_column(blockIdx.x, a.dimdw);
     w.chain = blockIdx.y;
-    w.slot = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+    w.slot = a.pbase + (int64_t)blockIdx.y * a.pstride + blockIdx.x;
     w.valid = true;
   } else {
     const int x = blockIdx.x & 7;
@@ -287,7 +367,7 @@ __device__ inline WgPos wg_pos(const HxvArgs &a) {
     const int q = p / a.ncb;
     w.idw = p - q * a.ncb;
     w.chain = x + 8 * q;
-    w.slot = (int64_t)w.chain * a.ncb + w.idw;
+    w.slot = a.pbase + (int64_t)w.chain * a.pstride + w.idw;
     w.valid = w.chain < a.nchains;
   }
   return w;
@@ -303,7 +383,6 @@ __device__ inline WgPos wg_pos(const HxvArgs &a) {
 //    independent coalesced loads of v(:, jdw) per hop.
 // BS (lanes) and KR (rows per lane) are picked per sector shape by the
 // autotuner (hxv_tune).
-__device__ __host__ constexpr int align16(int n) { return (n + 15) & ~15; }
 
 template <typename R, bool REAL_AMP, bool STAGE, int BS, int KR, int U>
 __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
@@ -346,7 +425,7 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
     // (STAGE) the input column, then one barrier
     const C *g_amp_up = static_cast<const C *>(a.amp_up);
     const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
-    const int nd = a.cnt_dw[idw];  // same address in every lane
+    const int nd = a.skip_dw ? 0 : a.cnt_dw[idw];  // same address in every lane
     for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
     for (int q = threadIdx.x; q < nd; q += BS) {
       const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
@@ -356,8 +435,8 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
     if (STAGE)
       for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
     __syncthreads();
-    const double edw = a.e_dw[idw];
-    const uint32_t sdw = a.map_dw[idw];
+    const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
+    const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
     for (int base = threadIdx.x; base < dimup; base += BS * KR) {
       int row[KR];
       R ar[KR], ai[KR];
@@ -368,7 +447,9 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
         row[j] = min(base + j * BS, dimup - 1);  // clamped: loads stay in bounds
         nu[j] = (base + j * BS < dimup) ? a.cnt_up[row[j]] : 0;
         x[j] = STAGE ? s_col[row[j]] : vcol[row[j]];
-        const double ed = a.e_up[row[j]] + edw + cross_energy(a.cross, a.map_up[row[j]], sdw);
+        const double ed =
+            a.skip_diag ? 0.0
+                        : a.e_up[row[j]] + edw + cross_energy(a.cross, a.map_up[row[j]], sdw);
         ar[j] = (R)ed * x[j].x;
         ai[j] = (R)ed * x[j].y;
       }
@@ -444,13 +525,11 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
   constexpr int U = KR >= 4 ? 4 : 8;
   extern __shared__ __align__(16) unsigned char smem[];
   const int maxnd = a.dw_maxnnz;
-  const int o_dw_amp = align16(a.namp_up * ESZ);
-  const int o_dw_soff = o_dw_amp + align16(maxnd * ESZ);
-  const int o_col = o_dw_soff + align16(maxnd * (int)sizeof(uint32_t));
+  const BufLds lds = buf_lds(a.namp_up, a.dimup, maxnd, ESZ);
   C *s_amp_up = reinterpret_cast<C *>(smem);
-  C *s_dw_amp = reinterpret_cast<C *>(smem + o_dw_amp);
-  uint32_t *s_dw_soff = reinterpret_cast<uint32_t *>(smem + o_dw_soff);
-  C *s_col = reinterpret_cast<C *>(smem + o_col);
+  C *s_col = reinterpret_cast<C *>(smem + lds.col);
+  C *s_dw_amp = reinterpret_cast<C *>(smem + lds.dw_amp);
+  uint32_t *s_dw_soff = reinterpret_cast<uint32_t *>(smem + lds.dw_soff);
 
   const int dimup = a.dimup;
   const WgPos wp = wg_pos(a);
@@ -477,8 +556,9 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
         const_cast<C *>(vin), (short)0, (int)(uint32_t)(a.vstride * ESZ), 0x00020000);
     const C *g_amp_up = static_cast<const C *>(a.amp_up);
     const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
-    const int nd = a.cnt_dw[idw];
+    const int nd = a.skip_dw ? 0 : a.cnt_dw[idw];
     for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
+    if (threadIdx.x == 0) s_amp_up[a.namp_up] = C{(R)0, (R)0};  // padding hops
     for (int q = threadIdx.x; q < nd; q += BS) {
       const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
       s_dw_soff[q] = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
@@ -486,9 +566,14 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
     }
     for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
     __syncthreads();
-    const double edw = a.e_dw[idw];
-    const uint32_t sdw = a.map_dw[idw];
-    for (int base = threadIdx.x; base < dimup; base += BS * KR) {
+    const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
+    const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
+    // a wavefront runs a row block while any of its lanes has a row there
+    // (its other lanes take a clamped row with no hops and store nothing),
+    // so the wavefront-wide reductions below see all 64 lanes
+    for (int base0 = 0; base0 < dimup; base0 += BS * KR) {
+      if (base0 + (int)(threadIdx.x & ~63u) >= dimup) break;
+      const int base = base0 + (int)threadIdx.x;
       int row[KR];
       R ar[KR], ai[KR];
       C x[KR];
@@ -499,52 +584,66 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
         const RowMeta m = a.meta_up[row[j]];
         nu[j] = (base + j * BS < dimup) ? m.cnt : 0;
         x[j] = s_col[row[j]];
-        const double ed = m.e + edw + cross_energy(a.cross, m.s, sdw);
+        const double ed = a.skip_diag ? 0.0 : m.e + edw + cross_energy(a.cross, m.s, sdw);
         ar[j] = (R)ed * x[j].x;
         ai[j] = (R)ed * x[j].y;
       }
-      int nu_max = 0;
+      // up-spin hops: 4 entries per 16-byte load, each entry the LDS byte
+      // addresses of its element and signed amplitude, so a gather costs a
+      // mask, a shift and two LDS reads.  The chunk count is the maximum
+      // over the wavefront (uniform loop); entries past a row's hops are
+      // padding that read row 0 and the zero amplitude.  The next chunk's
+      // entries are loaded while this one is summed.
+      int nc_w = 0;
 #pragma unroll
-      for (int j = 0; j < KR; ++j) nu_max = max(nu_max, nu[j]);
-      // up-spin hops: 4 entries per 16-byte load, gathers from LDS.  No
-      // branch per entry: a row's 4 gathers (and amplitudes) of a chunk are
-      // issued together and waited for once; padding entries (0 in the
-      // table: row 0, amplitude 0) are read like the others and weighted 0.
-      // The next chunk's entries are loaded while this one is summed.
+      for (int j = 0; j < KR; ++j) nc_w = max(nc_w, (nu[j] + 3) >> 2);
+      for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
+      nc_w = __builtin_amdgcn_readfirstlane(nc_w);
+      // The entries (L2 hits: every workgroup reads the same table) come
+      // through a ring of PF chunks in flight, so a chunk's load has PF - 1
+      // chunks of gathers to hide its latency behind.  Every ring load is
+      // issued, the chunk index clamped to the table's last chunk, so the
+      // count of loads in flight is the same on every path and the waits
+      // stay partial.  Depth 4 costs ~10 VGPRs over depth 2: taken only by
+      // the 1024-lane shape (picked where a large staged column holds one
+      // workgroup per CU anyway); at 256/512 lanes 64 VGPRs keep 8
+      // wavefronts per SIMD (the 3x4 sector's 924 workgroups in one round).
+      constexpr int PF = (BS == 1024 && KR == 1) ? 4 : 2;
+      const int last = a.up_nc - 1;  // >= 0 whenever nc_w > 0
       const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
-      uint4 e4[KR];
-      if (nu_max > 0) {
+      if (nc_w > 0) {
+        uint4 e4[PF][KR];
 #pragma unroll
-        for (int j = 0; j < KR; ++j) e4[j] = ent4[row[j]];
-      }
-      for (int c = 0; 4 * c < nu_max; ++c) {
-        const bool more = 4 * (c + 1) < nu_max;
-        uint4 nx[KR];
-        if (more) {
+        for (int p = 0; p < PF; ++p)
 #pragma unroll
-          for (int j = 0; j < KR; ++j) nx[j] = ent4[(int64_t)(c + 1) * dimup + row[j]];
-        }
+          for (int j = 0; j < KR; ++j) e4[p][j] = ent4[(int64_t)min(p, last) * dimup + row[j]];
+        for (int c0 = 0; c0 < nc_w; c0 += PF) {
 #pragma unroll
-        for (int j = 0; j < KR; ++j) {
-          const uint32_t ee[4] = {e4[j].x, e4[j].y, e4[j].z, e4[j].w};
-          C y[4], h[4];
+          for (int p = 0; p < PF; ++p) {
+            const int c = c0 + p;
+            if (c < nc_w) {
 #pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            y[t] = s_col[ee[t] & kColMask];
-            h[t] = s_amp_up[ee[t] >> kColBits];
+              for (int j = 0; j < KR; ++j) {
+                const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
+                C y[4], h[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                  y[t] = lds_load<R>(ee[t] & kLdsAddrMask);
+                  if constexpr (REAL_AMP) {
+                    h[t].x = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+                    h[t].y = (R)0;
+                  } else {
+                    h[t] = lds_load<R>(ee[t] >> kLdsAddrBits);
+                  }
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) cmac<R, REAL_AMP>(ar[j], ai[j], h[t], y[t]);
+              }
+            }
+            const int cn = min(c + PF, last);
+#pragma unroll
+            for (int j = 0; j < KR; ++j) e4[p][j] = ent4[(int64_t)cn * dimup + row[j]];
           }
-#pragma unroll
-          for (int t = 0; t < 4; ++t) {
-            const bool valid = 4 * c + t < nu[j];
-            C hv;
-            hv.x = valid ? h[t].x : (R)0;
-            hv.y = valid ? h[t].y : (R)0;
-            cmac<R, REAL_AMP>(ar[j], ai[j], hv, y[t]);
-          }
-        }
-        if (more) {
-#pragma unroll
-          for (int j = 0; j < KR; ++j) e4[j] = nx[j];
         }
       }
       // down-spin hops: buffer loads, column offset in an SGPR.  complex64:
@@ -616,7 +715,80 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
       }
     }
   }
-  if (lz) block_partial_t<BS>(dot, a.partials, wp.slot);
+  // the per-wave sums reuse the start of the LDS (at least 128 B + a column)
+  if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
+}
+
+// Transposed form of H x v, for sectors far larger than the on-chip caches.
+// The down-spin term sum_q H_dw(idw, jdw_q) v(iup, jdw_q) reads, for every
+// column, its ~nnz neighbour columns: on a 1.66e8-state sector (54 hops per
+// row) that is ~54 reads of the whole vector from HBM per product.  In the
+// layout vt(idw, iup) = v(iup, idw) (idw fastest) the same term gathers
+// inside one row of vt, like the up-spin term inside a column of v, so
+//   1. k_hxv_buf (skip_dw):        Hv = diag v + H_up v
+//   2. k_transpose:                 vt = v^T
+//   3. k_hxv_buf (skip_diag, skip_dw) on vt with the down-spin tables,
+//      each workgroup staging its row of vt in LDS and writing the result
+//      over it:                     vt = (H_dw v)^T
+//   4. k_transpose<ADD>:            Hv += vt^T
+// moves 9 vector lengths of HBM traffic per product (read and write in each
+// of the four) and keeps every gather in LDS.  The two gather passes write
+// their partials of <v|Hv> into one row per chain, summed by k_lz_alpha.
+//
+// Element (i, j) of the input sits at in[j * ni + i] (i fastest), of the
+// output at out[i * nj + j] (j fastest); a 64 x 64 tile goes through LDS so
+// that both the loads and the stores are 64-lane runs of consecutive
+// elements.  Lanczos mode (lz non-null): chain blockIdx.z reads buffer
+// in1 on odd steps, and a finished chain is skipped.
+struct TrArgs {
+  const void *in0, *in1;
+  void *out;
+  int ni, nj;
+  int64_t vstride;
+  const LanczosDev *lz;
+};
+
+constexpr int kTile = 64;
+
+template <typename R, bool ADD>
+__global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
+  using C = cplx<R>;
+  __shared__ C s[kTile][kTile + 1];
+  const int64_t off = (int64_t)blockIdx.z * a.vstride;
+  const C *in = static_cast<const C *>(a.in0) + off;
+  if (a.lz) {
+    const LanczosDev *lz = a.lz + blockIdx.z;
+    if (lz->done) return;
+    if (lz->k & 1) in = static_cast<const C *>(a.in1) + off;
+  }
+  C *out = static_cast<C *>(a.out) + off;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int i0 = blockIdx.x * kTile, j0 = blockIdx.y * kTile;
+  constexpr int kWaves = kBlock / 64;
+  const int i = i0 + lane;
+#pragma unroll
+  for (int t = 0; t < kTile / kWaves; ++t) {
+    const int jj = w + kWaves * t;
+    const int j = j0 + jj;
+    if (i < a.ni && j < a.nj) s[jj][lane] = in[(int64_t)j * a.ni + i];
+  }
+  __syncthreads();
+  const int j = j0 + lane;
+#pragma unroll
+  for (int t = 0; t < kTile / kWaves; ++t) {
+    const int ii = w + kWaves * t;
+    const int ir = i0 + ii;
+    if (ir < a.ni && j < a.nj) {
+      C *o = out + (int64_t)ir * a.nj + j;
+      C val = s[lane][ii];
+      if (ADD) {
+        const C prev = *o;
+        val.x += prev.x;
+        val.y += prev.y;
+      }
+      *o = val;
+    }
+  }
 }
 
 // ------------------------------------------------------ Lanczos update --
@@ -927,8 +1099,15 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
   const int nc = (sp.maxnnz + 3) / 4;
   VCA_HIP(hipMalloc(&sp.meta, sizeof(RowMeta) * sp.dim));
   VCA_HIP(hipMalloc(&sp.ent4, sizeof(uint32_t) * 4 * (size_t)nc * sp.dim));
+  // LDS-address entries for the sector's element size: the column must end
+  // below 2^18 bytes and the amplitudes (with the zero) within 2^14 bytes
+  const int esz = sec.precision == VCA_FP64 ? (int)sizeof(double2) : (int)sizeof(float2);
+  const int namp = 2 * sp.nterms;
+  sp.lds_form = (size_t)(namp + 1) * esz <= kLdsAmpLimit &&
+                (size_t)buf_lds(namp, 0, 0, esz).col + (size_t)sp.dim * esz <= kLdsAddrMask + 1;
   hipLaunchKernelGGL(k_pack_rows, dim3((sp.dim + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
-                     sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4);
+                     sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4, esz,
+                     namp, sp.lds_form ? 1 : 0);
   VCA_HIP(hipGetLastError());
   VCA_HIP(hipStreamSynchronize(st));
   VCA_HIP(hipFree(d_ab));
@@ -957,29 +1136,62 @@ static void ensure_partials(Sector &sec, int nblocks) {
 // the autotuner (sec.hxv_variant) or, before tuning, from the default below.
 struct HxvShape {
   bool onecol_kernel;
-  bool buf;  // buffer-load form (k_hxv_buf) when the column can be staged
+  bool buf;    // buffer-load form (k_hxv_buf) when the column can be staged
+  bool tform;  // transposed form (k_hxv_buf passes on v and v^T, k_transpose)
   int bs, kr, u, cpb, grid;
 };
 
 // Variants of the one-column kernel the autotuner times: (BS, KR, U, form),
-// form 0 = k_hxv_col, 1 = k_hxv_buf (U fixed by KR there).
-static const int kVariants[][4] = {{256, 4, 4, 0},  {256, 2, 8, 0},  {1024, 1, 8, 0},
-                                   {1024, 2, 8, 0}, {1024, 4, 4, 0}, {512, 2, 8, 0},
-                                   {256, 1, 16, 0}, {512, 1, 16, 0}, {1024, 1, 16, 0},
-                                   {256, 4, 4, 1},  {256, 2, 8, 1},  {512, 2, 8, 1},
-                                   {512, 1, 8, 1},  {1024, 1, 8, 1}, {1024, 2, 8, 1}};
+// form 0 = k_hxv_col, 1 = k_hxv_buf (U fixed by KR there), 2 = transposed
+// form (k_hxv_buf passes).  The form-2 variants come last and are timed only
+// on sectors where tform_fits holds.
+static const int kVariants[][4] = {
+    {256, 4, 4, 0}, {256, 2, 8, 0},  {1024, 1, 8, 0}, {1024, 2, 8, 0}, {1024, 4, 4, 0},
+    {512, 2, 8, 0}, {256, 1, 16, 0}, {512, 1, 16, 0}, {1024, 1, 16, 0}, {256, 4, 4, 1},
+    {256, 2, 8, 1}, {512, 2, 8, 1},  {512, 1, 8, 1},  {1024, 1, 8, 1}, {1024, 2, 8, 1},
+    {512, 1, 8, 2}, {1024, 1, 8, 2}, {512, 2, 8, 2},  {1024, 2, 8, 2}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+constexpr int kNumDirect = 15;  // variants of forms 0 and 1
+
+// The transposed form pays 9 vector lengths of HBM traffic per product
+// against ~2 + nnz_dw re-reads of neighbour columns by the direct form, which
+// only hit on-chip while the vector fits the 256 MB MALL: offered from 2^22
+// states, when both spins' columns can be staged in LDS and one vector is
+// addressable by the 32-bit offsets of the buffer loads.
+constexpr int64_t kTformMinDim = 1 << 22;
+constexpr size_t kLdsBig = 150 * 1024;
+
+static size_t hxv_esz(const Sector &sec) {
+  return sec.precision == VCA_FP64 ? sizeof(double2) : sizeof(float2);
+}
+
+static bool tform_fits(const Sector &sec) {
+  const size_t esz = hxv_esz(sec);
+  auto staged = [&](const DevSpin &sp) {
+    return sp.dim >= 128 && sp.lds_form &&
+           (size_t)buf_lds(2 * sp.nterms, sp.dim, 0, (int)esz).total <= kLdsBig;
+  };
+  // VCA_TFORM_MIN_DIM lowers the size threshold (tests run the form on
+  // sectors small enough for the oracle)
+  int64_t min_dim = kTformMinDim;
+  const char *e = std::getenv("VCA_TFORM_MIN_DIM");
+  if (e && *e) min_dim = std::atoll(e);
+  return sec.dim >= min_dim && staged(sec.up) && staged(sec.dw) &&
+         (uint64_t)sec.dim * esz < (1ull << 31);
+}
 
 static HxvShape hxv_shape(const Sector &sec) {
   HxvShape h{};
   if (sec.up.dim >= 128) {
     h.onecol_kernel = true;
     int v = sec.hxv_variant;
-    if (v < 0 || v >= kNumVariants) v = sec.up.dim >= 4096 ? 4 : 2;  // default
+    if (v < 0 || v >= kNumVariants || (v >= kNumDirect && !tform_fits(sec)))
+      v = sec.up.dim >= 4096 ? 4 : 2;  // default
     h.bs = kVariants[v][0];
     h.kr = kVariants[v][1];
     h.u = kVariants[v][2];
     h.buf = kVariants[v][3] != 0;
+    h.tform = kVariants[v][3] == 2;
     h.cpb = 1;
     h.grid = sec.dw.dim;
   } else {
@@ -992,15 +1204,43 @@ static HxvShape hxv_shape(const Sector &sec) {
   return h;
 }
 
-int hxv_num_variants(const Sector &sec) { return sec.up.dim >= 128 ? kNumVariants : 1; }
-
-// Allocates the partial-sum buffer for the largest grid of a Lanczos step
-// on nb chains, so that launches recorded into a graph never allocate.
-void ensure_step_partials(Sector &sec, int nb) {
-  ensure_partials(sec, std::max(hxv_shape(sec).grid, grid_for(sec.dim)) * nb);
+int hxv_num_variants(const Sector &sec) {
+  if (sec.up.dim < 128) return 1;
+  return tform_fits(sec) ? kNumVariants : kNumDirect;
 }
 
-constexpr size_t kLdsBig = 150 * 1024;
+int hxv_scratch_vectors(const Sector &sec) { return hxv_shape(sec).tform ? 1 : 0; }
+
+// Partial sums per chain of one H x v: one per column, plus one per row of
+// v^T in the transposed form.
+static int hxv_partials(const Sector &sec, const HxvShape &h) {
+  return h.tform ? sec.dw.dim + sec.up.dim : h.grid;
+}
+
+// Scratch vectors of the transposed form for nb chains; like the partials,
+// a larger request first drains the stream and drops the recorded graphs.
+static void ensure_tbuf(Sector &sec, int nb) {
+  if (!hxv_shape(sec).tform || sec.tbuf_cap >= nb) return;
+  if (sec.tbuf) {
+    VCA_HIP(hipStreamSynchronize(sec.stream));
+    clear_graphs(sec);
+    VCA_HIP(hipFree(sec.tbuf));
+    sec.tbuf = nullptr;
+    sec.tbuf_cap = 0;
+  }
+  VCA_HIP(hipMalloc(&sec.tbuf, sec.vec_bytes() * (size_t)nb));
+  sec.tbuf_cap = nb;
+}
+
+// Allocates the partial-sum buffer for the largest grid of a Lanczos step
+// on nb chains (and the transposed form's scratch), so that launches
+// recorded into a graph never allocate.
+void ensure_step_partials(Sector &sec, int nb) {
+  const HxvShape h = hxv_shape(sec);
+  ensure_partials(sec, std::max(hxv_partials(sec, h), grid_for(sec.dim)) * nb);
+  ensure_tbuf(sec, nb);
+}
+
 constexpr int kXcdChainsMin = 64;
 
 template <typename R, bool REAL_AMP, bool STAGE, int BS, int KR, int U>
@@ -1037,6 +1277,12 @@ static void launch_buf(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
   std::call_once(attr_once, [&]() {
     VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    // the hop tables hold LDS addresses relative to 0: the dynamic LDS
+    // must start there, i.e. the kernel has no static LDS
+    hipFuncAttributes fa{};
+    VCA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+    if (fa.sharedSizeBytes != 0)
+      fail(VCA_ERR_HIP, "k_hxv_buf has %zu bytes of static LDS", (size_t)fa.sharedSizeBytes);
   });
   hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
 }
@@ -1078,6 +1324,93 @@ static void launch_hxv_t(Sector &sec, HxvArgs &a, bool stage, bool onecol, dim3 
   }
 }
 
+template <typename R>
+static void launch_transpose(Sector &sec, const TrArgs &t, bool add, int nb) {
+  const dim3 g((unsigned)((t.ni + kTile - 1) / kTile), (unsigned)((t.nj + kTile - 1) / kTile),
+               (unsigned)nb);
+  if (add) hipLaunchKernelGGL((k_transpose<R, true>), g, dim3(kBlock), 0, sec.stream, t);
+  else hipLaunchKernelGGL((k_transpose<R, false>), g, dim3(kBlock), 0, sec.stream, t);
+  VCA_HIP(hipGetLastError());
+}
+
+template <typename R>
+static void launch_buf_pass(Sector &sec, const HxvArgs &a, const HxvShape &h, bool real_amp,
+                            dim3 grid, size_t smem) {
+  if (real_amp) launch_buf_s<R, true>(sec, a, h, grid, smem);
+  else launch_buf_s<R, false>(sec, a, h, grid, smem);
+  VCA_HIP(hipGetLastError());
+}
+
+// The four launches of the transposed form (see k_transpose) on nb chains
+// stacked [nb][dim]; the scratch vectors are sec.tbuf.  The one-column
+// passes run on the 2-D grid (columns, chains).
+template <typename R>
+static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_amp, int nb) {
+  constexpr size_t esz = sizeof(cplx<R>);
+  const bool fp64 = sec.precision == VCA_FP64;
+  a.xcd_chains = 0;
+  a.nchains = nb;
+  a.skip_dw = 1;
+  a.dw_maxnnz = 0;
+  // 1. Hv = diag v + H_up v (columns of v staged in LDS)
+  a.skip_diag = 0;
+  a.ncb = sec.dw.dim;
+  launch_buf_pass<R>(sec, a, h, real_amp, dim3(sec.dw.dim, nb),
+                     (size_t)buf_lds(a.namp_up, sec.up.dim, 0, (int)esz).total);
+  // 2. vt = v^T
+  TrArgs t{};
+  t.in0 = a.v0;
+  t.in1 = a.v1;
+  t.out = sec.tbuf;
+  t.ni = sec.up.dim;
+  t.nj = sec.dw.dim;
+  t.vstride = sec.dim;
+  t.lz = a.lz;
+  launch_transpose<R>(sec, t, false, nb);
+  // 3. vt = (H_dw v)^T: the down-spin tables as the gather tables, rows of
+  //    vt staged in LDS and overwritten by their result
+  HxvArgs b = a;
+  b.v0 = b.v1 = sec.tbuf;
+  b.hv = sec.tbuf;
+  b.dimup = sec.dw.dim;
+  b.dimdw = sec.up.dim;
+  b.map_up = sec.dw.map;
+  b.e_up = sec.dw.energy;
+  b.cnt_up = sec.dw.cnt;
+  b.ent_up = sec.dw.ent;
+  b.meta_up = sec.dw.meta;
+  b.ent4_up = sec.dw.ent4;
+  b.up_nc = (sec.dw.maxnnz + 3) / 4;
+  b.amp_up = fp64 ? (const void *)sec.dw.amp64 : (const void *)sec.dw.amp32;
+  b.namp_up = 2 * sec.dw.nterms;
+  b.skip_diag = 1;
+  b.ncb = sec.up.dim;
+  b.pbase = sec.dw.dim;
+  launch_buf_pass<R>(sec, b, h, real_amp, dim3(sec.up.dim, nb),
+                     (size_t)buf_lds(b.namp_up, sec.dw.dim, 0, (int)esz).total);
+  // 4. Hv += vt^T
+  TrArgs t2{};
+  t2.in0 = t2.in1 = sec.tbuf;
+  t2.out = a.hv;
+  t2.ni = sec.dw.dim;
+  t2.nj = sec.up.dim;
+  t2.vstride = sec.dim;
+  t2.lz = a.lz;
+  launch_transpose<R>(sec, t2, true, nb);
+  if (a.lz) {
+    hipLaunchKernelGGL(k_lz_alpha, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                       (int)a.pstride);
+    VCA_HIP(hipGetLastError());
+  }
+}
+
+static void launch_hxv_tform(Sector &sec, const HxvArgs &a, const HxvShape &h, bool real_amp,
+                             int nb) {
+  ensure_tbuf(sec, nb);
+  if (sec.precision == VCA_FP64) launch_tform_t<double>(sec, a, h, real_amp, nb);
+  else launch_tform_t<float>(sec, a, h, real_amp, nb);
+}
+
 void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos, int nb) {
   HxvArgs a{};
   a.vstride = sec.dim;
@@ -1109,18 +1442,27 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   a.dw_maxnnz = sec.dw.maxnnz;
   const bool real_amp = sec.up.real_amp && sec.dw.real_amp;
   if (shp.onecol_kernel) {
-    if (lanczos) ensure_partials(sec, grid * nb);
+    if (lanczos) ensure_partials(sec, hxv_partials(sec, shp) * nb);
     a.partials = sec.partials;
     a.meta_up = sec.up.meta;
     a.ent4_up = sec.up.ent4;
     a.up_nc = (sec.up.maxnnz + 3) / 4;
+    a.pbase = 0;
+    a.pstride = hxv_partials(sec, shp);
+    if (shp.tform) {
+      launch_hxv_tform(sec, a, shp, real_amp, nb);
+      return;
+    }
     const size_t fixed = (size_t)a.namp_up * esz + (size_t)a.dw_maxnnz * (esz + 8) + 48;
     const size_t col_bytes = (size_t)a.dimup * esz;
     const size_t budget = kLdsBig;
     const bool stage = fixed + col_bytes <= budget;
-    const size_t smem = fixed + (stage ? col_bytes : 0);
-    // buffer loads address one chain's vector with 32-bit offsets
-    const bool buf = shp.buf && stage && (uint64_t)sec.dim * esz < (1ull << 31);
+    // buffer-load form: its own LDS layout (buf_lds) and hop entries as LDS
+    // addresses; buffer loads address one chain's vector with 32-bit offsets
+    const size_t buf_smem = (size_t)buf_lds(a.namp_up, a.dimup, a.dw_maxnnz, (int)esz).total;
+    const bool buf = shp.buf && sec.up.lds_form && buf_smem <= budget &&
+                     (uint64_t)sec.dim * esz < (1ull << 31);
+    const size_t smem = buf ? buf_smem : fixed + (stage ? col_bytes : 0);
     // chain-affine batches from kXcdChainsMin chains up (fewer would leave
     // XCDs idle); VCA_XCD_CHAINS=0/1 forces the choice
     const char *xc = std::getenv("VCA_XCD_CHAINS");

@@ -142,8 +142,8 @@ Sector::~Sector() {
     for (void *q : {(void *)sp->map, (void *)sp->energy, (void *)sp->cnt, (void *)sp->ent,
                     (void *)sp->amp64, (void *)sp->amp32, (void *)sp->meta, (void *)sp->ent4})
       if (q) (void)hipFree(q);
-  for (void *q : {buf[0], buf[1], buf[2], xvec, (void *)partials, (void *)lz, (void *)lz_arrays,
-                  (void *)d_ops})
+  for (void *q : {buf[0], buf[1], buf[2], xvec, tbuf, (void *)partials, (void *)lz,
+                  (void *)lz_arrays, (void *)d_ops})
     if (q) (void)hipFree(q);
   if (stream) (void)hipStreamDestroy(stream);
 }
@@ -333,6 +333,12 @@ static void hxv_tune(Sector &sec) {
   dev_free_async(v, sec.stream);
   dev_free_async(hv, sec.stream);
   sec.hxv_variant = best;
+  if (sec.tbuf && hxv_scratch_vectors(sec) == 0) {  // timed, not picked
+    VCA_HIP(hipStreamSynchronize(sec.stream));
+    VCA_HIP(hipFree(sec.tbuf));
+    sec.tbuf = nullptr;
+    sec.tbuf_cap = 0;
+  }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[key] = best;
 }
@@ -1169,9 +1175,12 @@ static void build_gf(Solver &s) {
       const int nlanc = (int)std::min<int64_t>(js.dim, p.lanc_ngfiter);
       size_t free_b = 0, total_b = 0;
       VCA_HIP(hipMemGetInfo(&free_b, &total_b));
-      const size_t have = (size_t)js.nb_cap * 3 * js.vec_bytes();
+      // three Lanczos vectors per chain, plus the H x v shape's scratch
+      const size_t per_chain = (size_t)(3 + hxv_scratch_vectors(js)) * js.vec_bytes();
+      const size_t have = (size_t)js.nb_cap * 3 * js.vec_bytes() +
+                          (size_t)js.tbuf_cap * js.vec_bytes();
       const size_t budget = (free_b + have) / 2;
-      int nb_max = (int)std::max<size_t>(1, budget / (3 * js.vec_bytes()));
+      int nb_max = (int)std::max<size_t>(1, budget / per_chain);
       nb_max = std::min(nb_max, 512);
       const std::vector<int> &idx = g.second;
       for (size_t b0 = 0; b0 < idx.size(); b0 += nb_max) {

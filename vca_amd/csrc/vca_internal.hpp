@@ -29,6 +29,11 @@ constexpr int kColBits = 21;
 constexpr uint32_t kColMask = (1u << kColBits) - 1u;
 constexpr int kMaxTerms = 1024;   // 2*terms must fit the 11 high bits
 constexpr int kBinomN = kMaxNs + 1;
+// Hop entries of the LDS-staged H x v (DevSpin::ent4): bits 0-17 the LDS byte
+// address of the gathered element, bits 18-31 that of its signed amplitude.
+constexpr int kLdsAddrBits = 18;
+constexpr uint32_t kLdsAddrMask = (1u << kLdsAddrBits) - 1u;
+constexpr uint32_t kLdsAmpLimit = 1u << (32 - kLdsAddrBits);  // amplitude table bytes
 
 struct Error {
   int code;
@@ -190,7 +195,12 @@ struct DevSpin {
   int32_t *cnt = nullptr;
   uint32_t *ent = nullptr;
   RowMeta *meta = nullptr;   // [dim]
-  uint32_t *ent4 = nullptr;  // [ceil(maxnnz/4)][dim][4]: 4 hops of a row per 16 bytes
+  // [ceil(maxnnz/4)][dim][4]: 4 hops of a row per 16 bytes, as LDS byte
+  // addresses for the staged kernel of the sector's precision (amplitude
+  // table with a zero entry at LDS 0, then the column); padding entries
+  // address row 0 and the zero amplitude.  lds_form: the addresses fit.
+  uint32_t *ent4 = nullptr;
+  bool lds_form = false;
   double2 *amp64 = nullptr;  // [2*nterms]: +amp, -amp
   float2 *amp32 = nullptr;
 };
@@ -216,6 +226,9 @@ struct Sector {
   void *xvec = nullptr;                        // eigenvector accumulator
   double *partials = nullptr;
   int partials_cap = 0;
+  // transposed form of H x v: one scratch vector per chain [tbuf_cap][dim]
+  void *tbuf = nullptr;
+  int tbuf_cap = 0;
   LanczosDev *lz = nullptr;  // device structs [nb_cap]
   LanczosDev lzh{};          // host copy of chain 0's device array pointers
   std::vector<LanczosDev> lzh_all;  // host copies for every chain
@@ -244,6 +257,8 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb = 1);
 void ensure_step_partials(Sector &sec, int nb);
 int hxv_num_variants(const Sector &sec);
+// vectors of device scratch per chain that the sector's H x v shape needs
+int hxv_scratch_vectors(const Sector &sec);
 void clear_graphs(Sector &sec);
 void launch_start_vector(Sector &sec, uint64_t seed);
 void launch_norm0(Sector &sec);  // nrm[0] = ||buf[0]||, k = 0, done = 0
