@@ -98,8 +98,9 @@ def test_hxv_square_3x4_full_sector():
     assert err < 1e-12, err
 
 
-N_HXV_VARIANTS = 19  # launch shapes of the one-column kernels (kernels.hip kVariants)
-N_HXV_DIRECT = 15    # the first 15: direct form; the rest: transposed form
+N_HXV_VARIANTS = 21  # launch shapes of the one-column kernels (kernels.hip kVariants)
+N_HXV_DIRECT = 15    # the first 15: direct form; the rest: transposed form (the
+                     # last 2 with one real plane at a time: real amplitudes only)
 
 
 @pytest.mark.parametrize("variant", range(N_HXV_VARIANTS))
@@ -288,17 +289,19 @@ def test_hxv_device_batch_equals_single(prec, xcd, monkeypatch):
     assert err < (1e-12 if prec == "fp64" else 2e-6), err
 
 
+@pytest.mark.parametrize("variant", ["16", "20"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_transposed_form_in_lanczos_and_batches(prec, monkeypatch):
-    """The transposed H x v form (forced on small sectors) inside the
-    Lanczos steps: its two gather passes' partials of alpha, the buffer
+def test_transposed_form_in_lanczos_and_batches(prec, variant, monkeypatch):
+    """The transposed H x v form (forced on small sectors; variant 16 with
+    whole complex columns, 20 one real plane at a time) inside the Lanczos
+    steps: its two gather passes' partials of alpha, the buffer
     parity and the skipping of finished chains.  Bars: the oracle's Lanczos
     coefficients (12 steps, bath model); stacked products equal to the
     single ones bit for bit; a whole solve (GS, batched chains) equal to the
     direct form's within rounding."""
     from vca_amd.api import DeviceArray
     monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
-    monkeypatch.setenv("VCA_HXV_VARIANT", "16")
+    monkeypatch.setenv("VCA_HXV_VARIANT", variant)
     p, hloc, hk, bh, bv = M.bath_model(nlat=5, nb=5)
     om = M.oracle_model(p, hloc, bh, bv)
     s = _solver(p, hloc, bh, bv)
@@ -331,7 +334,7 @@ def test_transposed_form_in_lanczos_and_batches(prec, monkeypatch):
     pc, hlc, hkc, _, _ = M.chain(nlat=10, u=4.0, nk=4)
     pc.lanc_ngfiter = 80
     out = {}
-    for form, var in (("transposed", "16"), ("direct", "13")):
+    for form, var in (("transposed", variant), ("direct", "13")):
         monkeypatch.setenv("VCA_HXV_VARIANT", var)
         _say(f"10-site chain solve, {form} H x v")
         sc = VcaSolver(pc)

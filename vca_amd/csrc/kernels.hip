@@ -719,6 +719,99 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
   if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
 }
 
+// Gather pass of the transposed form (skip_dw) on one real plane at a time,
+// for real amplitudes: the real and imaginary parts of v transform
+// independently, so NPL = 2 planes of a complex vector are staged in LDS one
+// after the other (a 12,870-row complex128 column, 206 KB, does not fit the
+// LDS; one plane, 103 KB, does) and NPL = 1 is a real vector.  One workgroup
+// per column, BS lanes, one row per lane per block; LDS: the namp real
+// amplitudes and a zero at 0, then the staged plane, addressed by the
+// one-plane hop table (DevSpin::ent4p).  Each plane's results go straight
+// to that plane of the output (in place is safe: a plane is staged before
+// it is overwritten, and the other plane is not touched meanwhile).
+template <typename R, int NPL, int BS>
+__global__ __launch_bounds__(BS) void k_hxv_plane(const HxvArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int PF = BS == 1024 ? 4 : 2;
+  constexpr int RSZ = sizeof(R);
+  R *s_amp = reinterpret_cast<R *>(smem);
+  R *s_col = reinterpret_cast<R *>(smem + buf_lds(a.namp_up, 0, 0, RSZ).col);
+  const int dimup = a.dimup;
+  const WgPos wp = wg_pos(a);
+  if (!wp.valid) return;
+  const int idw = wp.idw;
+  const int64_t chain_off = (int64_t)wp.chain * a.vstride * NPL;
+  LanczosDev *lz = a.lz ? a.lz + wp.chain : nullptr;
+  bool done = false;
+  R scale = (R)1;
+  const R *vin = static_cast<const R *>(a.v0) + chain_off;
+  if (lz) {
+    const int k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const R *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  const R *vcol = vin + (int64_t)idw * dimup * NPL;
+  R *hcol = static_cast<R *>(a.hv) + chain_off + (int64_t)idw * dimup * NPL;
+  double dot = 0.0;
+
+  if (!done) {
+    const cplx<R> *g_amp = static_cast<const cplx<R> *>(a.amp_up);
+    for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp[t] = g_amp[t].x;
+    if (threadIdx.x == 0) s_amp[a.namp_up] = (R)0;  // padding hops
+    const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
+    const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
+    const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+    const int last = a.up_nc - 1;
+    for (int pl = 0; pl < NPL; ++pl) {
+      __syncthreads();  // the previous plane's gathers are done
+      for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[(int64_t)f * NPL + pl];
+      __syncthreads();
+      for (int base0 = 0; base0 < dimup; base0 += BS) {
+        if (base0 + (int)(threadIdx.x & ~63u) >= dimup) break;
+        const int base = base0 + (int)threadIdx.x;
+        const int row = min(base, dimup - 1);
+        const RowMeta m = a.meta_up[row];
+        const int nu = base < dimup ? m.cnt : 0;
+        const R x = s_col[row];
+        const double ed = a.skip_diag ? 0.0 : m.e + edw + cross_energy(a.cross, m.s, sdw);
+        R acc = (R)ed * x;
+        int nc_w = (nu + 3) >> 2;
+        for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
+        nc_w = __builtin_amdgcn_readfirstlane(nc_w);
+        if (nc_w > 0) {
+          uint4 e4[PF];
+#pragma unroll
+          for (int p = 0; p < PF; ++p) e4[p] = ent4[(int64_t)min(p, last) * dimup + row];
+          for (int c0 = 0; c0 < nc_w; c0 += PF) {
+#pragma unroll
+            for (int p = 0; p < PF; ++p) {
+              if (c0 + p < nc_w) {
+                const uint32_t ee[4] = {e4[p].x, e4[p].y, e4[p].z, e4[p].w};
+                R y[4], h[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                  y[t] = lds_load_re<R>(ee[t] & kLdsAddrMask);
+                  h[t] = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) acc = fma(h[t], y[t], acc);
+              }
+              e4[p] = ent4[(int64_t)min(c0 + p + PF, last) * dimup + row];
+            }
+          }
+        }
+        if (base < dimup) {
+          const R out = acc * scale;
+          hcol[(int64_t)row * NPL + pl] = out;
+          if (lz) dot += (double)(x * scale) * (double)out;
+        }
+      }
+    }
+  }
+  if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
+}
+
 // Transposed form of H x v, for sectors far larger than the on-chip caches.
 // The down-spin term sum_q H_dw(idw, jdw_q) v(iup, jdw_q) reads, for every
 // column, its ~nnz neighbour columns: on a 1.66e8-state sector (54 hops per
@@ -1109,6 +1202,17 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
                      sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4, esz,
                      namp, sp.lds_form ? 1 : 0);
   VCA_HIP(hipGetLastError());
+  // one-plane addresses (real amplitudes only)
+  const int rsz = esz / 2;
+  sp.plane_form = sp.real_amp && (size_t)(namp + 1) * rsz <= kLdsAmpLimit &&
+                  (size_t)buf_lds(namp, 0, 0, rsz).col + (size_t)sp.dim * rsz <= kLdsAddrMask + 1;
+  if (sp.plane_form) {
+    VCA_HIP(hipMalloc(&sp.ent4p, sizeof(uint32_t) * 4 * (size_t)nc * sp.dim));
+    hipLaunchKernelGGL(k_pack_rows, dim3((sp.dim + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4p,
+                       rsz, namp, 1);
+  }
+  VCA_HIP(hipGetLastError());
   VCA_HIP(hipStreamSynchronize(st));
   VCA_HIP(hipFree(d_ab));
   VCA_HIP(hipFree(d_eps));
@@ -1137,27 +1241,29 @@ static void ensure_partials(Sector &sec, int nblocks) {
 struct HxvShape {
   bool onecol_kernel;
   bool buf;    // buffer-load form (k_hxv_buf) when the column can be staged
-  bool tform;  // transposed form (k_hxv_buf passes on v and v^T, k_transpose)
+  bool tform;  // transposed form (gather passes on v and v^T, k_transpose)
+  bool plane;  // its gather passes one real plane at a time (k_hxv_plane)
   int bs, kr, u, cpb, grid;
 };
 
 // Variants of the one-column kernel the autotuner times: (BS, KR, U, form),
 // form 0 = k_hxv_col, 1 = k_hxv_buf (U fixed by KR there), 2 = transposed
-// form (k_hxv_buf passes).  The form-2 variants come last and are timed only
-// on sectors where tform_fits holds.
+// form with k_hxv_buf passes, 3 = transposed form with k_hxv_plane passes.
+// Forms 2 and 3 are timed only on sectors where tform_fits holds for them.
 static const int kVariants[][4] = {
     {256, 4, 4, 0}, {256, 2, 8, 0},  {1024, 1, 8, 0}, {1024, 2, 8, 0}, {1024, 4, 4, 0},
     {512, 2, 8, 0}, {256, 1, 16, 0}, {512, 1, 16, 0}, {1024, 1, 16, 0}, {256, 4, 4, 1},
     {256, 2, 8, 1}, {512, 2, 8, 1},  {512, 1, 8, 1},  {1024, 1, 8, 1}, {1024, 2, 8, 1},
-    {512, 1, 8, 2}, {1024, 1, 8, 2}, {512, 2, 8, 2},  {1024, 2, 8, 2}};
+    {512, 1, 8, 2}, {1024, 1, 8, 2}, {512, 2, 8, 2},  {1024, 2, 8, 2}, {512, 1, 8, 3},
+    {1024, 1, 8, 3}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
-constexpr int kNumDirect = 15;  // variants of forms 0 and 1
 
 // The transposed form pays 9 vector lengths of HBM traffic per product
 // against ~2 + nnz_dw re-reads of neighbour columns by the direct form, which
 // only hit on-chip while the vector fits the 256 MB MALL: offered from 2^22
-// states, when both spins' columns can be staged in LDS and one vector is
-// addressable by the 32-bit offsets of the buffer loads.
+// states, when both spins' columns (form 2; its buffer loads address one
+// vector with 32-bit offsets) or one real plane of them (form 3, real
+// amplitudes) can be staged in LDS.
 constexpr int64_t kTformMinDim = 1 << 22;
 constexpr size_t kLdsBig = 150 * 1024;
 
@@ -1165,11 +1271,13 @@ static size_t hxv_esz(const Sector &sec) {
   return sec.precision == VCA_FP64 ? sizeof(double2) : sizeof(float2);
 }
 
-static bool tform_fits(const Sector &sec) {
-  const size_t esz = hxv_esz(sec);
+static bool tform_fits(const Sector &sec, bool plane) {
+  const size_t esz = hxv_esz(sec), rsz = esz / 2;
   auto staged = [&](const DevSpin &sp) {
-    return sp.dim >= 128 && sp.lds_form &&
-           (size_t)buf_lds(2 * sp.nterms, sp.dim, 0, (int)esz).total <= kLdsBig;
+    if (sp.dim < 128) return false;
+    if (plane)
+      return sp.plane_form && (size_t)buf_lds(2 * sp.nterms, sp.dim, 0, (int)rsz).total <= kLdsBig;
+    return sp.lds_form && (size_t)buf_lds(2 * sp.nterms, sp.dim, 0, (int)esz).total <= kLdsBig;
   };
   // VCA_TFORM_MIN_DIM lowers the size threshold (tests run the form on
   // sectors small enough for the oracle)
@@ -1177,7 +1285,14 @@ static bool tform_fits(const Sector &sec) {
   const char *e = std::getenv("VCA_TFORM_MIN_DIM");
   if (e && *e) min_dim = std::atoll(e);
   return sec.dim >= min_dim && staged(sec.up) && staged(sec.dw) &&
-         (uint64_t)sec.dim * esz < (1ull << 31);
+         (plane || (uint64_t)sec.dim * esz < (1ull << 31));
+}
+
+bool hxv_variant_ok(const Sector &sec, int v) {
+  if (v < 0 || v >= kNumVariants) return false;
+  if (sec.up.dim < 128) return v == 0;  // one shape (k_hxv)
+  const int form = kVariants[v][3];
+  return form < 2 || tform_fits(sec, form == 3);
 }
 
 static HxvShape hxv_shape(const Sector &sec) {
@@ -1185,13 +1300,13 @@ static HxvShape hxv_shape(const Sector &sec) {
   if (sec.up.dim >= 128) {
     h.onecol_kernel = true;
     int v = sec.hxv_variant;
-    if (v < 0 || v >= kNumVariants || (v >= kNumDirect && !tform_fits(sec)))
-      v = sec.up.dim >= 4096 ? 4 : 2;  // default
+    if (!hxv_variant_ok(sec, v)) v = sec.up.dim >= 4096 ? 4 : 2;  // default
     h.bs = kVariants[v][0];
     h.kr = kVariants[v][1];
     h.u = kVariants[v][2];
     h.buf = kVariants[v][3] != 0;
-    h.tform = kVariants[v][3] == 2;
+    h.tform = kVariants[v][3] >= 2;
+    h.plane = kVariants[v][3] == 3;
     h.cpb = 1;
     h.grid = sec.dw.dim;
   } else {
@@ -1204,10 +1319,7 @@ static HxvShape hxv_shape(const Sector &sec) {
   return h;
 }
 
-int hxv_num_variants(const Sector &sec) {
-  if (sec.up.dim < 128) return 1;
-  return tform_fits(sec) ? kNumVariants : kNumDirect;
-}
+int hxv_num_variants(const Sector &sec) { return sec.up.dim < 128 ? 1 : kNumVariants; }
 
 int hxv_scratch_vectors(const Sector &sec) { return hxv_shape(sec).tform ? 1 : 0; }
 
@@ -1333,11 +1445,35 @@ static void launch_transpose(Sector &sec, const TrArgs &t, bool add, int nb) {
   VCA_HIP(hipGetLastError());
 }
 
+template <typename R, int NPL, int BS>
+static void launch_plane(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static std::once_flag attr_once;
+  auto fn = k_hxv_plane<R, NPL, BS>;
+  std::call_once(attr_once, [&]() {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    hipFuncAttributes fa{};  // LDS addresses from 0: no static LDS
+    VCA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+    if (fa.sharedSizeBytes != 0)
+      fail(VCA_ERR_HIP, "k_hxv_plane has %zu bytes of static LDS", (size_t)fa.sharedSizeBytes);
+  });
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
+}
+
+// One gather pass of the transposed form: k_hxv_buf on whole complex
+// columns, or (h.plane) k_hxv_plane on the two real planes of the columns.
 template <typename R>
 static void launch_buf_pass(Sector &sec, const HxvArgs &a, const HxvShape &h, bool real_amp,
-                            dim3 grid, size_t smem) {
-  if (real_amp) launch_buf_s<R, true>(sec, a, h, grid, smem);
-  else launch_buf_s<R, false>(sec, a, h, grid, smem);
+                            dim3 grid) {
+  if (h.plane) {
+    const size_t smem = (size_t)buf_lds(a.namp_up, a.dimup, 0, (int)sizeof(R)).total;
+    if (h.bs == 1024) launch_plane<R, 2, 1024>(sec, a, grid, smem);
+    else launch_plane<R, 2, 512>(sec, a, grid, smem);
+  } else {
+    const size_t smem = (size_t)buf_lds(a.namp_up, a.dimup, 0, (int)sizeof(cplx<R>)).total;
+    if (real_amp) launch_buf_s<R, true>(sec, a, h, grid, smem);
+    else launch_buf_s<R, false>(sec, a, h, grid, smem);
+  }
   VCA_HIP(hipGetLastError());
 }
 
@@ -1346,7 +1482,6 @@ static void launch_buf_pass(Sector &sec, const HxvArgs &a, const HxvShape &h, bo
 // passes run on the 2-D grid (columns, chains).
 template <typename R>
 static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_amp, int nb) {
-  constexpr size_t esz = sizeof(cplx<R>);
   const bool fp64 = sec.precision == VCA_FP64;
   a.xcd_chains = 0;
   a.nchains = nb;
@@ -1355,8 +1490,8 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   // 1. Hv = diag v + H_up v (columns of v staged in LDS)
   a.skip_diag = 0;
   a.ncb = sec.dw.dim;
-  launch_buf_pass<R>(sec, a, h, real_amp, dim3(sec.dw.dim, nb),
-                     (size_t)buf_lds(a.namp_up, sec.up.dim, 0, (int)esz).total);
+  if (h.plane) a.ent4_up = sec.up.ent4p;
+  launch_buf_pass<R>(sec, a, h, real_amp, dim3(sec.dw.dim, nb));
   // 2. vt = v^T
   TrArgs t{};
   t.in0 = a.v0;
@@ -1379,15 +1514,14 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   b.cnt_up = sec.dw.cnt;
   b.ent_up = sec.dw.ent;
   b.meta_up = sec.dw.meta;
-  b.ent4_up = sec.dw.ent4;
+  b.ent4_up = h.plane ? sec.dw.ent4p : sec.dw.ent4;
   b.up_nc = (sec.dw.maxnnz + 3) / 4;
   b.amp_up = fp64 ? (const void *)sec.dw.amp64 : (const void *)sec.dw.amp32;
   b.namp_up = 2 * sec.dw.nterms;
   b.skip_diag = 1;
   b.ncb = sec.up.dim;
   b.pbase = sec.dw.dim;
-  launch_buf_pass<R>(sec, b, h, real_amp, dim3(sec.up.dim, nb),
-                     (size_t)buf_lds(b.namp_up, sec.dw.dim, 0, (int)esz).total);
+  launch_buf_pass<R>(sec, b, h, real_amp, dim3(sec.up.dim, nb));
   // 4. Hv += vt^T
   TrArgs t2{};
   t2.in0 = t2.in1 = sec.tbuf;

@@ -140,7 +140,8 @@ Sector::~Sector() {
   clear_graphs(*this);
   for (DevSpin *sp : {&up, &dw})
     for (void *q : {(void *)sp->map, (void *)sp->energy, (void *)sp->cnt, (void *)sp->ent,
-                    (void *)sp->amp64, (void *)sp->amp32, (void *)sp->meta, (void *)sp->ent4})
+                    (void *)sp->amp64, (void *)sp->amp32, (void *)sp->meta, (void *)sp->ent4,
+                    (void *)sp->ent4p})
       if (q) (void)hipFree(q);
   for (void *q : {buf[0], buf[1], buf[2], xvec, tbuf, (void *)partials, (void *)lz,
                   (void *)lz_arrays, (void *)d_ops})
@@ -307,6 +308,7 @@ static void hxv_tune(Sector &sec) {
   int best = 0;
   float best_ms = std::numeric_limits<float>::max();
   for (int var = 0; var < nv; ++var) {
+    if (!hxv_variant_ok(sec, var)) continue;
     sec.hxv_variant = var;
     launch_hxv(sec, v, v, hv, false);
     float ms_min = std::numeric_limits<float>::max();
@@ -376,6 +378,7 @@ static void hxv_tune_batch(Sector &sec, int nb) {
     VCA_HIP(hipEventCreate(&e1));
     float best_ms = std::numeric_limits<float>::max();
     for (int var = 0; var < nv; ++var) {
+      if (!hxv_variant_ok(sec, var)) continue;
       sec.hxv_variant = var;
       launch_hxv(sec, sec.buf[0], sec.buf[0], sec.buf[2], false, nb);
       float ms_min = std::numeric_limits<float>::max();

@@ -201,6 +201,11 @@ struct DevSpin {
   // address row 0 and the zero amplitude.  lds_form: the addresses fit.
   uint32_t *ent4 = nullptr;
   bool lds_form = false;
+  // The same hops addressed for one real plane staged in LDS (elements and
+  // amplitudes of sizeof(real) bytes), read by the plane kernel of the
+  // transposed H x v on spins with real amplitudes; plane_form: usable.
+  uint32_t *ent4p = nullptr;
+  bool plane_form = false;
   double2 *amp64 = nullptr;  // [2*nterms]: +amp, -amp
   float2 *amp32 = nullptr;
 };
@@ -257,6 +262,7 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb = 1);
 void ensure_step_partials(Sector &sec, int nb);
 int hxv_num_variants(const Sector &sec);
+bool hxv_variant_ok(const Sector &sec, int v);  // launch shape v can run on sec
 // vectors of device scratch per chain that the sector's H x v shape needs
 int hxv_scratch_vectors(const Sector &sec);
 void clear_graphs(Sector &sec);
