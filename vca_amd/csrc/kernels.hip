@@ -843,9 +843,21 @@ struct TrArgs {
 
 constexpr int kTile = 64;
 
-template <typename R, bool ADD>
+__device__ inline void acc_elem(double &a, double b) { a += b; }
+__device__ inline void acc_elem(float &a, float b) { a += b; }
+__device__ inline void acc_elem(double2 &a, const double2 &b) {
+  a.x += b.x;
+  a.y += b.y;
+}
+__device__ inline void acc_elem(float2 &a, const float2 &b) {
+  a.x += b.x;
+  a.y += b.y;
+}
+
+// E: complex element (cplx<R>) or, in real-vector mode, a real.
+template <typename E, bool ADD>
 __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
-  using C = cplx<R>;
+  using C = E;
   __shared__ C s[kTile][kTile + 1];
   const int64_t off = (int64_t)blockIdx.z * a.vstride;
   const C *in = static_cast<const C *>(a.in0) + off;
@@ -874,11 +886,7 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
     if (ir < a.ni && j < a.nj) {
       C *o = out + (int64_t)ir * a.nj + j;
       C val = s[lane][ii];
-      if (ADD) {
-        const C prev = *o;
-        val.x += prev.x;
-        val.y += prev.y;
-      }
+      if (ADD) acc_elem(val, *o);
       *o = val;
     }
   }
@@ -1068,6 +1076,18 @@ __global__ __launch_bounds__(kBlock) void k_convert(const void *src, void *dst, 
     c.x = (RD)s[i].x;
     c.y = (RD)s[i].y;
     d[i] = c;
+  }
+}
+
+// n reals -> n complex numbers with zero imaginary parts (src, dst distinct).
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_expand_real(const R *src, cplx<R> *dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    cplx<R> c;
+    c.x = src[i];
+    c.y = (R)0;
+    dst[i] = c;
   }
 }
 
@@ -1326,13 +1346,19 @@ int hxv_scratch_vectors(const Sector &sec) { return hxv_shape(sec).tform ? 1 : 0
 // Partial sums per chain of one H x v: one per column, plus one per row of
 // v^T in the transposed form.
 static int hxv_partials(const Sector &sec, const HxvShape &h) {
-  return h.tform ? sec.dw.dim + sec.up.dim : h.grid;
+  return h.tform || sec.real_vec ? sec.dw.dim + sec.up.dim : h.grid;
 }
+
+bool real_mode_ok(const Sector &sec) {
+  return sec.up.dim >= 128 && sec.up.real_amp && sec.dw.real_amp && tform_fits(sec, true);
+}
+
+int64_t vec_elems(const Sector &sec) { return sec.real_vec ? (sec.dim + 1) / 2 : sec.dim; }
 
 // Scratch vectors of the transposed form for nb chains; like the partials,
 // a larger request first drains the stream and drops the recorded graphs.
 static void ensure_tbuf(Sector &sec, int nb) {
-  if (!hxv_shape(sec).tform || sec.tbuf_cap >= nb) return;
+  if ((!hxv_shape(sec).tform && !sec.real_vec) || sec.tbuf_cap >= nb) return;
   if (sec.tbuf) {
     VCA_HIP(hipStreamSynchronize(sec.stream));
     clear_graphs(sec);
@@ -1436,12 +1462,12 @@ static void launch_hxv_t(Sector &sec, HxvArgs &a, bool stage, bool onecol, dim3 
   }
 }
 
-template <typename R>
+template <typename E>
 static void launch_transpose(Sector &sec, const TrArgs &t, bool add, int nb) {
   const dim3 g((unsigned)((t.ni + kTile - 1) / kTile), (unsigned)((t.nj + kTile - 1) / kTile),
                (unsigned)nb);
-  if (add) hipLaunchKernelGGL((k_transpose<R, true>), g, dim3(kBlock), 0, sec.stream, t);
-  else hipLaunchKernelGGL((k_transpose<R, false>), g, dim3(kBlock), 0, sec.stream, t);
+  if (add) hipLaunchKernelGGL((k_transpose<E, true>), g, dim3(kBlock), 0, sec.stream, t);
+  else hipLaunchKernelGGL((k_transpose<E, false>), g, dim3(kBlock), 0, sec.stream, t);
   VCA_HIP(hipGetLastError());
 }
 
@@ -1501,7 +1527,7 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   t.nj = sec.dw.dim;
   t.vstride = sec.dim;
   t.lz = a.lz;
-  launch_transpose<R>(sec, t, false, nb);
+  launch_transpose<cplx<R>>(sec, t, false, nb);
   // 3. vt = (H_dw v)^T: the down-spin tables as the gather tables, rows of
   //    vt staged in LDS and overwritten by their result
   HxvArgs b = a;
@@ -1523,6 +1549,60 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   b.pbase = sec.dw.dim;
   launch_buf_pass<R>(sec, b, h, real_amp, dim3(sec.up.dim, nb));
   // 4. Hv += vt^T
+  TrArgs t2{};
+  t2.in0 = t2.in1 = sec.tbuf;
+  t2.out = a.hv;
+  t2.ni = sec.dw.dim;
+  t2.nj = sec.up.dim;
+  t2.vstride = sec.dim;
+  t2.lz = a.lz;
+  launch_transpose<cplx<R>>(sec, t2, true, nb);
+  if (a.lz) {
+    hipLaunchKernelGGL(k_lz_alpha, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                       (int)a.pstride);
+    VCA_HIP(hipGetLastError());
+  }
+}
+
+// Real-vector mode: the four launches of the transposed form on one real
+// plane (k_hxv_plane with NPL = 1, real transposes), 1024-lane passes.
+template <typename R>
+static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
+  a.xcd_chains = 0;
+  a.nchains = nb;
+  a.skip_dw = 1;
+  a.dw_maxnnz = 0;
+  a.skip_diag = 0;
+  a.ncb = sec.dw.dim;
+  a.ent4_up = sec.up.ent4p;
+  launch_plane<R, 1, 1024>(sec, a, dim3(sec.dw.dim, nb),
+                           (size_t)buf_lds(a.namp_up, sec.up.dim, 0, (int)sizeof(R)).total);
+  VCA_HIP(hipGetLastError());
+  TrArgs t{};
+  t.in0 = a.v0;
+  t.in1 = a.v1;
+  t.out = sec.tbuf;
+  t.ni = sec.up.dim;
+  t.nj = sec.dw.dim;
+  t.vstride = sec.dim;
+  t.lz = a.lz;
+  launch_transpose<R>(sec, t, false, nb);
+  HxvArgs b = a;
+  b.v0 = b.v1 = sec.tbuf;
+  b.hv = sec.tbuf;
+  b.dimup = sec.dw.dim;
+  b.dimdw = sec.up.dim;
+  b.meta_up = sec.dw.meta;
+  b.ent4_up = sec.dw.ent4p;
+  b.up_nc = (sec.dw.maxnnz + 3) / 4;
+  b.amp_up = sec.precision == VCA_FP64 ? (const void *)sec.dw.amp64 : (const void *)sec.dw.amp32;
+  b.namp_up = 2 * sec.dw.nterms;
+  b.skip_diag = 1;
+  b.ncb = sec.up.dim;
+  b.pbase = sec.dw.dim;
+  launch_plane<R, 1, 1024>(sec, b, dim3(sec.up.dim, nb),
+                           (size_t)buf_lds(b.namp_up, sec.dw.dim, 0, (int)sizeof(R)).total);
+  VCA_HIP(hipGetLastError());
   TrArgs t2{};
   t2.in0 = t2.in1 = sec.tbuf;
   t2.out = a.hv;
@@ -1575,6 +1655,20 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   const int grid = shp.grid;
   a.dw_maxnnz = sec.dw.maxnnz;
   const bool real_amp = sec.up.real_amp && sec.dw.real_amp;
+  if (sec.real_vec) {
+    if (nb != 1 || !real_mode_ok(sec))
+      fail(VCA_ERR_STATE, "real-vector H x v needs one chain on a sector that fits it");
+    if (lanczos) ensure_partials(sec, hxv_partials(sec, shp) * nb);
+    ensure_tbuf(sec, nb);
+    a.partials = sec.partials;
+    a.meta_up = sec.up.meta;
+    a.up_nc = (sec.up.maxnnz + 3) / 4;
+    a.pbase = 0;
+    a.pstride = hxv_partials(sec, shp);
+    if (fp64) launch_real_t<double>(sec, a, nb);
+    else launch_real_t<float>(sec, a, nb);
+    return;
+  }
   if (shp.onecol_kernel) {
     if (lanczos) ensure_partials(sec, hxv_partials(sec, shp) * nb);
     a.partials = sec.partials;
@@ -1650,9 +1744,9 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
   a.u1 = sec.buf[1];
   a.hv = sec.buf[2];
   a.x = accumulate_x ? sec.xvec : nullptr;
-  a.n = sec.dim;
+  a.n = vec_elems(sec);
   a.lz = sec.lz;
-  const int grid = grid_for(sec.dim);
+  const int grid = grid_for(a.n);
   ensure_partials(sec, grid * nb);
   a.partials = sec.partials;
   if (sec.precision == VCA_FP64)
@@ -1666,25 +1760,27 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
 }
 
 void launch_start_vector(Sector &sec, uint64_t seed) {
-  const int grid = grid_for(sec.dim);
+  const int64_t n = vec_elems(sec);
+  const int grid = grid_for(n);
   if (sec.precision == VCA_FP64)
-    hipLaunchKernelGGL(k_start<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
-                       sec.dim, seed);
+    hipLaunchKernelGGL(k_start<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0], n,
+                       seed);
   else
-    hipLaunchKernelGGL(k_start<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
-                       sec.dim, seed);
+    hipLaunchKernelGGL(k_start<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0], n,
+                       seed);
   VCA_HIP(hipGetLastError());
 }
 
 void launch_norm0(Sector &sec) {
-  const int grid = grid_for(sec.dim);
+  const int64_t n = vec_elems(sec);
+  const int grid = grid_for(n);
   ensure_partials(sec, grid);
   if (sec.precision == VCA_FP64)
-    hipLaunchKernelGGL(k_norm0<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
-                       sec.dim, sec.lz, sec.partials);
+    hipLaunchKernelGGL(k_norm0<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0], n,
+                       sec.lz, sec.partials);
   else
-    hipLaunchKernelGGL(k_norm0<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0],
-                       sec.dim, sec.lz, sec.partials);
+    hipLaunchKernelGGL(k_norm0<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.buf[0], n,
+                       sec.lz, sec.partials);
   VCA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_lz_start, dim3(1), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
                      grid);
@@ -1692,25 +1788,37 @@ void launch_norm0(Sector &sec) {
 }
 
 void launch_scale_x(Sector &sec) {
-  const int grid = grid_for(sec.dim);
+  const int64_t n = vec_elems(sec);
+  const int grid = grid_for(n);
   ensure_partials(sec, grid);
   // the norm lands in lz->y[0] (y is no longer needed after pass 2)
   const double *den = sec.lzh.y;
   if (sec.precision == VCA_FP64) {
     hipLaunchKernelGGL(k_normalize<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
-                       sec.dim, sec.lz, sec.partials);
+                       n, sec.lz, sec.partials);
     hipLaunchKernelGGL(k_lz_xnorm, dim3(1), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
                        grid);
     hipLaunchKernelGGL(k_scale_by<double>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
-                       sec.dim, den);
+                       n, den);
   } else {
     hipLaunchKernelGGL(k_normalize<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
-                       sec.dim, sec.lz, sec.partials);
+                       n, sec.lz, sec.partials);
     hipLaunchKernelGGL(k_lz_xnorm, dim3(1), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
                        grid);
     hipLaunchKernelGGL(k_scale_by<float>, dim3(grid), dim3(kBlock), 0, sec.stream, sec.xvec,
-                       sec.dim, den);
+                       n, den);
   }
+  VCA_HIP(hipGetLastError());
+}
+
+void launch_expand_real(Sector &sec, const void *src, void *dst) {
+  const int grid = grid_for(sec.dim);
+  if (sec.precision == VCA_FP64)
+    hipLaunchKernelGGL(k_expand_real<double>, dim3(grid), dim3(kBlock), 0, sec.stream,
+                       static_cast<const double *>(src), static_cast<double2 *>(dst), sec.dim);
+  else
+    hipLaunchKernelGGL(k_expand_real<float>, dim3(grid), dim3(kBlock), 0, sec.stream,
+                       static_cast<const float *>(src), static_cast<float2 *>(dst), sec.dim);
   VCA_HIP(hipGetLastError());
 }
 

@@ -475,7 +475,7 @@ static void ensure_lanczos(Sector &sec, int cap, int nb = 1) {
 // variable VCA_NO_GRAPHS=1 launches every step eagerly instead.
 static hipGraphExec_t step_graph(Sector &sec, int n, int nb, bool accx) {
   for (auto &g : sec.graphs)
-    if (g.nsteps == n && g.nb == nb && g.accx == accx) return g.exec;
+    if (g.nsteps == n && g.nb == nb && g.accx == accx && g.real == sec.real_vec) return g.exec;
   hipGraph_t graph;
   VCA_HIP(hipStreamBeginCapture(sec.stream, hipStreamCaptureModeThreadLocal));
   for (int i = 0; i < n; ++i) {
@@ -486,7 +486,7 @@ static hipGraphExec_t step_graph(Sector &sec, int n, int nb, bool accx) {
   hipGraphExec_t ex;
   VCA_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
   VCA_HIP(hipGraphDestroy(graph));
-  sec.graphs.push_back(Sector::StepGraph{n, nb, accx, ex});
+  sec.graphs.push_back(Sector::StepGraph{n, nb, accx, sec.real_vec, ex});
   return ex;
 }
 
@@ -548,22 +548,43 @@ struct GsOut {
   bool converged;
 };
 
+// Real-vector ground state (Sector::real_vec) for real Hamiltonians on the
+// sectors whose one-plane transposed H x v fits; VCA_REAL_GS=0 keeps the
+// complex vectors.
+static bool use_real_gs(const Sector &sec) {
+  const char *e = std::getenv("VCA_REAL_GS");
+  return !(e && e[0] == '0') && real_mode_ok(sec);
+}
+
 // Plain Lanczos ground state on sec (its precision), start = device vector
 // in that precision or null for the hash start vector.  The converged
 // eigenvector is left normalised in sec.xvec.  rel_tol: stop when
 // |dE| < tol*max(1,|E|) instead of |dE| < tol.  stall > 0: also stop once
 // |dE| has not reached a new minimum for `stall` steps (after 20 steps),
-// the noise floor of an FP32 run.
+// the noise floor of an FP32 run.  real: real-vector mode (start, if given,
+// holds dim reals); the eigenvector in sec.xvec is then expanded to complex
+// unless keep_real.
 static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, bool rel_tol,
-                        Solver *s, int stall = 0) {
+                        Solver *s, int stall = 0, bool real = false, bool keep_real = false) {
   niter = (int)std::min<int64_t>(niter, sec.dim);
   ensure_lanczos(sec, niter + 2);
+  struct RealMode {
+    Sector &sec;
+    ~RealMode() { sec.real_vec = false; }
+  } real_mode{sec};
+  sec.real_vec = real;
+  const size_t rsz = sec.precision == VCA_FP64 ? sizeof(double) : sizeof(float);
   auto init = [&]() {
     if (start)
-      VCA_HIP(hipMemcpyAsync(sec.buf[0], start, sec.vec_bytes(), hipMemcpyDeviceToDevice,
-                             sec.stream));
+      VCA_HIP(hipMemcpyAsync(sec.buf[0], start,
+                             real ? (size_t)(2 * vec_elems(sec)) * rsz : sec.vec_bytes(),
+                             hipMemcpyDeviceToDevice, sec.stream));
     else
       launch_start_vector(sec, 5001);
+    if (real && (sec.dim & 1))  // the pad real of the last complex pair
+      for (void *b : {sec.buf[0], sec.buf[1], sec.buf[2]})
+        VCA_HIP(hipMemsetAsync(static_cast<char *>(b) + (size_t)sec.dim * rsz, 0, rsz,
+                               sec.stream));
     launch_norm0(sec);
   };
   init();
@@ -614,6 +635,11 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
   VCA_HIP(hipMemsetAsync(sec.xvec, 0, sec.vec_bytes(), sec.stream));
   run_steps(sec, m, true, s);
   launch_scale_x(sec);
+  if (real && !keep_real) {  // dim reals -> dim complex numbers
+    launch_expand_real(sec, sec.xvec, sec.tbuf);
+    VCA_HIP(hipMemcpyAsync(sec.xvec, sec.tbuf, sec.vec_bytes(), hipMemcpyDeviceToDevice,
+                           sec.stream));
+  }
   VCA_HIP(hipStreamSynchronize(sec.stream));
   return GsOut{e_m, m, conv};
 }
@@ -624,13 +650,17 @@ static GsOut gs_mixed(Solver &s, Sector &sec32, Sector &sec64, double tol, int n
                       double *e32) {
   // FP32 to its noise floor (a near-FP32-accurate vector), so that the FP64
   // restart starts within ~1e-6 of the ground state and needs few steps
-  GsOut g32 = gs_lanczos(sec32, 1e-12, niter, nullptr, true, &s, 8);
+  // real Hamiltonian: both runs on real vectors, the FP32 eigenvector
+  // handed over as reals (converted as complex pairs)
+  const bool real = use_real_gs(sec32) && use_real_gs(sec64);
+  GsOut g32 = gs_lanczos(sec32, 1e-12, niter, nullptr, true, &s, 8, real, real);
   if (e32) *e32 = g32.e0;
   void *start64 = nullptr;
   VCA_HIP(hipMalloc(&start64, sec64.vec_bytes()));
-  launch_convert(sec32.xvec, VCA_MIXED, start64, VCA_FP64, sec32.dim, sec32.stream);
+  launch_convert(sec32.xvec, VCA_MIXED, start64, VCA_FP64,
+                 real ? (sec32.dim + 1) / 2 : sec32.dim, sec32.stream);
   VCA_HIP(hipStreamSynchronize(sec32.stream));
-  GsOut g64 = gs_lanczos(sec64, tol, niter, start64, false, &s);
+  GsOut g64 = gs_lanczos(sec64, tol, niter, start64, false, &s, 0, real, false);
   VCA_HIP(hipFree(start64));
   return g64;
 }
@@ -776,7 +806,8 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     VCA_HIP(hipStreamSynchronize(s64->stream));
   } else {
     auto s64 = make_sector(s, nup, ndw, VCA_FP64);
-    g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s);
+    g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s, 0,
+                   use_real_gs(*s64));
     VCA_HIP(hipMemcpyAsync(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice,
                            s64->stream));
     VCA_HIP(hipStreamSynchronize(s64->stream));
@@ -1913,7 +1944,7 @@ int vca_lanczos_gs(vca_sector *sv, double tol, int32_t niter, double *e0, double
     g = gs_mixed(*sec->solver, *sec, *twin, tol, niter, nullptr);
     res_sec = twin.get();
   } else {
-    g = gs_lanczos(*sec, tol, niter, nullptr, false, nullptr);
+    g = gs_lanczos(*sec, tol, niter, nullptr, false, nullptr, 0, use_real_gs(*sec));
   }
   *e0 = g.e0;
   if (iters) *iters = g.m;

@@ -239,10 +239,17 @@ struct Sector {
   std::vector<LanczosDev> lzh_all;  // host copies for every chain
   ExcOp *d_ops = nullptr;    // [nb_cap]
   double *lz_arrays = nullptr;
-  // captured chunks of Lanczos steps, keyed by (steps, chains, accumulate)
+  // Real-vector mode of a ground-state run (real H, real start vector):
+  // buf[*] and xvec hold dim reals instead of dim complex numbers, H x v is
+  // the transposed form on one real plane, and the element-wise kernels
+  // (update, norms, scaling, conversion) see the reals as ceil(dim/2)
+  // complex pairs, the odd pad element kept 0.  One chain only.
+  bool real_vec = false;
+  // captured chunks of Lanczos steps, keyed by (steps, chains, accumulate,
+  // real-vector mode)
   struct StepGraph {
     int nsteps, nb;
-    bool accx;
+    bool accx, real;
     hipGraphExec_t exec;
   };
   std::vector<StepGraph> graphs;
@@ -265,6 +272,14 @@ int hxv_num_variants(const Sector &sec);
 bool hxv_variant_ok(const Sector &sec, int v);  // launch shape v can run on sec
 // vectors of device scratch per chain that the sector's H x v shape needs
 int hxv_scratch_vectors(const Sector &sec);
+// Real-vector mode (Sector::real_vec) can run on the sector: real
+// amplitudes and the one-plane transposed H x v fits.
+bool real_mode_ok(const Sector &sec);
+// complex elements the element-wise kernels cover (ceil(dim/2) in
+// real-vector mode)
+int64_t vec_elems(const Sector &sec);
+// dst[i] = src[i] + 0i for the dim reals of src (same precision)
+void launch_expand_real(Sector &sec, const void *src, void *dst);
 void clear_graphs(Sector &sec);
 void launch_start_vector(Sector &sec, uint64_t seed);
 void launch_norm0(Sector &sec);  // nrm[0] = ||buf[0]||, k = 0, done = 0
