@@ -26,7 +26,9 @@ stream).  Reported beside it in the same JSON line:
     at beta=100 over 4096 frequencies, G on 512 real-axis points), on the
     GPU and by the oracle on one host core, with the differences;
   * bath16: the 8-site + 8-bath-level sector (165,636,900 states), H x v
-    FP64 and MIXED and the ground state both ways;
+    FP64 and MIXED on complex vectors (the tuned form: the transposed one on
+    this sector) and on real vectors (the Hamiltonian is real), and the
+    ground state both ways (it runs on real vectors);
   * roofline: of the FP64 H x v kernel, algorithmic bytes (one read of v and
     one write of Hv, 2 x 16 B per state) over the event-timed duration,
     against 8 TB/s; traffic = HBM bytes per launch measured with rocprofv3
@@ -36,7 +38,8 @@ stream).  Reported beside it in the same JSON line:
     cores with the reference's DW-slice split, and the Omega integrand
     (numpy) on a sample of frequencies.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--only hxv|omega]
+    python bench.py [--steps K] [--warmup W] [--gpus 1]
+                    [--only hxv|omega|full|c3|c1|bath16]
                     [--no-full] [--no-c3] [--no-bath16] [--no-cpu]
 """
 from __future__ import annotations
@@ -241,11 +244,12 @@ def main():
                     help="skip the 8-site + 8-bath-level sector (165,636,900 states)")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
-    ap.add_argument("--only", choices=["all", "hxv", "omega", "full", "c3", "c1"], default="all",
+    ap.add_argument("--only", choices=["all", "hxv", "omega", "full", "c3", "c1", "bath16"],
+                    default="all",
                     help="hxv: the H x v timing plus a 256 MiB copy that calibrates the HBM "
                          "byte counters; omega: the Omega integrand timing; full: the "
-                         "end-to-end 3x4 solve; c3: the every-sector 2x5 solve (rocprofv3 "
-                         "runs)")
+                         "end-to-end 3x4 solve; c3: the every-sector 2x5 solve; bath16: the "
+                         "1.66e8-state bath sector (rocprofv3 runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would "
@@ -263,6 +267,58 @@ def main():
         dt = np.complex128 if prec == "fp64" else np.complex64
         return (rng.standard_normal(n, dtype=np.float32)
                 + 1j * rng.standard_normal(n, dtype=np.float32)).astype(dt)
+
+    # ------------------------------- 8 sites + 8 bath levels, 1.66e8 states
+    def run_bath16():
+        pb, hlb, hkb, bhb, bvb = configs.config_bath_2x4(nk=2, precision="mixed")
+        sb = VcaSolver(pb)
+        sb.vca_set_bath(bhb, bvb)
+        sb.vca_set_hcluster(hlb)
+        out16 = {}
+        # FP64 first: the MIXED ground state builds an FP64 twin sector for
+        # its refinement, whose launch shape is then already tuned
+        for prec in ("fp64", "mixed"):
+            sec = sb.sector(8, 8, prec)
+            v = rand_vec(sec.dim, prec, 2)
+            dv = DeviceArray(v.shape, v.dtype).upload(v)
+            dh = DeviceArray(v.shape, v.dtype)
+            del v
+            sec.hxv_device(dv.ptr, dh.ptr, repeat=2)
+            device_synchronize()
+            ms = sec.hxv_device(dv.ptr, dh.ptr, repeat=5)
+            device_synchronize()
+            dv.free()
+            dh.free()
+            out16[prec] = {"ms_per_hxv": ms, "gstates_per_s": sec.dim / (ms * 1e-3) / 1e9,
+                           "hbm_gbs_algorithmic": 2 * (16 if prec == "fp64" else 8) * sec.dim
+                           / (ms * 1e-3) / 1e9}
+            log(f"bath16 {prec}: {ms:.3f} ms/apply, dim {sec.dim}")
+            # the real-vector product the ground state runs (real Hamiltonian)
+            rdt = np.float64 if prec == "fp64" else np.float32
+            rv = np.random.default_rng(3).standard_normal(sec.dim, dtype=np.float32).astype(rdt)
+            drv = DeviceArray(rv.shape, rdt).upload(rv)
+            drh = DeviceArray(rv.shape, rdt)
+            del rv
+            sec.hxv_device_real(drv.ptr, drh.ptr, repeat=2)
+            ms_r = sec.hxv_device_real(drv.ptr, drh.ptr, repeat=5)
+            device_synchronize()
+            drv.free()
+            drh.free()
+            rsz = 8 if prec == "fp64" else 4
+            out16[prec]["real_vector"] = {
+                "ms_per_hxv": ms_r, "gstates_per_s": sec.dim / (ms_r * 1e-3) / 1e9,
+                "hbm_gbs_algorithmic": 2 * rsz * sec.dim / (ms_r * 1e-3) / 1e9}
+            log(f"bath16 {prec} real vectors: {ms_r:.3f} ms/apply")
+            t0 = time.perf_counter()
+            e_, _, it_ = sec.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
+            out16["gs_" + prec] = {"e0": e_, "seconds": time.perf_counter() - t0, "iters": it_}
+            sec.close()
+        out16["e0_rel_err_mixed"] = abs(out16["gs_mixed"]["e0"] - out16["gs_fp64"]["e0"]) / abs(
+            out16["gs_fp64"]["e0"])
+        out16["gs_speedup_mixed"] = out16["gs_fp64"]["seconds"] / out16["gs_mixed"]["seconds"]
+        out16["dim"] = 12870 * 12870
+        log(f"bath16 GS: mixed {out16['gs_mixed']} fp64 {out16['gs_fp64']}")
+        return out16
 
     # --------------------------------------------------------- Omega -----
     def omega_section():
@@ -323,6 +379,9 @@ def main():
         return
     if args.only == "c1":
         print(json.dumps({"chain4": run_c1(cpu=not args.no_cpu)}), flush=True)
+        return
+    if args.only == "bath16":
+        print(json.dumps({"bath16": run_bath16()}), flush=True)
         return
 
     p, hloc, hk, _, _ = configs.config_square_3x4(nk=2)
@@ -404,42 +463,6 @@ def main():
         except Exception as exc:
             log(f"{name} failed: {exc}")
             return {"error": str(exc)}
-
-    # ------------------------------- 8 sites + 8 bath levels, 1.66e8 states
-    def run_bath16():
-        pb, hlb, hkb, bhb, bvb = configs.config_bath_2x4(nk=2, precision="mixed")
-        sb = VcaSolver(pb)
-        sb.vca_set_bath(bhb, bvb)
-        sb.vca_set_hcluster(hlb)
-        out16 = {}
-        # FP64 first: the MIXED ground state builds an FP64 twin sector for
-        # its refinement, whose launch shape is then already tuned
-        for prec in ("fp64", "mixed"):
-            sec = sb.sector(8, 8, prec)
-            v = rand_vec(sec.dim, prec, 2)
-            dv = DeviceArray(v.shape, v.dtype).upload(v)
-            dh = DeviceArray(v.shape, v.dtype)
-            del v
-            sec.hxv_device(dv.ptr, dh.ptr, repeat=2)
-            device_synchronize()
-            ms = sec.hxv_device(dv.ptr, dh.ptr, repeat=5)
-            device_synchronize()
-            dv.free()
-            dh.free()
-            out16[prec] = {"ms_per_hxv": ms, "gstates_per_s": sec.dim / (ms * 1e-3) / 1e9,
-                           "hbm_gbs_algorithmic": 2 * (16 if prec == "fp64" else 8) * sec.dim
-                           / (ms * 1e-3) / 1e9}
-            log(f"bath16 {prec}: {ms:.3f} ms/apply, dim {sec.dim}")
-            t0 = time.perf_counter()
-            e_, _, it_ = sec.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
-            out16["gs_" + prec] = {"e0": e_, "seconds": time.perf_counter() - t0, "iters": it_}
-            sec.close()
-        out16["e0_rel_err_mixed"] = abs(out16["gs_mixed"]["e0"] - out16["gs_fp64"]["e0"]) / abs(
-            out16["gs_fp64"]["e0"])
-        out16["gs_speedup_mixed"] = out16["gs_fp64"]["seconds"] / out16["gs_mixed"]["seconds"]
-        out16["dim"] = 12870 * 12870
-        log(f"bath16 GS: mixed {out16['gs_mixed']} fp64 {out16['gs_fp64']}")
-        return out16
 
     full = None if args.no_full else guarded("full 3x4 solve",
                                              lambda: run_full(cpu=not args.no_cpu))

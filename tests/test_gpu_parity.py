@@ -402,3 +402,37 @@ def test_real_vector_ground_state_in_a_solve(monkeypatch):
     assert abs(out["1"][0] - out["0"][0]) <= 1e-8 * abs(out["0"][0])
     for x, y in zip(out["1"][2:], out["0"][2:]):
         assert np.abs(x - y).max() <= 2e-6 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_hxv_device_real(prec, monkeypatch):
+    """vca_hxv_device_real on real vectors (size threshold lowered) gives the
+    oracle's H x v (1e-12 / 2e-6 relative), on an even and an odd sector;
+    a sector with complex amplitudes is refused with VCA_ERR_ARG."""
+    from vca_amd.api import DeviceArray, VcaError
+    monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
+    dt = np.float64 if prec == "fp64" else np.float32
+    for name, (p, hloc, hk, bh, bv), sector in (
+            ("bath5+5", M.bath_model(nlat=5, nb=5), (5, 5)),
+            ("chain11", M.chain(nlat=11, u=4.0), (3, 3))):
+        om = M.oracle_model(p, hloc, bh, bv)
+        s = _solver(p, hloc, bh, bv)
+        sec = s.sector(*sector, prec)
+        _say(f"real H x v {name} {sector} {prec}")
+        v = np.random.default_rng(3).normal(size=sec.dim).astype(dt)
+        dv = DeviceArray(v.shape, dt).upload(v)
+        dh = DeviceArray(v.shape, dt)
+        sec.hxv_device_real(dv.ptr, dh.ptr)
+        ref = om.hxv(*sector, v.astype(np.complex128))
+        assert np.abs(ref.imag).max() == 0.0
+        err = np.linalg.norm(dh.download() - ref.real) / np.linalg.norm(ref.real)
+        assert err < (1e-12 if prec == "fp64" else 2e-6), (name, err)
+        sec.close()
+    bhz = VcaParams(nlat=5, norb=2, nspin=2, uloc=(2.0, 2.0, 0, 0, 0))
+    bhz_h, _ = configs.bhz(5, 1, 2.0, 0.3, nk=(2, 1))
+    s = _solver(bhz, bhz_h)
+    sec = s.sector(5, 4, prec)
+    d = DeviceArray((sec.dim,), dt)
+    with pytest.raises(VcaError) as ei:
+        sec.hxv_device_real(d.ptr, d.ptr)
+    assert ei.value.code == -1

@@ -86,6 +86,7 @@ _SIGS = {
     "vca_hxv_device": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.POINTER(ctypes.c_float)]),
     "vca_hxv_device_batch": (ctypes.c_int, [_P, _I32, _P, _P, _I32,
                                             ctypes.POINTER(ctypes.c_float)]),
+    "vca_hxv_device_real": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.POINTER(ctypes.c_float)]),
     "vca_device_alloc": (ctypes.c_int, [ctypes.c_int64, ctypes.POINTER(_P)]),
     "vca_device_free": (ctypes.c_int, [_P]),
     "vca_device_synchronize": (ctypes.c_int, []),

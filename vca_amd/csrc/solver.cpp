@@ -1901,6 +1901,38 @@ int vca_hxv_device_batch(vca_sector *sv, int32_t nb, const void *v, void *hv, in
   VCA_GUARD_END
 }
 
+int vca_hxv_device_real(vca_sector *sv, const void *v, void *hv, int32_t repeat, float *ms) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !v || !hv) fail(VCA_ERR_ARG, "null argument");
+  if (!real_mode_ok(*sec))
+    fail(VCA_ERR_ARG,
+         "sector (%d,%d): the real-vector H x v needs real amplitudes and a sector large "
+         "enough for the transposed form",
+         sec->nup, sec->ndw);
+  if (repeat < 1) repeat = 1;
+  struct RealMode {
+    Sector &sec;
+    ~RealMode() { sec.real_vec = false; }
+  } real_mode{*sec};
+  sec->real_vec = true;
+  hipEvent_t e0, e1;
+  VCA_HIP(hipEventCreate(&e0));
+  VCA_HIP(hipEventCreate(&e1));
+  VCA_HIP(hipEventRecord(e0, sec->stream));
+  for (int r = 0; r < repeat; ++r) launch_hxv(*sec, v, v, hv, false);
+  VCA_HIP(hipEventRecord(e1, sec->stream));
+  VCA_HIP(hipEventSynchronize(e1));
+  if (ms) {
+    float t = 0;
+    VCA_HIP(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeat;
+  }
+  VCA_HIP(hipEventDestroy(e0));
+  VCA_HIP(hipEventDestroy(e1));
+  VCA_GUARD_END
+}
+
 int vca_sector_spin_coo(const vca_sector *sv, int32_t spin, int64_t *nnz, int32_t *rows,
                         int32_t *cols, double *amps) {
   VCA_GUARD_BEGIN
