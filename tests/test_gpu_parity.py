@@ -404,13 +404,17 @@ def test_real_vector_ground_state_in_a_solve(monkeypatch):
         assert np.abs(x - y).max() <= 2e-6 * np.abs(y).max()
 
 
+@pytest.mark.parametrize("two", ["1", "0"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_hxv_device_real(prec, monkeypatch):
+def test_hxv_device_real(prec, two, monkeypatch):
     """vca_hxv_device_real on real vectors (size threshold lowered) gives the
-    oracle's H x v (1e-12 / 2e-6 relative), on an even and an odd sector;
-    a sector with complex amplitudes is refused with VCA_ERR_ARG."""
+    oracle's H x v (1e-12 / 2e-6 relative), with two columns per workgroup
+    in the gather passes (k_hxv_plane2) or one (k_hxv_plane), on a sector
+    with an even and one with an odd number of columns; a sector with
+    complex amplitudes is refused with VCA_ERR_ARG."""
     from vca_amd.api import DeviceArray, VcaError
     monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
+    monkeypatch.setenv("VCA_PLANE2", two)
     dt = np.float64 if prec == "fp64" else np.float32
     for name, (p, hloc, hk, bh, bv), sector in (
             ("bath5+5", M.bath_model(nlat=5, nb=5), (5, 5)),

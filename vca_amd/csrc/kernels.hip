@@ -812,6 +812,118 @@ __global__ __launch_bounds__(BS) void k_hxv_plane(const HxvArgs a) {
   if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
 }
 
+// The real-vector gather pass with two columns per workgroup (columns 2p
+// and 2p + 1 of the pair p = workgroup), for real vectors whose two columns
+// fit the LDS together (FP32: 2 x 51.5 KB on the 12,870-row bath sector):
+// every hop entry and amplitude read from the table serves both columns,
+// halving the entry stream from L2 and a third of the LDS reads per output.
+// LDS: the amplitudes and a zero at 0, column 0 at the one-plane table's
+// offset, column 1 cstride bytes after it.  One partial of <v|Hv> per pair.
+template <typename R, int BS>
+__global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  constexpr int PF = BS == 1024 ? 4 : 2;
+  constexpr int RSZ = sizeof(R);
+  const int dimup = a.dimup;
+  const uint32_t o_col = (uint32_t)buf_lds(a.namp_up, 0, 0, RSZ).col;
+  const uint32_t cstride = (uint32_t)align16(dimup * RSZ);
+  R *s_amp = reinterpret_cast<R *>(smem);
+  R *s_col0 = reinterpret_cast<R *>(smem + o_col);
+  R *s_col1 = reinterpret_cast<R *>(smem + o_col + cstride);
+  const int npair = (a.dimdw + 1) / 2;
+  const int pair = xcd_column(blockIdx.x, npair);
+  const int idw0 = 2 * pair;
+  const bool two = idw0 + 1 < a.dimdw;
+  const int idw1 = two ? idw0 + 1 : idw0;
+  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
+  const int64_t slot = a.pbase + (int64_t)blockIdx.y * a.pstride + blockIdx.x;
+  LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
+  bool done = false;
+  R scale = (R)1;
+  const R *vin = static_cast<const R *>(a.v0) + chain_off;
+  if (lz) {
+    const int k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const R *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  const R *vcol0 = vin + (int64_t)idw0 * dimup;
+  const R *vcol1 = vin + (int64_t)idw1 * dimup;
+  R *hcol0 = static_cast<R *>(a.hv) + chain_off + (int64_t)idw0 * dimup;
+  R *hcol1 = static_cast<R *>(a.hv) + chain_off + (int64_t)idw1 * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    const cplx<R> *g_amp = static_cast<const cplx<R> *>(a.amp_up);
+    for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp[t] = g_amp[t].x;
+    if (threadIdx.x == 0) s_amp[a.namp_up] = (R)0;  // padding hops
+    for (int f = threadIdx.x; f < dimup; f += BS) {
+      s_col0[f] = vcol0[f];
+      s_col1[f] = vcol1[f];
+    }
+    __syncthreads();
+    const double edw0 = a.skip_diag ? 0.0 : a.e_dw[idw0];
+    const double edw1 = a.skip_diag ? 0.0 : a.e_dw[idw1];
+    const uint32_t sdw0 = a.skip_diag ? 0u : a.map_dw[idw0];
+    const uint32_t sdw1 = a.skip_diag ? 0u : a.map_dw[idw1];
+    const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+    const int last = a.up_nc - 1;
+    for (int base0 = 0; base0 < dimup; base0 += BS) {
+      if (base0 + (int)(threadIdx.x & ~63u) >= dimup) break;
+      const int base = base0 + (int)threadIdx.x;
+      const int row = min(base, dimup - 1);
+      const RowMeta m = a.meta_up[row];
+      const int nu = base < dimup ? m.cnt : 0;
+      const R x0 = s_col0[row], x1 = s_col1[row];
+      R acc0 = (R)0, acc1 = (R)0;
+      if (!a.skip_diag) {
+        acc0 = (R)(m.e + edw0 + cross_energy(a.cross, m.s, sdw0)) * x0;
+        acc1 = (R)(m.e + edw1 + cross_energy(a.cross, m.s, sdw1)) * x1;
+      }
+      int nc_w = (nu + 3) >> 2;
+      for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
+      nc_w = __builtin_amdgcn_readfirstlane(nc_w);
+      if (nc_w > 0) {
+        uint4 e4[PF];
+#pragma unroll
+        for (int p = 0; p < PF; ++p) e4[p] = ent4[(int64_t)min(p, last) * dimup + row];
+        for (int c0 = 0; c0 < nc_w; c0 += PF) {
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            if (c0 + p < nc_w) {
+              const uint32_t ee[4] = {e4[p].x, e4[p].y, e4[p].z, e4[p].w};
+              R y0[4], y1[4], h[4];
+#pragma unroll
+              for (int t = 0; t < 4; ++t) {
+                const uint32_t ya = ee[t] & kLdsAddrMask;
+                y0[t] = lds_load_re<R>(ya);
+                y1[t] = lds_load_re<R>(ya + cstride);
+                h[t] = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+              }
+#pragma unroll
+              for (int t = 0; t < 4; ++t) {
+                acc0 = fma(h[t], y0[t], acc0);
+                acc1 = fma(h[t], y1[t], acc1);
+              }
+            }
+            e4[p] = ent4[(int64_t)min(c0 + p + PF, last) * dimup + row];
+          }
+        }
+      }
+      if (base < dimup) {
+        const R out0 = acc0 * scale, out1 = acc1 * scale;
+        hcol0[row] = out0;
+        if (lz) dot += (double)(x0 * scale) * (double)out0;
+        if (two) {
+          hcol1[row] = out1;
+          if (lz) dot += (double)(x1 * scale) * (double)out1;
+        }
+      }
+    }
+  }
+  if (lz) block_partial_lds<BS>(dot, a.partials, slot, reinterpret_cast<double *>(smem));
+}
+
 // Transposed form of H x v, for sectors far larger than the on-chip caches.
 // The down-spin term sum_q H_dw(idw, jdw_q) v(iup, jdw_q) reads, for every
 // column, its ~nnz neighbour columns: on a 1.66e8-state sector (54 hops per
@@ -1345,8 +1457,29 @@ int hxv_scratch_vectors(const Sector &sec) { return hxv_shape(sec).tform ? 1 : 0
 
 // Partial sums per chain of one H x v: one per column, plus one per row of
 // v^T in the transposed form.
+// Real-vector gather pass over the columns of spin sp's table (column length
+// sp.dim): two columns per workgroup (k_hxv_plane2) when both fit the LDS
+// (VCA_PLANE2=0 keeps one).
+static bool real_pass_two(const Sector &sec, const DevSpin &sp) {
+  const char *e = std::getenv("VCA_PLANE2");
+  if (e && e[0] == '0') return false;
+  const int rsz = sec.precision == VCA_FP64 ? (int)sizeof(double) : (int)sizeof(float);
+  return (size_t)buf_lds(2 * sp.nterms, 0, 0, rsz).col + 2 * (size_t)align16(sp.dim * rsz) <=
+         kLdsBig;
+}
+
+// Workgroups of the real-vector passes: over the DimDw columns of v (up-spin
+// tables), then over the DimUp rows of v^T (down-spin tables).
+static int real_grid_up(const Sector &sec) {
+  return real_pass_two(sec, sec.up) ? (sec.dw.dim + 1) / 2 : sec.dw.dim;
+}
+static int real_grid_dw(const Sector &sec) {
+  return real_pass_two(sec, sec.dw) ? (sec.up.dim + 1) / 2 : sec.up.dim;
+}
+
 static int hxv_partials(const Sector &sec, const HxvShape &h) {
-  return h.tform || sec.real_vec ? sec.dw.dim + sec.up.dim : h.grid;
+  if (sec.real_vec) return real_grid_up(sec) + real_grid_dw(sec);
+  return h.tform ? sec.dw.dim + sec.up.dim : h.grid;
 }
 
 bool real_mode_ok(const Sector &sec) {
@@ -1564,8 +1697,39 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   }
 }
 
+template <typename R>
+static void launch_plane2(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static std::once_flag attr_once;
+  auto fn = k_hxv_plane2<R, 1024>;
+  std::call_once(attr_once, [&]() {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    hipFuncAttributes fa{};  // LDS addresses from 0: no static LDS
+    VCA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+    if (fa.sharedSizeBytes != 0)
+      fail(VCA_ERR_HIP, "k_hxv_plane2 has %zu bytes of static LDS", (size_t)fa.sharedSizeBytes);
+  });
+  hipLaunchKernelGGL(fn, grid, dim3(1024), smem, sec.stream, a);
+}
+
+// One real-vector gather pass on grid workgroups per chain: one column per
+// workgroup (k_hxv_plane, NPL = 1) or two (k_hxv_plane2).
+template <typename R>
+static void launch_real_pass(Sector &sec, const HxvArgs &a, bool two, int grid, int nb) {
+  constexpr int rsz = (int)sizeof(R);
+  if (two) {
+    const size_t smem = (size_t)buf_lds(a.namp_up, 0, 0, rsz).col +
+                        2 * (size_t)align16(a.dimup * rsz);
+    launch_plane2<R>(sec, a, dim3(grid, nb), smem);
+  } else {
+    launch_plane<R, 1, 1024>(sec, a, dim3(grid, nb),
+                             (size_t)buf_lds(a.namp_up, a.dimup, 0, rsz).total);
+  }
+  VCA_HIP(hipGetLastError());
+}
+
 // Real-vector mode: the four launches of the transposed form on one real
-// plane (k_hxv_plane with NPL = 1, real transposes), 1024-lane passes.
+// plane (real transposes), 1024-lane gather passes.
 template <typename R>
 static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   a.xcd_chains = 0;
@@ -1575,9 +1739,7 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   a.skip_diag = 0;
   a.ncb = sec.dw.dim;
   a.ent4_up = sec.up.ent4p;
-  launch_plane<R, 1, 1024>(sec, a, dim3(sec.dw.dim, nb),
-                           (size_t)buf_lds(a.namp_up, sec.up.dim, 0, (int)sizeof(R)).total);
-  VCA_HIP(hipGetLastError());
+  launch_real_pass<R>(sec, a, real_pass_two(sec, sec.up), real_grid_up(sec), nb);
   TrArgs t{};
   t.in0 = a.v0;
   t.in1 = a.v1;
@@ -1599,10 +1761,8 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   b.namp_up = 2 * sec.dw.nterms;
   b.skip_diag = 1;
   b.ncb = sec.up.dim;
-  b.pbase = sec.dw.dim;
-  launch_plane<R, 1, 1024>(sec, b, dim3(sec.up.dim, nb),
-                           (size_t)buf_lds(b.namp_up, sec.dw.dim, 0, (int)sizeof(R)).total);
-  VCA_HIP(hipGetLastError());
+  b.pbase = real_grid_up(sec);
+  launch_real_pass<R>(sec, b, real_pass_two(sec, sec.dw), real_grid_dw(sec), nb);
   TrArgs t2{};
   t2.in0 = t2.in1 = sec.tbuf;
   t2.out = a.hv;
