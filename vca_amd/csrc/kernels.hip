@@ -1271,7 +1271,7 @@ __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
 static int grid_for(int64_t n) {
   int64_t g = (n + kBlock * 4 - 1) / (kBlock * 4);
   if (g < 1) g = 1;
-  if (g > 4096) g = 4096;
+  if (g > 32768) g = 32768;
   return (int)g;
 }
 
