@@ -345,15 +345,18 @@ def test_transposed_form_in_lanczos_and_batches(prec, variant, monkeypatch):
         assert np.abs(x - y).max() <= 1e-9 * np.abs(y).max()
 
 
+@pytest.mark.parametrize("form", ["direct", "transposed2"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_real_vector_ground_state(prec, monkeypatch):
+def test_real_vector_ground_state(prec, form, monkeypatch):
     """Real Hamiltonians: the ground-state Lanczos on real vectors (forced on
     small sectors by lowering the transposed form's size threshold) against
     the same run on complex vectors: E0 within 1e-10 relative of each other,
     the eigenvector a normalised, real eigenvector of the oracle's H
     (residual 1e-5).  Sectors: 10-site chain
     (5,5), bath model (5,5), and the 11-site chain (3,3) whose 27,225 states
-    leave the pad real of the last complex pair."""
+    leave the pad real of the last complex pair.  Direct and transposed
+    real-vector H x v."""
+    _set_env(monkeypatch, REAL_FORMS[form])
     monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
     for name, (p, hloc, hk, bh, bv), sector in (
             ("chain10", M.chain(nlat=10, u=4.0), (5, 5)),
@@ -404,17 +407,30 @@ def test_real_vector_ground_state_in_a_solve(monkeypatch):
         assert np.abs(x - y).max() <= 2e-6 * np.abs(y).max()
 
 
-@pytest.mark.parametrize("two", ["1", "0"])
+REAL_FORMS = {
+    # the direct form (k_hxv_buf_real): small sectors keep the default shape
+    "direct": {},
+    # the transposed form on one real plane, two columns per workgroup
+    # (k_hxv_plane2) or one (k_hxv_plane)
+    "transposed2": {"VCA_TFORM_MIN_DIM": "0", "VCA_HXV_VARIANT": "20", "VCA_PLANE2": "1"},
+    "transposed1": {"VCA_TFORM_MIN_DIM": "0", "VCA_HXV_VARIANT": "20", "VCA_PLANE2": "0"},
+}
+
+
+def _set_env(monkeypatch, env):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+
+
+@pytest.mark.parametrize("form", sorted(REAL_FORMS))
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_hxv_device_real(prec, two, monkeypatch):
-    """vca_hxv_device_real on real vectors (size threshold lowered) gives the
-    oracle's H x v (1e-12 / 2e-6 relative), with two columns per workgroup
-    in the gather passes (k_hxv_plane2) or one (k_hxv_plane), on a sector
-    with an even and one with an odd number of columns; a sector with
-    complex amplitudes is refused with VCA_ERR_ARG."""
+def test_hxv_device_real(prec, form, monkeypatch):
+    """vca_hxv_device_real on real vectors gives the oracle's H x v (1e-12 /
+    2e-6 relative) in each real-vector form, on a sector with an even and
+    one with an odd number of columns; a sector with complex amplitudes is
+    refused with VCA_ERR_ARG."""
     from vca_amd.api import DeviceArray, VcaError
-    monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
-    monkeypatch.setenv("VCA_PLANE2", two)
+    _set_env(monkeypatch, REAL_FORMS[form])
     dt = np.float64 if prec == "fp64" else np.float32
     for name, (p, hloc, hk, bh, bv), sector in (
             ("bath5+5", M.bath_model(nlat=5, nb=5), (5, 5)),
@@ -440,3 +456,58 @@ def test_hxv_device_real(prec, two, monkeypatch):
     with pytest.raises(VcaError) as ei:
         sec.hxv_device_real(d.ptr, d.ptr)
     assert ei.value.code == -1
+
+
+@pytest.mark.parametrize("form", sorted(REAL_FORMS))
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_real_vector_chain_steps(prec, form, monkeypatch):
+    """A continued-fraction chain from a real start vector on a real
+    Hamiltonian runs on real vectors; its coefficients equal the oracle's
+    Lanczos recurrence (1e-10 FP64, 1e-5 MIXED) and those of the same chain
+    on complex vectors, on an even and an odd sector."""
+    _set_env(monkeypatch, REAL_FORMS[form])
+    for name, (p, hloc, hk, bh, bv), sector in (
+            ("bath5+5", M.bath_model(nlat=5, nb=5), (5, 5)),
+            ("chain11", M.chain(nlat=11, u=4.0), (3, 3))):
+        om = M.oracle_model(p, hloc, bh, bv)
+        s = _solver(p, hloc, bh, bv)
+        sec = s.sector(*sector, prec)
+        _say(f"real chain {name} {sector} {prec} {form}")
+        v0 = np.random.default_rng(11).normal(size=sec.dim) + 0j
+        v0 /= np.linalg.norm(v0)
+        a, b = sec.lanczos_tridiag(v0, 20)
+        ao, bo = R.lanczos_tridiag(lambda x: om.hxv(*sector, x), v0, 20)
+        tol = 1e-10 if prec == "fp64" else 1e-5
+        assert np.allclose(a, ao, rtol=tol, atol=tol), (name, np.abs(a - ao).max())
+        assert np.allclose(b, bo, rtol=tol, atol=tol), (name, np.abs(b - bo).max())
+        monkeypatch.setenv("VCA_REAL_CHAINS", "0")
+        ac, bc = sec.lanczos_tridiag(v0, 20)
+        monkeypatch.delenv("VCA_REAL_CHAINS")
+        assert np.allclose(a, ac, rtol=tol, atol=tol) and np.allclose(b, bc, rtol=tol, atol=tol)
+        sec.close()
+
+
+@pytest.mark.parametrize("xcd", ["0", "1"])
+def test_real_vector_chains_in_a_solve(xcd, monkeypatch):
+    """The excitation chains of a real Hamiltonian on real vectors (the
+    ground state's global phase removed), batched with the columns of each
+    chain spread over the XCDs or each chain on one XCD, against the same
+    solve on complex chains.  The ground state is the same complex vector in
+    both, so E0 agrees to rounding; G' and Omega differ by what dropping the
+    ground state's non-phase imaginary remainder (its Lanczos error) changes:
+    bars 2e-6 of G''s largest element and 1e-8 relative in Omega."""
+    monkeypatch.setenv("VCA_XCD_CHAINS", xcd)
+    p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
+    p.lanc_ngfiter = 80
+    out = {}
+    for real in ("1", "0"):
+        monkeypatch.setenv("VCA_REAL_CHAINS", real)
+        _say(f"10-site chain solve, real-vector chains={real}, xcd_chains={xcd}")
+        sc = VcaSolver(p)
+        om_ = sc.vca_solve(hloc, hk)
+        out[real] = (om_, sc.results()["omegaprime"], sc.vca_gf_cluster(0.3j),
+                     sc.vca_gf_cluster(1.0 + 0.5j))
+    assert abs(out["1"][1] - out["0"][1]) <= 1e-12 * abs(out["0"][1])
+    assert abs(out["1"][0] - out["0"][0]) <= 1e-8 * abs(out["0"][0])
+    for x, y in zip(out["1"][2:], out["0"][2:]):
+        assert np.abs(x - y).max() <= 2e-6 * np.abs(y).max(), np.abs(x - y).max()

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
@@ -812,6 +813,161 @@ __global__ __launch_bounds__(BS) void k_hxv_plane(const HxvArgs a) {
   if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
 }
 
+// Real-vector form of k_hxv_buf (real amplitudes, vectors of dim reals): the
+// same schedule on elements of sizeof(R) -- the column staged in LDS, the
+// up-spin hops gathered there through the one-plane hop table
+// (DevSpin::ent4p), the down-spin hops as buffer loads of whole neighbour
+// columns with the column offset in an SGPR.  Half the bytes and half the
+// LDS traffic of the complex form; the excitation chains of real
+// Hamiltonians run on it.  a.vstride counts reals between chains.
+template <typename R, int BS, int KR>
+__global__ __launch_bounds__(BS) void k_hxv_buf_real(const HxvArgs a) {
+  constexpr int ESZ = sizeof(R);
+  constexpr int U = KR >= 4 ? 4 : 8;
+  constexpr int UH = U / 2;
+  constexpr int PF = (BS == 1024 && KR == 1) ? 4 : 2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int maxnd = a.dw_maxnnz;
+  const BufLds lds = buf_lds(a.namp_up, a.dimup, maxnd, ESZ);
+  R *s_amp_up = reinterpret_cast<R *>(smem);
+  R *s_col = reinterpret_cast<R *>(smem + lds.col);
+  R *s_dw_amp = reinterpret_cast<R *>(smem + lds.dw_amp);
+  uint32_t *s_dw_soff = reinterpret_cast<uint32_t *>(smem + lds.dw_soff);
+
+  const int dimup = a.dimup;
+  const WgPos wp = wg_pos(a);
+  if (!wp.valid) return;
+  const int idw = wp.idw;
+  const int64_t chain_off = (int64_t)wp.chain * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + wp.chain : nullptr;
+  bool done = false;
+  R scale = (R)1;
+  const R *vin = static_cast<const R *>(a.v0) + chain_off;
+  if (lz) {
+    const int k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const R *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  R *hv = static_cast<R *>(a.hv) + chain_off + (int64_t)idw * dimup;
+  const R *vcol = vin + (int64_t)idw * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<R *>(vin), (short)0, (int)(uint32_t)(a.vstride * ESZ), 0x00020000);
+    const cplx<R> *g_amp_up = static_cast<const cplx<R> *>(a.amp_up);
+    const cplx<R> *g_amp_dw = static_cast<const cplx<R> *>(a.amp_dw);
+    const int nd = a.skip_dw ? 0 : a.cnt_dw[idw];
+    for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t].x;
+    if (threadIdx.x == 0) s_amp_up[a.namp_up] = (R)0;  // padding hops
+    for (int q = threadIdx.x; q < nd; q += BS) {
+      const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
+      s_dw_soff[q] = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
+      s_dw_amp[q] = g_amp_dw[e >> kColBits].x;
+    }
+    for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
+    __syncthreads();
+    const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
+    const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
+    for (int base0 = 0; base0 < dimup; base0 += BS * KR) {
+      if (base0 + (int)(threadIdx.x & ~63u) >= dimup) break;
+      const int base = base0 + (int)threadIdx.x;
+      int row[KR], nu[KR];
+      R ar[KR], x[KR];
+#pragma unroll
+      for (int j = 0; j < KR; ++j) {
+        row[j] = min(base + j * BS, dimup - 1);
+        const RowMeta m = a.meta_up[row[j]];
+        nu[j] = (base + j * BS < dimup) ? m.cnt : 0;
+        x[j] = s_col[row[j]];
+        const double ed = a.skip_diag ? 0.0 : m.e + edw + cross_energy(a.cross, m.s, sdw);
+        ar[j] = (R)ed * x[j];
+      }
+      // up-spin hops: LDS gathers, entries through a ring of PF chunks
+      int nc_w = 0;
+#pragma unroll
+      for (int j = 0; j < KR; ++j) nc_w = max(nc_w, (nu[j] + 3) >> 2);
+      for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
+      nc_w = __builtin_amdgcn_readfirstlane(nc_w);
+      const int last = a.up_nc - 1;
+      const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+      if (nc_w > 0) {
+        uint4 e4[PF][KR];
+#pragma unroll
+        for (int p = 0; p < PF; ++p)
+#pragma unroll
+          for (int j = 0; j < KR; ++j) e4[p][j] = ent4[(int64_t)min(p, last) * dimup + row[j]];
+        for (int c0 = 0; c0 < nc_w; c0 += PF) {
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            if (c0 + p < nc_w) {
+#pragma unroll
+              for (int j = 0; j < KR; ++j) {
+                const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
+                R y[4], h[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                  y[t] = lds_load_re<R>(ee[t] & kLdsAddrMask);
+                  h[t] = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) ar[j] = fma(h[t], y[t], ar[j]);
+              }
+            }
+            const int cn = min(c0 + p + PF, last);
+#pragma unroll
+            for (int j = 0; j < KR; ++j) e4[p][j] = ent4[(int64_t)cn * dimup + row[j]];
+          }
+        }
+      }
+      // down-spin hops: two stages of U/2 neighbour columns, one stage's
+      // buffer loads issued before the other is summed
+      R ya[UH][KR], yb[UH][KR];
+      auto load_stage = [&](R(&y)[UH][KR], int q0) {
+#pragma unroll
+        for (int u = 0; u < UH; ++u) {
+          const uint32_t so = __builtin_amdgcn_readfirstlane(s_dw_soff[min(q0 + u, nd - 1)]);
+#pragma unroll
+          for (int j = 0; j < KR; ++j) {
+            if constexpr (ESZ == 8)
+              y[u][j] = __builtin_bit_cast(
+                  R, __builtin_amdgcn_raw_buffer_load_b64(rv, row[j] * ESZ, so, 0));
+            else
+              y[u][j] = __builtin_bit_cast(
+                  R, __builtin_amdgcn_raw_buffer_load_b32(rv, row[j] * ESZ, so, 0));
+          }
+        }
+      };
+      auto sum_stage = [&](const R(&y)[UH][KR], int q0) {
+#pragma unroll
+        for (int u = 0; u < UH; ++u)
+          if (q0 + u < nd) {
+            const R h = s_dw_amp[q0 + u];
+#pragma unroll
+            for (int j = 0; j < KR; ++j) ar[j] = fma(h, y[u][j], ar[j]);
+          }
+      };
+      if (nd > 0) load_stage(ya, 0);
+      for (int q0 = 0; q0 < nd; q0 += 2 * UH) {
+        if (q0 + UH < nd) load_stage(yb, q0 + UH);
+        sum_stage(ya, q0);
+        if (q0 + 2 * UH < nd) load_stage(ya, q0 + 2 * UH);
+        if (q0 + UH < nd) sum_stage(yb, q0 + UH);
+      }
+#pragma unroll
+      for (int j = 0; j < KR; ++j) {
+        if (base + j * BS < dimup) {
+          const R out = ar[j] * scale;
+          hv[row[j]] = out;
+          if (lz) dot += (double)(x[j] * scale) * (double)out;
+        }
+      }
+    }
+  }
+  if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
+}
+
 // The real-vector gather pass with two columns per workgroup (columns 2p
 // and 2p + 1 of the pair p = workgroup), for real vectors whose two columns
 // fit the LDS together (FP32: 2 x 51.5 KB on the 12,870-row bath sector):
@@ -1223,11 +1379,16 @@ struct ExcArgs {
   int64_t vstride;
 };
 
-template <typename R>
+// RO (real-vector chains): psi holds real amplitudes (dim_i reals), every
+// coefficient is real, and out holds dim_j reals per chain with the pad real
+// of an odd dim_j set to 0.
+template <typename R, bool RO>
 __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
   using C = cplx<R>;
   const C *psi = static_cast<const C *>(a.psi);
+  const R *psi_r = static_cast<const R *>(a.psi);
   C *out = static_cast<C *>(a.out) + (int64_t)blockIdx.y * a.vstride;
+  R *out_r = static_cast<R *>(a.out) + (int64_t)blockIdx.y * a.vstride;
   const ExcOp op = a.ops[blockIdx.y];
   LanczosDev *lz = a.lz + blockIdx.y;
   const int64_t n = (int64_t)a.dimup_j * a.dimdw_j;
@@ -1252,19 +1413,71 @@ __global__ __launch_bounds__(kBlock) void k_excite(const ExcArgs a) {
       const uint32_t r = rank_string(s, a.binom);
       const int64_t i = a.spin == 1 ? (int64_t)r + (int64_t)jdw * a.dimup_i
                                     : (int64_t)jup + (int64_t)r * a.dimup_i;
-      const C p = psi[i];
-      // coef * sg * psi
-      vr += sg * (op.cre[o] * (double)p.x - op.cim[o] * (double)p.y);
-      vi += sg * (op.cre[o] * (double)p.y + op.cim[o] * (double)p.x);
+      if (RO) {
+        vr += sg * op.cre[o] * (double)psi_r[i];
+      } else {
+        const C p = psi[i];
+        // coef * sg * psi
+        vr += sg * (op.cre[o] * (double)p.x - op.cim[o] * (double)p.y);
+        vi += sg * (op.cre[o] * (double)p.y + op.cim[o] * (double)p.x);
+      }
     }
-    C c;
-    c.x = (R)vr;
-    c.y = (R)vi;
-    out[j] = c;
-    ss += (double)c.x * c.x + (double)c.y * c.y;
+    if (RO) {
+      const R c = (R)vr;
+      out_r[j] = c;
+      ss += (double)c * c;
+    } else {
+      C c;
+      c.x = (R)vr;
+      c.y = (R)vi;
+      out[j] = c;
+      ss += (double)c.x * c.x + (double)c.y * c.y;
+    }
   }
+  if (RO && (n & 1) && blockIdx.x == 0 && threadIdx.x == 0) out_r[n] = (R)0;
   (void)lz;
   block_partial(ss, a.partials + (int64_t)blockIdx.y * gridDim.x);
+}
+
+// Phase of a complex vector psi = e^{i phi} r (r real): sum_i psi_i^2 =
+// e^{2 i phi} |r|^2.  Two-stage: per-block partials of (Re, Im) of sum psi^2.
+__global__ __launch_bounds__(kBlock) void k_psi_sq(const double2 *psi, int64_t n,
+                                                   double *partials) {
+  double sr = 0, si = 0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double2 p = psi[i];
+    sr += p.x * p.x - p.y * p.y;
+    si += 2.0 * p.x * p.y;
+  }
+  const double br = block_sum(sr);
+  const double bi = block_sum(si);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = br;
+    partials[2 * blockIdx.x + 1] = bi;
+  }
+}
+
+// out_i = Re(psi_i e^{-i phi}) (cp = cos phi, sp = sin phi), in RD; per-block
+// partials of |Re|^2 and |Im|^2 of psi e^{-i phi}.
+template <typename RD>
+__global__ __launch_bounds__(kBlock) void k_psi_real(const double2 *psi, int64_t n, double cp,
+                                                     double sp, RD *out, double *partials) {
+  double sre = 0, sim = 0;
+  const int64_t stride = (int64_t)gridDim.x * kBlock;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const double2 p = psi[i];
+    const double re = p.x * cp + p.y * sp, im = p.y * cp - p.x * sp;
+    out[i] = (RD)re;
+    sre += re * re;
+    sim += im * im;
+  }
+  const double br = block_sum(sre);
+  const double bi = block_sum(sim);
+  if (threadIdx.x == 0) {
+    partials[2 * blockIdx.x] = br;
+    partials[2 * blockIdx.x + 1] = bi;
+  }
 }
 
 // ================================================== host launch wrappers
@@ -1403,6 +1616,13 @@ static size_t hxv_esz(const Sector &sec) {
   return sec.precision == VCA_FP64 ? sizeof(double2) : sizeof(float2);
 }
 
+// VCA_TFORM_MIN_DIM lowers the size threshold (tests run the form on
+// sectors small enough for the oracle)
+int64_t tform_min_dim() {
+  const char *e = std::getenv("VCA_TFORM_MIN_DIM");
+  return e && *e ? std::atoll(e) : kTformMinDim;
+}
+
 static bool tform_fits(const Sector &sec, bool plane) {
   const size_t esz = hxv_esz(sec), rsz = esz / 2;
   auto staged = [&](const DevSpin &sp) {
@@ -1411,12 +1631,7 @@ static bool tform_fits(const Sector &sec, bool plane) {
       return sp.plane_form && (size_t)buf_lds(2 * sp.nterms, sp.dim, 0, (int)rsz).total <= kLdsBig;
     return sp.lds_form && (size_t)buf_lds(2 * sp.nterms, sp.dim, 0, (int)esz).total <= kLdsBig;
   };
-  // VCA_TFORM_MIN_DIM lowers the size threshold (tests run the form on
-  // sectors small enough for the oracle)
-  int64_t min_dim = kTformMinDim;
-  const char *e = std::getenv("VCA_TFORM_MIN_DIM");
-  if (e && *e) min_dim = std::atoll(e);
-  return sec.dim >= min_dim && staged(sec.up) && staged(sec.dw) &&
+  return sec.dim >= tform_min_dim() && staged(sec.up) && staged(sec.dw) &&
          (plane || (uint64_t)sec.dim * esz < (1ull << 31));
 }
 
@@ -1477,13 +1692,28 @@ static int real_grid_dw(const Sector &sec) {
   return real_pass_two(sec, sec.dw) ? (sec.up.dim + 1) / 2 : sec.up.dim;
 }
 
+// Real-vector mode runs the transposed form (one real plane) where the
+// sector's tuned shape is transposed, else the direct form k_hxv_buf_real.
+static bool real_uses_tform(const Sector &sec) {
+  return hxv_shape(sec).tform && tform_fits(sec, true);
+}
+
+static bool direct_real_fits(const Sector &sec) {
+  const int rsz = sec.precision == VCA_FP64 ? (int)sizeof(double) : (int)sizeof(float);
+  return sec.up.plane_form &&
+         (size_t)buf_lds(2 * sec.up.nterms, sec.up.dim, sec.dw.maxnnz, rsz).total <= kLdsBig &&
+         2 * (uint64_t)sec.dim * rsz < (1ull << 31);
+}
+
 static int hxv_partials(const Sector &sec, const HxvShape &h) {
-  if (sec.real_vec) return real_grid_up(sec) + real_grid_dw(sec);
+  if (sec.real_vec)
+    return real_uses_tform(sec) ? real_grid_up(sec) + real_grid_dw(sec) : sec.dw.dim;
   return h.tform ? sec.dw.dim + sec.up.dim : h.grid;
 }
 
 bool real_mode_ok(const Sector &sec) {
-  return sec.up.dim >= 128 && sec.up.real_amp && sec.dw.real_amp && tform_fits(sec, true);
+  return sec.up.dim >= 128 && sec.up.real_amp && sec.dw.real_amp &&
+         (real_uses_tform(sec) || direct_real_fits(sec));
 }
 
 int64_t vec_elems(const Sector &sec) { return sec.real_vec ? (sec.dim + 1) / 2 : sec.dim; }
@@ -1491,7 +1721,7 @@ int64_t vec_elems(const Sector &sec) { return sec.real_vec ? (sec.dim + 1) / 2 :
 // Scratch vectors of the transposed form for nb chains; like the partials,
 // a larger request first drains the stream and drops the recorded graphs.
 static void ensure_tbuf(Sector &sec, int nb) {
-  if ((!hxv_shape(sec).tform && !sec.real_vec) || sec.tbuf_cap >= nb) return;
+  if (!hxv_shape(sec).tform || sec.tbuf_cap >= nb) return;
   if (sec.tbuf) {
     VCA_HIP(hipStreamSynchronize(sec.stream));
     clear_graphs(sec);
@@ -1728,6 +1958,51 @@ static void launch_real_pass(Sector &sec, const HxvArgs &a, bool two, int grid, 
   VCA_HIP(hipGetLastError());
 }
 
+template <typename R, int BS, int KR>
+static void launch_buf_real_k(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static std::once_flag attr_once;
+  auto fn = k_hxv_buf_real<R, BS, KR>;
+  std::call_once(attr_once, [&]() {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    hipFuncAttributes fa{};  // LDS addresses from 0: no static LDS
+    VCA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+    if (fa.sharedSizeBytes != 0)
+      fail(VCA_ERR_HIP, "k_hxv_buf_real has %zu bytes of static LDS", (size_t)fa.sharedSizeBytes);
+  });
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
+}
+
+// Real-vector mode, direct form: k_hxv_buf_real in the launch shape of the
+// sector's tuned buffer-load variant (512 lanes x 1 row otherwise), chains
+// spread or chain-affine as in the complex form.
+template <typename R>
+static void launch_buf_real(Sector &sec, HxvArgs a, const HxvShape &h, int nb) {
+  a.ent4_up = sec.up.ent4p;
+  a.ncb = sec.dw.dim;
+  a.nchains = nb;
+  const char *xc = std::getenv("VCA_XCD_CHAINS");
+  const bool xcd = xc && *xc ? (xc[0] == '1' && nb > 1) : nb >= kXcdChainsMin;
+  a.xcd_chains = xcd ? 1 : 0;
+  const dim3 g3 =
+      xcd ? dim3((unsigned)(sec.dw.dim * ((nb + 7) / 8 * 8)), 1) : dim3(sec.dw.dim, nb);
+  const size_t smem =
+      (size_t)buf_lds(a.namp_up, sec.up.dim, a.dw_maxnnz, (int)sizeof(R)).total;
+  const int bs = h.buf ? h.bs : 512, kr = h.buf ? h.kr : 1;
+  if (bs == 256 && kr == 4) launch_buf_real_k<R, 256, 4>(sec, a, g3, smem);
+  else if (bs == 256 && kr == 2) launch_buf_real_k<R, 256, 2>(sec, a, g3, smem);
+  else if (bs == 512 && kr == 2) launch_buf_real_k<R, 512, 2>(sec, a, g3, smem);
+  else if (bs == 1024 && kr == 1) launch_buf_real_k<R, 1024, 1>(sec, a, g3, smem);
+  else if (bs == 1024 && kr == 2) launch_buf_real_k<R, 1024, 2>(sec, a, g3, smem);
+  else launch_buf_real_k<R, 512, 1>(sec, a, g3, smem);
+  VCA_HIP(hipGetLastError());
+  if (a.lz) {
+    hipLaunchKernelGGL(k_lz_alpha, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                       (int)a.pstride);
+    VCA_HIP(hipGetLastError());
+  }
+}
+
 // Real-vector mode: the four launches of the transposed form on one real
 // plane (real transposes), 1024-lane gather passes.
 template <typename R>
@@ -1746,7 +2021,7 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   t.out = sec.tbuf;
   t.ni = sec.up.dim;
   t.nj = sec.dw.dim;
-  t.vstride = sec.dim;
+  t.vstride = a.vstride;
   t.lz = a.lz;
   launch_transpose<R>(sec, t, false, nb);
   HxvArgs b = a;
@@ -1768,7 +2043,7 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   t2.out = a.hv;
   t2.ni = sec.dw.dim;
   t2.nj = sec.up.dim;
-  t2.vstride = sec.dim;
+  t2.vstride = a.vstride;
   t2.lz = a.lz;
   launch_transpose<R>(sec, t2, true, nb);
   if (a.lz) {
@@ -1816,17 +2091,26 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   a.dw_maxnnz = sec.dw.maxnnz;
   const bool real_amp = sec.up.real_amp && sec.dw.real_amp;
   if (sec.real_vec) {
-    if (nb != 1 || !real_mode_ok(sec))
-      fail(VCA_ERR_STATE, "real-vector H x v needs one chain on a sector that fits it");
+    if (!real_mode_ok(sec))
+      fail(VCA_ERR_STATE, "real-vector H x v on a sector that does not fit it");
+    // chains sit 2*dim reals apart (the complex layout's stride), so the
+    // element-wise kernels see each chain as ceil(dim/2) complex pairs
+    a.vstride = 2 * sec.dim;
     if (lanczos) ensure_partials(sec, hxv_partials(sec, shp) * nb);
-    ensure_tbuf(sec, nb);
     a.partials = sec.partials;
     a.meta_up = sec.up.meta;
     a.up_nc = (sec.up.maxnnz + 3) / 4;
+    a.dw_maxnnz = sec.dw.maxnnz;
     a.pbase = 0;
     a.pstride = hxv_partials(sec, shp);
-    if (fp64) launch_real_t<double>(sec, a, nb);
-    else launch_real_t<float>(sec, a, nb);
+    if (real_uses_tform(sec)) {
+      ensure_tbuf(sec, nb);
+      if (fp64) launch_real_t<double>(sec, a, nb);
+      else launch_real_t<float>(sec, a, nb);
+    } else {
+      if (fp64) launch_buf_real<double>(sec, a, shp, nb);
+      else launch_buf_real<float>(sec, a, shp, nb);
+    }
     return;
   }
   if (shp.onecol_kernel) {
@@ -2014,14 +2298,60 @@ void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int 
   const int grid = grid_for(jsec.dim);
   ensure_partials(jsec, grid * nb);
   a.partials = jsec.partials;
-  if (jsec.precision == VCA_FP64)
-    hipLaunchKernelGGL(k_excite<double>, dim3(grid, nb), dim3(kBlock), 0, jsec.stream, a);
-  else
-    hipLaunchKernelGGL(k_excite<float>, dim3(grid, nb), dim3(kBlock), 0, jsec.stream, a);
+  const bool ro = jsec.real_vec;
+  if (ro) a.vstride = 2 * jsec.dim;  // reals between chains (real-vector mode)
+  if (jsec.precision == VCA_FP64) {
+    if (ro)
+      hipLaunchKernelGGL((k_excite<double, true>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
+                         a);
+    else
+      hipLaunchKernelGGL((k_excite<double, false>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
+                         a);
+  } else {
+    if (ro)
+      hipLaunchKernelGGL((k_excite<float, true>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
+                         a);
+    else
+      hipLaunchKernelGGL((k_excite<float, false>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
+                         a);
+  }
   VCA_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_lz_start, dim3(nb), dim3(kBlock), 0, jsec.stream, jsec.lz, jsec.partials,
                      grid);
   VCA_HIP(hipGetLastError());
+}
+
+double real_vector_of(const void *psi, int64_t n, void *out, int prec, hipStream_t st) {
+  const int grid = grid_for(n);
+  double *part = nullptr;
+  VCA_HIP(hipMallocAsync((void **)&part, sizeof(double) * 2 * grid, st));
+  std::vector<double> h(2 * (size_t)grid);
+  auto sums = [&](double &a, double &b) {
+    VCA_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * 2 * grid, hipMemcpyDeviceToHost, st));
+    VCA_HIP(hipStreamSynchronize(st));
+    a = b = 0;
+    for (int g = 0; g < grid; ++g) {
+      a += h[2 * g];
+      b += h[2 * g + 1];
+    }
+  };
+  const double2 *p = static_cast<const double2 *>(psi);
+  hipLaunchKernelGGL(k_psi_sq, dim3(grid), dim3(kBlock), 0, st, p, n, part);
+  VCA_HIP(hipGetLastError());
+  double qr, qi;
+  sums(qr, qi);
+  const double phi = 0.5 * std::atan2(qi, qr);
+  if (prec == VCA_FP64)
+    hipLaunchKernelGGL(k_psi_real<double>, dim3(grid), dim3(kBlock), 0, st, p, n, std::cos(phi),
+                       std::sin(phi), static_cast<double *>(out), part);
+  else
+    hipLaunchKernelGGL(k_psi_real<float>, dim3(grid), dim3(kBlock), 0, st, p, n, std::cos(phi),
+                       std::sin(phi), static_cast<float *>(out), part);
+  VCA_HIP(hipGetLastError());
+  double sre, sim;
+  sums(sre, sim);
+  VCA_HIP(hipFreeAsync(part, st));
+  return std::sqrt(sim / std::max(sre + sim, 1e-300));
 }
 
 }  // namespace vca

@@ -549,11 +549,12 @@ struct GsOut {
 };
 
 // Real-vector ground state (Sector::real_vec) for real Hamiltonians on the
-// sectors whose one-plane transposed H x v fits; VCA_REAL_GS=0 keeps the
-// complex vectors.
+// large sectors (those of the transposed H x v, where it halves seconds of
+// work); smaller ones keep the complex Lanczos (and its start vector).
+// VCA_REAL_GS=0 keeps complex vectors everywhere.
 static bool use_real_gs(const Sector &sec) {
   const char *e = std::getenv("VCA_REAL_GS");
-  return !(e && e[0] == '0') && real_mode_ok(sec);
+  return !(e && e[0] == '0') && sec.dim >= tform_min_dim() && real_mode_ok(sec);
 }
 
 // Plain Lanczos ground state on sec (its precision), start = device vector
@@ -635,9 +636,9 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
   VCA_HIP(hipMemsetAsync(sec.xvec, 0, sec.vec_bytes(), sec.stream));
   run_steps(sec, m, true, s);
   launch_scale_x(sec);
-  if (real && !keep_real) {  // dim reals -> dim complex numbers
-    launch_expand_real(sec, sec.xvec, sec.tbuf);
-    VCA_HIP(hipMemcpyAsync(sec.xvec, sec.tbuf, sec.vec_bytes(), hipMemcpyDeviceToDevice,
+  if (real && !keep_real) {  // dim reals -> dim complex numbers (via buf[2])
+    launch_expand_real(sec, sec.xvec, sec.buf[2]);
+    VCA_HIP(hipMemcpyAsync(sec.xvec, sec.buf[2], sec.vec_bytes(), hipMemcpyDeviceToDevice,
                            sec.stream));
   }
   VCA_HIP(hipStreamSynchronize(sec.stream));
@@ -1171,6 +1172,13 @@ static void parallel_for(int n, Fn &&fn) {
 // Chains of the same target sector run together: one launch per Lanczos
 // step covers a batch of chains (grid.y = chain), their vectors stacked
 // [chain][dim] in HBM, each chain with its own alpha/beta in device memory.
+// A kept state whose imaginary part, after its global phase is removed, is
+// below this fraction of its norm runs its chains on real vectors.  The
+// ground-state Lanczos stops at |dE| < tol, which fixes the vector to
+// ~sqrt(tol) (1e-6 at the default 1e-12); a complex combination of
+// degenerate states is O(1) away.
+constexpr double kRealStateTol = 1e-4;
+
 static void build_gf(Solver &s) {
   const vca_params &p = s.p;
   s.gf.clear();
@@ -1196,6 +1204,22 @@ static void build_gf(Solver &s) {
       psi = psi32;
     }
     VCA_HIP(hipStreamSynchronize(s.stream));
+    // Real Hamiltonian: the chains run on real vectors (k_hxv_buf_real or the
+    // one-plane transposed form), from the ground state with its global
+    // phase removed -- G' does not depend on it.  Kept complex when the
+    // state is not a real vector times a phase (a complex combination of
+    // degenerate states) or VCA_REAL_CHAINS=0.
+    void *psi_r = nullptr;
+    const char *rc_env = std::getenv("VCA_REAL_CHAINS");
+    if (!(rc_env && rc_env[0] == '0') && s.md.spin[0].real_amp && s.md.spin[1].real_amp) {
+      const size_t rsz = prec == VCA_FP64 ? sizeof(double) : sizeof(float);
+      VCA_HIP(hipMalloc(&psi_r, rsz * (size_t)dim_i));
+      const double imf = real_vector_of(st.d_vec, dim_i, psi_r, prec, s.stream);
+      if (!(imf < kRealStateTol)) {
+        VCA_HIP(hipFree(psi_r));
+        psi_r = nullptr;
+      }
+    }
     const std::vector<ChainSpec> specs = chain_list(s, st);
     std::vector<GfChan> res(specs.size());
     std::map<std::array<int, 2>, std::vector<int>> groups;
@@ -1216,7 +1240,18 @@ static void build_gf(Solver &s) {
       const size_t budget = (free_b + have) / 2;
       int nb_max = (int)std::max<size_t>(1, budget / per_chain);
       nb_max = std::min(nb_max, 512);
-      const std::vector<int> &idx = g.second;
+      // the chains whose operator has real coefficients run on real vectors
+      // (real Hamiltonian, a real state); the others (the c_i - i c_j
+      // channels) on complex ones, in batches of their own
+      std::vector<int> part[2];  // [0] complex, [1] real vectors
+      const bool real_ok = psi_r && real_mode_ok(js);
+      for (int c : g.second) {
+        bool real_op = real_ok;
+        for (int o = 0; o < specs[c].op.nops; ++o) real_op = real_op && specs[c].op.cim[o] == 0.0;
+        part[real_op ? 1 : 0].push_back(c);
+      }
+      for (int pr = 0; pr < 2; ++pr) {
+      const std::vector<int> &idx = part[pr];
       for (size_t b0 = 0; b0 < idx.size(); b0 += nb_max) {
         const int nb = (int)std::min<size_t>(nb_max, idx.size() - b0);
         ensure_lanczos(js, nlanc + 2, std::max(nb, js.nb_cap));
@@ -1225,7 +1260,16 @@ static void build_gf(Solver &s) {
         for (int c = 0; c < nb; ++c) ops[c] = specs[idx[b0 + c]].op;
         VCA_HIP(hipMemcpyAsync(js.d_ops, ops.data(), sizeof(ExcOp) * nb, hipMemcpyHostToDevice,
                                js.stream));
-        launch_excite(js, dimup_i, dimdw_i, psi, specs[idx[b0]].spin, js.d_ops, nb);
+        struct RealMode {
+          Sector &sec;
+          ~RealMode() { sec.real_vec = false; }
+        } real_mode{js};
+        js.real_vec = pr == 1;
+        if (js.real_vec && (js.dim & 1))  // the pad reals of Hv and u_{k-1} stay 0
+          for (int i = 1; i < 3; ++i)
+            VCA_HIP(hipMemsetAsync(js.buf[i], 0, js.vec_bytes() * nb, js.stream));
+        launch_excite(js, dimup_i, dimdw_i, js.real_vec ? psi_r : psi, specs[idx[b0]].spin,
+                      js.d_ops, nb);
         run_steps(js, nlanc, false, &s, nb);
         std::vector<double> al((size_t)nb * nlanc), be((size_t)nb * nlanc), nrm0(nb);
         for (int c = 0; c < nb; ++c) {
@@ -1269,6 +1313,7 @@ static void build_gf(Solver &s) {
         for (int c = 0; c < nb; ++c) s.max_exc = std::max(s.max_exc, cmax[c]);
         s.res.nchains += nb;
       }
+      }  // the complex and the real-vector part
     }
     for (size_t c = 0; c < specs.size(); ++c) {
       auto &el = s.gf[specs[c].key];
@@ -1276,6 +1321,7 @@ static void build_gf(Solver &s) {
       el[ist].push_back(std::move(res[c]));
     }
     if (psi32) VCA_HIP(hipFree(psi32));
+    if (psi_r) VCA_HIP(hipFree(psi_r));
   }
 }
 
@@ -2016,11 +2062,31 @@ int vca_lanczos_tridiag(vca_sector *sv, const double *v0, int32_t nlanc, double 
   Sector *sec = reinterpret_cast<Sector *>(sv);
   if (!sec || !v0 || !alpha || !beta || nlanc < 1) fail(VCA_ERR_ARG, "bad argument");
   ensure_lanczos(*sec, nlanc + 2);
+  // a real start vector on a real Hamiltonian: the chain on real vectors,
+  // as the excitation chains of such models run (VCA_REAL_CHAINS=0: complex)
+  bool real_v0 = true;
+  for (int64_t i = 0; i < sec->dim && real_v0; ++i) real_v0 = v0[2 * i + 1] == 0.0;
+  const char *rc_env = std::getenv("VCA_REAL_CHAINS");
+  struct RealMode {
+    Sector &sec;
+    ~RealMode() { sec.real_vec = false; }
+  } real_mode{*sec};
+  sec->real_vec = real_v0 && !(rc_env && rc_env[0] == '0') && real_mode_ok(*sec);
   void *d64 = nullptr;
   VCA_HIP(hipMalloc(&d64, sizeof(double2) * sec->dim));
-  VCA_HIP(hipMemcpyAsync(d64, v0, sizeof(double2) * sec->dim, hipMemcpyHostToDevice,
-                         sec->stream));
-  launch_convert(d64, VCA_FP64, sec->buf[0], sec->precision, sec->dim, sec->stream);
+  if (sec->real_vec) {
+    // dim reals, the pad of an odd dim 0, converted as complex pairs
+    std::vector<double> re(2 * (size_t)vec_elems(*sec), 0.0);
+    for (int64_t i = 0; i < sec->dim; ++i) re[i] = v0[2 * i];
+    VCA_HIP(hipMemcpy(d64, re.data(), sizeof(double) * re.size(), hipMemcpyHostToDevice));
+    if (sec->dim & 1)
+      for (int i = 1; i < 3; ++i)
+        VCA_HIP(hipMemsetAsync(sec->buf[i], 0, sec->vec_bytes(), sec->stream));
+  } else {
+    VCA_HIP(hipMemcpyAsync(d64, v0, sizeof(double2) * sec->dim, hipMemcpyHostToDevice,
+                           sec->stream));
+  }
+  launch_convert(d64, VCA_FP64, sec->buf[0], sec->precision, vec_elems(*sec), sec->stream);
   launch_norm0(*sec);
   double nrm0;
   tridiag_run(*sec, nlanc, alpha, beta, &nrm0, nullptr);

@@ -275,11 +275,19 @@ int hxv_scratch_vectors(const Sector &sec);
 // Real-vector mode (Sector::real_vec) can run on the sector: real
 // amplitudes and the one-plane transposed H x v fits.
 bool real_mode_ok(const Sector &sec);
+// sector size from which the transposed H x v and the real-vector ground
+// state are used (2^22 states; VCA_TFORM_MIN_DIM overrides)
+int64_t tform_min_dim();
 // complex elements the element-wise kernels cover (ceil(dim/2) in
 // real-vector mode)
 int64_t vec_elems(const Sector &sec);
 // dst[i] = src[i] + 0i for the dim reals of src (same precision)
 void launch_expand_real(Sector &sec, const void *src, void *dst);
+// out (n reals in precision prec) = Re(psi e^{-i phi}) for the complex128
+// device vector psi, phi = arg(sum psi_i^2) / 2 its global phase; returns
+// |Im(psi e^{-i phi})| / |psi| (rounding level when psi is a real vector
+// times a phase).
+double real_vector_of(const void *psi, int64_t n, void *out, int prec, hipStream_t st);
 void clear_graphs(Sector &sec);
 void launch_start_vector(Sector &sec, uint64_t seed);
 void launch_norm0(Sector &sec);  // nrm[0] = ||buf[0]||, k = 0, done = 0
