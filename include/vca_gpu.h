@@ -212,10 +212,10 @@ int vca_hxv_device_batch(vca_sector *sec, int32_t nb, const void *v, void *hv,
                          int32_t repeat, float *ms);
 /* Real form for real Hamiltonians (every hop amplitude real): v and hv are
  * dim reals on the device (double for FP64, float for MIXED sectors).  It
- * is the product the ground-state Lanczos runs on such sectors when they
- * are large enough for the transposed H x v form (about 2^22 states and
- * up, each spin's one-plane column staged in LDS); VCA_ERR_ARG on other
- * sectors.  No counterpart in the reference, whose vectors are complex
+ * is the product the excitation chains with real coefficients, and the
+ * ground state of large sectors, run on for such Hamiltonians.  Needs
+ * DimUp >= 128 and the staged column to fit the LDS (VCA_ERR_ARG
+ * otherwise).  No counterpart in the reference, whose vectors are complex
  * (cc_sparse_HxV, VCA_VARS_GLOBAL.f90:83-89); a real v gives the real part
  * of what vca_hxv_device gives for v + 0i. */
 int vca_hxv_device_real(vca_sector *sec, const void *v, void *hv, int32_t repeat,
