@@ -487,22 +487,29 @@ def test_real_vector_chain_steps(prec, form, monkeypatch):
         sec.close()
 
 
-@pytest.mark.parametrize("xcd", ["0", "1"])
-def test_real_vector_chains_in_a_solve(xcd, monkeypatch):
+@pytest.mark.parametrize("case", ["chain10_xcd0", "chain10_xcd1", "chain11_sector33"])
+def test_real_vector_chains_in_a_solve(case, monkeypatch):
     """The excitation chains of a real Hamiltonian on real vectors (the
     ground state's global phase removed), batched with the columns of each
     chain spread over the XCDs or each chain on one XCD, against the same
-    solve on complex chains.  The ground state is the same complex vector in
-    both, so E0 agrees to rounding; G' and Omega differ by what dropping the
-    ground state's non-phase imaginary remainder (its Lanczos error) changes:
-    bars 2e-6 of G''s largest element and 1e-8 relative in Omega."""
-    monkeypatch.setenv("VCA_XCD_CHAINS", xcd)
-    p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
+    solve on complex chains.  chain11_sector33: the 11-site chain restricted
+    to the sector (3,3), whose target sector (3,2) has an odd dimension
+    (165 x 55 = 9,075: the pad real of every chain in the batch).  The
+    ground state is the same complex vector in both, so E0 agrees to
+    rounding; G' and Omega differ by what dropping the ground state's
+    non-phase imaginary remainder (its Lanczos error) changes: bars 2e-6 of
+    G''s largest element and 1e-8 relative in Omega."""
+    if case == "chain11_sector33":
+        p, hloc, hk, _, _ = M.chain(nlat=11, u=4.0, nk=4)
+        p.sectors = [(3, 3)]
+    else:
+        monkeypatch.setenv("VCA_XCD_CHAINS", case[-1])
+        p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
     p.lanc_ngfiter = 80
     out = {}
     for real in ("1", "0"):
         monkeypatch.setenv("VCA_REAL_CHAINS", real)
-        _say(f"10-site chain solve, real-vector chains={real}, xcd_chains={xcd}")
+        _say(f"{case} solve, real-vector chains={real}")
         sc = VcaSolver(p)
         om_ = sc.vca_solve(hloc, hk)
         out[real] = (om_, sc.results()["omegaprime"], sc.vca_gf_cluster(0.3j),

@@ -171,8 +171,8 @@ class Sector:
 
     def hxv_device_real(self, v_ptr, hv_ptr, repeat=1):
         """H v on device buffers of dim reals (double for FP64, float for
-        MIXED), for real Hamiltonians on sectors large enough for the
-        transposed H x v form (VcaError otherwise)."""
+        MIXED), for real Hamiltonians on sectors with DimUp >= 128 (VcaError
+        otherwise)."""
         ms = ctypes.c_float()
         check(self._lib.vca_hxv_device_real(self._h, ctypes.c_void_p(v_ptr),
                                             ctypes.c_void_p(hv_ptr), repeat, ctypes.byref(ms)))
