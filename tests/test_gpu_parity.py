@@ -98,9 +98,10 @@ def test_hxv_square_3x4_full_sector():
     assert err < 1e-12, err
 
 
-N_HXV_VARIANTS = 21  # launch shapes of the one-column kernels (kernels.hip kVariants)
-N_HXV_DIRECT = 15    # the first 15: direct form; the rest: transposed form (the
-                     # last 2 with one real plane at a time: real amplitudes only)
+N_HXV_VARIANTS = 24  # launch shapes of the one-column kernels (kernels.hip kVariants)
+N_HXV_DIRECT = 15    # the first 15: direct form; then the transposed form (15-18
+                     # whole columns, 19-20 one real plane at a time: real
+                     # amplitudes only) and the overlapped short-column form (21-23)
 
 
 @pytest.mark.parametrize("variant", range(N_HXV_VARIANTS))

@@ -813,6 +813,175 @@ __global__ __launch_bounds__(BS) void k_hxv_plane(const HxvArgs a) {
   if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
 }
 
+// Overlapped form of k_hxv_buf for short columns (DimUp <= NR * BS, each
+// lane owning rows tid and tid + BS): the down-spin hops need nothing from
+// LDS -- their hop list is read with uniform scalar loads, one neighbour
+// column offset per hop in an SGPR -- so they run before the LDS barrier,
+// while the lane's share of the input column (NR elements, held in
+// registers) and the amplitude table are still in flight to LDS.  The
+// up-spin gathers and the diagonal follow the barrier.  On a latency-bound
+// launch (one round of workgroups) the column staging no longer adds to
+// the critical path.
+template <typename R, bool REAL_AMP, int BS, int NR>
+__global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
+  using C = cplx<R>;
+  constexpr int ESZ = sizeof(C);
+  constexpr int U = NR == 1 ? 8 : 4;
+  // entry ring depth: one chunk for two complex128 rows, so that the
+  // gathers fit 64 VGPRs (8 wavefronts per SIMD: the 3x4 sector's 924
+  // workgroups of 512 lanes in one round)
+  constexpr int PF = (NR == 2 && ESZ == 16) ? 1 : 2;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BufLds lds = buf_lds(a.namp_up, a.dimup, 0, ESZ);
+  C *s_amp_up = reinterpret_cast<C *>(smem);
+  C *s_col = reinterpret_cast<C *>(smem + lds.col);
+
+  const int dimup = a.dimup;
+  const WgPos wp = wg_pos(a);
+  if (!wp.valid) return;
+  const int idw = wp.idw;
+  const int64_t chain_off = (int64_t)wp.chain * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + wp.chain : nullptr;
+  bool done = false;
+  R scale = (R)1;
+  const C *vin = static_cast<const C *>(a.v0) + chain_off;
+  if (lz) {
+    const int k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const C *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  C *hv = static_cast<C *>(a.hv) + chain_off + (int64_t)idw * dimup;
+  const C *vcol = vin + (int64_t)idw * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<C *>(vin), (short)0, (int)(uint32_t)(a.vstride * ESZ), 0x00020000);
+    const C *g_amp_up = static_cast<const C *>(a.amp_up);
+    const C *g_amp_dw = static_cast<const C *>(a.amp_dw);
+    // the lane's rows, its share of the column and its row records, all
+    // requested before the down-spin loads
+    int row[NR], nu[NR];
+    bool in[NR];
+    C x[NR];
+    RowMeta m[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const int r = (int)threadIdx.x + j * BS;
+      in[j] = r < dimup;
+      row[j] = min(r, dimup - 1);
+      x[j] = vcol[row[j]];
+      m[j] = a.meta_up[row[j]];
+    }
+    C amp = C{(R)0, (R)0};
+    if ((int)threadIdx.x < a.namp_up) amp = g_amp_up[threadIdx.x];
+    R ar[NR], ai[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) ar[j] = ai[j] = (R)0;
+    // down-spin hops: whole neighbour columns, offsets and amplitudes from
+    // uniform loads of the column's hop list
+    const int nd = a.skip_dw ? 0 : a.cnt_dw[idw];
+    const uint32_t *entd = a.ent_dw + idw;
+    for (int q0 = 0; q0 < nd; q0 += U) {
+      C y[U][NR], h[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t e = entd[(int64_t)min(q0 + u, nd - 1) * a.dimdw];
+        const uint32_t so = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
+        h[u] = g_amp_dw[e >> kColBits];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+          if constexpr (ESZ == 16)
+            y[u][j] = __builtin_bit_cast(
+                C, __builtin_amdgcn_raw_buffer_load_b128(rv, row[j] * ESZ, so, 0));
+          else
+            y[u][j] = __builtin_bit_cast(
+                C, __builtin_amdgcn_raw_buffer_load_b64(rv, row[j] * ESZ, so, 0));
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (q0 + u < nd) {
+#pragma unroll
+          for (int j = 0; j < NR; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h[u], y[u][j]);
+        }
+    }
+    // diagonal, then the column and the amplitudes into LDS (the row
+    // records and the column elements die here: fewer VGPRs in the gathers)
+    const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
+    const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      nu[j] = in[j] ? m[j].cnt : 0;
+      const double ed = a.skip_diag ? 0.0 : m[j].e + edw + cross_energy(a.cross, m[j].s, sdw);
+      ar[j] = fma((R)ed, x[j].x, ar[j]);
+      ai[j] = fma((R)ed, x[j].y, ai[j]);
+      if (in[j]) s_col[row[j]] = x[j];
+    }
+    if ((int)threadIdx.x < a.namp_up) s_amp_up[threadIdx.x] = amp;
+    for (int t = (int)threadIdx.x + BS; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
+    if (threadIdx.x == 0) s_amp_up[a.namp_up] = C{(R)0, (R)0};  // padding hops
+    __syncthreads();
+    // up-spin hops: LDS gathers, entries through a ring of PF chunks
+    int nc_w = 0;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) nc_w = max(nc_w, (nu[j] + 3) >> 2);
+    for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
+    nc_w = __builtin_amdgcn_readfirstlane(nc_w);
+    const int last = a.up_nc - 1;
+    const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+    if (nc_w > 0) {
+      uint4 e4[PF][NR];
+#pragma unroll
+      for (int p = 0; p < PF; ++p)
+#pragma unroll
+        for (int j = 0; j < NR; ++j) e4[p][j] = ent4[(int64_t)min(p, last) * dimup + row[j]];
+      for (int c0 = 0; c0 < nc_w; c0 += PF) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+          if (c0 + p < nc_w) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+              const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
+              C y[4], h[4];
+#pragma unroll
+              for (int t = 0; t < 4; ++t) {
+                y[t] = lds_load<R>(ee[t] & kLdsAddrMask);
+                if constexpr (REAL_AMP) {
+                  h[t].x = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+                  h[t].y = (R)0;
+                } else {
+                  h[t] = lds_load<R>(ee[t] >> kLdsAddrBits);
+                }
+              }
+#pragma unroll
+              for (int t = 0; t < 4; ++t) cmac<R, REAL_AMP>(ar[j], ai[j], h[t], y[t]);
+            }
+          }
+          const int cn = min(c0 + p + PF, last);
+#pragma unroll
+          for (int j = 0; j < NR; ++j) e4[p][j] = ent4[(int64_t)cn * dimup + row[j]];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      if (in[j]) {
+        C out;
+        out.x = ar[j] * scale;
+        out.y = ai[j] * scale;
+        hv[row[j]] = out;
+        if (lz) {
+          const C xj = s_col[row[j]];
+          dot += (double)(xj.x * scale) * (double)out.x + (double)(xj.y * scale) * (double)out.y;
+        }
+      }
+    }
+  }
+  if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
+}
+
 // Real-vector form of k_hxv_buf (real amplitudes, vectors of dim reals): the
 // same schedule on elements of sizeof(R) -- the column staged in LDS, the
 // up-spin hops gathered there through the one-plane hop table
@@ -1588,19 +1757,21 @@ struct HxvShape {
   bool buf;    // buffer-load form (k_hxv_buf) when the column can be staged
   bool tform;  // transposed form (gather passes on v and v^T, k_transpose)
   bool plane;  // its gather passes one real plane at a time (k_hxv_plane)
+  bool ov;     // overlapped short-column form (k_hxv_ov)
   int bs, kr, u, cpb, grid;
 };
 
 // Variants of the one-column kernel the autotuner times: (BS, KR, U, form),
 // form 0 = k_hxv_col, 1 = k_hxv_buf (U fixed by KR there), 2 = transposed
-// form with k_hxv_buf passes, 3 = transposed form with k_hxv_plane passes.
-// Forms 2 and 3 are timed only on sectors where tform_fits holds for them.
+// form with k_hxv_buf passes, 3 = transposed form with k_hxv_plane passes,
+// 4 = k_hxv_ov (KR = rows per lane, NR; columns of at most NR * BS rows).
+// Forms 2 to 4 are timed only on sectors they fit (hxv_variant_ok).
 static const int kVariants[][4] = {
     {256, 4, 4, 0}, {256, 2, 8, 0},  {1024, 1, 8, 0}, {1024, 2, 8, 0}, {1024, 4, 4, 0},
     {512, 2, 8, 0}, {256, 1, 16, 0}, {512, 1, 16, 0}, {1024, 1, 16, 0}, {256, 4, 4, 1},
     {256, 2, 8, 1}, {512, 2, 8, 1},  {512, 1, 8, 1},  {1024, 1, 8, 1}, {1024, 2, 8, 1},
     {512, 1, 8, 2}, {1024, 1, 8, 2}, {512, 2, 8, 2},  {1024, 2, 8, 2}, {512, 1, 8, 3},
-    {1024, 1, 8, 3}};
+    {1024, 1, 8, 3}, {512, 2, 4, 4}, {1024, 1, 8, 4}, {256, 2, 4, 4}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 // The transposed form pays 9 vector lengths of HBM traffic per product
@@ -1639,6 +1810,12 @@ bool hxv_variant_ok(const Sector &sec, int v) {
   if (v < 0 || v >= kNumVariants) return false;
   if (sec.up.dim < 128) return v == 0;  // one shape (k_hxv)
   const int form = kVariants[v][3];
+  if (form == 4) {
+    const size_t esz = hxv_esz(sec);
+    return sec.up.lds_form && sec.up.dim <= kVariants[v][0] * kVariants[v][1] &&
+           (size_t)buf_lds(2 * sec.up.nterms, sec.up.dim, 0, (int)esz).total <= kLdsBig &&
+           (uint64_t)sec.dim * esz < (1ull << 31);
+  }
   return form < 2 || tform_fits(sec, form == 3);
 }
 
@@ -1652,8 +1829,9 @@ static HxvShape hxv_shape(const Sector &sec) {
     h.kr = kVariants[v][1];
     h.u = kVariants[v][2];
     h.buf = kVariants[v][3] != 0;
-    h.tform = kVariants[v][3] >= 2;
+    h.tform = kVariants[v][3] == 2 || kVariants[v][3] == 3;
     h.plane = kVariants[v][3] == 3;
+    h.ov = kVariants[v][3] == 4;
     h.cpb = 1;
     h.grid = sec.dw.dim;
   } else {
@@ -1798,6 +1976,30 @@ static void launch_buf_s(Sector &sec, const HxvArgs &a, const HxvShape &h, dim3 
   else if (bs == 512 && kr == 1) launch_buf<R, REAL_AMP, 512, 1>(sec, a, grid, smem);
   else if (bs == 1024 && kr == 1) launch_buf<R, REAL_AMP, 1024, 1>(sec, a, grid, smem);
   else launch_buf<R, REAL_AMP, 1024, 2>(sec, a, grid, smem);
+}
+
+template <typename R, bool REAL_AMP, int BS, int NR>
+static void launch_ov(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static std::once_flag attr_once;
+  auto fn = k_hxv_ov<R, REAL_AMP, BS, NR>;
+  std::call_once(attr_once, [&]() {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    hipFuncAttributes fa{};  // LDS addresses from 0: no static LDS
+    VCA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+    if (fa.sharedSizeBytes != 0)
+      fail(VCA_ERR_HIP, "k_hxv_ov has %zu bytes of static LDS", (size_t)fa.sharedSizeBytes);
+  });
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
+}
+
+template <typename R, bool REAL_AMP>
+static void launch_ov_s(Sector &sec, const HxvArgs &a, const HxvShape &h, dim3 grid) {
+  const size_t smem =
+      (size_t)buf_lds(a.namp_up, a.dimup, 0, (int)sizeof(cplx<R>)).total;
+  if (h.bs == 1024) launch_ov<R, REAL_AMP, 1024, 1>(sec, a, grid, smem);
+  else if (h.bs == 256) launch_ov<R, REAL_AMP, 256, 2>(sec, a, grid, smem);
+  else launch_ov<R, REAL_AMP, 512, 2>(sec, a, grid, smem);
 }
 
 template <typename R, bool REAL_AMP>
@@ -2143,7 +2345,15 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
     a.ncb = grid;
     a.nchains = nb;
     const dim3 g3 = xcd ? dim3((unsigned)(grid * ((nb + 7) / 8 * 8)), 1) : dim3(grid, nb);
-    if (fp64) {
+    if (shp.ov) {
+      if (fp64) {
+        if (real_amp) launch_ov_s<double, true>(sec, a, shp, g3);
+        else launch_ov_s<double, false>(sec, a, shp, g3);
+      } else {
+        if (real_amp) launch_ov_s<float, true>(sec, a, shp, g3);
+        else launch_ov_s<float, false>(sec, a, shp, g3);
+      }
+    } else if (fp64) {
       if (real_amp) launch_col_t<double, true>(sec, a, stage, buf, shp, g3, smem);
       else launch_col_t<double, false>(sec, a, stage, buf, shp, g3, smem);
     } else {
