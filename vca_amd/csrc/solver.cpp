@@ -590,7 +590,10 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
   };
   init();
   std::vector<double> al(niter + 4, 0.0), be(niter + 4, 0.0);
-  const int chunk = 16;
+  // steps between convergence checks: every step past convergence in the
+  // last chunk is wasted, so large sectors (ms per step, a check costs
+  // ~0.1 ms) check every 4 steps; small ones every 16 (launch-bound steps)
+  const int chunk = sec.dim >= (int64_t)1 << 22 ? 4 : 16;
   int k0 = 0, m = 0;
   double e_old = std::numeric_limits<double>::infinity(), e_m = 0;
   double best_de = std::numeric_limits<double>::infinity();
