@@ -569,11 +569,7 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
                         Solver *s, int stall = 0, bool real = false, bool keep_real = false) {
   niter = (int)std::min<int64_t>(niter, sec.dim);
   ensure_lanczos(sec, niter + 2);
-  struct RealMode {
-    Sector &sec;
-    ~RealMode() { sec.real_vec = false; }
-  } real_mode{sec};
-  sec.real_vec = real;
+  RealVecScope real_mode(sec, real);
   const size_t rsz = sec.precision == VCA_FP64 ? sizeof(double) : sizeof(float);
   auto init = [&]() {
     if (start)
@@ -1263,11 +1259,7 @@ static void build_gf(Solver &s) {
         for (int c = 0; c < nb; ++c) ops[c] = specs[idx[b0 + c]].op;
         VCA_HIP(hipMemcpyAsync(js.d_ops, ops.data(), sizeof(ExcOp) * nb, hipMemcpyHostToDevice,
                                js.stream));
-        struct RealMode {
-          Sector &sec;
-          ~RealMode() { sec.real_vec = false; }
-        } real_mode{js};
-        js.real_vec = pr == 1;
+        RealVecScope real_mode(js, pr == 1);
         if (js.real_vec && (js.dim & 1))  // the pad reals of Hv and u_{k-1} stay 0
           for (int i = 1; i < 3; ++i)
             VCA_HIP(hipMemsetAsync(js.buf[i], 0, js.vec_bytes() * nb, js.stream));
@@ -1960,11 +1952,7 @@ int vca_hxv_device_real(vca_sector *sv, const void *v, void *hv, int32_t repeat,
          "enough for the transposed form",
          sec->nup, sec->ndw);
   if (repeat < 1) repeat = 1;
-  struct RealMode {
-    Sector &sec;
-    ~RealMode() { sec.real_vec = false; }
-  } real_mode{*sec};
-  sec->real_vec = true;
+  RealVecScope real_mode(*sec, true);
   hipEvent_t e0, e1;
   VCA_HIP(hipEventCreate(&e0));
   VCA_HIP(hipEventCreate(&e1));
@@ -2070,11 +2058,7 @@ int vca_lanczos_tridiag(vca_sector *sv, const double *v0, int32_t nlanc, double 
   bool real_v0 = true;
   for (int64_t i = 0; i < sec->dim && real_v0; ++i) real_v0 = v0[2 * i + 1] == 0.0;
   const char *rc_env = std::getenv("VCA_REAL_CHAINS");
-  struct RealMode {
-    Sector &sec;
-    ~RealMode() { sec.real_vec = false; }
-  } real_mode{*sec};
-  sec->real_vec = real_v0 && !(rc_env && rc_env[0] == '0') && real_mode_ok(*sec);
+  RealVecScope real_mode(*sec, real_v0 && !(rc_env && rc_env[0] == '0') && real_mode_ok(*sec));
   void *d64 = nullptr;
   VCA_HIP(hipMalloc(&d64, sizeof(double2) * sec->dim));
   if (sec->real_vec) {

@@ -259,6 +259,16 @@ struct Sector {
   ~Sector();
 };
 
+// Real-vector mode of a sector for the lifetime of the scope (Sector::
+// real_vec), switched off again on every exit, errors included.
+struct RealVecScope {
+  Sector &sec;
+  RealVecScope(Sector &s, bool on) : sec(s) { sec.real_vec = on; }
+  ~RealVecScope() { sec.real_vec = false; }
+  RealVecScope(const RealVecScope &) = delete;
+  RealVecScope &operator=(const RealVecScope &) = delete;
+};
+
 // Kernel launch wrappers (kernels.hip)
 void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns,
                        int npart, const uint32_t *d_binom, double e_add);
