@@ -1329,6 +1329,76 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
   }
 }
 
+// Lanczos step of the real-vector transposed form: the transpose-add of the
+// down-spin result fused with the vector update, so Hv never goes through
+// HBM.  Per element n = idw*DimUp + iup of chain blockIdx.z:
+//   h = Hv_up[n] + Wt[iup*DimDw + idw]   (Wt: the down-spin pass, in v^T)
+//   u_{k+1}[n] = h - (alpha_k/nrm_k) u_k[n] - (beta_k/nrm_{k-1}) u_{k-1}[n]
+//   x[n] += (y_k/nrm_k) u_k[n]            (eigenvector pass, x non-null)
+// with the arithmetic of k_lanczos_update; a 64 x 64 tile of Wt is turned
+// through LDS as in k_transpose.  Partials of |u_{k+1}|^2 per tile, the pad
+// real of an odd dim untouched (it stays 0).
+struct TUpdArgs {
+  void *u0, *u1;
+  const void *hv, *wt;
+  void *x;
+  int dimup, dimdw;
+  int64_t vstride;  // reals between chains
+  LanczosDev *lz;
+  double *partials;
+};
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
+  __shared__ R s[kTile][kTile + 1];
+  const int64_t off = (int64_t)blockIdx.z * a.vstride;
+  LanczosDev *lz = a.lz + blockIdx.z;
+  const int k = lz->k;
+  const bool done = lz->done != 0;
+  double ss = 0.0;
+  if (!done) {
+    const R *wt = static_cast<const R *>(a.wt) + off;
+    const int ni = a.dimdw, nj = a.dimup;  // element (i = idw, j = iup)
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    constexpr int kWaves = kBlock / 64;
+    const int i0 = blockIdx.x * kTile, j0 = blockIdx.y * kTile;
+    const int i = i0 + lane;
+#pragma unroll
+    for (int t = 0; t < kTile / kWaves; ++t) {
+      const int jj = w + kWaves * t;
+      const int j = j0 + jj;
+      if (i < ni && j < nj) s[jj][lane] = wt[(int64_t)j * ni + i];
+    }
+    __syncthreads();
+    const R *hv = static_cast<const R *>(a.hv) + off;
+    const R *uk = static_cast<const R *>((k & 1) ? a.u1 : a.u0) + off;
+    R *up = static_cast<R *>((k & 1) ? a.u0 : a.u1) + off;
+    R *x = static_cast<R *>(a.x);
+    const R ca = (R)(lz->alpha[k] / lz->nrm[k]);
+    const R cb = k > 0 ? (R)(lz->beta[k] / lz->nrm[k - 1]) : (R)0;
+    const R cy = x ? (R)(lz->y[k] / lz->nrm[k]) : (R)0;
+    const int j = j0 + lane;
+#pragma unroll
+    for (int t = 0; t < kTile / kWaves; ++t) {
+      const int ii = w + kWaves * t;
+      const int ir = i0 + ii;
+      if (ir < ni && j < nj) {
+        const int64_t n = (int64_t)ir * nj + j;
+        const R h = hv[n] + s[lane][ii];
+        const R u = uk[n];
+        const R wv = k > 0 ? h - ca * u - cb * up[n] : h - ca * u;
+        up[n] = wv;
+        ss += (double)wv * (double)wv;
+        if (x) x[n] += cy * u;
+      }
+    }
+  }
+  const double b = block_sum(ss);
+  if (threadIdx.x == 0)
+    a.partials[(int64_t)blockIdx.z * gridDim.x * gridDim.y + blockIdx.y * gridDim.x +
+               blockIdx.x] = b;
+}
+
 // ------------------------------------------------------ Lanczos update --
 struct UpdArgs {
   void *u0, *u1;  // u_k in buffer k&1, u_{k-1} (and u_{k+1}) in the other
@@ -1911,12 +1981,19 @@ static void ensure_tbuf(Sector &sec, int nb) {
   sec.tbuf_cap = nb;
 }
 
+// Tiles of the fused transpose-add + update (k_tadd_update) per chain.
+static int tadd_tiles(const Sector &sec) {
+  return ((sec.dw.dim + kTile - 1) / kTile) * ((sec.up.dim + kTile - 1) / kTile);
+}
+
 // Allocates the partial-sum buffer for the largest grid of a Lanczos step
 // on nb chains (and the transposed form's scratch), so that launches
 // recorded into a graph never allocate.
 void ensure_step_partials(Sector &sec, int nb) {
   const HxvShape h = hxv_shape(sec);
-  ensure_partials(sec, std::max(hxv_partials(sec, h), grid_for(sec.dim)) * nb);
+  int n = std::max(hxv_partials(sec, h), grid_for(sec.dim));
+  if (sec.real_vec && real_uses_tform(sec)) n = std::max(n, tadd_tiles(sec));
+  ensure_partials(sec, n * nb);
   ensure_tbuf(sec, nb);
 }
 
@@ -2240,14 +2317,18 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   b.ncb = sec.up.dim;
   b.pbase = real_grid_up(sec);
   launch_real_pass<R>(sec, b, real_pass_two(sec, sec.dw), real_grid_dw(sec), nb);
-  TrArgs t2{};
-  t2.in0 = t2.in1 = sec.tbuf;
-  t2.out = a.hv;
-  t2.ni = sec.dw.dim;
-  t2.nj = sec.up.dim;
-  t2.vstride = a.vstride;
-  t2.lz = a.lz;
-  launch_transpose<R>(sec, t2, true, nb);
+  // Hv += (.)^T: in Lanczos mode fused into the vector update
+  // (k_tadd_update, launch_lanczos_update), so Hv_up is all H x v writes
+  if (!a.lz) {
+    TrArgs t2{};
+    t2.in0 = t2.in1 = sec.tbuf;
+    t2.out = a.hv;
+    t2.ni = sec.dw.dim;
+    t2.nj = sec.up.dim;
+    t2.vstride = a.vstride;
+    t2.lz = a.lz;
+    launch_transpose<R>(sec, t2, true, nb);
+  }
   if (a.lz) {
     hipLaunchKernelGGL(k_lz_alpha, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
                        (int)a.pstride);
@@ -2392,6 +2473,32 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
 }
 
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
+  if (sec.real_vec && real_uses_tform(sec)) {
+    TUpdArgs t{};
+    t.u0 = sec.buf[0];
+    t.u1 = sec.buf[1];
+    t.hv = sec.buf[2];
+    t.wt = sec.tbuf;
+    t.x = accumulate_x ? sec.xvec : nullptr;
+    t.dimup = sec.up.dim;
+    t.dimdw = sec.dw.dim;
+    t.vstride = 2 * sec.dim;
+    t.lz = sec.lz;
+    const int nblk = tadd_tiles(sec);
+    ensure_partials(sec, nblk * nb);
+    t.partials = sec.partials;
+    const dim3 g((unsigned)((sec.dw.dim + kTile - 1) / kTile),
+                 (unsigned)((sec.up.dim + kTile - 1) / kTile), (unsigned)nb);
+    if (sec.precision == VCA_FP64)
+      hipLaunchKernelGGL(k_tadd_update<double>, g, dim3(kBlock), 0, sec.stream, t);
+    else
+      hipLaunchKernelGGL(k_tadd_update<float>, g, dim3(kBlock), 0, sec.stream, t);
+    VCA_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_lz_beta, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
+                       nblk);
+    VCA_HIP(hipGetLastError());
+    return;
+  }
   UpdArgs a{};
   a.vstride = sec.dim;
   a.u0 = sec.buf[0];
