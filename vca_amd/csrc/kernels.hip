@@ -1137,6 +1137,146 @@ __global__ __launch_bounds__(BS) void k_hxv_buf_real(const HxvArgs a) {
   if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
 }
 
+// k_hxv_buf_real with two adjacent rows per lane (rows 2t and 2t + 1 of each
+// block of 2*BS rows): each down-spin hop is one 2-element load per lane
+// (16 B FP64, 8 B FP32) instead of two, and the results are stored in
+// pairs.  Needs an even DimUp (every column then starts on a pair
+// boundary).
+template <typename R, int BS>
+__global__ __launch_bounds__(BS) void k_hxv_buf_real2(const HxvArgs a) {
+  constexpr int ESZ = sizeof(R);
+  constexpr int U = 4;
+  constexpr int PF = 2;
+  using R2 = typename Vec2<R>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int maxnd = a.dw_maxnnz;
+  const BufLds lds = buf_lds(a.namp_up, a.dimup, maxnd, ESZ);
+  R *s_amp_up = reinterpret_cast<R *>(smem);
+  R *s_col = reinterpret_cast<R *>(smem + lds.col);
+  R *s_dw_amp = reinterpret_cast<R *>(smem + lds.dw_amp);
+  uint32_t *s_dw_soff = reinterpret_cast<uint32_t *>(smem + lds.dw_soff);
+
+  const int dimup = a.dimup;
+  const WgPos wp = wg_pos(a);
+  if (!wp.valid) return;
+  const int idw = wp.idw;
+  const int64_t chain_off = (int64_t)wp.chain * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + wp.chain : nullptr;
+  bool done = false;
+  R scale = (R)1;
+  const R *vin = static_cast<const R *>(a.v0) + chain_off;
+  if (lz) {
+    const int k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const R *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  R *hv = static_cast<R *>(a.hv) + chain_off + (int64_t)idw * dimup;
+  const R *vcol = vin + (int64_t)idw * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<R *>(vin), (short)0, (int)(uint32_t)(a.vstride * ESZ), 0x00020000);
+    const cplx<R> *g_amp_up = static_cast<const cplx<R> *>(a.amp_up);
+    const cplx<R> *g_amp_dw = static_cast<const cplx<R> *>(a.amp_dw);
+    const int nd = a.skip_dw ? 0 : a.cnt_dw[idw];
+    for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t].x;
+    if (threadIdx.x == 0) s_amp_up[a.namp_up] = (R)0;  // padding hops
+    for (int q = threadIdx.x; q < nd; q += BS) {
+      const uint32_t e = a.ent_dw[(int64_t)q * a.dimdw + idw];
+      s_dw_soff[q] = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
+      s_dw_amp[q] = g_amp_dw[e >> kColBits].x;
+    }
+    for (int f = threadIdx.x; f < dimup / 2; f += BS)
+      reinterpret_cast<R2 *>(s_col)[f] = reinterpret_cast<const R2 *>(vcol)[f];
+    __syncthreads();
+    const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
+    const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
+    for (int base0 = 0; base0 < dimup; base0 += 2 * BS) {
+      if (base0 + 2 * (int)(threadIdx.x & ~63u) >= dimup) break;
+      const int pr = base0 / 2 + (int)threadIdx.x;  // pair index
+      const bool in = 2 * pr < dimup;
+      const int r0 = min(2 * pr, dimup - 2);
+      int row[2], nu[2];
+      R ar[2], x[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        row[j] = r0 + j;
+        const RowMeta m = a.meta_up[row[j]];
+        nu[j] = in ? m.cnt : 0;
+        x[j] = s_col[row[j]];
+        const double ed = a.skip_diag ? 0.0 : m.e + edw + cross_energy(a.cross, m.s, sdw);
+        ar[j] = (R)ed * x[j];
+      }
+      int nc_w = max((nu[0] + 3) >> 2, (nu[1] + 3) >> 2);
+      for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
+      nc_w = __builtin_amdgcn_readfirstlane(nc_w);
+      const int last = a.up_nc - 1;
+      const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+      if (nc_w > 0) {
+        uint4 e4[PF][2];
+#pragma unroll
+        for (int p = 0; p < PF; ++p)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) e4[p][j] = ent4[(int64_t)min(p, last) * dimup + row[j]];
+        for (int c0 = 0; c0 < nc_w; c0 += PF) {
+#pragma unroll
+          for (int p = 0; p < PF; ++p) {
+            if (c0 + p < nc_w) {
+#pragma unroll
+              for (int j = 0; j < 2; ++j) {
+                const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
+                R y[4], h[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                  y[t] = lds_load_re<R>(ee[t] & kLdsAddrMask);
+                  h[t] = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+                }
+#pragma unroll
+                for (int t = 0; t < 4; ++t) ar[j] = fma(h[t], y[t], ar[j]);
+              }
+            }
+            const int cn = min(c0 + p + PF, last);
+#pragma unroll
+            for (int j = 0; j < 2; ++j) e4[p][j] = ent4[(int64_t)cn * dimup + row[j]];
+          }
+        }
+      }
+      // down-spin hops: one 2-element load per hop for the lane's row pair
+      for (int q0 = 0; q0 < nd; q0 += U) {
+        R2 y[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          const uint32_t so = __builtin_amdgcn_readfirstlane(s_dw_soff[min(q0 + u, nd - 1)]);
+          if constexpr (ESZ == 8)
+            y[u] = __builtin_bit_cast(
+                R2, __builtin_amdgcn_raw_buffer_load_b128(rv, r0 * ESZ, so, 0));
+          else
+            y[u] = __builtin_bit_cast(
+                R2, __builtin_amdgcn_raw_buffer_load_b64(rv, r0 * ESZ, so, 0));
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+          if (q0 + u < nd) {
+            const R h = s_dw_amp[q0 + u];
+            ar[0] = fma(h, y[u].x, ar[0]);
+            ar[1] = fma(h, y[u].y, ar[1]);
+          }
+      }
+      if (in) {
+        R2 out;
+        out.x = ar[0] * scale;
+        out.y = ar[1] * scale;
+        reinterpret_cast<R2 *>(hv)[r0 / 2] = out;
+        if (lz)
+          dot += (double)(x[0] * scale) * (double)out.x + (double)(x[1] * scale) * (double)out.y;
+      }
+    }
+  }
+  if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
+}
+
 // The real-vector gather pass with two columns per workgroup (columns 2p
 // and 2p + 1 of the pair p = workgroup), for real vectors whose two columns
 // fit the LDS together (FP32: 2 x 51.5 KB on the 12,870-row bath sector):
@@ -2252,6 +2392,22 @@ static void launch_buf_real_k(Sector &sec, const HxvArgs &a, dim3 grid, size_t s
   hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
 }
 
+template <typename R, int BS>
+static void launch_buf_real_k2(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static std::once_flag attr_once;
+  auto fn = k_hxv_buf_real2<R, BS>;
+  std::call_once(attr_once, [&]() {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    hipFuncAttributes fa{};  // LDS addresses from 0: no static LDS
+    VCA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+    if (fa.sharedSizeBytes != 0)
+      fail(VCA_ERR_HIP, "k_hxv_buf_real2 has %zu bytes of static LDS",
+           (size_t)fa.sharedSizeBytes);
+  });
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
+}
+
 // Real-vector mode, direct form: k_hxv_buf_real in the launch shape of the
 // sector's tuned buffer-load variant (512 lanes x 1 row otherwise), chains
 // spread or chain-affine as in the complex form.
@@ -2268,7 +2424,16 @@ static void launch_buf_real(Sector &sec, HxvArgs a, const HxvShape &h, int nb) {
   const size_t smem =
       (size_t)buf_lds(a.namp_up, sec.up.dim, a.dw_maxnnz, (int)sizeof(R)).total;
   const int bs = h.buf ? h.bs : 512, kr = h.buf ? h.kr : 1;
-  if (bs == 256 && kr == 4) launch_buf_real_k<R, 256, 4>(sec, a, g3, smem);
+  // FP64 with an even DimUp: two adjacent rows per lane (k_hxv_buf_real2),
+  // one 16-byte load per hop for both (3x4 chains: GF 3.35 -> 2.93 s).  In
+  // FP32 the pair is an 8-byte load and the one-row form is faster (GF 2.04
+  // against 2.17 s).  VCA_REAL_PAIRS=0 keeps one row per load.
+  const char *rp = std::getenv("VCA_REAL_PAIRS");
+  if (sizeof(R) == 8 && (sec.up.dim & 1) == 0 && !(rp && rp[0] == '0')) {
+    if (bs >= 1024) launch_buf_real_k2<R, 1024>(sec, a, g3, smem);
+    else if (bs == 256) launch_buf_real_k2<R, 256>(sec, a, g3, smem);
+    else launch_buf_real_k2<R, 512>(sec, a, g3, smem);
+  } else if (bs == 256 && kr == 4) launch_buf_real_k<R, 256, 4>(sec, a, g3, smem);
   else if (bs == 256 && kr == 2) launch_buf_real_k<R, 256, 2>(sec, a, g3, smem);
   else if (bs == 512 && kr == 2) launch_buf_real_k<R, 512, 2>(sec, a, g3, smem);
   else if (bs == 1024 && kr == 1) launch_buf_real_k<R, 1024, 1>(sec, a, g3, smem);
