@@ -2130,6 +2130,7 @@ static int tadd_tiles(const Sector &sec) {
 // on nb chains (and the transposed form's scratch), so that launches
 // recorded into a graph never allocate.
 void ensure_step_partials(Sector &sec, int nb) {
+  tune_real_direct(sec);  // before any launch is recorded
   const HxvShape h = hxv_shape(sec);
   int n = std::max(hxv_partials(sec, h), grid_for(sec.dim));
   if (sec.real_vec && real_uses_tform(sec)) n = std::max(n, tadd_tiles(sec));
@@ -2392,6 +2393,13 @@ static void launch_buf_real_k(Sector &sec, const HxvArgs &a, dim3 grid, size_t s
   hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
 }
 
+// Launch shapes of the real-vector direct form: (BS, rows per lane,
+// adjacent row pairs); pairs need an even DimUp.
+static const int kRealVariants[][3] = {{256, 4, 0}, {256, 2, 0},  {512, 2, 0},
+                                       {512, 1, 0}, {1024, 1, 0}, {1024, 2, 0},
+                                       {256, 1, 1}, {512, 1, 1},  {1024, 1, 1}};
+constexpr int kNumRealVariants = sizeof(kRealVariants) / sizeof(kRealVariants[0]);
+
 template <typename R, int BS>
 static void launch_buf_real_k2(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
   static std::once_flag attr_once;
@@ -2423,13 +2431,17 @@ static void launch_buf_real(Sector &sec, HxvArgs a, const HxvShape &h, int nb) {
       xcd ? dim3((unsigned)(sec.dw.dim * ((nb + 7) / 8 * 8)), 1) : dim3(sec.dw.dim, nb);
   const size_t smem =
       (size_t)buf_lds(a.namp_up, sec.up.dim, a.dw_maxnnz, (int)sizeof(R)).total;
-  const int bs = h.buf ? h.bs : 512, kr = h.buf ? h.kr : 1;
-  // FP64 with an even DimUp: two adjacent rows per lane (k_hxv_buf_real2),
-  // one 16-byte load per hop for both (3x4 chains: GF 3.35 -> 2.93 s).  In
-  // FP32 the pair is an 8-byte load and the one-row form is faster (GF 2.04
-  // against 2.17 s).  VCA_REAL_PAIRS=0 keeps one row per load.
-  const char *rp = std::getenv("VCA_REAL_PAIRS");
-  if (sizeof(R) == 8 && (sec.up.dim & 1) == 0 && !(rp && rp[0] == '0')) {
+  // launch shape: the tuned one (tune_real_direct) or, untuned, the complex
+  // form's BS x KR, with adjacent row pairs (k_hxv_buf_real2: one 16-byte
+  // load per down-spin hop for both rows) in FP64 when DimUp is even
+  int bs = h.buf ? h.bs : 512, kr = h.buf ? h.kr : 1;
+  bool pairs = sizeof(R) == 8 && (sec.up.dim & 1) == 0;
+  if (sec.real_variant >= 0) {
+    bs = kRealVariants[sec.real_variant][0];
+    kr = kRealVariants[sec.real_variant][1];
+    pairs = kRealVariants[sec.real_variant][2] != 0;
+  }
+  if (pairs) {
     if (bs >= 1024) launch_buf_real_k2<R, 1024>(sec, a, g3, smem);
     else if (bs == 256) launch_buf_real_k2<R, 256>(sec, a, g3, smem);
     else launch_buf_real_k2<R, 512>(sec, a, g3, smem);
@@ -2801,6 +2813,13 @@ void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int 
   hipLaunchKernelGGL(k_lz_start, dim3(nb), dim3(kBlock), 0, jsec.stream, jsec.lz, jsec.partials,
                      grid);
   VCA_HIP(hipGetLastError());
+}
+
+int real_num_variants() { return kNumRealVariants; }
+
+bool real_variant_ok(const Sector &sec, int var) {
+  return var >= 0 && var < kNumRealVariants && !(kRealVariants[var][2] && (sec.up.dim & 1)) &&
+         real_mode_ok(sec) && !real_uses_tform(sec);
 }
 
 double real_vector_of(const void *psi, int64_t n, void *out, int prec, hipStream_t st) {

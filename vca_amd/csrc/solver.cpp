@@ -276,6 +276,48 @@ static std::map<TuneKey, int> g_tuned;
 static std::mutex g_tune_mu;
 constexpr int64_t kTuneMinDim = (int64_t)1 << 17;  // one vector is timed from here up
 
+// Times one H x v (launch_hxv on a zero vector, as the sector's mode is
+// set) for every variant that pick(var) admits and sets, and returns the
+// fastest: a warm-up launch, then the best of three.
+template <typename Pick>
+static int time_variants(Sector &sec, int nv, Pick pick, const char *what) {
+  void *v = dev_alloc_async(sec.vec_bytes(), sec.stream);
+  void *hv = dev_alloc_async(sec.vec_bytes(), sec.stream);
+  VCA_HIP(hipMemsetAsync(v, 0, sec.vec_bytes(), sec.stream));
+  hipEvent_t e0, e1;
+  VCA_HIP(hipEventCreate(&e0));
+  VCA_HIP(hipEventCreate(&e1));
+  int best = 0;
+  float best_ms = std::numeric_limits<float>::max();
+  const char *verbose = std::getenv("VCA_TUNE_VERBOSE");
+  for (int var = 0; var < nv; ++var) {
+    if (!pick(var)) continue;
+    launch_hxv(sec, v, v, hv, false);
+    float ms_min = std::numeric_limits<float>::max();
+    for (int r = 0; r < 3; ++r) {
+      VCA_HIP(hipEventRecord(e0, sec.stream));
+      launch_hxv(sec, v, v, hv, false);
+      VCA_HIP(hipEventRecord(e1, sec.stream));
+      VCA_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
+      ms_min = std::min(ms_min, ms);
+    }
+    if (ms_min < best_ms) {
+      best_ms = ms_min;
+      best = var;
+    }
+    if (verbose && verbose[0] == '1')
+      std::fprintf(stderr, "%s tune dim %dx%d prec %d variant %d: %.2f us\n", what, sec.up.dim,
+                   sec.dw.dim, sec.precision, var, ms_min * 1e3);
+  }
+  VCA_HIP(hipEventDestroy(e0));
+  VCA_HIP(hipEventDestroy(e1));
+  dev_free_async(v, sec.stream);
+  dev_free_async(hv, sec.stream);
+  return best;
+}
+
 static void hxv_tune(Sector &sec) {
   const char *force = std::getenv("VCA_HXV_VARIANT");
   if (force && *force) {
@@ -299,41 +341,11 @@ static void hxv_tune(Sector &sec) {
       return;
     }
   }
-  void *v = dev_alloc_async(sec.vec_bytes(), sec.stream);
-  void *hv = dev_alloc_async(sec.vec_bytes(), sec.stream);
-  VCA_HIP(hipMemsetAsync(v, 0, sec.vec_bytes(), sec.stream));
-  hipEvent_t e0, e1;
-  VCA_HIP(hipEventCreate(&e0));
-  VCA_HIP(hipEventCreate(&e1));
-  int best = 0;
-  float best_ms = std::numeric_limits<float>::max();
-  for (int var = 0; var < nv; ++var) {
-    if (!hxv_variant_ok(sec, var)) continue;
+  const int best = time_variants(sec, nv, [&](int var) {
+    if (!hxv_variant_ok(sec, var)) return false;
     sec.hxv_variant = var;
-    launch_hxv(sec, v, v, hv, false);
-    float ms_min = std::numeric_limits<float>::max();
-    for (int r = 0; r < 3; ++r) {
-      VCA_HIP(hipEventRecord(e0, sec.stream));
-      launch_hxv(sec, v, v, hv, false);
-      VCA_HIP(hipEventRecord(e1, sec.stream));
-      VCA_HIP(hipEventSynchronize(e1));
-      float ms = 0;
-      VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
-      ms_min = std::min(ms_min, ms);
-    }
-    if (ms_min < best_ms) {
-      best_ms = ms_min;
-      best = var;
-    }
-    const char *verbose = std::getenv("VCA_TUNE_VERBOSE");
-    if (verbose && verbose[0] == '1')
-      std::fprintf(stderr, "hxv tune dim %dx%d prec %d variant %d: %.2f us\n", sec.up.dim,
-                   sec.dw.dim, sec.precision, var, ms_min * 1e3);
-  }
-  VCA_HIP(hipEventDestroy(e0));
-  VCA_HIP(hipEventDestroy(e1));
-  dev_free_async(v, sec.stream);
-  dev_free_async(hv, sec.stream);
+    return true;
+  }, "hxv");
   sec.hxv_variant = best;
   if (sec.tbuf && hxv_scratch_vectors(sec) == 0) {  // timed, not picked
     VCA_HIP(hipStreamSynchronize(sec.stream));
@@ -343,6 +355,34 @@ static void hxv_tune(Sector &sec) {
   }
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tuned[key] = best;
+}
+
+// The real-vector direct form (real vectors below the transposed form's
+// size) has launch shapes of its own (rows per lane, adjacent row pairs),
+// timed the same way with the sector in real mode.
+static std::map<TuneKey, int> g_tuned_real;
+
+void tune_real_direct(Sector &sec) {
+  if (sec.real_variant >= 0 || !sec.real_vec || !real_variant_ok(sec, 0)) return;
+  const char *t = std::getenv("VCA_TUNE");
+  if ((t && t[0] == '0') || sec.dim < kTuneMinDim) return;
+  const TuneKey key{sec.up.dim, sec.dw.dim, sec.precision, 1, sec.up.nterms, sec.dw.nterms};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned_real.find(key);
+    if (it != g_tuned_real.end()) {
+      sec.real_variant = it->second;
+      return;
+    }
+  }
+  const int best = time_variants(sec, real_num_variants(), [&](int var) {
+    if (!real_variant_ok(sec, var)) return false;
+    sec.real_variant = var;
+    return true;
+  }, "real hxv");
+  sec.real_variant = best;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tuned_real[key] = best;
 }
 
 // Launch shape for nb stacked vectors (the batched steps of the excitation
@@ -358,13 +398,19 @@ static std::map<std::pair<TuneKey, int>, int> g_tuned_batch;
 static void hxv_tune_batch(Sector &sec, int nb) {
   const char *force = std::getenv("VCA_HXV_VARIANT");
   const char *t = std::getenv("VCA_TUNE");
-  const int nv = hxv_num_variants(sec);
+  // real-vector mode picks among the real direct form's shapes
+  const bool real = sec.real_vec;
+  const int nv = real ? real_num_variants() : hxv_num_variants(sec);
   if ((force && *force) || (t && t[0] == '0') || nv <= 1 || nb <= 1) return;
-  if (sec.dim >= kTuneMinDim) return;
-  const TuneKey tk{sec.up.dim, sec.dw.dim, sec.precision,
-                   (int)(sec.up.real_amp && sec.dw.real_amp), sec.up.nterms, sec.dw.nterms};
+  // real vectors: timed with the batch at every size (the direct form's
+  // best shape for one vector is not the best for a batch of chains)
+  if (real ? !real_variant_ok(sec, 0) : sec.dim >= kTuneMinDim) return;
+  int &cur = real ? sec.real_variant : sec.hxv_variant;
+  const TuneKey tk{sec.up.dim,   sec.dw.dim,   sec.precision,
+                   real ? 2 : (int)(sec.up.real_amp && sec.dw.real_amp),
+                   sec.up.nterms, sec.dw.nterms};
   const auto key = std::make_pair(tk, nb);
-  const int orig = sec.hxv_variant;
+  const int orig = cur;
   int best = -1;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
@@ -378,8 +424,8 @@ static void hxv_tune_batch(Sector &sec, int nb) {
     VCA_HIP(hipEventCreate(&e1));
     float best_ms = std::numeric_limits<float>::max();
     for (int var = 0; var < nv; ++var) {
-      if (!hxv_variant_ok(sec, var)) continue;
-      sec.hxv_variant = var;
+      if (!(real ? real_variant_ok(sec, var) : hxv_variant_ok(sec, var))) continue;
+      cur = var;
       launch_hxv(sec, sec.buf[0], sec.buf[0], sec.buf[2], false, nb);
       float ms_min = std::numeric_limits<float>::max();
       for (int r = 0; r < 2; ++r) {
@@ -397,18 +443,18 @@ static void hxv_tune_batch(Sector &sec, int nb) {
       }
       const char *verbose = std::getenv("VCA_TUNE_VERBOSE");
       if (verbose && verbose[0] == '1')
-        std::fprintf(stderr, "hxv batch tune dim %dx%d x %d prec %d variant %d: %.2f us\n",
-                     sec.up.dim, sec.dw.dim, nb, sec.precision, var, ms_min * 1e3);
+        std::fprintf(stderr, "hxv batch tune dim %dx%d x %d prec %d real %d variant %d: %.2f us\n",
+                     sec.up.dim, sec.dw.dim, nb, sec.precision, (int)real, var, ms_min * 1e3);
     }
     VCA_HIP(hipEventDestroy(e0));
     VCA_HIP(hipEventDestroy(e1));
-    sec.hxv_variant = orig;
+    cur = orig;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tuned_batch[key] = best;
   }
-  if (best != sec.hxv_variant) {
+  if (best != cur) {
     clear_graphs(sec);
-    sec.hxv_variant = best;
+    cur = best;
   }
 }
 
@@ -1254,12 +1300,12 @@ static void build_gf(Solver &s) {
       for (size_t b0 = 0; b0 < idx.size(); b0 += nb_max) {
         const int nb = (int)std::min<size_t>(nb_max, idx.size() - b0);
         ensure_lanczos(js, nlanc + 2, std::max(nb, js.nb_cap));
-        hxv_tune_batch(js, nb);
         std::vector<ExcOp> ops(nb);
         for (int c = 0; c < nb; ++c) ops[c] = specs[idx[b0 + c]].op;
         VCA_HIP(hipMemcpyAsync(js.d_ops, ops.data(), sizeof(ExcOp) * nb, hipMemcpyHostToDevice,
                                js.stream));
         RealVecScope real_mode(js, pr == 1);
+        hxv_tune_batch(js, nb);
         if (js.real_vec && (js.dim & 1))  // the pad reals of Hv and u_{k-1} stay 0
           for (int i = 1; i < 3; ++i)
             VCA_HIP(hipMemsetAsync(js.buf[i], 0, js.vec_bytes() * nb, js.stream));
@@ -1953,6 +1999,7 @@ int vca_hxv_device_real(vca_sector *sv, const void *v, void *hv, int32_t repeat,
          sec->nup, sec->ndw);
   if (repeat < 1) repeat = 1;
   RealVecScope real_mode(*sec, true);
+  tune_real_direct(*sec);
   hipEvent_t e0, e1;
   VCA_HIP(hipEventCreate(&e0));
   VCA_HIP(hipEventCreate(&e1));

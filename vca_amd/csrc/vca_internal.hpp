@@ -227,6 +227,7 @@ struct Sector {
   // Lanczos workspace (lazily allocated) for nb_cap chains of lz_cap steps
   int lz_cap = 0, nb_cap = 0;
   int hxv_variant = -1;  // launch shape of H x v picked by the autotuner
+  int real_variant = -1;  // the same for the real-vector direct form
   void *buf[3] = {nullptr, nullptr, nullptr};  // u_even, u_odd, hv [nb][dim]
   void *xvec = nullptr;                        // eigenvector accumulator
   double *partials = nullptr;
@@ -285,6 +286,12 @@ int hxv_scratch_vectors(const Sector &sec);
 // Real-vector mode (Sector::real_vec) can run on the sector: real
 // amplitudes and the one-plane transposed H x v fits.
 bool real_mode_ok(const Sector &sec);
+// Launch shapes of the real-vector direct form (used when the transposed
+// form is not): their number, whether shape i runs this sector, and the
+// autotuner that picks one per sector shape (never inside a graph capture).
+int real_num_variants();
+bool real_variant_ok(const Sector &sec, int var);
+void tune_real_direct(Sector &sec);
 // sector size from which the transposed H x v and the real-vector ground
 // state are used (2^22 states; VCA_TFORM_MIN_DIM overrides)
 int64_t tform_min_dim();
