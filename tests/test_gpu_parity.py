@@ -415,7 +415,56 @@ REAL_FORMS = {
     # (k_hxv_plane2) or one (k_hxv_plane)
     "transposed2": {"VCA_TFORM_MIN_DIM": "0", "VCA_HXV_VARIANT": "20", "VCA_PLANE2": "1"},
     "transposed1": {"VCA_TFORM_MIN_DIM": "0", "VCA_HXV_VARIANT": "20", "VCA_PLANE2": "0"},
+    # the overlapped direct form (k_hxv_ov_real, 512 lanes x 2 rows)
+    "overlapped": {"VCA_REAL_VARIANT": "9"},
 }
+
+# launch shapes of the real direct form: (lanes, rows or slots per lane,
+# kind): 0 single rows, 1 row pairs, 2 overlapped, 3 overlapped row pairs
+REAL_SHAPES = [(256, 4, 0), (256, 2, 0), (512, 2, 0), (512, 1, 0), (1024, 1, 0), (1024, 2, 0),
+               (256, 1, 1), (512, 1, 1), (1024, 1, 1), (512, 2, 2), (1024, 1, 2), (256, 4, 2),
+               (512, 1, 3), (256, 2, 3), (1024, 1, 3), (128, 8, 2), (128, 4, 3)]
+
+
+def _real_shape_fits(var, dimup):
+    bs, nr, kind = REAL_SHAPES[var]
+    if kind in (1, 3) and dimup % 2:
+        return False
+    return kind < 2 or dimup <= bs * nr * (2 if kind == 3 else 1)
+
+
+@pytest.mark.parametrize("var", range(len(REAL_SHAPES)))
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_real_direct_shapes(prec, var, monkeypatch):
+    """Every launch shape of the real-vector direct form (VCA_REAL_VARIANT)
+    gives the oracle's H x v (1e-12 / 2e-6 relative) on the sectors it fits:
+    an even DimUp of 252 and an odd one of 165; a shape that does not fit is
+    refused with VCA_ERR_ARG."""
+    from vca_amd.api import DeviceArray, VcaError
+    monkeypatch.setenv("VCA_REAL_VARIANT", str(var))
+    dt = np.float64 if prec == "fp64" else np.float32
+    for name, (p, hloc, hk, bh, bv), sector in (
+            ("bath5+5", M.bath_model(nlat=5, nb=5), (5, 5)),
+            ("chain11", M.chain(nlat=11, u=4.0), (3, 3))):
+        s = _solver(p, hloc, bh, bv)
+        sec = s.sector(*sector, prec)
+        dimup = sec.dimup
+        v = np.random.default_rng(5).normal(size=sec.dim).astype(dt)
+        dv = DeviceArray(v.shape, dt).upload(v)
+        dh = DeviceArray(v.shape, dt)
+        _say(f"real H x v shape {var} {name} {sector} {prec} dimup {dimup}")
+        if not _real_shape_fits(var, dimup):
+            with pytest.raises(VcaError) as ei:
+                sec.hxv_device_real(dv.ptr, dh.ptr)
+            assert ei.value.code == -1
+            sec.close()
+            continue
+        sec.hxv_device_real(dv.ptr, dh.ptr)
+        om = M.oracle_model(p, hloc, bh, bv)
+        ref = om.hxv(*sector, v.astype(np.complex128)).real
+        err = np.linalg.norm(dh.download() - ref) / np.linalg.norm(ref)
+        assert err < (1e-12 if prec == "fp64" else 2e-6), (name, var, err)
+        sec.close()
 
 
 def _set_env(monkeypatch, env):
@@ -488,14 +537,17 @@ def test_real_vector_chain_steps(prec, form, monkeypatch):
         sec.close()
 
 
-@pytest.mark.parametrize("case", ["chain10_xcd0", "chain10_xcd1", "chain11_sector33"])
+@pytest.mark.parametrize("case", ["chain10_xcd0", "chain10_xcd1", "chain11_sector33",
+                                  "chain10_overlapped"])
 def test_real_vector_chains_in_a_solve(case, monkeypatch):
     """The excitation chains of a real Hamiltonian on real vectors (the
     ground state's global phase removed), batched with the columns of each
     chain spread over the XCDs or each chain on one XCD, against the same
     solve on complex chains.  chain11_sector33: the 11-site chain restricted
     to the sector (3,3), whose target sector (3,2) has an odd dimension
-    (165 x 55 = 9,075: the pad real of every chain in the batch).  The
+    (165 x 55 = 9,075: the pad real of every chain in the batch).
+    chain10_overlapped: the real batches on the overlapped direct form
+    (k_hxv_ov_real, 512 lanes x 2 rows), each chain on one XCD.  The
     ground state is the same complex vector in both, so E0 agrees to
     rounding; G' and Omega differ by what dropping the ground state's
     non-phase imaginary remainder (its Lanczos error) changes: bars 2e-6 of
@@ -504,7 +556,11 @@ def test_real_vector_chains_in_a_solve(case, monkeypatch):
         p, hloc, hk, _, _ = M.chain(nlat=11, u=4.0, nk=4)
         p.sectors = [(3, 3)]
     else:
-        monkeypatch.setenv("VCA_XCD_CHAINS", case[-1])
+        if case == "chain10_overlapped":
+            monkeypatch.setenv("VCA_XCD_CHAINS", "1")
+            monkeypatch.setenv("VCA_REAL_VARIANT", "9")
+        else:
+            monkeypatch.setenv("VCA_XCD_CHAINS", case[-1])
         p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
     p.lanc_ngfiter = 80
     out = {}

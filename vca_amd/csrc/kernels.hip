@@ -982,6 +982,188 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
   if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
 }
 
+// Real-vector form of k_hxv_ov (real amplitudes, vectors of dim reals) for
+// short columns: each lane owns NR slots of P adjacent rows (slot s holds
+// rows s*P .. s*P+P-1; P = 2 needs an even DimUp), DimUp <= BS * NR * P.
+// The down-spin hops are buffer loads of whole neighbour columns (one P-row
+// vector load per slot and hop) with the hop list read by uniform loads,
+// issued before the LDS barrier; the column and the amplitudes go to LDS
+// meanwhile, and the up-spin gathers use the one-plane hop table
+// (DevSpin::ent4p).  a.vstride counts reals between chains.
+template <typename R, int BS, int NR, int P>
+__global__ __launch_bounds__(BS) void k_hxv_ov_real(const HxvArgs a) {
+  constexpr int ESZ = sizeof(R);
+  constexpr int NRW = NR * P;  // rows per lane
+  constexpr int U = NRW <= 2 ? 8 : 4;
+  constexpr int PF = 2;
+  using V2 = typename Vec2<R>::T;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const BufLds lds = buf_lds(a.namp_up, a.dimup, 0, ESZ);
+  R *s_amp_up = reinterpret_cast<R *>(smem);
+  R *s_col = reinterpret_cast<R *>(smem + lds.col);
+
+  const int dimup = a.dimup;
+  const WgPos wp = wg_pos(a);
+  if (!wp.valid) return;
+  const int idw = wp.idw;
+  const int64_t chain_off = (int64_t)wp.chain * a.vstride;
+  LanczosDev *lz = a.lz ? a.lz + wp.chain : nullptr;
+  bool done = false;
+  R scale = (R)1;
+  const R *vin = static_cast<const R *>(a.v0) + chain_off;
+  if (lz) {
+    const int k = lz->k;
+    done = lz->done != 0;
+    if (k & 1) vin = static_cast<const R *>(a.v1) + chain_off;
+    if (!done) scale = (R)(1.0 / lz->nrm[k]);
+  }
+  R *hv = static_cast<R *>(a.hv) + chain_off + (int64_t)idw * dimup;
+  const R *vcol = vin + (int64_t)idw * dimup;
+  double dot = 0.0;
+
+  if (!done) {
+    const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<R *>(vin), (short)0, (int)(uint32_t)(a.vstride * ESZ), 0x00020000);
+    const cplx<R> *g_amp_up = static_cast<const cplx<R> *>(a.amp_up);
+    const cplx<R> *g_amp_dw = static_cast<const cplx<R> *>(a.amp_dw);
+    const int nslot = dimup / P;
+    int slot[NR], row[NRW], nu[NRW];
+    bool in[NR];
+    R x[NRW];
+    RowMeta m[NRW];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const int sl = (int)threadIdx.x + j * BS;
+      in[j] = sl < nslot;
+      slot[j] = min(sl, nslot - 1);
+      if constexpr (P == 2) {
+        const V2 xv = reinterpret_cast<const V2 *>(vcol)[slot[j]];
+        x[2 * j] = xv.x;
+        x[2 * j + 1] = xv.y;
+      } else {
+        x[j] = vcol[slot[j]];
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p) {
+        row[j * P + p] = slot[j] * P + p;
+        m[j * P + p] = a.meta_up[slot[j] * P + p];
+      }
+    }
+    R amp = (R)0;
+    if ((int)threadIdx.x < a.namp_up) amp = g_amp_up[threadIdx.x].x;
+    R ar[NRW];
+#pragma unroll
+    for (int j = 0; j < NRW; ++j) ar[j] = (R)0;
+    // down-spin hops: whole neighbour columns, offsets and amplitudes from
+    // uniform loads of the column's hop list
+    const int nd = a.skip_dw ? 0 : a.cnt_dw[idw];
+    const uint32_t *entd = a.ent_dw + idw;
+    for (int q0 = 0; q0 < nd; q0 += U) {
+      R y[U][NRW], h[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        const uint32_t e = entd[(int64_t)min(q0 + u, nd - 1) * a.dimdw];
+        const uint32_t so = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
+        h[u] = g_amp_dw[e >> kColBits].x;
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+          const int vo = slot[j] * P * ESZ;
+          if constexpr (P == 2 && ESZ == 8) {
+            const V2 yv = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b128(rv, vo, so, 0));
+            y[u][2 * j] = yv.x;
+            y[u][2 * j + 1] = yv.y;
+          } else if constexpr (P == 2) {
+            const V2 yv = __builtin_bit_cast(V2, __builtin_amdgcn_raw_buffer_load_b64(rv, vo, so, 0));
+            y[u][2 * j] = yv.x;
+            y[u][2 * j + 1] = yv.y;
+          } else if constexpr (ESZ == 8) {
+            y[u][j] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b64(rv, vo, so, 0));
+          } else {
+            y[u][j] = __builtin_bit_cast(R, __builtin_amdgcn_raw_buffer_load_b32(rv, vo, so, 0));
+          }
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (q0 + u < nd) {
+#pragma unroll
+          for (int j = 0; j < NRW; ++j) ar[j] = fma(h[u], y[u][j], ar[j]);
+        }
+    }
+    // diagonal, then the column and the amplitudes into LDS
+    const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
+    const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
+#pragma unroll
+    for (int j = 0; j < NRW; ++j) {
+      nu[j] = in[j / P] ? m[j].cnt : 0;
+      const double ed = a.skip_diag ? 0.0 : m[j].e + edw + cross_energy(a.cross, m[j].s, sdw);
+      ar[j] = fma((R)ed, x[j], ar[j]);
+      if (in[j / P]) s_col[row[j]] = x[j];
+    }
+    if ((int)threadIdx.x < a.namp_up) s_amp_up[threadIdx.x] = amp;
+    for (int t = (int)threadIdx.x + BS; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t].x;
+    if (threadIdx.x == 0) s_amp_up[a.namp_up] = (R)0;  // padding hops
+    __syncthreads();
+    // up-spin hops: LDS gathers, entries through a ring of PF chunks
+    int nc_w = 0;
+#pragma unroll
+    for (int j = 0; j < NRW; ++j) nc_w = max(nc_w, (nu[j] + 3) >> 2);
+    for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
+    nc_w = __builtin_amdgcn_readfirstlane(nc_w);
+    const int last = a.up_nc - 1;
+    const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+    if (nc_w > 0) {
+      uint4 e4[PF][NRW];
+#pragma unroll
+      for (int p = 0; p < PF; ++p)
+#pragma unroll
+        for (int j = 0; j < NRW; ++j) e4[p][j] = ent4[(int64_t)min(p, last) * dimup + row[j]];
+      for (int c0 = 0; c0 < nc_w; c0 += PF) {
+#pragma unroll
+        for (int p = 0; p < PF; ++p) {
+          if (c0 + p < nc_w) {
+#pragma unroll
+            for (int j = 0; j < NRW; ++j) {
+              const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
+              R y[4], h[4];
+#pragma unroll
+              for (int t = 0; t < 4; ++t) {
+                y[t] = lds_load_re<R>(ee[t] & kLdsAddrMask);
+                h[t] = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+              }
+#pragma unroll
+              for (int t = 0; t < 4; ++t) ar[j] = fma(h[t], y[t], ar[j]);
+            }
+          }
+          const int cn = min(c0 + p + PF, last);
+#pragma unroll
+          for (int j = 0; j < NRW; ++j) e4[p][j] = ent4[(int64_t)cn * dimup + row[j]];
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      if (in[j]) {
+        if constexpr (P == 2) {
+          V2 out;
+          out.x = ar[2 * j] * scale;
+          out.y = ar[2 * j + 1] * scale;
+          reinterpret_cast<V2 *>(hv)[slot[j]] = out;
+          if (lz) {
+            const V2 xv = *reinterpret_cast<const V2 *>(s_col + 2 * slot[j]);
+            dot += (double)(xv.x * scale) * (double)out.x + (double)(xv.y * scale) * (double)out.y;
+          }
+        } else {
+          const R out = ar[j] * scale;
+          hv[slot[j]] = out;
+          if (lz) dot += (double)(s_col[slot[j]] * scale) * (double)out;
+        }
+      }
+    }
+  }
+  if (lz) block_partial_lds<BS>(dot, a.partials, wp.slot, reinterpret_cast<double *>(smem));
+}
+
 // Real-vector form of k_hxv_buf (real amplitudes, vectors of dim reals): the
 // same schedule on elements of sizeof(R) -- the column staged in LDS, the
 // up-spin hops gathered there through the one-plane hop table
@@ -2393,11 +2575,29 @@ static void launch_buf_real_k(Sector &sec, const HxvArgs &a, dim3 grid, size_t s
   hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
 }
 
-// Launch shapes of the real-vector direct form: (BS, rows per lane,
-// adjacent row pairs); pairs need an even DimUp.
-static const int kRealVariants[][3] = {{256, 4, 0}, {256, 2, 0},  {512, 2, 0},
-                                       {512, 1, 0}, {1024, 1, 0}, {1024, 2, 0},
-                                       {256, 1, 1}, {512, 1, 1},  {1024, 1, 1}};
+template <typename R, int BS, int NR, int P>
+static void launch_ov_real_k(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
+  static std::once_flag attr_once;
+  auto fn = k_hxv_ov_real<R, BS, NR, P>;
+  std::call_once(attr_once, [&]() {
+    VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
+    hipFuncAttributes fa{};  // LDS addresses from 0: no static LDS
+    VCA_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void *>(fn)));
+    if (fa.sharedSizeBytes != 0)
+      fail(VCA_ERR_HIP, "k_hxv_ov_real has %zu bytes of static LDS", (size_t)fa.sharedSizeBytes);
+  });
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
+}
+
+// Launch shapes of the real-vector direct form: (BS, rows or slots per
+// lane, kind): kind 0 k_hxv_buf_real (KR rows), 1 k_hxv_buf_real2 (adjacent
+// row pairs), 2 and 3 k_hxv_ov_real with single rows and with adjacent row
+// pairs (DimUp <= BS x slots x rows per slot).  Pairs need an even DimUp.
+static const int kRealVariants[][3] = {
+    {256, 4, 0}, {256, 2, 0}, {512, 2, 0}, {512, 1, 0}, {1024, 1, 0}, {1024, 2, 0},
+    {256, 1, 1}, {512, 1, 1}, {1024, 1, 1}, {512, 2, 2}, {1024, 1, 2}, {256, 4, 2},
+    {512, 1, 3}, {256, 2, 3}, {1024, 1, 3}, {128, 8, 2}, {128, 4, 3}};
 constexpr int kNumRealVariants = sizeof(kRealVariants) / sizeof(kRealVariants[0]);
 
 template <typename R, int BS>
@@ -2436,12 +2636,27 @@ static void launch_buf_real(Sector &sec, HxvArgs a, const HxvShape &h, int nb) {
   // load per down-spin hop for both rows) in FP64 when DimUp is even
   int bs = h.buf ? h.bs : 512, kr = h.buf ? h.kr : 1;
   bool pairs = sizeof(R) == 8 && (sec.up.dim & 1) == 0;
+  int kind = pairs ? 1 : 0;
   if (sec.real_variant >= 0) {
     bs = kRealVariants[sec.real_variant][0];
     kr = kRealVariants[sec.real_variant][1];
-    pairs = kRealVariants[sec.real_variant][2] != 0;
+    kind = kRealVariants[sec.real_variant][2];
+    pairs = kind == 1;
   }
-  if (pairs) {
+  if (kind >= 2) {
+    const int id = bs * 100 + kr * 10 + (kind - 2);
+    switch (id) {
+      case 51220: launch_ov_real_k<R, 512, 2, 1>(sec, a, g3, smem); break;
+      case 102410: launch_ov_real_k<R, 1024, 1, 1>(sec, a, g3, smem); break;
+      case 25640: launch_ov_real_k<R, 256, 4, 1>(sec, a, g3, smem); break;
+      case 51211: launch_ov_real_k<R, 512, 1, 2>(sec, a, g3, smem); break;
+      case 25621: launch_ov_real_k<R, 256, 2, 2>(sec, a, g3, smem); break;
+      case 102411: launch_ov_real_k<R, 1024, 1, 2>(sec, a, g3, smem); break;
+      case 12880: launch_ov_real_k<R, 128, 8, 1>(sec, a, g3, smem); break;
+      case 12841: launch_ov_real_k<R, 128, 4, 2>(sec, a, g3, smem); break;
+      default: fail(VCA_ERR_ARG, "no real overlapped shape %d x %d", bs, kr);
+    }
+  } else if (pairs) {
     if (bs >= 1024) launch_buf_real_k2<R, 1024>(sec, a, g3, smem);
     else if (bs == 256) launch_buf_real_k2<R, 256>(sec, a, g3, smem);
     else launch_buf_real_k2<R, 512>(sec, a, g3, smem);
@@ -2818,8 +3033,13 @@ void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int 
 int real_num_variants() { return kNumRealVariants; }
 
 bool real_variant_ok(const Sector &sec, int var) {
-  return var >= 0 && var < kNumRealVariants && !(kRealVariants[var][2] && (sec.up.dim & 1)) &&
-         real_mode_ok(sec) && !real_uses_tform(sec);
+  if (var < 0 || var >= kNumRealVariants || !real_mode_ok(sec) || real_uses_tform(sec))
+    return false;
+  const int kind = kRealVariants[var][2];
+  if ((kind == 1 || kind == 3) && (sec.up.dim & 1)) return false;  // row pairs
+  if (kind >= 2)  // overlapped: the column in one round of lanes
+    return sec.up.dim <= kRealVariants[var][0] * kRealVariants[var][1] * (kind == 3 ? 2 : 1);
+  return true;
 }
 
 double real_vector_of(const void *psi, int64_t n, void *out, int prec, hipStream_t st) {

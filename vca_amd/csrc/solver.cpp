@@ -364,6 +364,14 @@ static std::map<TuneKey, int> g_tuned_real;
 
 void tune_real_direct(Sector &sec) {
   if (sec.real_variant >= 0 || !sec.real_vec || !real_variant_ok(sec, 0)) return;
+  const char *force = std::getenv("VCA_REAL_VARIANT");  // a shape this sector can run
+  if (force && *force) {
+    const int var = std::atoi(force);
+    if (!real_variant_ok(sec, var))
+      fail(VCA_ERR_ARG, "VCA_REAL_VARIANT=%d does not fit sector (%d,%d)", var, sec.nup, sec.ndw);
+    sec.real_variant = var;
+    return;
+  }
   const char *t = std::getenv("VCA_TUNE");
   if ((t && t[0] == '0') || sec.dim < kTuneMinDim) return;
   const TuneKey key{sec.up.dim, sec.dw.dim, sec.precision, 1, sec.up.nterms, sec.dw.nterms};
@@ -396,10 +404,10 @@ void tune_real_direct(Sector &sec) {
 static std::map<std::pair<TuneKey, int>, int> g_tuned_batch;
 
 static void hxv_tune_batch(Sector &sec, int nb) {
-  const char *force = std::getenv("VCA_HXV_VARIANT");
-  const char *t = std::getenv("VCA_TUNE");
   // real-vector mode picks among the real direct form's shapes
   const bool real = sec.real_vec;
+  const char *force = std::getenv(real ? "VCA_REAL_VARIANT" : "VCA_HXV_VARIANT");
+  const char *t = std::getenv("VCA_TUNE");
   const int nv = real ? real_num_variants() : hxv_num_variants(sec);
   if ((force && *force) || (t && t[0] == '0') || nv <= 1 || nb <= 1) return;
   // real vectors: timed with the batch at every size (the direct form's
