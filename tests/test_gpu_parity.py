@@ -98,10 +98,12 @@ def test_hxv_square_3x4_full_sector():
     assert err < 1e-12, err
 
 
-N_HXV_VARIANTS = 24  # launch shapes of the one-column kernels (kernels.hip kVariants)
+N_HXV_VARIANTS = 27  # launch shapes of the one-column kernels (kernels.hip kVariants)
 N_HXV_DIRECT = 15    # the first 15: direct form; then the transposed form (15-18
                      # whole columns, 19-20 one real plane at a time: real
-                     # amplitudes only) and the overlapped short-column form (21-23)
+                     # amplitudes only), the overlapped short-column form (21-23)
+                     # and its signed-column variant (24-26: one real hopping
+                     # amplitude)
 
 
 @pytest.mark.parametrize("variant", range(N_HXV_VARIANTS))
@@ -288,6 +290,48 @@ def test_hxv_device_batch_equals_single(prec, xcd, monkeypatch):
     ref = om.hxv(5, 5, v[1].astype(np.complex128))
     err = np.linalg.norm(hv[1] - ref) / np.linalg.norm(ref)
     assert err < (1e-12 if prec == "fp64" else 2e-6), err
+
+
+@pytest.mark.parametrize("variant", ["24", "25", "26"])
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_signed_column_form_in_lanczos_and_batches(prec, variant, monkeypatch):
+    """The signed-column overlapped H x v (one real hopping amplitude: the
+    10-site chain; the column staged as +t x and -t x, no amplitude gather)
+    inside the Lanczos steps, where alpha's partials reread the input column:
+    the oracle's Lanczos coefficients (12 steps; 1e-10 FP64, 1e-5 MIXED), and
+    stacked products equal to the single ones bit for bit."""
+    from vca_amd.api import DeviceArray
+    monkeypatch.setenv("VCA_HXV_VARIANT", variant)
+    p, hloc, hk, bh, bv = M.chain(nlat=10, u=4.0)
+    om = M.oracle_model(p, hloc, bh, bv)
+    s = _solver(p, hloc, bh, bv)
+    sec = s.sector(5, 5, prec)
+    _say(f"signed-column form {variant}: tridiag chain10 (5,5) {prec}")
+    v0 = _rand(sec.dim, 5)
+    v0 /= np.linalg.norm(v0)
+    a, b = sec.lanczos_tridiag(v0, 12)
+    ao, bo = R.lanczos_tridiag(lambda x: om.hxv(5, 5, x), v0, 12)
+    tol = 1e-10 if prec == "fp64" else 1e-5
+    assert np.allclose(a, ao, rtol=tol, atol=tol) and np.allclose(b, bo, rtol=tol, atol=tol)
+    _say(f"signed-column form {variant}: batched products chain10 (5,5) nb=13 {prec}")
+    dt = np.complex128 if prec == "fp64" else np.complex64
+    nb = 13
+    rng = np.random.default_rng(8)
+    v = (rng.normal(size=(nb, sec.dim)) + 1j * rng.normal(size=(nb, sec.dim))).astype(dt)
+    dv = DeviceArray(v.shape, dt).upload(v)
+    dh = DeviceArray(v.shape, dt)
+    sec.hxv_device_batch(nb, dv.ptr, dh.ptr)
+    hv = dh.download()
+    d1 = DeviceArray((sec.dim,), dt)
+    h1 = DeviceArray((sec.dim,), dt)
+    for c in (0, 6, 12):
+        d1.upload(v[c])
+        sec.hxv_device(d1.ptr, h1.ptr)
+        assert np.array_equal(h1.download(), hv[c])
+        ref = om.hxv(5, 5, v[c].astype(np.complex128))
+        err = np.linalg.norm(hv[c] - ref) / np.linalg.norm(ref)
+        assert err < (1e-12 if prec == "fp64" else 2e-6), (c, err)
+    sec.close()
 
 
 @pytest.mark.parametrize("variant", ["16", "20"])

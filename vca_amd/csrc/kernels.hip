@@ -110,11 +110,27 @@ __device__ __host__ inline BufLds buf_lds(int namp, int dimup, int maxnd, int es
   return l;
 }
 
+// LDS layout of the signed-column form: the zero element at 0, the column
+// scaled by +t, the column scaled by -t.
+struct SgnLds {
+  int colp, coln, total;
+};
+__device__ __host__ inline SgnLds sgn_lds(int dimup, int esz) {
+  SgnLds l;
+  l.colp = align16(esz);
+  l.coln = l.colp + align16(dimup * esz);
+  l.total = l.coln + align16(dimup * esz);
+  return l;
+}
+
 // Row records and 4-hop packing of the ELL table for the buffer-load H x v,
 // each hop as the two LDS byte addresses its gather reads (element of the
 // staged column, signed amplitude) for elements of esz bytes and namp
 // amplitudes; padding entries read row 0 and the zero amplitude.  lds_form
-// 0 (addresses would not fit the entry): entries left 0, never read.
+// 0 (addresses would not fit the entry): entries left 0, never read;
+// lds_form 2: signed-column entries (sgn_lds; every term's amplitude is the
+// same real t, so amplitude 2k is +t and 2k + 1 is -t), padding entries the
+// zero element.
 __global__ void k_pack_rows(int dim, int maxnnz, const uint32_t *map, const double *energy,
                             const int32_t *cnt, const uint32_t *ent, RowMeta *meta,
                             uint32_t *ent4, int esz, int namp, int lds_form) {
@@ -132,7 +148,14 @@ __global__ void k_pack_rows(int dim, int maxnnz, const uint32_t *map, const doub
     for (int t = 0; t < 4; ++t) {
       const int q = 4 * c + t;
       uint32_t w = 0u;
-      if (lds_form) {
+      if (lds_form == 2) {
+        if (q < m.cnt) {
+          const SgnLds sl = sgn_lds(dim, esz);
+          const uint32_t e = ent[(int64_t)q * dim + i];
+          w = (uint32_t)(((e >> kColBits) & 1u) ? sl.coln : sl.colp) +
+              (e & kColMask) * (uint32_t)esz;
+        }
+      } else if (lds_form) {
         w = pad;
         if (q < m.cnt) {
           const uint32_t e = ent[(int64_t)q * dim + i];
@@ -822,7 +845,11 @@ __global__ __launch_bounds__(BS) void k_hxv_plane(const HxvArgs a) {
 // up-spin gathers and the diagonal follow the barrier.  On a latency-bound
 // launch (one round of workgroups) the column staging no longer adds to
 // the critical path.
-template <typename R, bool REAL_AMP, int BS, int NR>
+//
+// SGN: the signed-column form (one real hopping amplitude t; the table is
+// DevSpin::ent4s): the column is staged twice, scaled by +t and by -t, and
+// each up-spin hop is one LDS load and two adds -- no amplitude gather.
+template <typename R, bool REAL_AMP, int BS, int NR, bool SGN = false>
 __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
   using C = cplx<R>;
   constexpr int ESZ = sizeof(C);
@@ -830,11 +857,14 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
   // entry ring depth: one chunk for two complex128 rows, so that the
   // gathers fit 64 VGPRs (8 wavefronts per SIMD: the 3x4 sector's 924
   // workgroups of 512 lanes in one round)
-  constexpr int PF = (NR == 2 && ESZ == 16) ? 1 : 2;
+  constexpr int PF = (NR == 2 && ESZ == 16 && !SGN) ? 1 : 2;
+  static_assert(!SGN || REAL_AMP, "the signed-column form needs a real amplitude");
   extern __shared__ __align__(16) unsigned char smem[];
   const BufLds lds = buf_lds(a.namp_up, a.dimup, 0, ESZ);
+  const SgnLds sl = sgn_lds(a.dimup, ESZ);
   C *s_amp_up = reinterpret_cast<C *>(smem);
-  C *s_col = reinterpret_cast<C *>(smem + lds.col);
+  C *s_col = reinterpret_cast<C *>(smem + (SGN ? sl.colp : lds.col));
+  C *s_coln = reinterpret_cast<C *>(smem + sl.coln);
 
   const int dimup = a.dimup;
   const WgPos wp = wg_pos(a);
@@ -875,7 +905,8 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
       m[j] = a.meta_up[row[j]];
     }
     C amp = C{(R)0, (R)0};
-    if ((int)threadIdx.x < a.namp_up) amp = g_amp_up[threadIdx.x];
+    if (SGN) amp = g_amp_up[0];  // +t
+    else if ((int)threadIdx.x < a.namp_up) amp = g_amp_up[threadIdx.x];
     R ar[NR], ai[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) ar[j] = ai[j] = (R)0;
@@ -917,11 +948,22 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
       const double ed = a.skip_diag ? 0.0 : m[j].e + edw + cross_energy(a.cross, m[j].s, sdw);
       ar[j] = fma((R)ed, x[j].x, ar[j]);
       ai[j] = fma((R)ed, x[j].y, ai[j]);
-      if (in[j]) s_col[row[j]] = x[j];
+      if constexpr (SGN) {
+        if (in[j]) {
+          s_col[row[j]] = C{amp.x * x[j].x, amp.x * x[j].y};
+          s_coln[row[j]] = C{-amp.x * x[j].x, -amp.x * x[j].y};
+        }
+      } else {
+        if (in[j]) s_col[row[j]] = x[j];
+      }
     }
-    if ((int)threadIdx.x < a.namp_up) s_amp_up[threadIdx.x] = amp;
-    for (int t = (int)threadIdx.x + BS; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
-    if (threadIdx.x == 0) s_amp_up[a.namp_up] = C{(R)0, (R)0};  // padding hops
+    if constexpr (SGN) {
+      if (threadIdx.x == 0) s_amp_up[0] = C{(R)0, (R)0};  // padding hops
+    } else {
+      if ((int)threadIdx.x < a.namp_up) s_amp_up[threadIdx.x] = amp;
+      for (int t = (int)threadIdx.x + BS; t < a.namp_up; t += BS) s_amp_up[t] = g_amp_up[t];
+      if (threadIdx.x == 0) s_amp_up[a.namp_up] = C{(R)0, (R)0};  // padding hops
+    }
     __syncthreads();
     // up-spin hops: LDS gathers, entries through a ring of PF chunks
     int nc_w = 0;
@@ -944,6 +986,17 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
               const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
+              if constexpr (SGN) {
+                C y[4];
+#pragma unroll
+                for (int t = 0; t < 4; ++t) y[t] = lds_load<R>(ee[t]);
+#pragma unroll
+                for (int t = 0; t < 4; ++t) {
+                  ar[j] += y[t].x;
+                  ai[j] += y[t].y;
+                }
+                continue;
+              }
               C y[4], h[4];
 #pragma unroll
               for (int t = 0; t < 4; ++t) {
@@ -973,7 +1026,7 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
         out.y = ai[j] * scale;
         hv[row[j]] = out;
         if (lz) {
-          const C xj = s_col[row[j]];
+          const C xj = SGN ? vcol[row[j]] : s_col[row[j]];
           dot += (double)(xj.x * scale) * (double)out.x + (double)(xj.y * scale) * (double)out.y;
         }
       }
@@ -2108,6 +2161,17 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
                      sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4, esz,
                      namp, sp.lds_form ? 1 : 0);
   VCA_HIP(hipGetLastError());
+  // signed-column addresses (every hopping term the same real amplitude t)
+  bool one_amp = sp.real_amp && sp.nterms >= 1;
+  for (int t = 0; t < sp.nterms && one_amp; ++t)
+    one_amp = amp64[2 * t].x == amp64[0].x && amp64[2 * t].y == 0.0;
+  sp.sgn_form = one_amp && (size_t)sgn_lds(sp.dim, esz).total <= kLdsAddrMask + 1;
+  if (sp.sgn_form) {
+    VCA_HIP(hipMalloc(&sp.ent4s, sizeof(uint32_t) * 4 * (size_t)nc * sp.dim));
+    hipLaunchKernelGGL(k_pack_rows, dim3((sp.dim + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
+                       sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4s,
+                       esz, namp, 2);
+  }
   // one-plane addresses (real amplitudes only)
   const int rsz = esz / 2;
   sp.plane_form = sp.real_amp && (size_t)(namp + 1) * rsz <= kLdsAmpLimit &&
@@ -2150,6 +2214,7 @@ struct HxvShape {
   bool tform;  // transposed form (gather passes on v and v^T, k_transpose)
   bool plane;  // its gather passes one real plane at a time (k_hxv_plane)
   bool ov;     // overlapped short-column form (k_hxv_ov)
+  bool sgn;    // ... with the signed-column up-spin table
   int bs, kr, u, cpb, grid;
 };
 
@@ -2163,7 +2228,8 @@ static const int kVariants[][4] = {
     {512, 2, 8, 0}, {256, 1, 16, 0}, {512, 1, 16, 0}, {1024, 1, 16, 0}, {256, 4, 4, 1},
     {256, 2, 8, 1}, {512, 2, 8, 1},  {512, 1, 8, 1},  {1024, 1, 8, 1}, {1024, 2, 8, 1},
     {512, 1, 8, 2}, {1024, 1, 8, 2}, {512, 2, 8, 2},  {1024, 2, 8, 2}, {512, 1, 8, 3},
-    {1024, 1, 8, 3}, {512, 2, 4, 4}, {1024, 1, 8, 4}, {256, 2, 4, 4}};
+    {1024, 1, 8, 3}, {512, 2, 4, 4}, {1024, 1, 8, 4}, {256, 2, 4, 4}, {512, 2, 4, 5},
+    {1024, 1, 8, 5}, {256, 2, 4, 5}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 // The transposed form pays 9 vector lengths of HBM traffic per product
@@ -2208,6 +2274,13 @@ bool hxv_variant_ok(const Sector &sec, int v) {
            (size_t)buf_lds(2 * sec.up.nterms, sec.up.dim, 0, (int)esz).total <= kLdsBig &&
            (uint64_t)sec.dim * esz < (1ull << 31);
   }
+  if (form == 5) {  // signed-column overlapped form
+    const size_t esz = hxv_esz(sec);
+    return sec.up.sgn_form && sec.up.real_amp && sec.dw.real_amp &&
+           sec.up.dim <= kVariants[v][0] * kVariants[v][1] &&
+           (size_t)sgn_lds(sec.up.dim, (int)esz).total <= kLdsBig &&
+           (uint64_t)sec.dim * esz < (1ull << 31);
+  }
   return form < 2 || tform_fits(sec, form == 3);
 }
 
@@ -2223,7 +2296,8 @@ static HxvShape hxv_shape(const Sector &sec) {
     h.buf = kVariants[v][3] != 0;
     h.tform = kVariants[v][3] == 2 || kVariants[v][3] == 3;
     h.plane = kVariants[v][3] == 3;
-    h.ov = kVariants[v][3] == 4;
+    h.ov = kVariants[v][3] == 4 || kVariants[v][3] == 5;
+    h.sgn = kVariants[v][3] == 5;
     h.cpb = 1;
     h.grid = sec.dw.dim;
   } else {
@@ -2378,10 +2452,10 @@ static void launch_buf_s(Sector &sec, const HxvArgs &a, const HxvShape &h, dim3 
   else launch_buf<R, REAL_AMP, 1024, 2>(sec, a, grid, smem);
 }
 
-template <typename R, bool REAL_AMP, int BS, int NR>
+template <typename R, bool REAL_AMP, int BS, int NR, bool SGN = false>
 static void launch_ov(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
   static std::once_flag attr_once;
-  auto fn = k_hxv_ov<R, REAL_AMP, BS, NR>;
+  auto fn = k_hxv_ov<R, REAL_AMP, BS, NR, SGN>;
   std::call_once(attr_once, [&]() {
     VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
@@ -2395,6 +2469,17 @@ static void launch_ov(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
 
 template <typename R, bool REAL_AMP>
 static void launch_ov_s(Sector &sec, const HxvArgs &a, const HxvShape &h, dim3 grid) {
+  if constexpr (REAL_AMP) {
+    if (h.sgn) {
+      HxvArgs b = a;
+      b.ent4_up = sec.up.ent4s;
+      const size_t ss = (size_t)sgn_lds(a.dimup, (int)sizeof(cplx<R>)).total;
+      if (h.bs == 1024) launch_ov<R, true, 1024, 1, true>(sec, b, grid, ss);
+      else if (h.bs == 256) launch_ov<R, true, 256, 2, true>(sec, b, grid, ss);
+      else launch_ov<R, true, 512, 2, true>(sec, b, grid, ss);
+      return;
+    }
+  }
   const size_t smem =
       (size_t)buf_lds(a.namp_up, a.dimup, 0, (int)sizeof(cplx<R>)).total;
   if (h.bs == 1024) launch_ov<R, REAL_AMP, 1024, 1>(sec, a, grid, smem);

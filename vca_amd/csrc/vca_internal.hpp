@@ -205,6 +205,12 @@ struct DevSpin {
   // amplitudes of sizeof(real) bytes), read by the plane kernel of the
   // transposed H x v on spins with real amplitudes; plane_form: usable.
   uint32_t *ent4p = nullptr;
+  // Signed-column entries (every hopping term the same real amplitude t, so
+  // a hop's amplitude is +t or -t): each hop is the LDS address of its element in a staged copy
+  // of the column scaled by +t or by -t, padding hops the zero at address 0.
+  // sgn_form: built (the two copies fit the address bits).
+  uint32_t *ent4s = nullptr;
+  bool sgn_form = false;
   bool plane_form = false;
   double2 *amp64 = nullptr;  // [2*nterms]: +amp, -amp
   float2 *amp32 = nullptr;
