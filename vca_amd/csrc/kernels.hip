@@ -914,7 +914,48 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
     // uniform loads of the column's hop list
     const int nd = a.skip_dw ? 0 : a.cnt_dw[idw];
     const uint32_t *entd = a.ent_dw + idw;
-    for (int q0 = 0; q0 < nd; q0 += U) {
+    // signed-column complex64: two stages of U/2 neighbour columns, one
+    // stage's loads issued before the other is summed (as many loads in
+    // flight, no gaps; in complex128 the second stage would not fit 64
+    // VGPRs)
+    constexpr bool kStaged = SGN && ESZ == 8;
+    if constexpr (kStaged) {
+      constexpr int UH = U / 2;
+      C ya[UH][NR], yb[UH][NR], ha[UH], hb[UH];
+      auto load_stage = [&](C(&y)[UH][NR], C(&h)[UH], int q0) {
+#pragma unroll
+        for (int u = 0; u < UH; ++u) {
+          const uint32_t e = entd[(int64_t)min(q0 + u, nd - 1) * a.dimdw];
+          const uint32_t so = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
+          h[u] = g_amp_dw[e >> kColBits];
+#pragma unroll
+          for (int j = 0; j < NR; ++j) {
+            if constexpr (ESZ == 16)
+              y[u][j] = __builtin_bit_cast(
+                  C, __builtin_amdgcn_raw_buffer_load_b128(rv, row[j] * ESZ, so, 0));
+            else
+              y[u][j] = __builtin_bit_cast(
+                  C, __builtin_amdgcn_raw_buffer_load_b64(rv, row[j] * ESZ, so, 0));
+          }
+        }
+      };
+      auto sum_stage = [&](const C(&y)[UH][NR], const C(&h)[UH], int q0) {
+#pragma unroll
+        for (int u = 0; u < UH; ++u)
+          if (q0 + u < nd) {
+#pragma unroll
+            for (int j = 0; j < NR; ++j) cmac<R, REAL_AMP>(ar[j], ai[j], h[u], y[u][j]);
+          }
+      };
+      if (nd > 0) load_stage(ya, ha, 0);
+      for (int q0 = 0; q0 < nd; q0 += 2 * UH) {
+        if (q0 + UH < nd) load_stage(yb, hb, q0 + UH);
+        sum_stage(ya, ha, q0);
+        if (q0 + 2 * UH < nd) load_stage(ya, ha, q0 + 2 * UH);
+        if (q0 + UH < nd) sum_stage(yb, hb, q0 + UH);
+      }
+    }
+    for (int q0 = 0; q0 < (kStaged ? 0 : nd); q0 += U) {
       C y[U][NR], h[U];
 #pragma unroll
       for (int u = 0; u < U; ++u) {
