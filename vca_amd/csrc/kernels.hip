@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <mutex>
+#include <type_traits>
 
 #include "vca_internal.hpp"
 
@@ -129,8 +130,8 @@ __device__ __host__ inline SgnLds sgn_lds(int dimup, int esz) {
 // amplitudes; padding entries read row 0 and the zero amplitude.  lds_form
 // 0 (addresses would not fit the entry): entries left 0, never read;
 // lds_form 2: signed-column entries (sgn_lds; every term's amplitude is the
-// same real t, so amplitude 2k is +t and 2k + 1 is -t), padding entries the
-// zero element.
+// same real t, so amplitude 2k is +t and 2k + 1 is -t) as 16-bit indices of
+// esz-byte elements, 4 per 8 bytes; padding entries index the zero element.
 __global__ void k_pack_rows(int dim, int maxnnz, const uint32_t *map, const double *energy,
                             const int32_t *cnt, const uint32_t *ent, RowMeta *meta,
                             uint32_t *ent4, int esz, int namp, int lds_form) {
@@ -148,13 +149,15 @@ __global__ void k_pack_rows(int dim, int maxnnz, const uint32_t *map, const doub
     for (int t = 0; t < 4; ++t) {
       const int q = 4 * c + t;
       uint32_t w = 0u;
-      if (lds_form == 2) {
+      if (lds_form == 2) {  // 16-bit element indices, 4 to 8 bytes
+        uint32_t idx = 0u;
         if (q < m.cnt) {
           const SgnLds sl = sgn_lds(dim, esz);
           const uint32_t e = ent[(int64_t)q * dim + i];
-          w = (uint32_t)(((e >> kColBits) & 1u) ? sl.coln : sl.colp) +
-              (e & kColMask) * (uint32_t)esz;
+          idx = (uint32_t)((((e >> kColBits) & 1u) ? sl.coln : sl.colp) / esz) + (e & kColMask);
         }
+        reinterpret_cast<uint16_t *>(ent4)[((int64_t)c * dim + i) * 4 + t] = (uint16_t)idx;
+        continue;
       } else if (lds_form) {
         w = pad;
         if (q < m.cnt) {
@@ -1013,9 +1016,11 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
     for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
     nc_w = __builtin_amdgcn_readfirstlane(nc_w);
     const int last = a.up_nc - 1;
-    const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
+    // signed-column entries: four 16-bit element indices per 8 bytes
+    using EntT = std::conditional_t<SGN, uint2, uint4>;
+    const EntT *ent4 = reinterpret_cast<const EntT *>(a.ent4_up);
     if (nc_w > 0) {
-      uint4 e4[PF][NR];
+      EntT e4[PF][NR];
 #pragma unroll
       for (int p = 0; p < PF; ++p)
 #pragma unroll
@@ -1026,31 +1031,33 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
           if (c0 + p < nc_w) {
 #pragma unroll
             for (int j = 0; j < NR; ++j) {
-              const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
               if constexpr (SGN) {
+                const uint32_t ee[4] = {e4[p][j].x & 0xffffu, e4[p][j].x >> 16,
+                                        e4[p][j].y & 0xffffu, e4[p][j].y >> 16};
                 C y[4];
 #pragma unroll
-                for (int t = 0; t < 4; ++t) y[t] = lds_load<R>(ee[t]);
+                for (int t = 0; t < 4; ++t) y[t] = lds_load<R>(ee[t] * (uint32_t)ESZ);
 #pragma unroll
                 for (int t = 0; t < 4; ++t) {
                   ar[j] += y[t].x;
                   ai[j] += y[t].y;
                 }
-                continue;
-              }
-              C y[4], h[4];
+              } else {
+                const uint32_t ee[4] = {e4[p][j].x, e4[p][j].y, e4[p][j].z, e4[p][j].w};
+                C y[4], h[4];
 #pragma unroll
-              for (int t = 0; t < 4; ++t) {
-                y[t] = lds_load<R>(ee[t] & kLdsAddrMask);
-                if constexpr (REAL_AMP) {
-                  h[t].x = lds_load_re<R>(ee[t] >> kLdsAddrBits);
-                  h[t].y = (R)0;
-                } else {
-                  h[t] = lds_load<R>(ee[t] >> kLdsAddrBits);
+                for (int t = 0; t < 4; ++t) {
+                  y[t] = lds_load<R>(ee[t] & kLdsAddrMask);
+                  if constexpr (REAL_AMP) {
+                    h[t].x = lds_load_re<R>(ee[t] >> kLdsAddrBits);
+                    h[t].y = (R)0;
+                  } else {
+                    h[t] = lds_load<R>(ee[t] >> kLdsAddrBits);
+                  }
                 }
-              }
 #pragma unroll
-              for (int t = 0; t < 4; ++t) cmac<R, REAL_AMP>(ar[j], ai[j], h[t], y[t]);
+                for (int t = 0; t < 4; ++t) cmac<R, REAL_AMP>(ar[j], ai[j], h[t], y[t]);
+              }
             }
           }
           const int cn = min(c0 + p + PF, last);
@@ -2206,9 +2213,9 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
   bool one_amp = sp.real_amp && sp.nterms >= 1;
   for (int t = 0; t < sp.nterms && one_amp; ++t)
     one_amp = amp64[2 * t].x == amp64[0].x && amp64[2 * t].y == 0.0;
-  sp.sgn_form = one_amp && (size_t)sgn_lds(sp.dim, esz).total <= kLdsAddrMask + 1;
+  sp.sgn_form = one_amp && (size_t)sgn_lds(sp.dim, esz).total / esz <= 0xffff;
   if (sp.sgn_form) {
-    VCA_HIP(hipMalloc(&sp.ent4s, sizeof(uint32_t) * 4 * (size_t)nc * sp.dim));
+    VCA_HIP(hipMalloc(&sp.ent4s, sizeof(uint16_t) * 4 * (size_t)nc * sp.dim));
     hipLaunchKernelGGL(k_pack_rows, dim3((sp.dim + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
                        sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4s,
                        esz, namp, 2);
