@@ -1833,9 +1833,22 @@ struct UpdArgs {
   int64_t vstride;
 };
 
-template <typename R>
+// NT: nontemporal loads and stores (batches far larger than the 256 MB MALL,
+// whose vectors are not reread from it).
+template <bool NT, typename V>
+__device__ inline V ld_stream(const V *p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename V>
+__device__ inline void st_stream(V *p, V v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+template <typename R, bool NT = false>
 __global__ __launch_bounds__(kBlock) void k_lanczos_update(const UpdArgs a) {
-  using C = cplx<R>;
+  using C = typename Vec2<R>::T;
   const int64_t off = (int64_t)blockIdx.y * a.vstride;
   LanczosDev *lz = a.lz + blockIdx.y;
   const int k = lz->k;
@@ -1851,17 +1864,17 @@ __global__ __launch_bounds__(kBlock) void k_lanczos_update(const UpdArgs a) {
     const R cy = x ? (R)(lz->y[k] / lz->nrm[k]) : (R)0;
     const int64_t stride = (int64_t)gridDim.x * kBlock;
     for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < a.n; i += stride) {
-      const C u = uk[i], h = hv[i];
+      const C u = ld_stream<NT>(uk + i), h = ld_stream<NT>(hv + i);
       C w;
       if (k > 0) {
-        const C p = up[i];
+        const C p = ld_stream<NT>(up + i);
         w.x = h.x - ca * u.x - cb * p.x;
         w.y = h.y - ca * u.y - cb * p.y;
       } else {
         w.x = h.x - ca * u.x;
         w.y = h.y - ca * u.y;
       }
-      up[i] = w;
+      st_stream<NT>(up + i, w);
       ss += (double)w.x * (double)w.x + (double)w.y * (double)w.y;
       if (x) {
         C xv = x[i];
@@ -3035,7 +3048,19 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
   const int grid = grid_for(a.n);
   ensure_partials(sec, grid * nb);
   a.partials = sec.partials;
-  if (sec.precision == VCA_FP64)
+  // nontemporal from 256 MB of vectors per stream (the MALL's size) up;
+  // VCA_UPDATE_NT=0/1 forces the choice
+  const char *nt_env = std::getenv("VCA_UPDATE_NT");
+  const bool nt = nt_env && *nt_env ? nt_env[0] == '1'
+                                    : (size_t)nb * sec.vec_bytes() >= ((size_t)256 << 20);
+  if (nt) {
+    if (sec.precision == VCA_FP64)
+      hipLaunchKernelGGL((k_lanczos_update<double, true>), dim3(grid, nb), dim3(kBlock), 0,
+                         sec.stream, a);
+    else
+      hipLaunchKernelGGL((k_lanczos_update<float, true>), dim3(grid, nb), dim3(kBlock), 0,
+                         sec.stream, a);
+  } else if (sec.precision == VCA_FP64)
     hipLaunchKernelGGL(k_lanczos_update<double>, dim3(grid, nb), dim3(kBlock), 0, sec.stream, a);
   else
     hipLaunchKernelGGL(k_lanczos_update<float>, dim3(grid, nb), dim3(kBlock), 0, sec.stream, a);
