@@ -197,6 +197,7 @@ struct HxvArgs {
   // form (skip_dw, skip_diag) share one row of partials per chain
   int64_t pbase, pstride;
   int skip_dw, skip_diag;
+  int nt_store;  // Hv written with nontemporal stores (batches beyond the MALL)
 };
 
 __device__ inline double cross_energy(const CrossTerm &c, uint32_t su, uint32_t sd) {
@@ -1072,7 +1073,11 @@ __global__ __launch_bounds__(BS) void k_hxv_ov(const HxvArgs a) {
         C out;
         out.x = ar[j] * scale;
         out.y = ai[j] * scale;
-        hv[row[j]] = out;
+        if (a.nt_store)
+          __builtin_nontemporal_store(__builtin_bit_cast(typename Vec2<R>::T, out),
+                                      reinterpret_cast<typename Vec2<R>::T *>(hv) + row[j]);
+        else
+          hv[row[j]] = out;
         if (lz) {
           const C xj = SGN ? vcol[row[j]] : s_col[row[j]];
           dot += (double)(xj.x * scale) * (double)out.x + (double)(xj.y * scale) * (double)out.y;
@@ -1249,14 +1254,16 @@ __global__ __launch_bounds__(BS) void k_hxv_ov_real(const HxvArgs a) {
           V2 out;
           out.x = ar[2 * j] * scale;
           out.y = ar[2 * j + 1] * scale;
-          reinterpret_cast<V2 *>(hv)[slot[j]] = out;
+          if (a.nt_store) __builtin_nontemporal_store(out, reinterpret_cast<V2 *>(hv) + slot[j]);
+          else reinterpret_cast<V2 *>(hv)[slot[j]] = out;
           if (lz) {
             const V2 xv = *reinterpret_cast<const V2 *>(s_col + 2 * slot[j]);
             dot += (double)(xv.x * scale) * (double)out.x + (double)(xv.y * scale) * (double)out.y;
           }
         } else {
           const R out = ar[j] * scale;
-          hv[slot[j]] = out;
+          if (a.nt_store) __builtin_nontemporal_store(out, hv + slot[j]);
+          else hv[slot[j]] = out;
           if (lz) dot += (double)(s_col[slot[j]] * scale) * (double)out;
         }
       }
@@ -2887,6 +2894,12 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   a.v0 = v0;
   a.v1 = v1;
   a.hv = hv;
+  {
+    const char *nt_env = std::getenv("VCA_NT_STREAMS");
+    a.nt_store = (nt_env && *nt_env ? nt_env[0] == '1'
+                                    : (size_t)nb * sec.vec_bytes() >= ((size_t)256 << 20))
+                     ? 1 : 0;
+  }
   a.dimup = sec.up.dim;
   a.dimdw = sec.dw.dim;
   a.map_up = sec.up.map;
@@ -3049,8 +3062,8 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
   ensure_partials(sec, grid * nb);
   a.partials = sec.partials;
   // nontemporal from 256 MB of vectors per stream (the MALL's size) up;
-  // VCA_UPDATE_NT=0/1 forces the choice
-  const char *nt_env = std::getenv("VCA_UPDATE_NT");
+  // VCA_NT_STREAMS=0/1 forces the choice
+  const char *nt_env = std::getenv("VCA_NT_STREAMS");
   const bool nt = nt_env && *nt_env ? nt_env[0] == '1'
                                     : (size_t)nb * sec.vec_bytes() >= ((size_t)256 << 20);
   if (nt) {
