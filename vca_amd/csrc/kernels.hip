@@ -2580,6 +2580,17 @@ static void launch_hxv_t(Sector &sec, HxvArgs &a, bool stage, bool onecol, dim3 
   }
 }
 
+// Nontemporal loads and stores for the batched update and the overlapped
+// H x v's output from 256 MB of vectors per launch (the MALL's size) up:
+// nothing is found there again.  (The transposed form's transposes and
+// fused update ran slower with them: bath sector real H x v 8.21 -> 8.35 ms.)
+// VCA_NT_STREAMS=0/1 forces the choice.
+static bool nt_streams(const Sector &sec, int nb) {
+  const char *e = std::getenv("VCA_NT_STREAMS");
+  if (e && *e) return e[0] == '1';
+  return (size_t)nb * sec.vec_bytes() >= ((size_t)256 << 20);
+}
+
 template <typename E>
 static void launch_transpose(Sector &sec, const TrArgs &t, bool add, int nb) {
   const dim3 g((unsigned)((t.ni + kTile - 1) / kTile), (unsigned)((t.nj + kTile - 1) / kTile),
@@ -2894,12 +2905,7 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
   a.v0 = v0;
   a.v1 = v1;
   a.hv = hv;
-  {
-    const char *nt_env = std::getenv("VCA_NT_STREAMS");
-    a.nt_store = (nt_env && *nt_env ? nt_env[0] == '1'
-                                    : (size_t)nb * sec.vec_bytes() >= ((size_t)256 << 20))
-                     ? 1 : 0;
-  }
+  a.nt_store = nt_streams(sec, nb) ? 1 : 0;
   a.dimup = sec.up.dim;
   a.dimdw = sec.dw.dim;
   a.map_up = sec.up.map;
@@ -3061,12 +3067,7 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
   const int grid = grid_for(a.n);
   ensure_partials(sec, grid * nb);
   a.partials = sec.partials;
-  // nontemporal from 256 MB of vectors per stream (the MALL's size) up;
-  // VCA_NT_STREAMS=0/1 forces the choice
-  const char *nt_env = std::getenv("VCA_NT_STREAMS");
-  const bool nt = nt_env && *nt_env ? nt_env[0] == '1'
-                                    : (size_t)nb * sec.vec_bytes() >= ((size_t)256 << 20);
-  if (nt) {
+  if (nt_streams(sec, nb)) {
     if (sec.precision == VCA_FP64)
       hipLaunchKernelGGL((k_lanczos_update<double, true>), dim3(grid, nb), dim3(kBlock), 0,
                          sec.stream, a);
