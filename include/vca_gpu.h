@@ -67,8 +67,9 @@ extern "C" {
 
 /* ground-state method, reference LANC_METHOD (VCA_DIAG.f90:98-107) */
 #define VCA_LANC_LANCZOS 0 /* one state per sector                        */
-#define VCA_LANC_ARPACK 1  /* up to lanc_nstates_sector states per sector
-                              (dense sectors only in this build)           */
+#define VCA_LANC_ARPACK 1  /* up to lanc_nstates_sector states per sector:
+                              dense eigh up to lanc_dim_threshold, GPU
+                              thick-restart Lanczos (FP64) above it        */
 
 /* frequency integration of Omega */
 #define VCA_OMEGA_QUAD 0      /* adaptive Gauss-Kronrod on [0,inf), T=0     */
@@ -185,6 +186,12 @@ int vca_sector_create(vca_solver *s, int32_t nup, int32_t ndw,
 void vca_sector_destroy(vca_sector *sec);
 int vca_sector_dims(const vca_sector *sec, int64_t *dim, int32_t *dimup,
                     int32_t *dimdw);
+/* The launch shapes the autotuner picked for the sector's complex H x v
+ * and for its real-vector direct form (-1: not tuned / not used).  No
+ * counterpart in the reference; it names the kernels a measurement of
+ * this sector ran, so stored profiler counters can be matched to them. */
+int vca_sector_variant(const vca_sector *sec, int32_t *hxv_variant,
+                       int32_t *real_variant);
 /* Hv = H v, host complex128 arrays of length n = dim (state index
  * i = iup + idw*DimUp, 0-based).  In MIXED precision the product runs on
  * complex64 and is widened back. */

@@ -444,6 +444,38 @@ int64_t orc_hxv_parallel(const orc_model *m, int nup, int ndw, const zc *v,
   return bad;
 }
 
+/* Slices [first, first + count) of an nslices-way split of one H x v, as
+ * the reference's DW decomposition distributes it (VCA_HAMILTONIAN.f90:77-90):
+ * slice k applies the local and up parts to the output columns of its DW
+ * slice and the down part to the rows of its UP slice.  The slices run on
+ * OpenMP threads (one slice per thread at a time), local and up parts of
+ * every slice first, then the down parts, so that no two threads write one
+ * element at once.  hv must hold zeros where the slices write.  A bounded
+ * sample of a sector too large to apply whole on one core: count/nslices
+ * of the work of one application.  Returns the number of absent targets. */
+int64_t orc_hxv_slices(const orc_model *m, int nup, int ndw, const zc *v, zc *hv,
+                       int nslices, int first, int count, int nthreads) {
+  orc_sector s;
+  if (nslices < 1 || first < 0 || count < 1 || first + count > nslices) return -1;
+  if (sector_init(m, nup, ndw, &s)) return -1;
+  int64_t bad = 0;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1) reduction(+ : bad)
+#endif
+  for (int k = first; k < first + count; ++k)
+    bad += hxv_range(m, &s, v, hv, (int)((int64_t)s.dimdw * k / nslices),
+                     (int)((int64_t)s.dimdw * (k + 1) / nslices));
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(nthreads) schedule(static, 1) reduction(+ : bad)
+#endif
+  for (int k = first; k < first + count; ++k)
+    bad += hxv_down(m, &s, v, hv, (int)((int64_t)s.dimup * k / nslices),
+                    (int)((int64_t)s.dimup * (k + 1) / nslices));
+  (void)nthreads;
+  sector_free(&s);
+  return bad;
+}
+
 /* ------------------------------------------------------ excitation vectors */
 /* out(j) += coef * sgn * psi(i) for every state i of sector (nup, ndw) on
  * which the operator acts, following VCA_GF_NORMAL.f90:184-198 (dagger=1,

@@ -13,7 +13,9 @@ against an independent Jordan-Wigner construction; spectra against the
 Hubbard dimer, U=0 and atomic-limit closed forms and particle-hole symmetry;
 the Green's function against the U=0 resolvent and the Hubbard atom; the
 off-diagonal recombination against exact Lehmann sums; Omega against the
-U=0 SFT exactness identity; the quadrature against closed-form integrals.
+U=0 SFT exactness identity; the Matsubara-grid sum against the U=0
+cosh-product closed form with its tail beyond the last frequency summed
+exactly; the quadrature against closed-form integrals.
 Parity unpinned (third-party numerics the reference calls but does not
 ship): the exact SciFortran Lanczos conventions (start vector, stopping
 rule), ARPACK's ordering of near-degenerate pairs, QUADPACK's default
@@ -45,6 +47,7 @@ def lib():
         L.orc_binary_search.restype = ctypes.c_int
         L.orc_hxv.restype = ctypes.c_int64
         L.orc_hxv_parallel.restype = ctypes.c_int64
+        L.orc_hxv_slices.restype = ctypes.c_int64
         L.orc_spin_hops.restype = ctypes.c_int64
         L.orc_diag.restype = ctypes.c_int
         L.orc_excite.restype = ctypes.c_int
@@ -226,6 +229,20 @@ class Model:
                                 hv.ctypes.data_as(ctypes.c_void_p))
         if bad != 0:
             raise RuntimeError(f"oracle hxv: {bad} hop targets outside the sector")
+        return hv
+
+    def hxv_slices(self, nup, ndw, v, hv, nslices, first, count, threads=1):
+        """Slices [first, first+count) of an nslices-way DW decomposition of
+        one H x v (VCA_HAMILTONIAN.f90:77-90) accumulated into hv: a bounded
+        sample of the work of one application (count/nslices of it)."""
+        assert v.dtype == np.complex128 and hv.dtype == np.complex128
+        assert v.flags.c_contiguous and hv.flags.c_contiguous
+        bad = lib().orc_hxv_slices(ctypes.byref(self._c), nup, ndw,
+                                   v.ctypes.data_as(ctypes.c_void_p),
+                                   hv.ctypes.data_as(ctypes.c_void_p), nslices, first, count,
+                                   threads)
+        if bad != 0:
+            raise RuntimeError(f"oracle hxv_slices: status {bad}")
         return hv
 
     def diag(self, nup, ndw):

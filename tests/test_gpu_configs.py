@@ -1,0 +1,383 @@
+"""GPU parity on the five benchmark workloads at their stated sizes, and on
+the Omega forms they use (the Matsubara sum, the MIXED log-dets, MIXED
+solves).
+
+Bars (FP64): E0 1e-10 relative, G'(z) and the Omega integrand 1e-8 of the
+largest value, Omega 1e-8 relative, H x v 1e-12.  MIXED bars are stated per
+test.  Where the oracle cannot run at full size (the 2x5 every-sector
+solve, the 2x2 BHZ cluster) the comparison is against fixtures the oracle
+wrote (tests/golden/make_config_golden.py); at 1.66e8 states the checks are
+the oracle's hop tables, sampled rows of the oracle's H x v, and
+size-independent properties (Hermiticity, real vs complex product, FP64 vs
+MIXED, the residual of the ground state under the oracle's H).
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from oracle import vca_ref as R
+from vca_amd import configs
+from vca_amd.api import DeviceArray, VcaSolver, device_synchronize
+
+from . import _models as M
+from ._closed_forms import matsubara_u0_exact
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _golden(name):
+    with open(os.path.join(GOLDEN, f"config_{name}.json")) as f:
+        return json.load(f)
+
+
+def _cplx(a):
+    a = np.asarray(a, dtype=float)
+    return a[..., 0] + 1j * a[..., 1]
+
+
+def gpu_gfdata(s: VcaSolver, p, nstates):
+    """The GPU's pole store as an oracle GFData (poles and weights read back
+    through vca_gf_poles), so that the oracle's k-sum and log-dets can run
+    on the GPU's G' at sizes where the oracle's own chains would take hours."""
+    gd = R.GFData(M.to_oracle(p), 0.0, 1.0)
+    for ispin in range(1, p.nspin + 1):
+        for il in range(1, p.nlat + 1):
+            for jl in range(1, p.nlat + 1):
+                for io in range(1, p.norb + 1):
+                    for jo in range(1, p.norb + 1):
+                        diag = il == jl and io == jo
+                        nch = 2 if diag else (2 if p.gf_symmetric else 4)
+                        gd.elems[(il, jl, ispin, io, jo)] = [
+                            [s.gf_poles(il, jl, ispin, io, jo, st, c) for c in range(1, nch + 1)]
+                            for st in range(1, nstates + 1)]
+    return gd
+
+
+# ------------------------------------------------------------------- C1 --
+def test_chain4_as_stated():
+    """The 4-site chain exactly as the workload states it: U=4, half
+    filling, Nk=32, every sector, Omega by the Matsubara sum at beta=100 over
+    4096 frequencies, G on 512 real-axis points with eta=0.05."""
+    p, hloc, hk, _, _ = configs.config_chain4(nk=32)
+    p.omega_mode, p.omega_nfreq, p.beta = "matsubara", 4096, 100.0
+    p.lreal, p.eps = 512, 0.05
+    _say("chain4 as stated: Matsubara Omega, 512 real-axis points")
+    s = VcaSolver(p)
+    om = s.vca_solve(hloc, hk)
+    res = s.results()
+    greal = s.gf_grid("realaxis", ("g",))["g"]
+    ref = R.vca_solve(M.to_oracle(p), hloc, hk, omega_mode="matsubara", nfreq=4096)
+    g_cpu, _, _ = R.sigma_grid(ref["model"], ref["gf"], 1)
+    assert abs(res["omegaprime"] - ref["e0"]) < 1e-10 * abs(ref["e0"])
+    assert abs(res["omega_integral"] - ref["omega_integral"]) < 1e-10 * max(
+        1.0, abs(ref["omega_integral"]))
+    assert abs(om - ref["sft_potential"]) < 1e-10 * abs(ref["sft_potential"])
+    assert np.abs(greal - g_cpu).max() < 1e-8 * np.abs(g_cpu).max()
+
+
+@pytest.mark.parametrize("case", ["chain4_tprime", "bhz"])
+def test_matsubara_mode_exact_at_u0(case):
+    """The GPU's Matsubara-sum Omega at U=0 equals the cosh-product closed
+    form minus the exactly summed tail beyond the last frequency."""
+    if case == "bhz":
+        p, hloc, hk, _, _ = M.bhz_small(nk=4)
+        p.uloc = (0.0, 0.0, 0, 0, 0)
+        beta, nfreq = 20.0, 300
+    else:
+        p, hloc, hk, _, _ = M.chain(nlat=4, u=0.0, nk=16, tprime=-0.6)
+        beta, nfreq = 100.0, 4096
+    p.beta, p.omega_mode, p.omega_nfreq = beta, "matsubara", nfreq
+    for prec in ("fp64", "mixed"):
+        p.precision = prec
+        _say(f"matsubara U=0 {case} {prec}")
+        s = VcaSolver(p)
+        s.vca_solve(hloc, hk)
+        po = M.to_oracle(p)
+        exact = matsubara_u0_exact(R.nnn2lso(hloc, p.nlat, p.nspin, p.norb),
+                                   R.hk_to_lso(po, hk), p.nspin, beta, nfreq)
+        got = s.results()["omega_integral"]
+        # MIXED: the chains of these 16- to 36-state sectors run FP32 Lanczos to
+        # their full length; G' carries ~1e-6 of FP32 error into the sum
+        bar = 1e-10 if prec == "fp64" else 5e-6
+        _say(f"  integral {got:.15f}, closed form {exact:.15f}, rel {abs(got - exact) / abs(exact):.2e}")
+        assert abs(got - exact) < bar * max(1.0, abs(exact)), (prec, got, exact)
+
+
+# --------------------------------------------------------- MIXED solves --
+MIXED_CASES = {
+    "chain4": lambda: configs.config_chain4(nk=16),
+    "chain8": lambda: M.chain(nlat=8, u=4.0, nk=8),
+    "bath": lambda: M.bath_model(nlat=2, nb=2),
+    "bhz": lambda: M.bhz_small(nk=4),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MIXED_CASES))
+def test_mixed_solve_matches_oracle(name):
+    """vca_solve in the MIXED form (FP32 Lanczos with FP64 alpha/beta and an
+    FP64 refinement of the ground state, FP32 chains, FP32 log-dets) against
+    the FP64 oracle.  Bars: E0 1e-10 relative (the refinement is FP64); G'(z)
+    2e-5 of its largest element; the Omega integrand 2e-5 of its largest value
+    over the frequencies (at high frequency the integrand is the small
+    remainder of cancelling terms, so its own relative error grows there);
+    Omega 5e-6 relative.  The FP32 chains of these small sectors (16 to 4900
+    states) run to their full length, which sets the G' and Omega errors."""
+    p, hloc, hk, bh, bv = MIXED_CASES[name]()
+    p.precision = "mixed"
+    _say(f"mixed vca_solve {name}")
+    s = VcaSolver(p)
+    om = s.vca_solve(hloc, hk, bh, bv)
+    ref = R.vca_solve(M.to_oracle(p), hloc, hk, bh, bv, epsabs=1e-11, epsrel=1e-12)
+    res = s.results()
+    gz = [(s.vca_gf_cluster(z), R.gf_cluster(ref["gf"], z))
+          for z in (0.3j, 1.7j, 0.5 + 0.2j, -2.0 + 1.0j)]
+    gerr = max(np.abs(g - go).max() / np.abs(go).max() for g, go in gz)
+    hkl = R.hk_to_lso(ref["model"].p, hk)
+    zs = np.array([0.05j, 0.4j, 1.0j, 3.0j, 20.0j])
+    fo = np.array([R.sum_kmesh(ref["model"], ref["gf"], hkl, z) for z in zs])
+    f = s.omega_integrand(zs, "mixed")
+    ferr = np.max(np.abs(f - fo)) / np.abs(fo).max()
+    oerr = abs(om - ref["sft_potential"]) / abs(ref["sft_potential"])
+    e0err = abs(res["omegaprime"] - ref["e0"]) / max(1, abs(ref["e0"]))
+    _say(f"  E0 {e0err:.1e}, G' {gerr:.1e}, integrand {ferr:.1e}, Omega {oerr:.1e}")
+    assert e0err < 1e-10
+    assert gerr < 2e-5
+    assert ferr < 2e-5
+    assert oerr < 5e-6
+
+
+# ------------------------------------------------------------------- C5 --
+def test_bhz2x2_full_kgrid_fp64_and_mixed():
+    """The 2x2 BHZ workload: E0 and G' of the sector (4,4) against the
+    oracle's fixture; the Omega integrand over the full 128x128 k-grid at
+    eight Matsubara frequencies (beta=200, from the first to the 8192nd),
+    FP64 within 1e-10 and the FP32 log-dets (k_logdet<float>) within 1e-6
+    relative of the oracle's FP64 values; and the Matsubara-sum integral over
+    all 8192 frequencies, MIXED against FP64, within 1e-8 relative."""
+    gold = _golden("bhz2x2")
+    p, hloc, hk, _, _ = configs.config_bhz_2x2(nk=128, nfreq=8192, precision="fp64")
+    _say("bhz2x2 FP64 solve, 16384 k x 8192 Matsubara frequencies")
+    s = VcaSolver(p)
+    s.vca_solve(hloc, hk)
+    res = s.results()
+    assert abs(res["omegaprime"] - gold["e0"]) < 1e-10 * abs(gold["e0"])
+    for z, gz in zip(_cplx(gold["gf_z"]), gold["gf"]):
+        g = s.vca_gf_cluster(z)
+        go = _cplx(gz)
+        assert np.abs(g - go).max() < 1e-8 * np.abs(go).max(), z
+    wn = np.array([(2 * n + 1) * math.pi / p.beta for n in gold["matsubara_n"]])
+    fo = np.array(gold["integrand"])
+    f64 = s.omega_integrand(1j * wn, "fp64")
+    f32 = s.omega_integrand(1j * wn, "mixed")
+    _say(f"  FP64 vs oracle {np.max(np.abs(f64 - fo) / np.abs(fo)):.1e}, "
+         f"FP32 vs oracle {np.max(np.abs(f32 - fo) / np.abs(fo)):.1e} (per frequency)")
+    assert np.allclose(f64, fo, rtol=1e-10, atol=1e-12), (f64, fo)
+    assert np.all(np.abs(f32 - fo) <= 1e-6 * np.abs(fo)), (f32, fo)
+    zs = 1j * (2 * np.arange(8192) + 1) * math.pi / p.beta
+    i64 = s.omega_integrand(zs, "fp64").sum()
+    i32 = s.omega_integrand(zs, "mixed").sum()
+    _say(f"  Matsubara sum MIXED vs FP64 {abs(i32 - i64) / abs(i64):.2e}")
+    assert abs(i32 - i64) < 1e-8 * abs(i64), (i32, i64)
+    # the solve's own Matsubara-sum integral is the FP64 sum above
+    assert abs(res["omega_integral"] - (3 - p.nspin) * 2.0 / p.beta * i64) < 1e-10 * abs(
+        res["omega_integral"])
+
+
+def test_bhz2x2_mixed_solve():
+    """The 2x2 BHZ workload in the MIXED form end to end against the FP64
+    solve: E0 1e-10, Omega 1e-8 relative (the Matsubara sum of FP32 log-dets
+    with FP64 accumulation)."""
+    out = {}
+    for prec in ("fp64", "mixed"):
+        p, hloc, hk, _, _ = configs.config_bhz_2x2(nk=128, nfreq=8192, precision=prec)
+        _say(f"bhz2x2 solve {prec}")
+        s = VcaSolver(p)
+        out[prec] = (s.vca_solve(hloc, hk), s.results()["omegaprime"])
+    _say(f"  E0 {abs(out['mixed'][1] - out['fp64'][1]) / abs(out['fp64'][1]):.1e}, Omega "
+         f"{abs(out['mixed'][0] - out['fp64'][0]) / abs(out['fp64'][0]):.1e}")
+    assert abs(out["mixed"][1] - out["fp64"][1]) < 1e-10 * abs(out["fp64"][1])
+    assert abs(out["mixed"][0] - out["fp64"][0]) < 1e-8 * abs(out["fp64"][0])
+
+
+# ------------------------------------------------------------------- C3 --
+def test_2x5_every_sector_matches_fixture():
+    """The 2x5 every-sector workload (121 sectors, up to 6 eigenpairs each,
+    760 chains batched per target sector, 16 concurrent sector workers)
+    against the oracle's fixture: the lowest eigenvalues of every sector
+    above the dense threshold, E0, the kept states and zeta, G'(z), the Omega
+    integrand and Omega."""
+    gold = _golden("2x5")
+    p, hloc, hk, _, _ = configs.config_batched_2x5(nk=8)
+    _say("2x5 every-sector solve")
+    s = VcaSolver(p)
+    om = s.vca_solve(hloc, hk)
+    res = s.results()
+    assert abs(res["omegaprime"] - gold["e0"]) < 1e-10 * abs(gold["e0"])
+    assert res["nstates"] == len(gold["states"])
+    assert abs(res["zeta"] - gold["zeta"]) < 1e-12
+    for z, gz in zip(_cplx(gold["gf_z"]), gold["gf"]):
+        g = s.vca_gf_cluster(z)
+        go = _cplx(gz)
+        assert np.abs(g - go).max() < 1e-8 * np.abs(go).max(), z
+    f = s.omega_integrand(_cplx(gold["integrand_z"]))
+    assert np.allclose(f, gold["integrand"], rtol=1e-8, atol=1e-10)
+    assert abs(om - gold["sft_potential"]) < 1e-8 * abs(gold["sft_potential"])
+    # every sector's eigenvalues (thick-restart Lanczos, ARPACK semantics)
+    _say("2x5 per-sector eigenvalues")
+    s2 = VcaSolver(p)
+    s2.vca_set_hcluster(hloc)
+    nchecked = 0
+    for key, ev in gold["sector_evals"].items():
+        nup, ndw = map(int, key.split(","))
+        if math.comb(10, nup) * math.comb(10, ndw) <= p.lanc_dim_threshold:
+            continue
+        sec = s2.sector(nup, ndw)
+        e, _, nconv = sec.lanczos_eigs(len(ev), ncv=min(sec.dim, 3 * 6 + 5), want_vecs=False)
+        assert nconv >= len(ev)
+        assert np.allclose(e, ev, rtol=1e-10, atol=1e-10), (key, e, ev)
+        sec.close()
+        nchecked += 1
+    assert nchecked > 90
+
+
+# ------------------------------------------------------------------- C2 --
+def test_square3x4_full_size():
+    """The 3x4 workload at its size (853,776 states, 300-step chains on the
+    731,808-state N+-1 sectors, 64x64 k-grid): the ground state under the
+    oracle's H x v (Rayleigh quotient and residual), the first 30 Lanczos
+    coefficients of a c^+_1 chain against the oracle's recurrence, and the
+    Omega integrand of the solve against the oracle's k-sum and log-dets on
+    the solve's own G'."""
+    p, hloc, hk, _, _ = configs.config_square_3x4(nk=64)
+    po = M.to_oracle(p)
+    om = R.Model(po, hloc)
+    workers = min(16, len(os.sched_getaffinity(0)))
+    _say("square3x4 full solve")
+    s = VcaSolver(p)
+    s.vca_solve(hloc, hk)
+    res = s.results()
+    # ground state: same E0 from the operator-level Lanczos, checked with the
+    # oracle's H
+    sec = s.sector(6, 6)
+    e0, psi, _ = sec.lanczos_gs(tol=1e-12, niter=512)
+    assert abs(e0 - res["omegaprime"]) < 1e-12 * abs(e0)
+    hpsi = om.hxv(6, 6, psi, workers=workers)
+    rq = np.vdot(psi, hpsi).real / np.vdot(psi, psi).real
+    assert abs(rq - e0) < 1e-10 * abs(e0), (rq, e0)
+    assert np.linalg.norm(hpsi - e0 * psi) < 1e-5
+    sec.close()
+    # a chain's first steps: c^+_1 |psi> in (7,6)
+    _say("square3x4 chain alpha/beta vs oracle")
+    v = np.zeros(math.comb(12, 7) * math.comb(12, 6), np.complex128)
+    om.excite(6, 6, psi, 1, 1, True, 1.0, v)
+    v /= np.linalg.norm(v)
+    sj = s.sector(7, 6)
+    a, b = sj.lanczos_tridiag(v, 30)
+    ao, bo = R.lanczos_tridiag(lambda x: om.hxv(7, 6, x, workers=workers), v, 30)
+    assert np.allclose(a, ao, rtol=1e-10, atol=1e-10) and np.allclose(b, bo, rtol=1e-10,
+                                                                      atol=1e-10)
+    sj.close()
+    # Omega integrand on the 4096 k-points from the solve's G'
+    _say("square3x4 integrand vs oracle k-sum")
+    gd = gpu_gfdata(s, p, res["nstates"])
+    hkl = R.hk_to_lso(po, hk)
+    zs = np.array([0.05j, 0.7j, 4.0j])
+    fo = np.array([R.sum_kmesh(om, gd, hkl, z) for z in zs])
+    assert np.allclose(s.omega_integrand(zs), fo, rtol=1e-8, atol=1e-10)
+    assert np.allclose(s.omega_integrand(zs, "mixed"), fo, rtol=2e-5, atol=1e-7)
+
+
+# ------------------------------------------------------------------- C4 --
+def _sampled_rows(om, nup, ndw, v, rows, diag):
+    """(H v)[rows] from the oracle's one-spin hop tables and diagonal:
+    H = diag + 1 (x) H_up + H_dw (x) 1, state i = iup + idw DimUp."""
+    du, dd = om.dims(nup, ndw)
+    iup, idw = rows % du, rows // du
+    out = diag[rows] * v[rows]
+    for spin, npart, own, other, stride_own, stride_other in (
+            (1, nup, iup, idw, 1, du), (2, ndw, idw, iup, du, 1)):
+        r, c, a = om.spin_hops(spin, npart)
+        order = np.argsort(r, kind="stable")
+        r, c, a = r[order], c[order], a[order]
+        n = du if spin == 1 else dd
+        ptr = np.searchsorted(r, np.arange(n + 1))
+        cnt = ptr[own + 1] - ptr[own]
+        rep = np.repeat(np.arange(len(rows)), cnt)
+        pos = np.repeat(ptr[own], cnt) + (np.arange(cnt.sum()) - np.repeat(np.cumsum(cnt) - cnt,
+                                                                               cnt))
+        src = c[pos].astype(np.int64) * stride_own + other[rep].astype(np.int64) * stride_other
+        np.add.at(out, rep, a[pos] * v[src])
+    return out
+
+
+def test_bath16_full_sector():
+    """The 8-site + 8-bath-level sector (8,8), 165,636,900 states (vectors of
+    2.65 GB): hop tables bit-exact against the oracle's; 100,000 sampled rows
+    of the FP64 product against the oracle's matrix elements (1e-12);
+    Hermiticity <w|Hv> = <Hw|v> (1e-12 of |w||Hv|); the real-vector product
+    equal to the complex one on a real vector (1e-14); MIXED within 2e-6 of
+    FP64; the ground state FP64 against MIXED (E0 within 1e-10) and under the
+    oracle's H on the sampled rows (residual)."""
+    p, hloc, hk, bh, bv = configs.config_bath_2x4(nk=2, precision="fp64")
+    om = M.oracle_model(p, hloc, bh, bv)
+    s = VcaSolver(p)
+    s.vca_set_bath(bh, bv)
+    s.vca_set_hcluster(hloc)
+    _say("bath16: hop tables")
+    sec = s.sector(8, 8)
+    assert sec.dim == 165636900
+    for spin in (1, 2):
+        g = sorted(zip(*[x.tolist() for x in sec.spin_coo(spin)]))
+        o = sorted(zip(*[x.tolist() for x in om.spin_hops(spin, 8)]))
+        assert g == o
+    rng = np.random.default_rng(16)
+    rows = rng.choice(sec.dim, 100_000, replace=False)
+    _say("bath16: diagonal (oracle)")
+    diag = om.diag(8, 8)
+    _say("bath16: sampled rows of the FP64 product")
+    v = rng.standard_normal(sec.dim) + 1j * rng.standard_normal(sec.dim)
+    hv = sec.hxv(v)
+    ref = _sampled_rows(om, 8, 8, v, rows, diag)
+    assert np.linalg.norm(hv[rows] - ref) < 1e-12 * np.linalg.norm(ref)
+    _say("bath16: Hermiticity")
+    w = rng.standard_normal(sec.dim) + 1j * rng.standard_normal(sec.dim)
+    hw = sec.hxv(w)
+    lhs, rhs = np.vdot(w, hv), np.vdot(hw, v)
+    assert abs(lhs - rhs) < 1e-12 * np.linalg.norm(w) * np.linalg.norm(hv)
+    del w, hw
+    _say("bath16: real-vector product")
+    x = rng.standard_normal(sec.dim)
+    hx = sec.hxv(x.astype(np.complex128))
+    dx = DeviceArray(x.shape, np.float64).upload(x)
+    dhx = DeviceArray(x.shape, np.float64)
+    sec.hxv_device_real(dx.ptr, dhx.ptr)
+    device_synchronize()
+    hxr = dhx.download()
+    dx.free()
+    dhx.free()
+    assert np.linalg.norm(hx.real - hxr) < 1e-14 * np.linalg.norm(hxr)
+    assert np.abs(hx.imag).max() == 0.0
+    del x, hx, hxr
+    _say("bath16: MIXED product")
+    sec32 = s.sector(8, 8, "mixed")
+    hv32 = sec32.hxv(v)
+    assert np.linalg.norm(hv32 - hv) < 2e-6 * np.linalg.norm(hv)
+    del hv32, v, hv
+    _say("bath16: ground state FP64 and MIXED")
+    e64, psi, _ = sec.lanczos_gs(tol=1e-12, niter=512)
+    e32, _, _ = sec32.lanczos_gs(tol=1e-12, niter=512, want_vec=False)
+    _say(f"  E0 FP64 {e64:.14f} MIXED {e32:.14f} rel {abs(e32 - e64) / abs(e64):.1e}")
+    assert abs(e32 - e64) < 1e-10 * abs(e64), (e32, e64)
+    hpsi_s = _sampled_rows(om, 8, 8, psi, rows, diag)
+    assert np.linalg.norm(hpsi_s - e64 * psi[rows]) < 1e-5 * np.linalg.norm(e64 * psi[rows])
+    sec.close()
+    sec32.close()

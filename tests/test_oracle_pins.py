@@ -10,6 +10,8 @@ from oracle import vca_ref as R
 from vca_amd import configs
 
 from . import _models as M
+from ._closed_forms import matsubara_tail as _matsubara_tail
+from ._closed_forms import matsubara_u0_exact as _matsubara_u0_exact
 
 
 # ------------------------------------------------------------ integers --
@@ -157,6 +159,23 @@ def test_hxv_direct_sweeps_equal_dense():
         v = rng.normal(size=h.shape[0]) + 1j * rng.normal(size=h.shape[0])
         assert np.abs(h @ v - model.hxv(nup, ndw, v)).max() < 1e-12
         assert np.abs(h @ v - model.hxv(nup, ndw, v, workers=3)).max() < 1e-12
+
+
+def test_hxv_slices_sum_to_one_application():
+    """The slices of the DW decomposition (the bounded CPU-baseline sample)
+    add up to the dense H v, whether run together on threads or one at a
+    time."""
+    p, hloc, hk, bh, bv = M.bath_model(nlat=3, nb=3)
+    model = M.oracle_model(p, hloc, bh, bv)
+    h = model.dense_h(3, 3)
+    v = np.random.default_rng(1).normal(size=h.shape[0]) * (1 + 0.5j)
+    for nsl, threads in ((7, 1), (7, 3)):
+        hv = model.hxv_slices(3, 3, v, np.zeros_like(v), nsl, 0, nsl, threads)
+        assert np.abs(h @ v - hv).max() < 1e-12
+    hv = np.zeros_like(v)
+    for k in range(5):
+        model.hxv_slices(3, 3, v, hv, 5, k, 1)
+    assert np.abs(h @ v - hv).max() < 1e-12
 
 
 # ------------------------------------------------------------- spectra --
@@ -354,6 +373,47 @@ def test_sft_exact_u0(tprime):
     hkl = R.hk_to_lso(po, hk)
     exact = 2 * np.mean([np.sum(np.minimum(np.linalg.eigvalsh(h), 0)) for h in hkl])
     assert abs(r["sft_potential"] - exact) < 1e-9 * abs(exact)
+
+
+def test_matsubara_tail_series():
+    """The tail series against direct summation far out plus the midpoint
+    rule's integral beyond it."""
+    beta, n0, c = 10.0, 40, 7.3
+    n = np.arange(n0, 4_000_000)
+    w = (2 * n + 1) * math.pi / beta
+    direct = np.log1p(c / w ** 2).sum()
+    w_end = 2 * 4_000_000 * math.pi / beta
+    direct += beta / (2 * math.pi) * c / w_end  # int_{w_end}^inf c/w^2 dw / (2pi/beta)
+    assert abs(direct - _matsubara_tail(c, beta, n0)) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["chain4", "chain4_tprime", "bhz"])
+def test_matsubara_sum_exact_u0(case):
+    """The fixed-grid Omega (matsubara_integration: (3-Nspin)(2/beta)
+    sum_{n<N} f(i w_n), w_n = (2n+1)pi/beta) at U=0 against the cosh
+    product minus the exactly summed tail beyond N: pins the odd
+    frequencies, the 2/beta weight, the (3-Nspin) factor and the truncation
+    at N together (a grid (2n-1)pi/beta from n=0, a weight 1/beta or a factor
+    2 for Nspin=2 each miss by orders of magnitude more than the bar)."""
+    if case == "bhz":
+        p, hloc, hk, _, _ = M.bhz_small(nk=4)
+        p.uloc = (0.0, 0.0, 0, 0, 0)
+        beta, nfreq = 20.0, 300
+    else:
+        tp = -1.0 if case == "chain4" else -0.6
+        p, hloc, hk, _, _ = M.chain(nlat=4, u=0.0, nk=8, tprime=tp)
+        beta, nfreq = 100.0, 4096
+    p.beta = beta
+    po = M.to_oracle(p)
+    r = R.vca_solve(po, hloc, hk, omega_mode="matsubara", nfreq=nfreq)
+    hkl = R.hk_to_lso(po, hk)
+    hll = R.nnn2lso(hloc, po.nlat, po.nspin, po.norb)
+    exact = _matsubara_u0_exact(hll, hkl, po.nspin, beta, nfreq)
+    assert abs(r["omega_integral"] - exact) < 1e-10 * max(1.0, abs(exact)), (
+        r["omega_integral"], exact)
+    # the truncation matters at this N: the untruncated closed form differs
+    inf = _matsubara_u0_exact(hll, hkl, po.nspin, beta, 10 ** 9)
+    assert abs(inf - exact) > 1e3 * 1e-10 * max(1.0, abs(exact))
 
 
 def test_sft_exact_u0_bhz():

@@ -80,6 +80,7 @@ _SIGS = {
     "vca_get_results": (ctypes.c_int, [_P, ctypes.POINTER(CResults)]),
     "vca_sector_create": (ctypes.c_int, [_P, _I32, _I32, _I32, ctypes.POINTER(_P)]),
     "vca_sector_destroy": (None, [_P]),
+    "vca_sector_variant": (ctypes.c_int, [_P, ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "vca_sector_dims": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int64),
                                        ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "vca_hxv": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P]),

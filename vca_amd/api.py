@@ -147,6 +147,13 @@ class Sector:
         except Exception:
             pass
 
+    def variant(self):
+        """(complex H x v launch shape, real-vector direct shape) the
+        autotuner picked; -1 where none."""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        check(self._lib.vca_sector_variant(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def hxv(self, v):
         """Hv = H v (cc_sparse_HxV semantics: Nloc = dim)."""
         v = np.ascontiguousarray(v, dtype=np.complex128)

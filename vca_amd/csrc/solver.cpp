@@ -1894,6 +1894,15 @@ int vca_sector_dims(const vca_sector *sv, int64_t *dim, int32_t *dimup, int32_t 
   VCA_GUARD_END
 }
 
+int vca_sector_variant(const vca_sector *sv, int32_t *hxv_variant, int32_t *real_variant) {
+  VCA_GUARD_BEGIN
+  const Sector *sec = reinterpret_cast<const Sector *>(sv);
+  if (!sec) fail(VCA_ERR_ARG, "null sector");
+  if (hxv_variant) *hxv_variant = sec->hxv_variant;
+  if (real_variant) *real_variant = sec->real_variant;
+  VCA_GUARD_END
+}
+
 int vca_hxv(vca_sector *sv, int64_t n, const double *v, double *hv) {
   VCA_GUARD_BEGIN
   Sector *sec = reinterpret_cast<Sector *>(sv);

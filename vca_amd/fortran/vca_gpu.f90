@@ -4,6 +4,14 @@
 !    vca_init_solver([comm,] [bath_h, bath_v])    VCA_MAIN.f90:51-115
 !    vca_solve(Hloc, Hk [, bath_h, bath_v])       VCA_MAIN.f90:124-215
 !    vca_get_sft_potential(sft)                   VCA_IO/get_sft_potential.f90:1-4
+!  the operator plugin point, for reference-style solvers that drive H x v
+!  themselves:
+!    cc_sparse_HxV (abstract interface) and the procedure pointer
+!    spHtimesV_p                                  VCA_VARS_GLOBAL.f90:83-89,137
+!    build_Hv_sector(isector) / delete_Hv_sector() /
+!    vecDim_Hv_sector(isector)                    VCA_HAMILTONIAN.f90:38-217
+!    vca_set_Hcluster(Hloc), vca_set_bath(h, v)   VCA_AUX_FUNX.f90:124-131,
+!                                                 VCA_BATH_SETUP.f90:112-138
 !  and the input variables of VCA_INPUT_VARS.f90:14-60 under the same
 !  names, and calls libvca_gpu.so through its C ABI (include/vca_gpu.h)
 !  with iso_c_binding.  A nonzero status from the library stops the
@@ -126,7 +134,60 @@ module vca_gpu
      end function c_get_gf_grid
   end interface
 
+  ! The operator plugin point (VCA_VARS_GLOBAL.f90:83-89,137): solvers call
+  ! spHtimesV_p(Nloc, v, Hv) on the sector build_Hv_sector made current.
+  abstract interface
+     subroutine cc_sparse_HxV(Nloc, v, Hv)
+       integer :: Nloc
+       complex(8), dimension(Nloc) :: v
+       complex(8), dimension(Nloc) :: Hv
+     end subroutine cc_sparse_HxV
+  end interface
+  procedure(cc_sparse_HxV), pointer, public :: spHtimesV_p => null()
+
+  interface
+     function c_set_hcluster(s, hloc) bind(C, name="vca_set_hcluster") result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s, hloc
+       integer(c_int) :: rc
+     end function c_set_hcluster
+     function c_set_bath(s, bh, bv) bind(C, name="vca_set_bath") result(rc)
+       import :: c_ptr, c_int
+       type(c_ptr), value :: s, bh, bv
+       integer(c_int) :: rc
+     end function c_set_bath
+     function c_sector_create(s, nup, ndw, prec, out) bind(C, name="vca_sector_create") &
+          result(rc)
+       import :: c_ptr, c_int, c_int32_t
+       type(c_ptr), value :: s
+       integer(c_int32_t), value :: nup, ndw, prec
+       type(c_ptr), intent(out) :: out
+       integer(c_int) :: rc
+     end function c_sector_create
+     subroutine c_sector_destroy(sec) bind(C, name="vca_sector_destroy")
+       import :: c_ptr
+       type(c_ptr), value :: sec
+     end subroutine c_sector_destroy
+     function c_sector_dims(sec, dim, dimup, dimdw) bind(C, name="vca_sector_dims") result(rc)
+       import :: c_ptr, c_int, c_int32_t, c_int64_t
+       type(c_ptr), value :: sec
+       integer(c_int64_t), intent(out) :: dim
+       integer(c_int32_t), intent(out) :: dimup, dimdw
+       integer(c_int) :: rc
+     end function c_sector_dims
+     function c_hxv(sec, n, v, hv) bind(C, name="vca_hxv") result(rc)
+       import :: c_ptr, c_int, c_int64_t, c_double_complex
+       type(c_ptr), value :: sec
+       integer(c_int64_t), value :: n
+       complex(c_double_complex), intent(in) :: v(*)
+       complex(c_double_complex), intent(out) :: hv(*)
+       integer(c_int) :: rc
+     end function c_hxv
+  end interface
+
   type(c_ptr), save :: handle = c_null_ptr
+  type(c_ptr), save :: sector_handle = c_null_ptr
+  integer, save :: Hsector = 0
   type(vca_params_t), save, target :: cparams
   integer(c_int32_t), allocatable, target, save :: csectors(:)
   real(8), save :: last_omegaprime = 0d0, last_sft = 0d0
@@ -164,6 +225,8 @@ module vca_gpu
   public :: vca_get_gimp_matsubara, vca_get_gimp_realaxis
   public :: vca_get_sigma_matsubara, vca_get_sigma_realaxis
   public :: vca_params_t, vca_results_t
+  public :: cc_sparse_HxV, build_Hv_sector, delete_Hv_sector, vecDim_Hv_sector, vca_HxV
+  public :: vca_set_Hcluster, vca_set_bath
 
 contains
 
@@ -221,13 +284,24 @@ contains
     end if
   end subroutine fill_params
 
-  ! VCA_MAIN.f90:51-76
+  ! VCA_MAIN.f90:51-76.  With bath_h and bath_v the bath dimensions must be
+  ! positive (the reference stops otherwise, :55-56); without them the
+  ! cluster has no bath: Nlat_bath = Norb_bath = 0 (:59-62).  The arrays
+  ! themselves are the model input of vca_solve / vca_set_bath, as in the
+  ! reference, where init only allocates the bath structure.
   subroutine vca_init_solver_serial(bath_h, bath_v)
     complex(8), intent(in), optional :: bath_h(:,:,:,:,:,:), bath_v(:,:,:,:,:,:)
+    if (present(bath_h) .neqv. present(bath_v)) stop "vca_init_solver: bath_h and bath_v go together"
+    if (present(bath_h) .and. present(bath_v)) then
+       if (Nlat_bath < 1 .or. Norb_bath < 1) stop "VCA_INIT_SOLVER error: wrong bath dimensions"
+    else
+       write(*, "(A)") "Bath not present, setting Nbath to 0"
+       Nlat_bath = 0
+       Norb_bath = 0
+    end if
     if (c_associated(handle)) return   ! dimensions are fixed once, as isetup does
     call fill_params()
     call check(c_init_solver(cparams, handle), "vca_init_solver")
-    if (present(bath_h) .neqv. present(bath_v)) stop "vca_init_solver: bath_h and bath_v go together"
   end subroutine vca_init_solver_serial
 
   subroutine vca_init_solver_comm(comm, bath_h, bath_v)
@@ -398,7 +472,99 @@ contains
     call grid_ij(1, 2, Sreal, ilat, jlat, "vca_get_sigma_realaxis")
   end subroutine vca_get_sigma_real_ij
 
+  ! set_Hcluster_nnn, VCA_AUX_FUNX.f90:124-131
+  subroutine vca_set_Hcluster(Hloc)
+    complex(8), intent(in), target, contiguous :: Hloc(:,:,:,:,:,:)
+    if (.not. c_associated(handle)) stop "vca_set_Hcluster: solver not initialised"
+    if (any(shape(Hloc) /= [Nlat, Nlat, Nspin, Nspin, Norb, Norb])) &
+         stop "set_Hcluster_nnn: wrong Hloc shape"
+    call check(c_set_hcluster(handle, c_loc(Hloc)), "vca_set_Hcluster")
+  end subroutine vca_set_Hcluster
+
+  ! VCA_BATH_SETUP.f90:112-138 (the shape checks of :117-118)
+  subroutine vca_set_bath(h_in, v_in)
+    complex(8), intent(in), target, contiguous :: h_in(:,:,:,:,:,:), v_in(:,:,:,:,:,:)
+    if (.not. c_associated(handle)) stop "vca_set_bath: solver not initialised"
+    if (any(shape(h_in) /= [Nlat_bath, Nlat_bath, Nspin, Nspin, Norb_bath, Norb_bath])) &
+         stop "vca_set_bath: wrong bath_h shape"
+    if (any(shape(v_in) /= [Nlat, Nlat_bath, Nspin, Nspin, Norb, Norb_bath])) &
+         stop "vca_set_bath: wrong bath_v shape"
+    call check(c_set_bath(handle, c_loc(h_in), c_loc(v_in)), "vca_set_bath")
+  end subroutine vca_set_bath
+
+  ! Sector numbering of get_Sector / get_Nup / get_Ndw (VCA_SETUP.f90:404-450)
+  ! with one up and one down string: isector = 1 + Ndw + Nup (Ns + 1).
+  subroutine sector_n(isector, nup, ndw)
+    integer, intent(in) :: isector
+    integer, intent(out) :: nup, ndw
+    integer :: ns
+    ns = Nlat * Norb + Nlat_bath * Norb_bath
+    if (isector < 1 .or. isector > (ns + 1)**2) stop "build_Hv_sector: isector out of range"
+    ndw = mod(isector - 1, ns + 1)
+    nup = (isector - 1) / (ns + 1)
+  end subroutine sector_n
+
+  ! build_Hv_sector, VCA_HAMILTONIAN.f90:38-146: makes sector isector of the
+  ! current Hloc (and bath) the operator's sector and points spHtimesV_p at
+  ! its H x v; the hop tables are built on the GPU.  The precision is
+  ! vca_precision.
+  subroutine build_Hv_sector(isector)
+    integer, intent(in) :: isector
+    integer :: nup, ndw
+    if (.not. c_associated(handle)) stop "build_Hv_sector: solver not initialised"
+    call sector_n(isector, nup, ndw)
+    if (c_associated(sector_handle)) call delete_Hv_sector()
+    call check(c_sector_create(handle, int(nup, c_int32_t), int(ndw, c_int32_t), &
+         int(vca_precision, c_int32_t), sector_handle), "build_Hv_sector")
+    Hsector = isector
+    spHtimesV_p => vca_HxV
+  end subroutine build_Hv_sector
+
+  ! delete_Hv_sector, VCA_HAMILTONIAN.f90:149-186
+  subroutine delete_Hv_sector()
+    if (c_associated(sector_handle)) call c_sector_destroy(sector_handle)
+    sector_handle = c_null_ptr
+    Hsector = 0
+    spHtimesV_p => null()
+  end subroutine delete_Hv_sector
+
+  ! vecDim_Hv_sector, VCA_HAMILTONIAN.f90:193-217 (one GPU: the whole sector)
+  integer function vecDim_Hv_sector(isector) result(vecDim)
+    integer, intent(in) :: isector
+    integer :: nup, ndw, ns
+    call sector_n(isector, nup, ndw)
+    ns = Nlat * Norb + Nlat_bath * Norb_bath
+    vecDim = binom(ns, nup) * binom(ns, ndw)
+  end function vecDim_Hv_sector
+
+  integer function binom(n, k)
+    integer, intent(in) :: n, k
+    integer :: i
+    binom = 1
+    do i = 1, k
+       binom = binom * (n - k + i) / i
+    end do
+  end function binom
+
+  ! The operator of the current sector with the cc_sparse_HxV interface
+  ! (VCA_VARS_GLOBAL.f90:83-89): Hv = H v, state i = iup + (idw-1) DimUp
+  ! (1-based), the product on the GPU (vca_hxv).  It is what
+  ! build_Hv_sector binds spHtimesV_p to, in place of directMatVec_main
+  ! (VCA_HAMILTONIAN_DIRECT_HxV.f90:42-78).
+  subroutine vca_HxV(Nloc, v, Hv)
+    integer :: Nloc
+    complex(8), dimension(Nloc) :: v
+    complex(8), dimension(Nloc) :: Hv
+    integer(c_int64_t) :: dim
+    integer(c_int32_t) :: du, dd
+    if (.not. c_associated(sector_handle)) stop "vca_HxV: no sector (build_Hv_sector first)"
+    call check(c_sector_dims(sector_handle, dim, du, dd), "vca_HxV")
+    if (int(Nloc, c_int64_t) /= dim) stop "vca_HxV: Nloc differs from the sector dimension"
+    call check(c_hxv(sector_handle, dim, v, Hv), "vca_HxV")
+  end subroutine vca_HxV
+
   subroutine vca_finalize_solver()
+    if (c_associated(sector_handle)) call delete_Hv_sector()
     if (c_associated(handle)) call c_finalize_solver(handle)
     handle = c_null_ptr
   end subroutine vca_finalize_solver

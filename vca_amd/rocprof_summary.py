@@ -1,14 +1,17 @@
 """Summaries of rocprofv3 result databases (kernel trace and PMC passes).
 
     python -m vca_amd.rocprof_summary trace  TRACE.db
-    python -m vca_amd.rocprof_summary pmc    FETCH.db WRITE.db KERNEL_SUBSTRING COPY_BYTES OUT.json
+    python -m vca_amd.rocprof_summary pmc    FETCH.db WRITE.db KERNELS COPY_BYTES OUT.json KEY VARIANT
 
 `trace` prints the per-kernel table (calls, average and total duration).
 `pmc` reads a FETCH_SIZE and a WRITE_SIZE pass (KiB per dispatch), finds
 the calibration copy kernel of COPY_BYTES bytes each way (the largest
 __amd_rocclr_copyBuffer / elementwise copy dispatch) to turn the counters
-into bytes, and writes the HBM traffic per dispatch of the kernel whose
-name contains KERNEL_SUBSTRING (the most-dispatched match) to OUT.json.
+into bytes, and stores under KEY in OUT.json the HBM traffic of one
+application: the sum over the comma-separated KERNELS substrings of the
+per-dispatch traffic of the most-dispatched kernel matching each (one
+launch of each per application), with the launch shape VARIANT the run
+picked, so that a later run can tell whether the counters are its own.
 """
 from __future__ import annotations
 
@@ -43,7 +46,7 @@ def _per_kernel(db: str, counter: str) -> dict:
     return res
 
 
-def pmc_traffic(fetch_db: str, write_db: str, kernel: str, copy_bytes: int) -> dict:
+def pmc_traffic(fetch_db: str, write_db: str, kernels: str, copy_bytes: int) -> dict:
     fetch = _per_kernel(fetch_db, "FETCH_SIZE")
     write = _per_kernel(write_db, "WRITE_SIZE")
     # calibration: the copy dispatch with the largest counts
@@ -55,15 +58,19 @@ def pmc_traffic(fetch_db: str, write_db: str, kernel: str, copy_bytes: int) -> d
     cw_kib = write.get(ck, (0, 0, 0))[2]
     f_scale = copy_bytes / (cf_kib * 1024.0)
     w_scale = copy_bytes / (cw_kib * 1024.0) if cw_kib else None
-    cand = [k for k in fetch if kernel in k]
-    if not cand:
-        raise SystemExit(f"no kernel matching {kernel!r}")
-    kn = max(cand, key=lambda k: fetch[k][0])
-    f_kib = fetch[kn][1]
-    w_kib = write[kn][1]
+    names, f_kib, w_kib, disp = [], 0.0, 0.0, []
+    for sub in kernels.split(","):
+        cand = [k for k in fetch if sub in k]
+        if not cand:
+            raise SystemExit(f"no kernel matching {sub!r}")
+        kn = max(cand, key=lambda k: fetch[k][0])
+        names.append(kn)
+        disp.append(fetch[kn][0])
+        f_kib += fetch[kn][1]
+        w_kib += write[kn][1]
     return {
-        "kernel": kn,
-        "dispatches": fetch[kn][0],
+        "kernels": names,
+        "dispatches": disp,
         "fetch_kib_per_dispatch": f_kib,
         "write_kib_per_dispatch": w_kib,
         "calibration_kernel": ck,
@@ -77,10 +84,17 @@ def pmc_traffic(fetch_db: str, write_db: str, kernel: str, copy_bytes: int) -> d
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace_table(argv[1]))
-    elif len(argv) >= 6 and argv[0] == "pmc":
+    elif len(argv) >= 8 and argv[0] == "pmc":
         d = pmc_traffic(argv[1], argv[2], argv[3], int(argv[4]))
+        d["variant"] = int(argv[7])
+        try:
+            with open(argv[5]) as f:
+                store = json.load(f)
+        except FileNotFoundError:
+            store = {}
+        store[argv[6]] = d
         with open(argv[5], "w") as f:
-            json.dump(d, f, indent=1)
+            json.dump(store, f, indent=1)
         print(json.dumps(d, indent=1))
     else:
         print(__doc__)
