@@ -212,6 +212,18 @@ int vca_device_free(void *ptr);
 int vca_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind);
 /* Waits for all work the library queued on the device. */
 int vca_device_synchronize(void);
+/* No counterpart in the reference: a form of vca_hxv_device for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  It allocates nothing: the transposed
+ * form's scratch vector (v^T) lives in a device array of 256 MiB that the
+ * library holds on each GPU it runs on until the process ends, and the
+ * direct forms need none.  A sector whose product needs more scratch than
+ * that (complex128 sectors above 16,777,216 states on the transposed form)
+ * is refused with VCA_ERR_ARG.  The product is enqueued on `stream` (a
+ * hipStream_t; NULL: the sector's stream) and not waited for.  Every call on
+ * one GPU shares the scratch: calls that overlap on one GPU race.  The
+ * sector must have been created (and its launch shape tuned) beforehand. */
+int vca_hxv_device_noalloc(vca_sector *sec, const void *v, void *hv, void *stream);
 /* The same for nb vectors stacked [nb][dim] in v and hv: one launch per
  * application covers all nb products (the batched form the excitation
  * chains use); *ms is the average time of one application to all nb. */

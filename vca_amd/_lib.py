@@ -12,6 +12,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libvca_gpu.so")
 
 VCA_OK = 0
+VCA_ERR_ARG = -1
+VCA_ERR_HIP = -2
+VCA_ERR_NOMEM = -3
+VCA_ERR_STATE = -4
+VCA_ERR_NOCONV = -5
 VCA_FP64 = 0
 VCA_MIXED = 1
 VCA_LANC_LANCZOS = 0
@@ -85,6 +90,7 @@ _SIGS = {
                                        ctypes.POINTER(_I32), ctypes.POINTER(_I32)]),
     "vca_hxv": (ctypes.c_int, [_P, ctypes.c_int64, _P, _P]),
     "vca_hxv_device": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.POINTER(ctypes.c_float)]),
+    "vca_hxv_device_noalloc": (ctypes.c_int, [_P, _P, _P, _P]),
     "vca_hxv_device_batch": (ctypes.c_int, [_P, _I32, _P, _P, _I32,
                                             ctypes.POINTER(ctypes.c_float)]),
     "vca_hxv_device_real": (ctypes.c_int, [_P, _P, _P, _I32, ctypes.POINTER(ctypes.c_float)]),

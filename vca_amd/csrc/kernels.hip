@@ -2633,10 +2633,12 @@ static void launch_buf_pass(Sector &sec, const HxvArgs &a, const HxvShape &h, bo
 }
 
 // The four launches of the transposed form (see k_transpose) on nb chains
-// stacked [nb][dim]; the scratch vectors are sec.tbuf.  The one-column
-// passes run on the 2-D grid (columns, chains).
+// stacked [nb][dim]; the scratch vectors [nb][dim] are at tb (tb, or
+// the caller's scratch).  The one-column passes run on the 2-D grid
+// (columns, chains).
 template <typename R>
-static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_amp, int nb) {
+static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_amp, int nb,
+                           void *tb) {
   const bool fp64 = sec.precision == VCA_FP64;
   a.xcd_chains = 0;
   a.nchains = nb;
@@ -2651,7 +2653,7 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   TrArgs t{};
   t.in0 = a.v0;
   t.in1 = a.v1;
-  t.out = sec.tbuf;
+  t.out = tb;
   t.ni = sec.up.dim;
   t.nj = sec.dw.dim;
   t.vstride = sec.dim;
@@ -2660,8 +2662,8 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   // 3. vt = (H_dw v)^T: the down-spin tables as the gather tables, rows of
   //    vt staged in LDS and overwritten by their result
   HxvArgs b = a;
-  b.v0 = b.v1 = sec.tbuf;
-  b.hv = sec.tbuf;
+  b.v0 = b.v1 = tb;
+  b.hv = tb;
   b.dimup = sec.dw.dim;
   b.dimdw = sec.up.dim;
   b.map_up = sec.dw.map;
@@ -2679,7 +2681,7 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   launch_buf_pass<R>(sec, b, h, real_amp, dim3(sec.up.dim, nb));
   // 4. Hv += vt^T
   TrArgs t2{};
-  t2.in0 = t2.in1 = sec.tbuf;
+  t2.in0 = t2.in1 = tb;
   t2.out = a.hv;
   t2.ni = sec.dw.dim;
   t2.nj = sec.up.dim;
@@ -2892,14 +2894,20 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   }
 }
 
+// scratch: nb vectors of the caller's for v^T, or null for sec.tbuf
+// (allocated here on first use).
 static void launch_hxv_tform(Sector &sec, const HxvArgs &a, const HxvShape &h, bool real_amp,
-                             int nb) {
-  ensure_tbuf(sec, nb);
-  if (sec.precision == VCA_FP64) launch_tform_t<double>(sec, a, h, real_amp, nb);
-  else launch_tform_t<float>(sec, a, h, real_amp, nb);
+                             int nb, void *scratch) {
+  if (!scratch) {
+    ensure_tbuf(sec, nb);
+    scratch = sec.tbuf;
+  }
+  if (sec.precision == VCA_FP64) launch_tform_t<double>(sec, a, h, real_amp, nb, scratch);
+  else launch_tform_t<float>(sec, a, h, real_amp, nb, scratch);
 }
 
-void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos, int nb) {
+void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos, int nb,
+                void *scratch) {
   HxvArgs a{};
   a.vstride = sec.dim;
   a.v0 = v0;
@@ -2962,7 +2970,7 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
     a.pbase = 0;
     a.pstride = hxv_partials(sec, shp);
     if (shp.tform) {
-      launch_hxv_tform(sec, a, shp, real_amp, nb);
+      launch_hxv_tform(sec, a, shp, real_amp, nb, scratch);
       return;
     }
     const size_t fixed = (size_t)a.namp_up * esz + (size_t)a.dw_maxnnz * (esz + 8) + 48;

@@ -1697,6 +1697,34 @@ static double now() {
 // ================================================================ C ABI ==
 using namespace vca;
 
+// ---------------------------------------- no-allocation H x v scratch ---
+// Device scratch of vca_hxv_device_noalloc: the transposed form's v^T for
+// one vector, in a file-scope device array instead of the sector's lazily
+// allocated Sector::tbuf.  The runtime places one copy per GPU when it loads
+// this code object; its address on the current device is looked up once
+// (hipGetSymbolAddress) by vca_init_solver, and host code only ever uses
+// that looked-up pointer.  Every call on one GPU shares it, so calls that
+// overlap on one GPU race; nothing in it needs to start at zero (the
+// transpose writes every element before the gather pass reads it).
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// The current device's address of `scratch`, looked up on its first use.
+static void resolve_scratch() {
+  int dev = 0;
+  VCA_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) fail(VCA_ERR_ARG, "device %d out of range", dev);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[dev]) return;
+  void *ptr = nullptr;
+  VCA_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  if (!ptr) fail(VCA_ERR_HIP, "no device address for the scratch array");
+  g_scratch_dev[dev] = ptr;
+}
+
 #define VCA_GUARD_BEGIN try {
 #define VCA_GUARD_END                         \
   }                                           \
@@ -1777,6 +1805,7 @@ int vca_init_solver(const vca_params *p, vca_solver **out) {
   VCA_HIP(hipGetDeviceCount(&ndev));
   if (ndev < 1) fail(VCA_ERR_HIP, "no HIP device");
   VCA_HIP(hipSetDevice(0));
+  resolve_scratch();
   VCA_HIP(hipStreamCreate(&s->stream));
   Binom bn;
   VCA_HIP(hipMalloc(&s->d_binom, sizeof(bn.c)));
@@ -1979,6 +2008,37 @@ int vca_device_copy(void *dst, const void *src, int64_t bytes, int32_t kind) {
     default: fail(VCA_ERR_ARG, "unknown copy kind %d", kind);
   }
   VCA_HIP(hipMemcpy(dst, src, (size_t)bytes, k));
+  VCA_GUARD_END
+}
+
+int vca_hxv_device_noalloc(vca_sector *sv, const void *v, void *hv, void *stream) {
+  VCA_GUARD_BEGIN
+  Sector *sec = reinterpret_cast<Sector *>(sv);
+  if (!sec || !v || !hv) fail(VCA_ERR_ARG, "null argument");
+  int dev = 0;
+  VCA_HIP(hipGetDevice(&dev));
+  void *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (dev >= 0 && dev < kMaxDevices) ptr = g_scratch_dev[dev];
+  }
+  if (!ptr) fail(VCA_ERR_STATE, "no scratch on device %d (vca_init_solver sets it up)", dev);
+  const size_t need = (size_t)hxv_scratch_vectors(*sec) * sec->vec_bytes();
+  if (need > kNoAllocScratchBytes)
+    fail(VCA_ERR_ARG,
+         "sector (%d,%d) needs %zu bytes of scratch; the no-allocation H x v holds %zu "
+         "(use vca_hxv_device)", sec->nup, sec->ndw, need, kNoAllocScratchBytes);
+  // launch on the caller's stream (e.g. one being captured), the sector's
+  // otherwise
+  const hipStream_t own = sec->stream;
+  if (stream) sec->stream = static_cast<hipStream_t>(stream);
+  try {
+    launch_hxv(*sec, v, v, hv, false, 1, need ? ptr : nullptr);
+  } catch (...) {
+    sec->stream = own;
+    throw;
+  }
+  sec->stream = own;
   VCA_GUARD_END
 }
 

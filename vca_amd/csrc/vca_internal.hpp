@@ -281,8 +281,11 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns,
                        int npart, const uint32_t *d_binom, double e_add);
 CrossTerm sector_cross(const Sector &sec);
 const uint32_t *solver_binom(const Solver *s);
+// scratch (complex vectors, transposed form only): nb vectors of the
+// sector's element size for v^T, or null for the sector's own (sec.tbuf,
+// allocated on first use)
 void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanczos,
-                int nb = 1);
+                int nb = 1, void *scratch = nullptr);
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb = 1);
 void ensure_step_partials(Sector &sec, int nb);
 int hxv_num_variants(const Sector &sec);
