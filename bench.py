@@ -482,14 +482,27 @@ def omega_section(args, cpu):
     r2 = so.results()
     k64 = r1["omega_seconds"] - r0["omega_seconds"]  # log-det kernel time (events)
     k32 = r2["omega_seconds"] - r1["omega_seconds"]
+    # the same FP32 log-dets with the contraction on the vector ALUs
+    os.environ["VCA_OMEGA_MFMA"] = "0"
+    try:
+        so.omega_integrand(zs[:64], "mixed")
+        r3 = so.results()
+        t0 = time.perf_counter()
+        f32v = so.omega_integrand(zs, "mixed")
+        t32v = time.perf_counter() - t0
+        k32v = so.results()["omega_seconds"] - r3["omega_seconds"]
+    finally:
+        os.environ.pop("VCA_OMEGA_MFMA", None)
+    errv = abs((3 - po.nspin) * 2.0 / po.beta * f32v.sum() - (3 - po.nspin) * 2.0 / po.beta
+               * f64.sum()) / abs((3 - po.nspin) * 2.0 / po.beta * f64.sum())
     s64 = (3 - po.nspin) * 2.0 / po.beta * f64.sum()
     s32 = (3 - po.nspin) * 2.0 / po.beta * f32.sum()
     err = abs(s32 - s64) / abs(s64)
     evals = nk * args.omega_nfreq
     fl = logdet_flops(8, 2)  # two spin blocks of 8 (Nlat*Norb = 8)
     log(f"Omega BHZ: {evals} (k,w) evals FP64 {t64:.3f}s (kernel {k64:.3f}s), MIXED "
-        f"{t32:.3f}s (kernel {k32:.3f}s); integral rel err mixed {err:.2e}; "
-        f"full solve {t_solve:.2f}s Omega={om64:.10f}")
+        f"{t32:.3f}s (kernel {k32:.3f}s, vector-ALU contraction {k32v:.3f}s); integral rel err "
+        f"mixed {err:.2e} (vector ALU {errv:.2e}); full solve {t_solve:.2f}s Omega={om64:.10f}")
     a64 = evals * fl / k64 / 1e12
     a32 = evals * fl / k32 / 1e12
     out = {
@@ -497,7 +510,13 @@ def omega_section(args, cpu):
                     f"{args.omega_nfreq} Matsubara frequencies, beta=200",
         "evals": evals, "evals_per_s_fp64": evals / t64, "evals_per_s_mixed": evals / t32,
         "kernel_seconds_fp64": k64, "kernel_seconds_mixed": k32,
-        "integral_rel_err_mixed": err, "omega_full_solve": om64,
+        "integral_rel_err_mixed": err,
+        "mixed_vector_alu": {"evals_per_s": evals / t32v, "kernel_seconds": k32v,
+                             "integral_rel_err": errv,
+                             "note": "FP32 log-dets with the contraction on the vector ALUs "
+                                     "(k_logdet<float, 8>) instead of the matrix cores "
+                                     "(k_logdet_mfma<8>)"},
+        "omega_full_solve": om64,
         "full_solve_seconds": t_solve,
         "roofline": {"bound": "valu", "flops_per_eval": fl,
                      "fp64": {"achieved": a64, "peak": FP64_PEAK_TFLOPS,

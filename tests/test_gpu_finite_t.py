@@ -168,3 +168,32 @@ def test_logdet_qr_equals_pivoted_lu(monkeypatch):
     monkeypatch.setenv("VCA_OMEGA_LU", "1")
     f_lu = s.omega_integrand_complex(zs)
     assert np.allclose(f_qr, f_lu, rtol=1e-12, atol=1e-13), (f_qr, f_lu)
+
+
+@pytest.mark.parametrize("case", ["bhz_blocks_of_8", "chain12_block_of_16"])
+def test_mfma_logdet_equals_vector_alu_logdet(case, monkeypatch):
+    """The FP32 integrand with the contraction on the matrix cores
+    (k_logdet_mfma) and on the vector ALUs (k_logdet<float>): log|det| and
+    the principal phase agree to FP32 rounding, on 8-row spin blocks and on
+    a 12-site block padded to 16 rows; both within 1e-6 of the FP64
+    integrand at the frequencies where it is not ill-conditioned."""
+    if case == "bhz_blocks_of_8":
+        p, hloc, hk, _, _ = configs.config_bhz_2x2(nk=16, nfreq=64, precision="fp64")
+    else:
+        p, hloc, hk, _, _ = M.chain(nlat=12, u=0.0, nk=64, tprime=-0.8)
+        p.sectors = [(6, 6)]
+    _say(f"MFMA vs VALU log-det, {case}")
+    s = VcaSolver(p)
+    s.vca_solve(hloc, hk)
+    zs = np.array([0.3j, 1.1j, 3.0j, 40.0j, 0.4 + 0.3j, -2.0 + 0.2j, 5.0 * np.exp(2.0j)])
+    f64 = s.omega_integrand_complex(zs, "fp64")
+    monkeypatch.setenv("VCA_OMEGA_MFMA", "0")
+    f_valu = s.omega_integrand_complex(zs, "mixed")
+    monkeypatch.setenv("VCA_OMEGA_MFMA", "1")
+    f_mfma = s.omega_integrand_complex(zs, "mixed")
+    scale = np.abs(f64).max()
+    _say(f"  MFMA vs VALU {np.abs(f_mfma - f_valu).max() / scale:.1e}, MFMA vs FP64 "
+         f"{np.abs(f_mfma - f64).max() / scale:.1e}")
+    assert np.abs(f_mfma - f_valu).max() < 2e-6 * scale
+    assert np.abs(f_mfma - f64).max() < 2e-6 * scale
+    assert np.allclose(f_mfma.real, s.omega_integrand(zs, "mixed"), rtol=1e-12, atol=1e-13)

@@ -251,6 +251,86 @@ __global__ __launch_bounds__(64) void k_prep(const PrepArgs a) {
 }
 
 // ------------------------------------------------------- k x z log-det --
+// Pivoted LU of the NBT x NBT matrices held one row per lane (lane gl of
+// each NBT-lane group): row r is X(r, :) = (ar, ai) with M(r, :) = X(r, :) +
+// e_cu (the identity's unit kept exact in column cu, -1 once eliminated).
+// The pivot row is the active row of largest |M(r, kk)| (ties to the lowest
+// row), found with an NBT-lane butterfly and broadcast to the group; rows
+// used as pivots go inactive, so no swap is needed.  Returns log|det M|;
+// with want_phase also the unit phase (phr, phi) of the pivot product and
+// the parity of the pivot order's inversions.
+template <typename R, int NBT>
+__device__ __forceinline__ double lu_logdet_rows(R (&ar)[NBT], R (&ai)[NBT], int cu, int gl,
+                                                 int lane, bool want_phase, double &phr,
+                                                 double &phi, int &inv_parity) {
+  bool active = true;
+  // prod |pivot|^2 kept as mantissa * 2^e (frexp after every factor, so
+  // no overflow), one log per matrix at the end
+  double pm_acc = 1.0;
+  int pm_exp = 0;
+#pragma unroll
+  for (int kk = 0; kk < NBT; ++kk) {
+    const R mkr = ar[kk] + (cu == kk ? (R)1 : (R)0);  // M(r, kk)
+    float mag = active ? (float)(mkr * mkr + ai[kk] * ai[kk]) : -1.0f;
+    int idx = gl;
+#pragma unroll
+    for (int off = NBT / 2; off > 0; off >>= 1) {
+      const float m2 = __shfl_xor(mag, off, NBT);
+      const int i2 = __shfl_xor(idx, off, NBT);
+      if (m2 > mag || (m2 == mag && i2 < idx)) {
+        mag = m2;
+        idx = i2;
+      }
+    }
+    const int src = (lane & ~(NBT - 1)) | idx;
+    R pr[NBT], pim[NBT];
+#pragma unroll
+    for (int j = kk; j < NBT; ++j) {
+      pr[j] = __shfl(ar[j], src, 64);
+      pim[j] = __shfl(ai[j], src, 64);
+    }
+    const int pcu = __shfl(cu, src, 64);
+    // pivot = X(p, kk) + [its unit is in column kk], formed in FP64
+    const double pvr = (double)pr[kk] + (pcu == kk ? 1.0 : 0.0);
+    const double pvi = (double)pim[kk];
+    const double pm = pvr * pvr + pvi * pvi;
+    {
+      int ex;
+      pm_acc = frexp(pm_acc * pm, &ex);
+      pm_exp += ex;
+    }
+    if (want_phase) {
+      // rows still active with a lower index than the pivot row are
+      // eliminated after it: one inversion each
+      const uint64_t below = __ballot(active && gl < idx);
+      const int gbase = lane & ~(NBT - 1);
+      const uint64_t gm = (below >> gbase) & ((NBT == 64) ? ~0ull : ((1ull << NBT) - 1ull));
+      inv_parity ^= __popcll(gm) & 1;
+      const double rm = 1.0 / sqrt(pm);
+      const double ur = pvr * rm, ui = pvi * rm;
+      const double nr = phr * ur - phi * ui;
+      phi = phr * ui + phi * ur;
+      phr = nr;
+    }
+    if (gl == idx) active = false;
+    if (active) {
+      const R pfr = (R)pvr, pfi = pim[kk];
+      const R inv = (R)1 / (pfr * pfr + pfi * pfi);
+      const R lr = (mkr * pfr + ai[kk] * pfi) * inv;  // l = M(r, kk) / pivot
+      const R li = (ai[kk] * pfr - mkr * pfi) * inv;
+#pragma unroll
+      for (int j = kk + 1; j < NBT; ++j) {
+        const R prj = pr[j] + (pcu == j ? (R)1 : (R)0);  // pivot row's unit
+        ar[j] -= lr * prj - li * pim[j];
+        ai[j] -= lr * pim[j] + li * prj;
+      }
+      if (cu == kk) cu = -1;  // this row's unit sat in the eliminated column
+    }
+  }
+  // identical in the lanes of a group
+  return 0.5 * (log(pm_acc) + (double)pm_exp * 0.69314718055994530942);
+}
+
 // Each 16-lane group owns one matrix M = 1 + P(z) - H_k G'(z), lane r
 // holding row r: the lane forms its row of H_k G' from its register copy
 // of H_k's row and G'(z) read from LDS (the same address for the whole
@@ -324,74 +404,10 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
         ai[j] -= xr * g.y + xi * g.x;
       }
     }
-    bool active = true;
-    // prod |pivot|^2 kept as mantissa * 2^e (frexp after every factor, so
-    // no overflow), one log per matrix at the end
-    double pm_acc = 1.0;
-    int pm_exp = 0;
     double phr = 1.0, phi = 0.0;  // unit phase of det(M)
     int inv_parity = 0;
-#pragma unroll
-    for (int kk = 0; kk < NBT; ++kk) {
-      const R mkr = ar[kk] + (cu == kk ? (R)1 : (R)0);  // M(r, kk)
-      float mag = active ? (float)(mkr * mkr + ai[kk] * ai[kk]) : -1.0f;
-      int idx = gl;
-#pragma unroll
-      for (int off = NBT / 2; off > 0; off >>= 1) {
-        const float m2 = __shfl_xor(mag, off, NBT);
-        const int i2 = __shfl_xor(idx, off, NBT);
-        if (m2 > mag || (m2 == mag && i2 < idx)) {
-          mag = m2;
-          idx = i2;
-        }
-      }
-      const int src = (lane & ~(NBT - 1)) | idx;
-      R pr[NBT], pim[NBT];
-#pragma unroll
-      for (int j = kk; j < NBT; ++j) {
-        pr[j] = __shfl(ar[j], src, 64);
-        pim[j] = __shfl(ai[j], src, 64);
-      }
-      const int pcu = __shfl(cu, src, 64);
-      // pivot = X(p, kk) + [its unit is in column kk], formed in FP64
-      const double pvr = (double)pr[kk] + (pcu == kk ? 1.0 : 0.0);
-      const double pvi = (double)pim[kk];
-      const double pm = pvr * pvr + pvi * pvi;
-      {
-        int ex;
-        pm_acc = frexp(pm_acc * pm, &ex);
-        pm_exp += ex;
-      }
-      if (phase) {
-        // rows still active with a lower index than the pivot row are
-        // eliminated after it: one inversion each
-        const uint64_t below = __ballot(active && gl < idx);
-        const int gbase = lane & ~(NBT - 1);
-        const uint64_t gm = (below >> gbase) & ((NBT == 64) ? ~0ull : ((1ull << NBT) - 1ull));
-        inv_parity ^= __popcll(gm) & 1;
-        const double rm = 1.0 / sqrt(pm);
-        const double ur = pvr * rm, ui = pvi * rm;
-        const double nr = phr * ur - phi * ui;
-        phi = phr * ui + phi * ur;
-        phr = nr;
-      }
-      if (gl == idx) active = false;
-      if (active) {
-        const R pfr = (R)pvr, pfi = pim[kk];
-        const R inv = (R)1 / (pfr * pfr + pfi * pfi);
-        const R lr = (mkr * pfr + ai[kk] * pfi) * inv;  // l = M(r, kk) / pivot
-        const R li = (ai[kk] * pfr - mkr * pfi) * inv;
-#pragma unroll
-        for (int j = kk + 1; j < NBT; ++j) {
-          const R prj = pr[j] + (pcu == j ? (R)1 : (R)0);  // pivot row's unit
-          ar[j] -= lr * prj - li * pim[j];
-          ai[j] -= lr * pim[j] + li * prj;
-        }
-        if (cu == kk) cu = -1;  // this row's unit sat in the eliminated column
-      }
-    }
-    // ld is identical in the lanes of a group; invalid k contribute 0
-    const double ld = 0.5 * (log(pm_acc) + (double)pm_exp * 0.69314718055994530942);
+    const double ld = lu_logdet_rows<R, NBT>(ar, ai, cu, gl, lane, phase != nullptr, phr, phi,
+                                              inv_parity);
     if (gl == 0) s_red[grp] = kvalid ? ld : 0.0;
     if (phase && gl == 0 && kvalid) {
       const double sg = inv_parity ? -1.0 : 1.0;
@@ -402,6 +418,361 @@ __global__ __launch_bounds__(kOBlock) void k_logdet(int nk, int nz, int zchunk, 
       double t = 0;
       for (int g = 0; g < MPB; ++g) t += s_red[g];
       partials[((int64_t)iz * nspin_blk + sb) * gridDim.x + kb] = t;
+    }
+  }
+}
+
+// ----------------------------------------- k x z log-det, MFMA contraction --
+// The FP32 integrand with the contraction on the matrix cores: for each
+// frequency, X_k = P(z) - H_k G'(z) for the 64/NBT matrices of a wavefront
+// is one real GEMM on 16-row tiles of the stacked H_k rows,
+//   [X_re | X_im] = [P_re | P_im] + [-H_re | -H_im] [[G_re, G_im], [-G_im, G_re]],
+// (16 x 2NBT) += (16 x 2NBT) (2NBT x 2NBT), as v_mfma_f32_16x16x4f32 steps
+// of 4 inner indices: per tile 2NBT/4 steps for each of the 2NBT/16 output
+// column tiles.  The -H_k operand stays in registers across the chunk of
+// frequencies, G'(z) and P(z) are staged in LDS per frequency, and each
+// tile goes through LDS into the one-row-per-lane layout of the pivoted LU
+// (lu_logdet_rows), which runs on the vector ALUs.  Operand lanes (16x16x4
+// f32): A(i, k) from lane i + 16k, B(k, j) from lane j + 16k, D(4q + r, j)
+// in accumulator r of lane j + 16q.  Same arguments and partial sums as
+// k_logdet<float, NBT>.
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+template <int NBT>
+__global__ __launch_bounds__(kOBlock) void k_logdet_mfma(int nk, int nz, int zchunk,
+                                                         int nspin_blk, const float2 *hk,
+                                                         const float2 *gmat, const float2 *pmat,
+                                                         double *partials, double2 *phase) {
+  static_assert(NBT == 8 || NBT == 16, "8- or 16-row blocks");
+  constexpr int MPB = kOBlock / NBT;  // matrices per block
+  constexpr int MPW = 64 / NBT;       // matrices per wavefront
+  constexpr int NT = 4;               // 16-row tiles per wavefront (64 stacked rows)
+  constexpr int NCT = 2 * NBT / 16;   // 16-column tiles of [re | im]
+  constexpr int KC = 2 * NBT / 4;     // 4-deep steps over the inner [re | im] index
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int gl = lane & (NBT - 1);
+  const int grp = threadIdx.x / NBT;
+  const int kb = blockIdx.x;
+  const int sb = blockIdx.z;
+  const int k = kb * MPB + grp;
+  const bool kvalid = k < nk;
+  const int ti = lane & 15, tq = lane >> 4;
+  __shared__ double s_red[MPB];
+  __shared__ float2 s_gz[NBT * NBT], s_pz[NBT * NBT];
+  __shared__ float s_x[kOBlock / 64][NT * NCT][16 * 17];  // rows padded to 17
+  // A operand (-H_k), constant over the frequencies: tile t row ti is row
+  // (t * 16 + ti) % NBT of the wavefront's matrix (t * 16 + ti) / NBT
+  float a_op[NT][KC];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int row = t * 16 + ti;
+    const int kk = kb * MPB + wave * MPW + row / NBT;
+    const int r = row % NBT;
+#pragma unroll
+    for (int c = 0; c < KC; ++c) {
+      const int kin = 4 * c + tq;  // inner index: [re 0..NBT-1 | im 0..NBT-1]
+      float v = 0.0f;
+      if (kk < nk) {
+        const float2 h = hk[(((int64_t)kk * nspin_blk + sb) * NBT + r) * NBT + (kin % NBT)];
+        v = kin < NBT ? h.x : h.y;
+      }
+      a_op[t][c] = -v;
+    }
+  }
+  const int zbeg = blockIdx.y * zchunk;
+  const int zend = min(zbeg + zchunk, nz);
+  for (int iz = zbeg; iz < zend; ++iz) {
+    const int64_t zoff = ((int64_t)iz * nspin_blk + sb) * kNB * kNB;
+    __syncthreads();  // the previous frequency's LU reads of s_x are done
+    for (int e = threadIdx.x; e < NBT * NBT; e += kOBlock) {
+      const int r = e / NBT, c = e % NBT;
+      s_gz[e] = gmat[zoff + r * kNB + c];
+      s_pz[e] = pmat[zoff + r * kNB + c];
+    }
+    __syncthreads();
+    // B operand: B(kin, col) for kin = 4c + tq, col = 16 ct + ti
+    float b_op[NCT][KC];
+    mfma_f4 c_in[NCT];
+#pragma unroll
+    for (int ct = 0; ct < NCT; ++ct) {
+      const int col = 16 * ct + ti;
+      const int j = col % NBT;
+      const bool oim = col >= NBT;
+#pragma unroll
+      for (int c = 0; c < KC; ++c) {
+        const int kin = 4 * c + tq;
+        const float2 g = s_gz[(kin % NBT) * NBT + j];
+        b_op[ct][c] = kin < NBT ? (oim ? g.y : g.x) : (oim ? g.x : -g.y);
+      }
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const float2 pz = s_pz[((4 * tq + rr) % NBT) * NBT + j];
+        c_in[ct][rr] = oim ? pz.y : pz.x;
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+#pragma unroll
+      for (int ct = 0; ct < NCT; ++ct) {
+        mfma_f4 acc = c_in[ct];
+#pragma unroll
+        for (int c = 0; c < KC; ++c)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[t][c], b_op[ct][c], acc, 0, 0, 0);
+#pragma unroll
+        for (int rr = 0; rr < 4; ++rr) s_x[wave][t * NCT + ct][(4 * tq + rr) * 17 + ti] = acc[rr];
+      }
+    }
+    __syncthreads();
+    // this lane's row of its matrix: wavefront row q * NBT + gl
+    const int wrow = (lane / NBT) * NBT + gl;
+    const int t = wrow / 16, rt = wrow % 16;
+    float ar[NBT], ai[NBT];
+#pragma unroll
+    for (int j = 0; j < NBT; ++j) {
+      ar[j] = s_x[wave][t * NCT + j / 16][rt * 17 + (j % 16)];
+      ai[j] = s_x[wave][t * NCT + (NBT + j) / 16][rt * 17 + ((NBT + j) % 16)];
+    }
+    double phr = 1.0, phi = 0.0;
+    int inv_parity = 0;
+    const double ld = lu_logdet_rows<float, NBT>(ar, ai, gl, gl, lane, phase != nullptr, phr,
+                                                 phi, inv_parity);
+    if (gl == 0) s_red[grp] = kvalid ? ld : 0.0;
+    if (phase && gl == 0 && kvalid) {
+      const double sg = inv_parity ? -1.0 : 1.0;
+      phase[((int64_t)iz * nspin_blk + sb) * nk + k] = make_double2(sg * phr, sg * phi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double tsum = 0;
+      for (int g = 0; g < MPB; ++g) tsum += s_red[g];
+      partials[((int64_t)iz * nspin_blk + sb) * gridDim.x + kb] = tsum;
+    }
+  }
+}
+
+// ------------------------ k x z log-det, MFMA contraction, a matrix per lane --
+// The FP32 integrand for 8-row blocks with no cross-lane step in the LU: a
+// wavefront holds 64 matrices, lane l the one of k-point (l % KPW) of its
+// KPW and frequency (l / KPW) of the ZPI in flight.  Per frequency the KPW
+// contractions X_k = P(z) - H_k G'(z) are KPW/2 tiles of 16 stacked rows
+// (two k-points each) on the matrix cores, as in k_logdet_mfma; each tile
+// is stored to the wavefront's LDS region column-major (a lane's four
+// accumulators are four consecutive rows of one column: one 16-byte
+// store), and once all ZPI frequencies are stored each lane reads its whole
+// 8 x 8 matrix into registers.  Each lane then runs the LU with partial
+// pivoting by itself: the pivot row is found by compares and brought up by
+// selects, the unit of the identity kept exact per row (M(r, :) = X(r, :) +
+// e_cu(r)), the pivot formed and the pivot product kept in FP64 (mantissa
+// and exponent).  Partial sums per (z, block of KPB k-points); phases as in
+// k_logdet.
+constexpr int kLane8Kpw = 8;  // k-points per wavefront of k_logdet_lane8 (x 8 frequencies)
+
+template <int KPW, int ZPI>
+__global__ __launch_bounds__(kOBlock) void k_logdet_lane8(int nk, int nz, int zchunk,
+                                                          int nspin_blk, const float2 *hk,
+                                                          const float2 *gmat, const float2 *pmat,
+                                                          double *partials, double2 *phase) {
+  static_assert(KPW * ZPI == 64 && KPW % 2 == 0, "one matrix per lane");
+  constexpr int NW = kOBlock / 64;
+  constexpr int KPB = NW * KPW;    // k-points per block
+  constexpr int NT = KPW / 2;      // 16-row tiles of one frequency
+  constexpr int MS = 8 * 16 + 4;   // LDS floats per matrix: 16 columns (re, im) of 8 rows
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int ti = lane & 15, tq = lane >> 4;
+  const int kl = lane % KPW, zl = lane / KPW;  // this lane's matrix
+  const int sb = blockIdx.z;
+  const int kw0 = blockIdx.x * KPB + wave * KPW;
+  const int k = kw0 + kl;
+  const bool kvalid = k < nk;
+  __shared__ float2 s_g[ZPI][64], s_p[ZPI][64];
+  __shared__ __align__(16) float s_x[NW][32 * MS];  // one round: half the matrices
+  __shared__ double s_red[NW][ZPI];
+  float *wx = s_x[wave];
+  const int zbeg = blockIdx.y * zchunk;
+  const int zend = min(zbeg + zchunk, nz);
+  for (int z0 = zbeg; z0 < zend; z0 += ZPI) {
+    __syncthreads();  // the previous frequencies are done with s_g, s_p, s_red
+    for (int e = threadIdx.x; e < ZPI * 64; e += kOBlock) {
+      const int zz = e / 64, rc = e % 64;
+      const int iz = z0 + zz;
+      float2 g = make_float2(0.f, 0.f), pz = g;
+      if (iz < zend) {
+        const int64_t zoff = ((int64_t)iz * nspin_blk + sb) * kNB * kNB + (rc / 8) * kNB + rc % 8;
+        g = gmat[zoff];
+        pz = pmat[zoff];
+      }
+      s_g[zz][rc] = g;
+      s_p[zz][rc] = pz;
+    }
+    // A operand (-H_k, reloaded per chunk from cache so that it is not held
+    // through the LU): tile t row ti is row (ti & 7) of k-point 2t + (ti >> 3)
+    float a_op[NT][4];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int kk = kw0 + 2 * t + (ti >> 3);
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int kin = 4 * c + tq;
+        float v = 0.0f;
+        if (kk < nk) {
+          const float2 h = hk[(((int64_t)kk * nspin_blk + sb) * 8 + (ti & 7)) * 8 + (kin & 7)];
+          v = kin < 8 ? h.x : h.y;
+        }
+        a_op[t][c] = -v;
+      }
+    }
+    __syncthreads();
+    const int j = ti & 7;
+    const bool oim = ti >= 8;
+    float xr[8][8], xi[8][8];
+    // two rounds of ZPI/2 frequencies (32 matrices each): the lanes of a
+    // round read their matrices before the next round's tiles overwrite
+    // the region (the wavefront's own LDS: its stores and loads run in order)
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+#pragma unroll 2
+    for (int zq = 0; zq < ZPI / 2; ++zq) {
+      const int zp = half * (ZPI / 2) + zq;
+      // B(kin, col) for kin = 4c + tq, col = ti; C: P(z) rows
+      float b_op[4];
+      mfma_f4 c_in;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int kin = 4 * c + tq;
+        const float2 g = s_g[zp][(kin & 7) * 8 + j];
+        b_op[c] = kin < 8 ? (oim ? g.y : g.x) : (oim ? g.x : -g.y);
+      }
+#pragma unroll
+      for (int rr = 0; rr < 4; ++rr) {
+        const float2 pz = s_p[zp][((4 * tq + rr) & 7) * 8 + j];
+        c_in[rr] = oim ? pz.y : pz.x;
+      }
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        mfma_f4 acc = c_in;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+          acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[t][c], b_op[c], acc, 0, 0, 0);
+        // D(4 tq + rr, ti): rows 4 (tq & 1) .. +3 of k-point 2t + tq / 2,
+        // column (j, re/im): matrix zq * KPW + 2t + tq / 2 of the round
+        const int m = zq * KPW + 2 * t + (tq >> 1);
+        *reinterpret_cast<float4 *>(&wx[m * MS + (2 * j + (oim ? 1 : 0)) * 8 + 4 * (tq & 1)]) =
+            make_float4(acc[0], acc[1], acc[2], acc[3]);
+      }
+    }
+    if ((zl >= ZPI / 2) == (half == 1)) {
+      const float4 *m4 =
+          reinterpret_cast<const float4 *>(&wx[(lane - half * (ZPI / 2) * KPW) * MS]);
+#pragma unroll
+      for (int jj = 0; jj < 8; ++jj) {
+        const float4 r0 = m4[(2 * jj) * 2], r1 = m4[(2 * jj) * 2 + 1];
+        const float4 i0 = m4[(2 * jj + 1) * 2], i1 = m4[(2 * jj + 1) * 2 + 1];
+        xr[0][jj] = r0.x; xr[1][jj] = r0.y; xr[2][jj] = r0.z; xr[3][jj] = r0.w;
+        xr[4][jj] = r1.x; xr[5][jj] = r1.y; xr[6][jj] = r1.z; xr[7][jj] = r1.w;
+        xi[0][jj] = i0.x; xi[1][jj] = i0.y; xi[2][jj] = i0.z; xi[3][jj] = i0.w;
+        xi[4][jj] = i1.x; xi[5][jj] = i1.y; xi[6][jj] = i1.z; xi[7][jj] = i1.w;
+      }
+    }
+    }  // round
+    // LU with partial pivoting of M = X + (unit of row r in column cu[r])
+    int cu[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) cu[r] = r;
+    double pm_acc = 1.0, phr = 1.0, phi = 0.0;
+    int pm_exp = 0, parity = 0;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      int p = kk;
+      float best;
+      {
+        const float mr = xr[kk][kk] + (cu[kk] == kk ? 1.0f : 0.0f);
+        best = mr * mr + xi[kk][kk] * xi[kk][kk];
+      }
+#pragma unroll
+      for (int r = kk + 1; r < 8; ++r) {
+        const float mr = xr[r][kk] + (cu[r] == kk ? 1.0f : 0.0f);
+        const float m2 = mr * mr + xi[r][kk] * xi[r][kk];
+        if (m2 > best) {
+          best = m2;
+          p = r;
+        }
+      }
+      // rows kk and p trade places; skipped when no lane of the wavefront
+      // pivots (the diagonal wins wherever M is close to the identity)
+      if (__any(p != kk)) {
+#pragma unroll
+        for (int r = kk + 1; r < 8; ++r) {
+          const bool sw = p == r;
+#pragma unroll
+          for (int jj = kk; jj < 8; ++jj) {
+            const float ar = xr[kk][jj], br = xr[r][jj], ai = xi[kk][jj], bi = xi[r][jj];
+            xr[kk][jj] = sw ? br : ar;
+            xr[r][jj] = sw ? ar : br;
+            xi[kk][jj] = sw ? bi : ai;
+            xi[r][jj] = sw ? ai : bi;
+          }
+          const int ca = cu[kk], cb = cu[r];
+          cu[kk] = sw ? cb : ca;
+          cu[r] = sw ? ca : cb;
+        }
+        parity ^= p != kk ? 1 : 0;
+      }
+      // the pivot with its unit formed in FP64 (1 + a small X keeps X's digits)
+      const double pvd = (double)xr[kk][kk] + (cu[kk] == kk ? 1.0 : 0.0);
+      const float pvr = (float)pvd;
+      const float pvi = xi[kk][kk];
+      {
+        const double pm = pvd * pvd + (double)pvi * pvi;
+        int ex;
+        pm_acc = frexp(pm_acc * pm, &ex);
+        pm_exp += ex;
+        if (phase) {
+          const double rm = 1.0 / sqrt(pm);
+          const double ur = pvd * rm, ui = pvi * rm;
+          const double nr = phr * ur - phi * ui;
+          phi = phr * ui + phi * ur;
+          phr = nr;
+        }
+      }
+      const float inv = 1.0f / (pvr * pvr + pvi * pvi);
+      float pr[8], pim[8];  // the pivot row with its unit
+#pragma unroll
+      for (int jj = kk + 1; jj < 8; ++jj) {
+        pr[jj] = xr[kk][jj] + (cu[kk] == jj ? 1.0f : 0.0f);
+        pim[jj] = xi[kk][jj];
+      }
+#pragma unroll
+      for (int r = kk + 1; r < 8; ++r) {
+        const float mr = xr[r][kk] + (cu[r] == kk ? 1.0f : 0.0f);
+        const float mi = xi[r][kk];
+        const float lr = (mr * pvr + mi * pvi) * inv;  // l = M(r, kk) / pivot
+        const float li = (mi * pvr - mr * pvi) * inv;
+#pragma unroll
+        for (int jj = kk + 1; jj < 8; ++jj) {
+          xr[r][jj] -= lr * pr[jj] - li * pim[jj];
+          xi[r][jj] -= lr * pim[jj] + li * pr[jj];
+        }
+        if (cu[r] == kk) cu[r] = -1;  // the unit sat in the eliminated column
+      }
+    }
+    const int iz = z0 + zl;
+    double ld = 0.5 * (log(pm_acc) + (double)pm_exp * 0.69314718055994530942);
+    if (!kvalid || iz >= zend) ld = 0.0;
+    if (phase && kvalid && iz < zend) {
+      const double sg = parity ? -1.0 : 1.0;
+      phase[((int64_t)iz * nspin_blk + sb) * nk + k] = make_double2(sg * phr, sg * phi);
+    }
+    // sum over the wavefront's KPW k-points of each frequency, then the block's
+#pragma unroll
+    for (int off = KPW / 2; off > 0; off >>= 1) ld += __shfl_xor(ld, off, KPW);
+    if (kl == 0) s_red[wave][zl] = ld;
+    __syncthreads();
+    if (threadIdx.x < ZPI && z0 + (int)threadIdx.x < zend) {
+      double tsum = 0;
+      for (int w = 0; w < NW; ++w) tsum += s_red[w][threadIdx.x];
+      partials[((int64_t)(z0 + threadIdx.x) * nspin_blk + sb) * gridDim.x + blockIdx.x] = tsum;
     }
   }
 }
@@ -1038,8 +1409,14 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   // in FP32 on the 2x2 BHZ grid).
   const char *lu_env = std::getenv("VCA_OMEGA_LU");
   const bool use_qr = nbt == 8 && precision == VCA_FP64 && !(lu_env && lu_env[0] == '1');
+  // FP32: the contraction on the matrix cores (VCA_OMEGA_MFMA=0: on the
+  // vector ALUs, k_logdet); 8-row blocks with one matrix per lane
+  // (VCA_OMEGA_MFMA=row: a row per lane, k_logdet_mfma<8>)
+  const char *mf_env = std::getenv("VCA_OMEGA_MFMA");
+  const bool mfma = precision != VCA_FP64 && !(mf_env && mf_env[0] == '0');
+  const bool lane8 = mfma && nbt == 8 && !(mf_env && mf_env[0] == 'r');
   const int lanes = use_qr ? 4 : nbt;
-  const int mpb = kOBlock / lanes;
+  const int mpb = lane8 ? (kOBlock / 64) * kLane8Kpw : kOBlock / lanes;
   const int nkb = (host.nk + mpb - 1) / mpb;
   const int zchunk = 16;
   const dim3 grid(nkb, (nz + zchunk - 1) / zchunk, host.nspin_blk);
@@ -1066,12 +1443,24 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
     hipLaunchKernelGGL(k_b_to_f32, dim3((unsigned)((nm + 255) / 256)), dim3(256), 0, st, nm,
                        d_p64, d_p32);
     if (e0) VCA_HIP(hipEventRecord(e0, st));
-    if (nbt == 8)
-      hipLaunchKernelGGL((k_logdet<float, 8>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
-                         host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
-    else
-      hipLaunchKernelGGL((k_logdet<float, 16>), grid, dim3(kOBlock), 0, st, host.nk, nz,
+    if (lane8) {
+      hipLaunchKernelGGL((k_logdet_lane8<kLane8Kpw, 64 / kLane8Kpw>), grid, dim3(kOBlock), 0, st, host.nk, nz,
                          zchunk, host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
+    } else if (nbt == 8) {
+      if (mfma)
+        hipLaunchKernelGGL((k_logdet_mfma<8>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
+                           host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
+      else
+        hipLaunchKernelGGL((k_logdet<float, 8>), grid, dim3(kOBlock), 0, st, host.nk, nz,
+                           zchunk, host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
+    } else {
+      if (mfma)
+        hipLaunchKernelGGL((k_logdet_mfma<16>), grid, dim3(kOBlock), 0, st, host.nk, nz, zchunk,
+                           host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
+      else
+        hipLaunchKernelGGL((k_logdet<float, 16>), grid, dim3(kOBlock), 0, st, host.nk, nz,
+                           zchunk, host.nspin_blk, d_hk32, d_g32, d_p32, d_part, phase);
+    }
   }
   VCA_HIP(hipGetLastError());
   if (e1) VCA_HIP(hipEventRecord(e1, st));
