@@ -518,11 +518,14 @@ def omega_section(args, cpu):
                                      "(k_logdet_mfma<8>)"},
         "omega_full_solve": om64,
         "full_solve_seconds": t_solve,
-        "roofline": {"bound": "valu", "flops_per_eval": fl,
-                     "fp64": {"achieved": a64, "peak": FP64_PEAK_TFLOPS,
+        "roofline": {"flops_per_eval": fl,
+                     "fp64": {"bound": "valu", "achieved": a64, "peak": FP64_PEAK_TFLOPS,
                               "unit": "TFLOP/s", "frac": a64 / FP64_PEAK_TFLOPS},
-                     "mixed": {"achieved": a32, "peak": FP32_PEAK_TFLOPS,
-                               "unit": "TFLOP/s", "frac": a32 / FP32_PEAK_TFLOPS}},
+                     "mixed": {"bound": "mfma", "achieved": a32, "peak": FP32_PEAK_TFLOPS,
+                               "unit": "TFLOP/s", "frac": a32 / FP32_PEAK_TFLOPS,
+                               "note": "contraction (8 nb^3 per block) on the FP32 matrix "
+                                       "cores, LU (8 nb^3 / 3) on the vector ALUs; peak: "
+                                       "dense FP32 MFMA"}},
     }
     if cpu:
         try:

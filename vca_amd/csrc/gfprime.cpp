@@ -85,6 +85,12 @@ struct Cursor {
   const std::vector<char> &b;
   size_t pos = 0;
   const std::string &path;
+  // n items of at least `bytes` bytes each must fit the rest of the record
+  // (checked before anything is sized from a count read from the file)
+  void need(int64_t n, size_t bytes) const {
+    if ((uint64_t)n > (b.size() - pos) / bytes)
+      fail(VCA_ERR_ARG, "%s: GF entry shorter than its counts", path.c_str());
+  }
   int32_t i32() {
     if (pos + 4 > b.size()) fail(VCA_ERR_ARG, "%s: GF entry shorter than its counts", path.c_str());
     int32_t v;
@@ -270,21 +276,22 @@ std::vector<GfEntry> gfprime_read(const std::string &path, bool formatted, size_
     char buf[1 << 16];
     size_t n;
     while ((n = std::fread(buf, 1, sizeof buf, f)) > 0) in.s.append(buf, n);
+    // the vectors grow as values are read, so a count larger than the data
+    // that follows it fails at the end of the text, not in an allocation
     for (size_t k = 0; k < nelem; ++k) {
       const int32_t ns = in.i32();
       check_count(ns, "state", path);
-      el[k].resize(ns);
-      for (auto &st : el[k]) {
+      for (int32_t a = 0; a < ns; ++a) {
+        el[k].emplace_back();
         const int32_t nc = in.i32();
         check_count(nc, "channel", path);
-        st.resize(nc);
-        for (auto &ch : st) {
+        for (int32_t b = 0; b < nc; ++b) {
+          el[k].back().emplace_back();
+          GfChanIO &ch = el[k].back().back();
           const int32_t ne = in.i32();
           check_count(ne, "pole", path);
-          ch.p.resize(ne);
-          ch.w.resize(ne);
-          for (auto &v : ch.p) v = in.c128();
-          for (auto &v : ch.w) v = in.c128();
+          for (int32_t e = 0; e < ne; ++e) ch.p.push_back(in.c128());
+          for (int32_t e = 0; e < ne; ++e) ch.w.push_back(in.c128());
         }
       }
     }
@@ -298,14 +305,17 @@ std::vector<GfEntry> gfprime_read(const std::string &path, bool formatted, size_
       Cursor c{rec, 0, path};
       const int32_t ns = c.i32();
       check_count(ns, "state", path);
+      c.need(ns, 4);  // a state holds at least its channel count
       el[k].resize(ns);
       for (auto &st : el[k]) {
         const int32_t nc = c.i32();
         check_count(nc, "channel", path);
+        c.need(nc, 4);  // a channel holds at least its pole count
         st.resize(nc);
         for (auto &ch : st) {
           const int32_t ne = c.i32();
           check_count(ne, "pole", path);
+          c.need(ne, 32);  // a pole and its weight, complex(8) each
           ch.p.resize(ne);
           ch.w.resize(ne);
           for (auto &v : ch.p) v = c.c128();

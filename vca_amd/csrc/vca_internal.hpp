@@ -181,7 +181,7 @@ struct LanczosDev {
 };
 
 // Per-row diagonal energy, string and hop count in one 16-byte record.
-struct __align__(16) RowMeta {
+struct alignas(16) RowMeta {
   double e;
   uint32_t s;
   int32_t cnt;
