@@ -792,9 +792,34 @@ __global__ __launch_bounds__(BS) void k_hxv_plane(const HxvArgs a) {
     const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
     const uint4 *ent4 = reinterpret_cast<const uint4 *>(a.ent4_up);
     const int last = a.up_nc - 1;
+    // two planes of a column of at most kPlaneRows rows per lane: each
+    // complex element is loaded once, its imaginary part held in registers
+    // for the second plane (no second strided pass over the column)
+    constexpr int kPlaneRows = 13;
+    const bool hold = NPL == 2 && dimup <= kPlaneRows * BS;
+    R xim[kPlaneRows];
     for (int pl = 0; pl < NPL; ++pl) {
       __syncthreads();  // the previous plane's gathers are done
-      for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[(int64_t)f * NPL + pl];
+      if (hold && pl == 0) {
+        const cplx<R> *vc = reinterpret_cast<const cplx<R> *>(vcol);
+#pragma unroll
+        for (int q = 0; q < kPlaneRows; ++q) {
+          const int f = (int)threadIdx.x + q * BS;
+          if (f < dimup) {
+            const cplx<R> x = vc[f];
+            s_col[f] = x.x;
+            xim[q] = x.y;
+          }
+        }
+      } else if (hold) {
+#pragma unroll
+        for (int q = 0; q < kPlaneRows; ++q) {
+          const int f = (int)threadIdx.x + q * BS;
+          if (f < dimup) s_col[f] = xim[q];
+        }
+      } else {
+        for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[(int64_t)f * NPL + pl];
+      }
       __syncthreads();
       for (int base0 = 0; base0 < dimup; base0 += BS) {
         if (base0 + (int)(threadIdx.x & ~63u) >= dimup) break;
@@ -1723,10 +1748,20 @@ __device__ inline void acc_elem(float2 &a, const float2 &b) {
 }
 
 // E: complex element (cplx<R>) or, in real-vector mode, a real.
+// Tile edge of k_transpose: 64 for elements up to 8 bytes; 32 for 16-byte
+// (complex128) elements, whose 64 x 64 tile (66.5 KB of LDS) would leave
+// two workgroups per CU to keep the HBM streams busy.
+template <typename E>
+constexpr int tr_tile() {
+  return sizeof(E) > 8 ? 32 : kTile;
+}
+
 template <typename E, bool ADD>
 __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
   using C = E;
-  __shared__ C s[kTile][kTile + 1];
+  constexpr int T = tr_tile<E>();
+  constexpr int kRows = kBlock / T;  // rows of the tile per pass of the workgroup
+  __shared__ C s[T][T + 1];
   const int64_t off = (int64_t)blockIdx.z * a.vstride;
   const C *in = static_cast<const C *>(a.in0) + off;
   if (a.lz) {
@@ -1735,21 +1770,20 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
     if (lz->k & 1) in = static_cast<const C *>(a.in1) + off;
   }
   C *out = static_cast<C *>(a.out) + off;
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int i0 = blockIdx.x * kTile, j0 = blockIdx.y * kTile;
-  constexpr int kWaves = kBlock / 64;
+  const int lane = threadIdx.x % T, w = threadIdx.x / T;
+  const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
   const int i = i0 + lane;
 #pragma unroll
-  for (int t = 0; t < kTile / kWaves; ++t) {
-    const int jj = w + kWaves * t;
+  for (int t = 0; t < T / kRows; ++t) {
+    const int jj = w + kRows * t;
     const int j = j0 + jj;
     if (i < a.ni && j < a.nj) s[jj][lane] = in[(int64_t)j * a.ni + i];
   }
   __syncthreads();
   const int j = j0 + lane;
 #pragma unroll
-  for (int t = 0; t < kTile / kWaves; ++t) {
-    const int ii = w + kWaves * t;
+  for (int t = 0; t < T / kRows; ++t) {
+    const int ii = w + kRows * t;
     const int ir = i0 + ii;
     if (ir < a.ni && j < a.nj) {
       C *o = out + (int64_t)ir * a.nj + j;
@@ -2688,8 +2722,8 @@ static bool nt_streams(const Sector &sec, int nb) {
 
 template <typename E>
 static void launch_transpose(Sector &sec, const TrArgs &t, bool add, int nb) {
-  const dim3 g((unsigned)((t.ni + kTile - 1) / kTile), (unsigned)((t.nj + kTile - 1) / kTile),
-               (unsigned)nb);
+  constexpr int T = tr_tile<E>();
+  const dim3 g((unsigned)((t.ni + T - 1) / T), (unsigned)((t.nj + T - 1) / T), (unsigned)nb);
   if (add) hipLaunchKernelGGL((k_transpose<E, true>), g, dim3(kBlock), 0, sec.stream, t);
   else hipLaunchKernelGGL((k_transpose<E, false>), g, dim3(kBlock), 0, sec.stream, t);
   VCA_HIP(hipGetLastError());
