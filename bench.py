@@ -42,7 +42,8 @@ on the sector's stream).  Reported beside it in the same JSON line:
     and all cores; the Omega integrand (numpy) on a sample of frequencies.
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
-                    [--only c4|hxv|omega|full|c3|c1|square3x4]
+                    [--only c4|hxv|omega|full|c3|c1|square3x4|c4_solve]
+                    [--c4-form plain|fp64|mixed] [--save F.npz] [--compare A.npz B.npz]
                     [--no-full] [--no-c3] [--no-cpu]
 """
 from __future__ import annotations
@@ -257,6 +258,79 @@ def run_c3():
     log(f"2x5 every-sector solve: {c3['seconds']:.1f}s Omega={omc:.10f} "
         f"(diag {r['t_diag']:.2f}s, GF {r['t_gf']:.2f}s, {r['nchains']} chains)")
     return c3
+
+
+C4_FORMS = {
+    # the plain FP64 GPU form: complex128 vectors everywhere, one chain per
+    # launch, every Lanczos step launched eagerly (no graphs)
+    "plain": ("fp64", {"VCA_CHAIN_BATCH": "1", "VCA_NO_GRAPHS": "1", "VCA_REAL_CHAINS": "0",
+                       "VCA_REAL_GS": "0"}),
+    # FP64 with the batching form: chains of a target sector in one launch,
+    # 16-step graphs, real vectors where the operator and state are real
+    "fp64": ("fp64", {}),
+    # the batched mixed-precision form
+    "mixed": ("mixed", {}),
+}
+
+
+def run_c4_solve(form, save=None):
+    """vca_solve of the 16-site bath cluster end to end: the Lanczos ground
+    state of the 165,636,900-state sector (8,8), the 240 excitation chains of
+    200 steps on the 147,232,800-state sectors (9,8) and (7,8), and Omega on
+    the 16x16 k-grid, in one of C4_FORMS; with save, E0, Omega and G'_ij at 64
+    Matsubara and 64 real-axis points go to an .npz for the comparison of
+    two forms (each form needs minutes, so they run as separate commands)."""
+    import numpy as np
+    from vca_amd import configs
+    from vca_amd.api import VcaSolver
+
+    prec, env = C4_FORMS[form]
+    os.environ.update(env)
+    os.environ.setdefault("VCA_PROGRESS", "1")
+    p, hloc, hk, bh, bv = configs.config_bath_2x4(nk=16, precision=prec)
+    s = VcaSolver(p)
+    log(f"C4 solve, form {form} ({prec}; {env or 'default knobs'})")
+    t0 = time.perf_counter()
+    om = s.vca_solve(hloc, hk, bh, bv)
+    secs = time.perf_counter() - t0
+    r = s.results()
+    out = {"workload": "8-site 2x4 Hubbard cluster + 8 bath levels, U=4: ground state of "
+                       "sector (8,8) (165,636,900 states), 240 excitation chains of 200 steps "
+                       "on sectors (9,8) and (7,8) (147,232,800 states each), Omega on the "
+                       "16x16 k-grid",
+           "form": form, "precision": prec, "knobs": env, "seconds": secs, "sft": om,
+           **{k: r[k] for k in ("omegaprime", "omega_integral", "e0_mixed", "nstates",
+                                "nchains", "hxv_count", "t_diag", "t_gf", "t_omega",
+                                "omega_evals")},
+           "hxv_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9}
+    log(f"C4 solve {form}: {secs:.1f} s (diag {r['t_diag']:.1f}, chains {r['t_gf']:.1f}, Omega "
+        f"{r['t_omega']:.2f}), {r['nchains']} chains, Omega'={r['omegaprime']:.12f}, "
+        f"SFT={om:.12f}")
+    if save:
+        wm = np.pi / 100.0 * (2 * np.arange(64) + 1)
+        wr = np.linspace(-6.0, 6.0, 64)
+        gm = np.stack([s.vca_gf_cluster(1j * w)[:, :, 0, 0, 0, 0] for w in wm])
+        gr = np.stack([s.vca_gf_cluster(w + 0.05j)[:, :, 0, 0, 0, 0] for w in wr])
+        np.savez(save, gm=gm, gr=gr, wm=wm, wr=wr, omegaprime=r["omegaprime"],
+                 omega_integral=r["omega_integral"], sft=om)
+        out["saved"] = save
+    return out
+
+
+def compare_c4_solves(ref_path, path):
+    """Differences of a C4 solve saved by run_c4_solve against another (the
+    reference form): E0, Omega, and each G'_ij against its own largest value
+    over the points."""
+    import numpy as np
+    a, b = np.load(ref_path), np.load(path)
+    out = {"e0_rel_err": float(abs(b["omegaprime"] - a["omegaprime"]) / abs(a["omegaprime"])),
+           "sft_rel_err": float(abs(b["sft"] - a["sft"]) / abs(a["sft"])),
+           "omega_integral_rel_err": float(abs(b["omega_integral"] - a["omega_integral"]) /
+                                           abs(a["omega_integral"]))}
+    for key in ("gm", "gr"):
+        scale = np.maximum(np.abs(a[key]).max(axis=0), 1e-8 * np.abs(a[key]).max())
+        out[f"{key}_max_rel_err"] = float((np.abs(b[key] - a[key]) / scale).max())
+    return out
 
 
 def c4_section(args, cpu, gs=True):
@@ -560,14 +634,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--omega-nk", type=int, default=128)
     ap.add_argument("--omega-nfreq", type=int, default=8192)
+    ap.add_argument("--c4-form", choices=sorted(C4_FORMS), default="mixed",
+                    help="the form of the c4_solve run")
+    ap.add_argument("--save", default=None, help="c4_solve: .npz of E0, Omega and G'")
+    ap.add_argument("--compare", nargs=2, metavar=("REF", "OTHER"), default=None,
+                    help="the differences of two saved c4_solve runs (no GPU)")
     ap.add_argument("--only", choices=["all", "c4", "hxv", "omega", "full", "c3", "c1",
-                                       "square3x4"],
+                                       "square3x4", "c4_solve"],
                     default="all",
                     help="c4: the headline H x v and ground state on the 165,636,900-state "
                          "sector; hxv: the 3x4 and 165,636,900-state H x v plus a 256 MiB "
                          "copy that calibrates the HBM byte counters; omega: the Omega "
                          "integrand; full: the end-to-end 3x4 solve; c3: the every-sector "
-                         "2x5 solve; c1: the 4-site chain; square3x4: the 3x4 sector")
+                         "2x5 solve; c1: the 4-site chain; square3x4: the 3x4 sector; "
+                         "c4_solve: the 16-site bath cluster end to end in --c4-form")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); how the path would "
@@ -576,6 +656,12 @@ def main():
         sys.exit(2)
     cpu = not args.no_cpu
 
+    if args.compare:
+        print(json.dumps(compare_c4_solves(*args.compare)), flush=True)
+        return
+    if args.only == "c4_solve":
+        print(json.dumps({"c4_solve": run_c4_solve(args.c4_form, args.save)}), flush=True)
+        return
     if args.only == "omega":
         print(json.dumps({"omega": omega_section(args, cpu)}), flush=True)
         return

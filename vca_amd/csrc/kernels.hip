@@ -1597,9 +1597,13 @@ __global__ __launch_bounds__(BS) void k_hxv_buf_real2(const HxvArgs a) {
 // and 2p + 1 of the pair p = workgroup), for real vectors whose two columns
 // fit the LDS together (FP32: 2 x 51.5 KB on the 12,870-row bath sector):
 // every hop entry and amplitude read from the table serves both columns,
-// halving the entry stream from L2 and a third of the LDS reads per output.
-// LDS: the amplitudes and a zero at 0, column 0 at the one-plane table's
-// offset, column 1 cstride bytes after it.  One partial of <v|Hv> per pair.
+// halving the entry stream from L2.  The two columns are staged interleaved
+// (element f of both at o_col + 2 f RSZ), so one 2 RSZ-byte LDS read per hop
+// fetches both: the LDS serves 32 lanes per cycle whatever the width up to
+// 8 bytes, so a 4-byte read per column would take twice the cycles.  The
+// one-plane table's address ya of element f (o_col + f RSZ) becomes
+// 2 ya - o_col.  LDS: the amplitudes and a zero at 0, then the column pair.
+// One partial of <v|Hv> per pair.
 template <typename R, int BS>
 __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1607,10 +1611,8 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
   constexpr int RSZ = sizeof(R);
   const int dimup = a.dimup;
   const uint32_t o_col = (uint32_t)buf_lds(a.namp_up, 0, 0, RSZ).col;
-  const uint32_t cstride = (uint32_t)align16(dimup * RSZ);
   R *s_amp = reinterpret_cast<R *>(smem);
-  R *s_col0 = reinterpret_cast<R *>(smem + o_col);
-  R *s_col1 = reinterpret_cast<R *>(smem + o_col + cstride);
+  cplx<R> *s_pair = reinterpret_cast<cplx<R> *>(smem + o_col);
   const int npair = (a.dimdw + 1) / 2;
   const int pair = xcd_column(blockIdx.x, npair);
   const int idw0 = 2 * pair;
@@ -1639,8 +1641,10 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
     for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp[t] = g_amp[t].x;
     if (threadIdx.x == 0) s_amp[a.namp_up] = (R)0;  // padding hops
     for (int f = threadIdx.x; f < dimup; f += BS) {
-      s_col0[f] = vcol0[f];
-      s_col1[f] = vcol1[f];
+      cplx<R> y;
+      y.x = vcol0[f];
+      y.y = vcol1[f];
+      s_pair[f] = y;
     }
     __syncthreads();
     const double edw0 = a.skip_diag ? 0.0 : a.e_dw[idw0];
@@ -1655,7 +1659,8 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
       const int row = min(base, dimup - 1);
       const RowMeta m = a.meta_up[row];
       const int nu = base < dimup ? m.cnt : 0;
-      const R x0 = s_col0[row], x1 = s_col1[row];
+      const cplx<R> xp = s_pair[row];
+      const R x0 = xp.x, x1 = xp.y;
       R acc0 = (R)0, acc1 = (R)0;
       if (!a.skip_diag) {
         acc0 = (R)(m.e + edw0 + cross_energy(a.cross, m.s, sdw0)) * x0;
@@ -1673,18 +1678,17 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
           for (int p = 0; p < PF; ++p) {
             if (c0 + p < nc_w) {
               const uint32_t ee[4] = {e4[p].x, e4[p].y, e4[p].z, e4[p].w};
-              R y0[4], y1[4], h[4];
+              cplx<R> y[4];
+              R h[4];
 #pragma unroll
               for (int t = 0; t < 4; ++t) {
-                const uint32_t ya = ee[t] & kLdsAddrMask;
-                y0[t] = lds_load_re<R>(ya);
-                y1[t] = lds_load_re<R>(ya + cstride);
+                y[t] = lds_load<R>(2 * (ee[t] & kLdsAddrMask) - o_col);
                 h[t] = lds_load_re<R>(ee[t] >> kLdsAddrBits);
               }
 #pragma unroll
               for (int t = 0; t < 4; ++t) {
-                acc0 = fma(h[t], y0[t], acc0);
-                acc1 = fma(h[t], y1[t], acc1);
+                acc0 = fma(h[t], y[t].x, acc0);
+                acc1 = fma(h[t], y[t].y, acc1);
               }
             }
             e4[p] = ent4[(int64_t)min(c0 + p + PF, last) * dimup + row];
@@ -2205,25 +2209,37 @@ static int grid_for(int64_t n) {
   return (int)g;
 }
 
-// Hop order of a one-plane table (DevSpin::ent4p), chosen per wavefront
-// (64 consecutive rows, one per lane) so that the LDS gathers of one slot
-// spread over the banks.  The LDS serves 256 bytes per cycle from 64 banks
-// of 4 bytes: 32 lanes of 8-byte (FP64) or 64 lanes of 4-byte (FP32)
-// reads; lanes reading one address share it (broadcast), distinct
-// addresses in one bank group are served one after another.  The order of
-// a row's hops is free (they are summed), so slot by slot each lane (the
-// one with most hops left first) takes the hop whose element and amplitude
-// addresses add the fewest new addresses to already used bank groups.  On
-// the 12,870-row bath sector this cuts the modelled LDS cycles of a gather
-// pass from 1.65x to 1.17x of conflict-free.  Rows keep their hops; only
-// the slots move.  VCA_BANK_ORDER=0 keeps the table's order.
+// Hop order of a one-plane table (DevSpin::ent4p), chosen per half
+// wavefront (32 consecutive rows, one per lane) so that the LDS gathers of
+// one slot spread over the banks.  The LDS has 64 banks of 4 bytes and
+// serves 32 lanes per cycle, for 4- and 8-byte reads alike; lanes reading
+// one address share it (broadcast), distinct addresses in one bank are
+// served one after another.  The reads of a slot: the element, 8 bytes
+// (FP64 one-plane, or the FP32 column pair of k_hxv_plane2 at
+// 2 ya - col), and the amplitude, rsz bytes.  The order of a row's hops is
+// free (they are summed), so slot by slot each lane (the one with most hops
+// left first) takes the hop whose element and amplitude addresses add the
+// fewest new addresses to already used banks.  Rows keep their hops; only
+// the slots move: perm[slot] is the slot of the original table the entry
+// came from (a padding slot where the row has no hop left), so that the
+// complex table (DevSpin::ent4, the same hops at complex element addresses,
+// whose 8-byte FP32 reads fall in the same bank pairs up to a shift) can
+// take the same order.  VCA_BANK_ORDER=0 keeps the tables' order.
 static void order_plane_hops(std::vector<uint32_t> &ent4, const std::vector<int32_t> &cnt,
-                             int dim, int nc, int rsz, uint32_t pad) {
-  const int per = rsz / 4 > 0 ? rsz / 4 : 1;  // 4-byte banks per element
-  const int groups = 64 / per;                // bank groups of one LDS cycle
-  const int dom = groups;                     // lanes per LDS cycle
-  std::vector<std::vector<uint32_t>> rem(dom);
-  std::vector<int> load_e(groups), load_a(groups), order(dom);
+                             int dim, int nc, int rsz, uint32_t pad, uint32_t col,
+                             std::vector<int32_t> &perm) {
+  constexpr int dom = 32;  // lanes served in one LDS cycle
+  // bank groups: an 8-byte element read takes a pair of banks (32 groups);
+  // an amplitude read a pair (FP64) or one bank (FP32, 64 groups)
+  auto group_e = [&](uint32_t ya) {
+    const uint32_t addr = rsz == 8 ? ya : 2 * ya - col;
+    return (int)((addr / 8) % 32);
+  };
+  auto group_a = [&](uint32_t aa) { return (int)((aa / rsz) % (uint32_t)(rsz == 8 ? 32 : 64)); };
+  std::vector<std::vector<std::pair<uint32_t, int>>> rem(dom);
+  std::vector<int> load_e(32), load_a(64), order(dom);
+  perm.resize(ent4.size());
+  for (size_t i = 0; i < perm.size(); ++i) perm[i] = (int32_t)(i / 4 / dim * 4 + i % 4);
   std::vector<int64_t> stamp_e((size_t)(kLdsAddrMask + 1) / rsz + 1, -1);
   std::vector<int64_t> stamp_a((size_t)kLdsAmpLimit / rsz + 1, -1);
   int64_t gen = 0;
@@ -2237,7 +2253,7 @@ static void order_plane_hops(std::vector<uint32_t> &ent4, const std::vector<int3
       const int nl = std::min(i1, d0 + dom) - d0;
       for (int l = 0; l < nl; ++l) {
         rem[l].clear();
-        for (int q = 0; q < cnt[d0 + l]; ++q) rem[l].push_back(ent4[slot_of(q, d0 + l)]);
+        for (int q = 0; q < cnt[d0 + l]; ++q) rem[l].push_back({ent4[slot_of(q, d0 + l)], q});
       }
       for (int q = 0; q < nslot; ++q) {
         ++gen;
@@ -2249,32 +2265,36 @@ static void order_plane_hops(std::vector<uint32_t> &ent4, const std::vector<int3
         for (int k = 0; k < nl; ++k) {
           const int l = order[k];
           uint32_t w = pad;
+          int from = cnt[d0 + l];  // a padding slot of the row
           if (!rem[l].empty()) {
             size_t best = 0;
             int bc = std::numeric_limits<int>::max();
             for (size_t c = 0; c < rem[l].size() && bc > 0; ++c) {
-              const uint32_t e = (rem[l][c] & kLdsAddrMask) / rsz, a = (rem[l][c] >> kLdsAddrBits) / rsz;
-              const int cost = (stamp_e[e] == gen ? 0 : load_e[e % groups]) +
-                               (stamp_a[a] == gen ? 0 : load_a[a % groups]);
+              const uint32_t e = rem[l][c].first;
+              const uint32_t ya = e & kLdsAddrMask, aa = e >> kLdsAddrBits;
+              const int cost = (stamp_e[ya / rsz] == gen ? 0 : load_e[group_e(ya)]) +
+                               (stamp_a[aa / rsz] == gen ? 0 : load_a[group_a(aa)]);
               if (cost < bc) {
                 bc = cost;
                 best = c;
               }
             }
-            w = rem[l][best];
+            w = rem[l][best].first;
+            from = rem[l][best].second;
             rem[l][best] = rem[l].back();
             rem[l].pop_back();
           }
-          const uint32_t e = (w & kLdsAddrMask) / rsz, a = (w >> kLdsAddrBits) / rsz;
-          if (stamp_e[e] != gen) {
-            stamp_e[e] = gen;
-            ++load_e[e % groups];
+          const uint32_t ya = w & kLdsAddrMask, aa = w >> kLdsAddrBits;
+          if (stamp_e[ya / rsz] != gen) {
+            stamp_e[ya / rsz] = gen;
+            ++load_e[group_e(ya)];
           }
-          if (stamp_a[a] != gen) {
-            stamp_a[a] = gen;
-            ++load_a[a % groups];
+          if (stamp_a[aa / rsz] != gen) {
+            stamp_a[aa / rsz] = gen;
+            ++load_a[group_a(aa)];
           }
           ent4[slot_of(q, d0 + l)] = w;
+          perm[slot_of(q, d0 + l)] = from;
         }
       }
     }
@@ -2373,9 +2393,23 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
       VCA_HIP(hipStreamSynchronize(st));
       const uint32_t pad = (uint32_t)buf_lds(namp, 0, 0, rsz).col |
                            ((uint32_t)(namp * rsz) << kLdsAddrBits);
-      order_plane_hops(h4, hc, sp.dim, nc, rsz, pad);
+      std::vector<int32_t> perm;
+      order_plane_hops(h4, hc, sp.dim, nc, rsz, pad, (uint32_t)buf_lds(namp, 0, 0, rsz).col,
+                       perm);
       VCA_HIP(hipMemcpyAsync(sp.ent4p, h4.data(), sizeof(uint32_t) * h4.size(),
                              hipMemcpyHostToDevice, st));
+      if (sp.lds_form && esz == 8) {  // the complex64 table in the same order
+        std::vector<uint32_t> c4(h4.size()), o4(h4.size());
+        VCA_HIP(hipMemcpyAsync(c4.data(), sp.ent4, sizeof(uint32_t) * c4.size(),
+                               hipMemcpyDeviceToHost, st));
+        VCA_HIP(hipStreamSynchronize(st));
+        for (size_t i = 0; i < o4.size(); ++i) {
+          const int row = (int)(i / 4 % (size_t)sp.dim), q = perm[i];
+          o4[i] = c4[((size_t)(q / 4) * sp.dim + row) * 4 + q % 4];
+        }
+        VCA_HIP(hipMemcpyAsync(sp.ent4, o4.data(), sizeof(uint32_t) * o4.size(),
+                               hipMemcpyHostToDevice, st));
+      }
       VCA_HIP(hipStreamSynchronize(st));
     }
   }

@@ -99,8 +99,9 @@ struct Solver {
   double emin = 0, zeta = 1, max_exc = 0;
   std::unique_ptr<OmegaDev> om;
   bool om_ready = false;
-  bool gf_loaded = false;
-  std::mutex res_mu;  // res updates from the concurrent sector solves  // pole store read from a file (vca_read_gfprime)
+  bool gf_loaded = false;  // pole store read from a file (vca_read_gfprime)
+  std::mutex res_mu;       // res updates from the concurrent sector solves
+  double t_start = 0;      // steady-clock seconds at the start of vca_solve
   std::vector<KeptState> states;
   ~Solver();
 };
@@ -1222,6 +1223,38 @@ static void parallel_for(int n, Fn &&fn) {
   for (auto &x : th) x.join();
 }
 
+// VCA_PROGRESS=1: one line on stderr per finished stage and chain batch,
+// with the seconds since the solve began (long sectors run for minutes).
+static bool progress_on() {
+  static const bool on = [] {
+    const char *e = std::getenv("VCA_PROGRESS");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+static void progress(double t0, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static void progress(double t0, const char *fmt, ...) {
+  if (!progress_on()) return;
+  const double t =
+      std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  std::fprintf(stderr, "vca [%8.2f s] ", t - t0);
+  va_list ap;
+  va_start(ap, fmt);
+  std::vfprintf(stderr, fmt, ap);
+  va_end(ap);
+  std::fputc('\n', stderr);
+  std::fflush(stderr);
+}
+
+// VCA_CHAIN_BATCH=n: at most n chains per launch (1: one chain at a time,
+// the unbatched form); unset or 0: as many as half the free memory holds.
+static int chain_batch_cap() {
+  const char *e = std::getenv("VCA_CHAIN_BATCH");
+  const int n = e ? std::atoi(e) : 0;
+  return n > 0 ? n : 512;
+}
+
 // Chains of the same target sector run together: one launch per Lanczos
 // step covers a batch of chains (grid.y = chain), their vectors stacked
 // [chain][dim] in HBM, each chain with its own alpha/beta in device memory.
@@ -1292,7 +1325,7 @@ static void build_gf(Solver &s) {
                           (size_t)js.tbuf_cap * js.vec_bytes();
       const size_t budget = (free_b + have) / 2;
       int nb_max = (int)std::max<size_t>(1, budget / per_chain);
-      nb_max = std::min(nb_max, 512);
+      nb_max = std::min(nb_max, chain_batch_cap());
       // the chains whose operator has real coefficients run on real vectors
       // (real Hamiltonian, a real state); the others (the c_i - i c_j
       // channels) on complex ones, in batches of their own
@@ -1361,6 +1394,9 @@ static void build_gf(Solver &s) {
         parallel_for(nb, post);
         for (int c = 0; c < nb; ++c) s.max_exc = std::max(s.max_exc, cmax[c]);
         s.res.nchains += nb;
+        progress(s.t_start, "state %zu sector (%d,%d) %s chains %zu-%zu of %zu, %d steps",
+                 ist, g.first[0], g.first[1], pr ? "real" : "complex", b0 + 1, b0 + nb,
+                 idx.size(), nlanc);
       }
       }  // the complex and the real-vector part
     }
@@ -1871,12 +1907,16 @@ int vca_solve(vca_solver *sv, const double *hloc, const double *hk, int64_t nk,
   s->hk.assign(reinterpret_cast<const zc *>(hk), reinterpret_cast<const zc *>(hk) + per_k * nk);
   s->nk = nk;
   double t0 = now();
+  s->t_start = t0;
   diagonalize(*s);
   double t1 = now();
+  progress(t0, "ground state: %zu kept states", s->states.size());
   build_gf(*s);
   double t2 = now();
+  progress(t0, "excitation chains: %d", s->res.nchains);
   omega_stage(*s);
   double t3 = now();
+  progress(t0, "Omega: %lld (k, z) evaluations", (long long)s->res.omega_evals);
   s->res.t_diag = t1 - t0;
   s->res.t_gf = t2 - t1;
   s->res.t_omega = t3 - t2;
