@@ -125,6 +125,16 @@ __device__ __host__ inline SgnLds sgn_lds(int dimup, int esz) {
   return l;
 }
 
+// One-plane hop entries (element at o_col + f rsz) readdressed for the
+// interleaved column pair of k_hxv_plane2 (element f at o_col + 2 f rsz):
+// the element address ya becomes 2 ya - o_col, the amplitude bits stay.
+__global__ void k_pair_addr(const uint32_t *ent4p, uint32_t *ent4q, int64_t n, uint32_t col) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t e = ent4p[i];
+  ent4q[i] = (2 * (e & kLdsAddrMask) - col) | (e & ~kLdsAddrMask);
+}
+
 // Row records and 4-hop packing of the ELL table for the buffer-load H x v,
 // each hop as the two LDS byte addresses its gather reads (element of the
 // staged column, signed amplitude) for elements of esz bytes and namp
@@ -247,6 +257,30 @@ __device__ inline void cmac(R &ar, R &ai, const cplx<R> &h, const cplx<R> &y) {
   }
 }
 
+// Copies n elements into LDS with each lane's loads of a chunk of U
+// elements issued before any of its stores (indices clamped to n - 1, so
+// the loads are unconditional and need no branch): the HBM latency of a
+// staged column is paid once per chunk instead of once per element.
+template <int BS, typename T>
+constexpr int stage_unroll() {
+  return sizeof(T) >= 16 ? 8 : 16;
+}
+
+template <int BS, typename T, typename Ld, typename St>
+__device__ inline void stage_copy(int n, Ld ld, St st) {
+  constexpr int U = stage_unroll<BS, T>();
+  for (int f0 = 0; f0 < n; f0 += U * BS) {
+    T x[U];
+#pragma unroll
+    for (int q = 0; q < U; ++q) x[q] = ld(min(f0 + (int)threadIdx.x + q * BS, n - 1));
+#pragma unroll
+    for (int q = 0; q < U; ++q) {
+      const int f = f0 + (int)threadIdx.x + q * BS;
+      if (f < n) st(f, x[q]);
+    }
+  }
+}
+
 template <typename R, bool REAL_AMP, bool STAGE, bool ONECOL>
 __global__ __launch_bounds__(kBlock) void k_hxv(const HxvArgs a) {
   using C = cplx<R>;
@@ -282,7 +316,8 @@ __global__ __launch_bounds__(kBlock) void k_hxv(const HxvArgs a) {
 
   if (!done) {
     if (STAGE)
-      for (int f = threadIdx.x; f < nel; f += kBlock) s_col[f] = vcol[f];
+      stage_copy<kBlock, C>(nel, [&](int f) { return vcol[f]; },
+                            [&](int f, const C &x) { s_col[f] = x; });
     __syncthreads();
     for (int f = threadIdx.x; f < nel; f += kBlock) {
       int c, iup;
@@ -462,7 +497,8 @@ __global__ __launch_bounds__(BS) void k_hxv_col(const HxvArgs a) {
       s_dw_amp[q] = g_amp_dw[e >> kColBits];
     }
     if (STAGE)
-      for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
+      stage_copy<BS, C>(dimup, [&](int f) { return vcol[f]; },
+                        [&](int f, const C &x) { s_col[f] = x; });
     __syncthreads();
     const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
     const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
@@ -593,7 +629,8 @@ __global__ __launch_bounds__(BS) void k_hxv_buf(const HxvArgs a) {
       s_dw_soff[q] = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
       s_dw_amp[q] = g_amp_dw[e >> kColBits];
     }
-    for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
+    stage_copy<BS, C>(dimup, [&](int f) { return vcol[f]; },
+                      [&](int f, const C &x) { s_col[f] = x; });
     __syncthreads();
     const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
     const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
@@ -818,7 +855,8 @@ __global__ __launch_bounds__(BS) void k_hxv_plane(const HxvArgs a) {
           if (f < dimup) s_col[f] = xim[q];
         }
       } else {
-        for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[(int64_t)f * NPL + pl];
+        stage_copy<BS, R>(dimup, [&](int f) { return vcol[(int64_t)f * NPL + pl]; },
+                          [&](int f, const R &x) { s_col[f] = x; });
       }
       __syncthreads();
       for (int base0 = 0; base0 < dimup; base0 += BS) {
@@ -1351,7 +1389,8 @@ __global__ __launch_bounds__(BS) void k_hxv_buf_real(const HxvArgs a) {
       s_dw_soff[q] = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
       s_dw_amp[q] = g_amp_dw[e >> kColBits].x;
     }
-    for (int f = threadIdx.x; f < dimup; f += BS) s_col[f] = vcol[f];
+    stage_copy<BS, R>(dimup, [&](int f) { return vcol[f]; },
+                      [&](int f, const R &x) { s_col[f] = x; });
     __syncthreads();
     const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
     const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
@@ -1504,8 +1543,8 @@ __global__ __launch_bounds__(BS) void k_hxv_buf_real2(const HxvArgs a) {
       s_dw_soff[q] = (e & kColMask) * (uint32_t)dimup * (uint32_t)ESZ;
       s_dw_amp[q] = g_amp_dw[e >> kColBits].x;
     }
-    for (int f = threadIdx.x; f < dimup / 2; f += BS)
-      reinterpret_cast<R2 *>(s_col)[f] = reinterpret_cast<const R2 *>(vcol)[f];
+    stage_copy<BS, R2>(dimup / 2, [&](int f) { return reinterpret_cast<const R2 *>(vcol)[f]; },
+                       [&](int f, const R2 &x) { reinterpret_cast<R2 *>(s_col)[f] = x; });
     __syncthreads();
     const double edw = a.skip_diag ? 0.0 : a.e_dw[idw];
     const uint32_t sdw = a.skip_diag ? 0u : a.map_dw[idw];
@@ -1601,9 +1640,9 @@ __global__ __launch_bounds__(BS) void k_hxv_buf_real2(const HxvArgs a) {
 // (element f of both at o_col + 2 f RSZ), so one 2 RSZ-byte LDS read per hop
 // fetches both: the LDS serves 32 lanes per cycle whatever the width up to
 // 8 bytes, so a 4-byte read per column would take twice the cycles.  The
-// one-plane table's address ya of element f (o_col + f RSZ) becomes
-// 2 ya - o_col.  LDS: the amplitudes and a zero at 0, then the column pair.
-// One partial of <v|Hv> per pair.
+// hop table is DevSpin::ent4q, the one-plane table's hops addressed for the
+// pair (a mask and a shift per hop).  LDS: the amplitudes and a zero at 0,
+// then the column pair.  One partial of <v|Hv> per pair.
 template <typename R, int BS>
 __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
@@ -1640,12 +1679,15 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
     const cplx<R> *g_amp = static_cast<const cplx<R> *>(a.amp_up);
     for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp[t] = g_amp[t].x;
     if (threadIdx.x == 0) s_amp[a.namp_up] = (R)0;  // padding hops
-    for (int f = threadIdx.x; f < dimup; f += BS) {
-      cplx<R> y;
-      y.x = vcol0[f];
-      y.y = vcol1[f];
-      s_pair[f] = y;
-    }
+    stage_copy<BS, cplx<R>>(
+        dimup,
+        [&](int f) {
+          cplx<R> y;
+          y.x = vcol0[f];
+          y.y = vcol1[f];
+          return y;
+        },
+        [&](int f, const cplx<R> &y) { s_pair[f] = y; });
     __syncthreads();
     const double edw0 = a.skip_diag ? 0.0 : a.e_dw[idw0];
     const double edw1 = a.skip_diag ? 0.0 : a.e_dw[idw1];
@@ -1682,7 +1724,7 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
               R h[4];
 #pragma unroll
               for (int t = 0; t < 4; ++t) {
-                y[t] = lds_load<R>(2 * (ee[t] & kLdsAddrMask) - o_col);
+                y[t] = lds_load<R>(ee[t] & kLdsAddrMask);
                 h[t] = lds_load_re<R>(ee[t] >> kLdsAddrBits);
               }
 #pragma unroll
@@ -1798,67 +1840,98 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
   }
 }
 
-// Lanczos step of the real-vector transposed form: the transpose-add of the
-// down-spin result fused with the vector update, so Hv never goes through
-// HBM.  Per element n = idw*DimUp + iup of chain blockIdx.z:
+// Lanczos step of the transposed form: the transpose-add of the down-spin
+// result fused with the vector update, so Hv never goes through HBM.  Per
+// element n = idw*DimUp + iup of chain blockIdx.z:
 //   h = Hv_up[n] + Wt[iup*DimDw + idw]   (Wt: the down-spin pass, in v^T)
 //   u_{k+1}[n] = h - (alpha_k/nrm_k) u_k[n] - (beta_k/nrm_{k-1}) u_{k-1}[n]
 //   x[n] += (y_k/nrm_k) u_k[n]            (eigenvector pass, x non-null)
-// with the arithmetic of k_lanczos_update; a 64 x 64 tile of Wt is turned
-// through LDS as in k_transpose.  Partials of |u_{k+1}|^2 per tile, the pad
-// real of an odd dim untouched (it stays 0).
+// with the arithmetic of k_lanczos_update (the same sums in the same order,
+// so the result equals the transpose-add followed by that update bit for
+// bit); a tile of Wt is turned through LDS as in k_transpose.  E: a real
+// (real-vector mode, the pad real of an odd dim untouched: it stays 0) or
+// a complex element.  Partials of |u_{k+1}|^2 per tile.
 struct TUpdArgs {
   void *u0, *u1;
   const void *hv, *wt;
   void *x;
   int dimup, dimdw;
-  int64_t vstride;  // reals between chains
+  int64_t vstride;  // elements between chains
   LanczosDev *lz;
   double *partials;
 };
 
 template <typename R>
+__device__ inline R tupd(R h, R u, R p, R ca, R cb, bool first) {
+  return first ? h - ca * u : h - ca * u - cb * p;
+}
+template <typename R, typename C>
+__device__ inline C tupd(const C &h, const C &u, const C &p, R ca, R cb, bool first) {
+  C w;
+  w.x = tupd(h.x, u.x, p.x, ca, cb, first);
+  w.y = tupd(h.y, u.y, p.y, ca, cb, first);
+  return w;
+}
+__device__ inline double tnorm2(float w) { return (double)w * (double)w; }
+__device__ inline double tnorm2(double w) { return w * w; }
+template <typename C>
+__device__ inline double tnorm2(const C &w) {
+  return (double)w.x * (double)w.x + (double)w.y * (double)w.y;
+}
+template <typename R>
+__device__ inline void taxpy(R &x, R c, R u) {
+  x += c * u;
+}
+template <typename R, typename C>
+__device__ inline void taxpy(C &x, R c, const C &u) {
+  x.x += c * u.x;
+  x.y += c * u.y;
+}
+
+template <typename R, typename E>
 __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
-  __shared__ R s[kTile][kTile + 1];
+  constexpr int T = tr_tile<E>();
+  constexpr int kRows = kBlock / T;  // rows of the tile per pass of the workgroup
+  __shared__ E s[T][T + 1];
   const int64_t off = (int64_t)blockIdx.z * a.vstride;
   LanczosDev *lz = a.lz + blockIdx.z;
   const int k = lz->k;
   const bool done = lz->done != 0;
   double ss = 0.0;
   if (!done) {
-    const R *wt = static_cast<const R *>(a.wt) + off;
+    const E *wt = static_cast<const E *>(a.wt) + off;
     const int ni = a.dimdw, nj = a.dimup;  // element (i = idw, j = iup)
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    constexpr int kWaves = kBlock / 64;
-    const int i0 = blockIdx.x * kTile, j0 = blockIdx.y * kTile;
+    const int lane = threadIdx.x % T, w = threadIdx.x / T;
+    const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
     const int i = i0 + lane;
 #pragma unroll
-    for (int t = 0; t < kTile / kWaves; ++t) {
-      const int jj = w + kWaves * t;
+    for (int t = 0; t < T / kRows; ++t) {
+      const int jj = w + kRows * t;
       const int j = j0 + jj;
       if (i < ni && j < nj) s[jj][lane] = wt[(int64_t)j * ni + i];
     }
     __syncthreads();
-    const R *hv = static_cast<const R *>(a.hv) + off;
-    const R *uk = static_cast<const R *>((k & 1) ? a.u1 : a.u0) + off;
-    R *up = static_cast<R *>((k & 1) ? a.u0 : a.u1) + off;
-    R *x = static_cast<R *>(a.x);
+    const E *hv = static_cast<const E *>(a.hv) + off;
+    const E *uk = static_cast<const E *>((k & 1) ? a.u1 : a.u0) + off;
+    E *up = static_cast<E *>((k & 1) ? a.u0 : a.u1) + off;
+    E *x = static_cast<E *>(a.x);
     const R ca = (R)(lz->alpha[k] / lz->nrm[k]);
     const R cb = k > 0 ? (R)(lz->beta[k] / lz->nrm[k - 1]) : (R)0;
     const R cy = x ? (R)(lz->y[k] / lz->nrm[k]) : (R)0;
     const int j = j0 + lane;
 #pragma unroll
-    for (int t = 0; t < kTile / kWaves; ++t) {
-      const int ii = w + kWaves * t;
+    for (int t = 0; t < T / kRows; ++t) {
+      const int ii = w + kRows * t;
       const int ir = i0 + ii;
       if (ir < ni && j < nj) {
         const int64_t n = (int64_t)ir * nj + j;
-        const R h = hv[n] + s[lane][ii];
-        const R u = uk[n];
-        const R wv = k > 0 ? h - ca * u - cb * up[n] : h - ca * u;
+        E h = s[lane][ii];
+        acc_elem(h, hv[n]);
+        const E u = uk[n];
+        const E wv = tupd<R>(h, u, k > 0 ? up[n] : u, ca, cb, k == 0);
         up[n] = wv;
-        ss += (double)wv * (double)wv;
-        if (x) x[n] += cy * u;
+        ss += tnorm2(wv);
+        if (x) taxpy<R>(x[n], cy, u);
       }
     }
   }
@@ -2211,10 +2284,11 @@ static int grid_for(int64_t n) {
 
 // Hop order of a one-plane table (DevSpin::ent4p), chosen per half
 // wavefront (32 consecutive rows, one per lane) so that the LDS gathers of
-// one slot spread over the banks.  The LDS has 64 banks of 4 bytes and
-// serves 32 lanes per cycle, for 4- and 8-byte reads alike; lanes reading
-// one address share it (broadcast), distinct addresses in one bank are
-// served one after another.  The reads of a slot: the element, 8 bytes
+// one slot spread over the banks.  The LDS serves 32 lanes per cycle: an
+// 8-byte read from 64 banks of 4 bytes (bank pair = address / 8 mod 32), a
+// 4-byte read from 32 (bank = address / 4 mod 32); lanes reading one
+// address share it (broadcast), distinct addresses in one bank are served
+// one after another.  The reads of a slot: the element, 8 bytes
 // (FP64 one-plane, or the FP32 column pair of k_hxv_plane2 at
 // 2 ya - col), and the amplitude, rsz bytes.  The order of a row's hops is
 // free (they are summed), so slot by slot each lane (the one with most hops
@@ -2229,15 +2303,15 @@ static void order_plane_hops(std::vector<uint32_t> &ent4, const std::vector<int3
                              int dim, int nc, int rsz, uint32_t pad, uint32_t col,
                              std::vector<int32_t> &perm) {
   constexpr int dom = 32;  // lanes served in one LDS cycle
-  // bank groups: an 8-byte element read takes a pair of banks (32 groups);
-  // an amplitude read a pair (FP64) or one bank (FP32, 64 groups)
+  // bank groups (32 each): the element read is 8 bytes, the amplitude read
+  // rsz bytes
   auto group_e = [&](uint32_t ya) {
     const uint32_t addr = rsz == 8 ? ya : 2 * ya - col;
     return (int)((addr / 8) % 32);
   };
-  auto group_a = [&](uint32_t aa) { return (int)((aa / rsz) % (uint32_t)(rsz == 8 ? 32 : 64)); };
+  auto group_a = [&](uint32_t aa) { return (int)((aa / rsz) % 32); };
   std::vector<std::vector<std::pair<uint32_t, int>>> rem(dom);
-  std::vector<int> load_e(32), load_a(64), order(dom);
+  std::vector<int> load_e(32), load_a(32), order(dom);
   perm.resize(ent4.size());
   for (size_t i = 0; i < perm.size(); ++i) perm[i] = (int32_t)(i / 4 / dim * 4 + i % 4);
   std::vector<int64_t> stamp_e((size_t)(kLdsAddrMask + 1) / rsz + 1, -1);
@@ -2412,6 +2486,15 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
       }
       VCA_HIP(hipStreamSynchronize(st));
     }
+    // the column-pair addresses of k_hxv_plane2, where a pair fits the LDS
+    const uint32_t col = (uint32_t)buf_lds(namp, 0, 0, rsz).col;
+    if ((size_t)col + 2 * (size_t)align16(sp.dim * rsz) <= kLdsBig) {
+      const int64_t n = (int64_t)4 * nc * sp.dim;
+      VCA_HIP(hipMalloc(&sp.ent4q, sizeof(uint32_t) * n));
+      hipLaunchKernelGGL(k_pair_addr, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock),
+                         0, st, sp.ent4p, sp.ent4q, n, col);
+      VCA_HIP(hipGetLastError());
+    }
   }
   VCA_HIP(hipGetLastError());
   VCA_HIP(hipStreamSynchronize(st));
@@ -2470,7 +2553,6 @@ constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 // vector with 32-bit offsets) or one real plane of them (form 3, real
 // amplitudes) can be staged in LDS.
 constexpr int64_t kTformMinDim = 1 << 22;
-constexpr size_t kLdsBig = 150 * 1024;
 
 static size_t hxv_esz(const Sector &sec) {
   return sec.precision == VCA_FP64 ? sizeof(double2) : sizeof(float2);
@@ -2609,8 +2691,22 @@ static void ensure_tbuf(Sector &sec, int nb) {
 }
 
 // Tiles of the fused transpose-add + update (k_tadd_update) per chain.
+// The Lanczos update fused with the transpose-add (k_tadd_update): in
+// real-vector mode on the transposed form, and for complex vectors wherever
+// the H x v is the transposed form.
+static bool fused_tadd(const Sector &sec) {
+  return sec.real_vec ? real_uses_tform(sec) : hxv_shape(sec).tform;
+}
+
+// Tile edge of k_tadd_update for the sector's element (tr_tile).
+static int tadd_edge(const Sector &sec) {
+  const size_t rsz = sec.precision == VCA_FP64 ? sizeof(double) : sizeof(float);
+  return (sec.real_vec ? rsz : 2 * rsz) > 8 ? 32 : kTile;
+}
+
 static int tadd_tiles(const Sector &sec) {
-  return ((sec.dw.dim + kTile - 1) / kTile) * ((sec.up.dim + kTile - 1) / kTile);
+  const int t = tadd_edge(sec);
+  return ((sec.dw.dim + t - 1) / t) * ((sec.up.dim + t - 1) / t);
 }
 
 // Allocates the partial-sum buffer for the largest grid of a Lanczos step
@@ -2620,7 +2716,7 @@ void ensure_step_partials(Sector &sec, int nb) {
   tune_real_direct(sec);  // before any launch is recorded
   const HxvShape h = hxv_shape(sec);
   int n = std::max(hxv_partials(sec, h), grid_for(sec.dim));
-  if (sec.real_vec && real_uses_tform(sec)) n = std::max(n, tadd_tiles(sec));
+  if (fused_tadd(sec)) n = std::max(n, tadd_tiles(sec));
   ensure_partials(sec, n * nb);
   ensure_tbuf(sec, nb);
 }
@@ -2842,15 +2938,18 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   b.ncb = sec.up.dim;
   b.pbase = sec.dw.dim;
   launch_buf_pass<R>(sec, b, h, real_amp, dim3(sec.up.dim, nb));
-  // 4. Hv += vt^T
-  TrArgs t2{};
-  t2.in0 = t2.in1 = tb;
-  t2.out = a.hv;
-  t2.ni = sec.dw.dim;
-  t2.nj = sec.up.dim;
-  t2.vstride = sec.dim;
-  t2.lz = a.lz;
-  launch_transpose<cplx<R>>(sec, t2, true, nb);
+  // 4. Hv += vt^T; in Lanczos mode fused into the vector update
+  //    (k_tadd_update, launch_lanczos_update), so Hv holds diag v + H_up v
+  if (!a.lz) {
+    TrArgs t2{};
+    t2.in0 = t2.in1 = tb;
+    t2.out = a.hv;
+    t2.ni = sec.dw.dim;
+    t2.nj = sec.up.dim;
+    t2.vstride = sec.dim;
+    t2.lz = a.lz;
+    launch_transpose<cplx<R>>(sec, t2, true, nb);
+  }
   if (a.lz) {
     hipLaunchKernelGGL(k_lz_alpha, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
                        (int)a.pstride);
@@ -2876,9 +2975,12 @@ static void launch_plane2(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem)
 // One real-vector gather pass on grid workgroups per chain: one column per
 // workgroup (k_hxv_plane, NPL = 1) or two (k_hxv_plane2).
 template <typename R>
-static void launch_real_pass(Sector &sec, const HxvArgs &a, bool two, int grid, int nb) {
+static void launch_real_pass(Sector &sec, HxvArgs a, const DevSpin &sp, bool two, int grid,
+                             int nb) {
   constexpr int rsz = (int)sizeof(R);
+  a.ent4_up = two ? sp.ent4q : sp.ent4p;
   if (two) {
+    if (!sp.ent4q) fail(VCA_ERR_STATE, "no column-pair hop table");
     const size_t smem = (size_t)buf_lds(a.namp_up, 0, 0, rsz).col +
                         2 * (size_t)align16(a.dimup * rsz);
     launch_plane2<R>(sec, a, dim3(grid, nb), smem);
@@ -3014,7 +3116,7 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   a.skip_diag = 0;
   a.ncb = sec.dw.dim;
   a.ent4_up = sec.up.ent4p;
-  launch_real_pass<R>(sec, a, real_pass_two(sec, sec.up), real_grid_up(sec), nb);
+  launch_real_pass<R>(sec, a, sec.up, real_pass_two(sec, sec.up), real_grid_up(sec), nb);
   TrArgs t{};
   t.in0 = a.v0;
   t.in1 = a.v1;
@@ -3037,7 +3139,7 @@ static void launch_real_t(Sector &sec, HxvArgs a, int nb) {
   b.skip_diag = 1;
   b.ncb = sec.up.dim;
   b.pbase = real_grid_up(sec);
-  launch_real_pass<R>(sec, b, real_pass_two(sec, sec.dw), real_grid_dw(sec), nb);
+  launch_real_pass<R>(sec, b, sec.dw, real_pass_two(sec, sec.dw), real_grid_dw(sec), nb);
   // Hv += (.)^T: in Lanczos mode fused into the vector update
   // (k_tadd_update, launch_lanczos_update), so Hv_up is all H x v writes
   if (!a.lz) {
@@ -3201,7 +3303,7 @@ void launch_hxv(Sector &sec, const void *v0, const void *v1, void *hv, bool lanc
 }
 
 void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
-  if (sec.real_vec && real_uses_tform(sec)) {
+  if (fused_tadd(sec)) {
     TUpdArgs t{};
     t.u0 = sec.buf[0];
     t.u1 = sec.buf[1];
@@ -3210,17 +3312,23 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
     t.x = accumulate_x ? sec.xvec : nullptr;
     t.dimup = sec.up.dim;
     t.dimdw = sec.dw.dim;
-    t.vstride = 2 * sec.dim;
+    t.vstride = sec.real_vec ? 2 * sec.dim : sec.dim;  // chains 2 dim reals apart
     t.lz = sec.lz;
     const int nblk = tadd_tiles(sec);
     ensure_partials(sec, nblk * nb);
     t.partials = sec.partials;
-    const dim3 g((unsigned)((sec.dw.dim + kTile - 1) / kTile),
-                 (unsigned)((sec.up.dim + kTile - 1) / kTile), (unsigned)nb);
-    if (sec.precision == VCA_FP64)
-      hipLaunchKernelGGL(k_tadd_update<double>, g, dim3(kBlock), 0, sec.stream, t);
+    const int te = tadd_edge(sec);
+    const dim3 g((unsigned)((sec.dw.dim + te - 1) / te), (unsigned)((sec.up.dim + te - 1) / te),
+                 (unsigned)nb);
+    const bool fp64 = sec.precision == VCA_FP64;
+    if (sec.real_vec && fp64)
+      hipLaunchKernelGGL((k_tadd_update<double, double>), g, dim3(kBlock), 0, sec.stream, t);
+    else if (sec.real_vec)
+      hipLaunchKernelGGL((k_tadd_update<float, float>), g, dim3(kBlock), 0, sec.stream, t);
+    else if (fp64)
+      hipLaunchKernelGGL((k_tadd_update<double, double2>), g, dim3(kBlock), 0, sec.stream, t);
     else
-      hipLaunchKernelGGL(k_tadd_update<float>, g, dim3(kBlock), 0, sec.stream, t);
+      hipLaunchKernelGGL((k_tadd_update<float, float2>), g, dim3(kBlock), 0, sec.stream, t);
     VCA_HIP(hipGetLastError());
     hipLaunchKernelGGL(k_lz_beta, dim3(nb), dim3(kBlock), 0, sec.stream, sec.lz, sec.partials,
                        nblk);

@@ -142,7 +142,7 @@ Sector::~Sector() {
   for (DevSpin *sp : {&up, &dw})
     for (void *q : {(void *)sp->map, (void *)sp->energy, (void *)sp->cnt, (void *)sp->ent,
                     (void *)sp->amp64, (void *)sp->amp32, (void *)sp->meta, (void *)sp->ent4,
-                    (void *)sp->ent4p, (void *)sp->ent4s})
+                    (void *)sp->ent4p, (void *)sp->ent4q, (void *)sp->ent4s})
       if (q) (void)hipFree(q);
   for (void *q : {buf[0], buf[1], buf[2], xvec, tbuf, (void *)partials, (void *)lz,
                   (void *)lz_arrays, (void *)d_ops})

@@ -34,6 +34,8 @@ constexpr int kBinomN = kMaxNs + 1;
 constexpr int kLdsAddrBits = 18;
 constexpr uint32_t kLdsAddrMask = (1u << kLdsAddrBits) - 1u;
 constexpr uint32_t kLdsAmpLimit = 1u << (32 - kLdsAddrBits);  // amplitude table bytes
+// dynamic LDS of the large-column kernels (of the 160 KB per CU)
+constexpr size_t kLdsBig = 150 * 1024;
 
 struct Error {
   int code;
@@ -205,6 +207,10 @@ struct DevSpin {
   // amplitudes of sizeof(real) bytes), read by the plane kernel of the
   // transposed H x v on spins with real amplitudes; plane_form: usable.
   uint32_t *ent4p = nullptr;
+  // ent4p's hops addressed for two columns staged interleaved (element f of
+  // both at the column offset + 2 f sizeof(real)), read by the two-column
+  // plane kernel; null where a column pair does not fit the LDS.
+  uint32_t *ent4q = nullptr;
   // Signed-column entries (every hopping term the same real amplitude t, so
   // a hop's amplitude is +t or -t): each hop is the LDS address of its element in a staged copy
   // of the column scaled by +t or by -t, padding hops the zero at address 0.
