@@ -428,11 +428,13 @@ def test_real_vector_ground_state_in_a_solve(monkeypatch):
     """A whole solve whose ground state runs on real vectors, against the
     solve on complex vectors (the oracle's own solve of this 10-site cluster
     takes over 10 minutes on one core; the pipeline tests pin smaller ones
-    to it).  The two runs start from different vectors, so they agree to
-    what the Lanczos stopping rule |dE| < 1e-12 fixes, not bit for bit:
-    E0 within 1e-10 relative, Omega within 1e-8 (1.1e-10 seen), and G',
-    linear in the ground-state vector that the rule fixes to ~1e-6, within
-    2e-6 of its largest element (5.9e-7 seen).  The GPU's complex path
+    to it).  The two ground-state runs start from different vectors, so
+    they agree to what the Lanczos stopping rule |dE| < 1e-12 fixes, not bit
+    for bit: E0 within 1e-10 relative, Omega within 1e-8 (1.1e-10 seen), and
+    G', first order in the ground-state vector that the rule fixes to ~1e-6
+    (the eigenvector error is ~sqrt of the energy error), within 2e-6 of its
+    largest element (5.9e-7 seen).  The chains themselves add only rounding
+    (test_real_vector_chains_in_a_solve: 1e-10).  The GPU's complex path
     matches the oracle to 1e-8 in G' because both follow the same start
     vector."""
     monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
@@ -593,9 +595,13 @@ def test_real_vector_chains_in_a_solve(case, monkeypatch):
     chain10_overlapped: the real batches on the overlapped direct form
     (k_hxv_ov_real, 512 lanes x 2 rows), each chain on one XCD.  The
     ground state is the same complex vector in both, so E0 agrees to
-    rounding; G' and Omega differ by what dropping the ground state's
-    non-phase imaginary remainder (its Lanczos error) changes: bars 2e-6 of
-    G''s largest element and 1e-8 relative in Omega."""
+    rounding.  With psi = e^{i phi}(r + i s), a real H and an operator O of
+    real coefficients, <O psi|R(z)|O psi> = <Or|R|Or> + <Os|R|Os> exactly
+    (R = (z - H)^-1 is complex-symmetric, so the cross terms cancel): the
+    real chains, which drop s, differ by a term second order in the state's
+    imaginary remainder, and otherwise by rounding, which 80-step chains do
+    not amplify (no loss of orthogonality yet).  Bars 1e-10 of G''s largest
+    element and 1e-12 relative in Omega (up to 6e-14 and 2e-14 seen)."""
     if case == "chain11_sector33":
         p, hloc, hk, _, _ = M.chain(nlat=11, u=4.0, nk=4)
         p.sectors = [(3, 3)]
@@ -616,6 +622,9 @@ def test_real_vector_chains_in_a_solve(case, monkeypatch):
         out[real] = (om_, sc.results()["omegaprime"], sc.vca_gf_cluster(0.3j),
                      sc.vca_gf_cluster(1.0 + 0.5j))
     assert abs(out["1"][1] - out["0"][1]) <= 1e-12 * abs(out["0"][1])
-    assert abs(out["1"][0] - out["0"][0]) <= 1e-8 * abs(out["0"][0])
+    print(f"{case}: dOmega {abs(out['1'][0] - out['0'][0]) / abs(out['0'][0]):.2e}, dG' " +
+          ", ".join(f"{np.abs(x - y).max() / np.abs(y).max():.2e}"
+                    for x, y in zip(out["1"][2:], out["0"][2:])), flush=True)
+    assert abs(out["1"][0] - out["0"][0]) <= 1e-12 * abs(out["0"][0])
     for x, y in zip(out["1"][2:], out["0"][2:]):
-        assert np.abs(x - y).max() <= 2e-6 * np.abs(y).max(), np.abs(x - y).max()
+        assert np.abs(x - y).max() <= 1e-10 * np.abs(y).max(), np.abs(x - y).max()

@@ -1804,39 +1804,53 @@ constexpr int tr_tile() {
 
 template <typename E, bool ADD>
 __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
-  using C = E;
   constexpr int T = tr_tile<E>();
   constexpr int kRows = kBlock / T;  // rows of the tile per pass of the workgroup
-  __shared__ C s[T][T + 1];
+  constexpr int NP = T / kRows;      // passes
+  __shared__ E s[T][T + 1];
   const int64_t off = (int64_t)blockIdx.z * a.vstride;
-  const C *in = static_cast<const C *>(a.in0) + off;
+  const E *in = static_cast<const E *>(a.in0) + off;
   if (a.lz) {
     const LanczosDev *lz = a.lz + blockIdx.z;
     if (lz->done) return;
-    if (lz->k & 1) in = static_cast<const C *>(a.in1) + off;
+    if (lz->k & 1) in = static_cast<const E *>(a.in1) + off;
   }
-  C *out = static_cast<C *>(a.out) + off;
+  E *out = static_cast<E *>(a.out) + off;
   const int lane = threadIdx.x % T, w = threadIdx.x / T;
   const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
-  const int i = i0 + lane;
+  // every load of the lane is issued before the first LDS store (indices
+  // clamped into the matrix, the extra elements discarded), so a tile costs
+  // one HBM round trip rather than NP; the same on the way out
+  const int i = i0 + lane, ic = min(i, a.ni - 1);
+  E x[NP];
 #pragma unroll
-  for (int t = 0; t < T / kRows; ++t) {
+  for (int t = 0; t < NP; ++t) {
+    const int jc = min(j0 + w + kRows * t, a.nj - 1);
+    x[t] = in[(int64_t)jc * a.ni + ic];
+  }
+#pragma unroll
+  for (int t = 0; t < NP; ++t) {
     const int jj = w + kRows * t;
-    const int j = j0 + jj;
-    if (i < a.ni && j < a.nj) s[jj][lane] = in[(int64_t)j * a.ni + i];
+    if (i < a.ni && j0 + jj < a.nj) s[jj][lane] = x[t];
   }
   __syncthreads();
-  const int j = j0 + lane;
+  const int j = j0 + lane, jc = min(j, a.nj - 1);
 #pragma unroll
-  for (int t = 0; t < T / kRows; ++t) {
-    const int ii = w + kRows * t;
-    const int ir = i0 + ii;
-    if (ir < a.ni && j < a.nj) {
-      C *o = out + (int64_t)ir * a.nj + j;
-      C val = s[lane][ii];
-      if (ADD) acc_elem(val, *o);
-      *o = val;
+  for (int t = 0; t < NP; ++t) x[t] = s[lane][w + kRows * t];
+  if (ADD) {
+    E o[NP];
+#pragma unroll
+    for (int t = 0; t < NP; ++t) {
+      const int irc = min(i0 + w + kRows * t, a.ni - 1);
+      o[t] = out[(int64_t)irc * a.nj + jc];
     }
+#pragma unroll
+    for (int t = 0; t < NP; ++t) acc_elem(x[t], o[t]);
+  }
+#pragma unroll
+  for (int t = 0; t < NP; ++t) {
+    const int ir = i0 + w + kRows * t;
+    if (ir < a.ni && j < a.nj) out[(int64_t)ir * a.nj + j] = x[t];
   }
 }
 
@@ -1892,6 +1906,9 @@ template <typename R, typename E>
 __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
   constexpr int T = tr_tile<E>();
   constexpr int kRows = kBlock / T;  // rows of the tile per pass of the workgroup
+  constexpr int NP = T / kRows;      // passes
+  // elements per batch of loads in the update (registers: 4 or 5 per element)
+  constexpr int CH = sizeof(E) >= 8 ? NP / 2 : NP;
   __shared__ E s[T][T + 1];
   const int64_t off = (int64_t)blockIdx.z * a.vstride;
   LanczosDev *lz = a.lz + blockIdx.z;
@@ -1903,12 +1920,17 @@ __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
     const int ni = a.dimdw, nj = a.dimup;  // element (i = idw, j = iup)
     const int lane = threadIdx.x % T, w = threadIdx.x / T;
     const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
-    const int i = i0 + lane;
+    // loads batched as in k_transpose (clamped indices)
+    {
+      const int i = i0 + lane, ic = min(i, ni - 1);
+      E y[NP];
 #pragma unroll
-    for (int t = 0; t < T / kRows; ++t) {
-      const int jj = w + kRows * t;
-      const int j = j0 + jj;
-      if (i < ni && j < nj) s[jj][lane] = wt[(int64_t)j * ni + i];
+      for (int t = 0; t < NP; ++t) y[t] = wt[(int64_t)min(j0 + w + kRows * t, nj - 1) * ni + ic];
+#pragma unroll
+      for (int t = 0; t < NP; ++t) {
+        const int jj = w + kRows * t;
+        if (i < ni && j0 + jj < nj) s[jj][lane] = y[t];
+      }
     }
     __syncthreads();
     const E *hv = static_cast<const E *>(a.hv) + off;
@@ -1918,20 +1940,31 @@ __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
     const R ca = (R)(lz->alpha[k] / lz->nrm[k]);
     const R cb = k > 0 ? (R)(lz->beta[k] / lz->nrm[k - 1]) : (R)0;
     const R cy = x ? (R)(lz->y[k] / lz->nrm[k]) : (R)0;
-    const int j = j0 + lane;
+    const int j = j0 + lane, jc = min(j, nj - 1);
 #pragma unroll
-    for (int t = 0; t < T / kRows; ++t) {
-      const int ii = w + kRows * t;
-      const int ir = i0 + ii;
-      if (ir < ni && j < nj) {
-        const int64_t n = (int64_t)ir * nj + j;
-        E h = s[lane][ii];
-        acc_elem(h, hv[n]);
-        const E u = uk[n];
-        const E wv = tupd<R>(h, u, k > 0 ? up[n] : u, ca, cb, k == 0);
-        up[n] = wv;
-        ss += tnorm2(wv);
-        if (x) taxpy<R>(x[n], cy, u);
+    for (int t0 = 0; t0 < NP; t0 += CH) {
+      E h[CH], hw[CH], u[CH], p[CH];
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int ii = w + kRows * (t0 + c);
+        const int64_t n = (int64_t)min(i0 + ii, ni - 1) * nj + jc;
+        h[c] = s[lane][ii];
+        hw[c] = hv[n];
+        u[c] = uk[n];
+        if (k > 0) p[c] = up[n];
+      }
+#pragma unroll
+      for (int c = 0; c < CH; ++c) {
+        const int ii = w + kRows * (t0 + c);
+        const int ir = i0 + ii;
+        if (ir < ni && j < nj) {
+          const int64_t n = (int64_t)ir * nj + j;
+          acc_elem(h[c], hw[c]);
+          const E wv = tupd<R>(h[c], u[c], k > 0 ? p[c] : u[c], ca, cb, k == 0);
+          up[n] = wv;
+          ss += tnorm2(wv);
+          if (x) taxpy<R>(x[n], cy, u[c]);
+        }
       }
     }
   }
