@@ -8,9 +8,10 @@
 the calibration copy kernel of COPY_BYTES bytes each way (the largest
 __amd_rocclr_copyBuffer / elementwise copy dispatch) to turn the counters
 into bytes, and stores under KEY in OUT.json the HBM traffic of one
-application: the sum over the comma-separated KERNELS substrings of the
-per-dispatch traffic of the most-dispatched kernel matching each (one
-launch of each per application), with the launch shape VARIANT the run
+application: the sum over the KERNELS entries (separated by ';', each a
+substring of a kernel name with an optional '*N', N launches of it per
+application, default 1) of N times the per-dispatch traffic of the
+most-dispatched kernel matching it, with the launch shape VARIANT the run
 picked, so that a later run can tell whether the counters are its own.
 """
 from __future__ import annotations
@@ -58,18 +59,21 @@ def pmc_traffic(fetch_db: str, write_db: str, kernels: str, copy_bytes: int) -> 
     cw_kib = write.get(ck, (0, 0, 0))[2]
     f_scale = copy_bytes / (cf_kib * 1024.0)
     w_scale = copy_bytes / (cw_kib * 1024.0) if cw_kib else None
-    names, f_kib, w_kib, disp = [], 0.0, 0.0, []
-    for sub in kernels.split(","):
+    names, f_kib, w_kib, disp, per_app = [], 0.0, 0.0, [], []
+    for ent in kernels.split(";"):
+        sub, _, n = ent.rpartition("*") if "*" in ent else (ent, "", "1")
         cand = [k for k in fetch if sub in k]
         if not cand:
             raise SystemExit(f"no kernel matching {sub!r}")
         kn = max(cand, key=lambda k: fetch[k][0])
         names.append(kn)
         disp.append(fetch[kn][0])
-        f_kib += fetch[kn][1]
-        w_kib += write[kn][1]
+        per_app.append(int(n))
+        f_kib += int(n) * fetch[kn][1]
+        w_kib += int(n) * write[kn][1]
     return {
         "kernels": names,
+        "launches_per_application": per_app,
         "dispatches": disp,
         "fetch_kib_per_dispatch": f_kib,
         "write_kib_per_dispatch": w_kib,
