@@ -240,6 +240,8 @@ def test_2x5_every_sector_matches_fixture():
     s2 = VcaSolver(p)
     s2.vca_set_hcluster(hloc)
     nchecked = 0
+    nlanc = sum(1 for key in gold["sector_evals"]
+                if math.prod(math.comb(10, int(n)) for n in key.split(",")) > p.lanc_dim_threshold)
     for key, ev in gold["sector_evals"].items():
         nup, ndw = map(int, key.split(","))
         if math.comb(10, nup) * math.comb(10, ndw) <= p.lanc_dim_threshold:
@@ -250,7 +252,8 @@ def test_2x5_every_sector_matches_fixture():
         assert np.allclose(e, ev, rtol=1e-10, atol=1e-10), (key, e, ev)
         sec.close()
         nchecked += 1
-    assert nchecked > 90
+    # 77 of the 121 sectors are above the dense threshold (256 states)
+    assert nchecked == nlanc == 77
 
 
 # ------------------------------------------------------------------- C2 --
