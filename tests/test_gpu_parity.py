@@ -215,6 +215,7 @@ def test_chain_affine_batches_in_a_solve(monkeypatch):
     """The batched chain steps with each chain on one XCD (forced) give the
     same solve as with the columns spread over the XCDs: only the order of
     the alpha partial sums differs."""
+    monkeypatch.setenv("VCA_CHAIN_SET_MAX_DIM", "0")  # one batch per target sector
     p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
     p.lanc_ngfiter = 80
     out = {}
@@ -376,6 +377,7 @@ def test_transposed_form_in_lanczos_and_batches(prec, variant, monkeypatch):
     sec.close()
     if prec != "fp64":
         return
+    monkeypatch.setenv("VCA_CHAIN_SET_MAX_DIM", "0")  # chains on the sector's own H x v
     pc, hlc, hkc, _, _ = M.chain(nlat=10, u=4.0, nk=4)
     pc.lanc_ngfiter = 80
     out = {}
@@ -602,6 +604,7 @@ def test_real_vector_chains_in_a_solve(case, monkeypatch):
     imaginary remainder, and otherwise by rounding, which 80-step chains do
     not amplify (no loss of orthogonality yet).  Bars 1e-10 of G''s largest
     element and 1e-12 relative in Omega (up to 6e-14 and 2e-14 seen)."""
+    monkeypatch.setenv("VCA_CHAIN_SET_MAX_DIM", "0")  # one batch per target sector
     if case == "chain11_sector33":
         p, hloc, hk, _, _ = M.chain(nlat=11, u=4.0, nk=4)
         p.sectors = [(3, 3)]

@@ -15,7 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libvca_gpu.so")
-SOURCES = ["kernels.hip", "omega.hip", "eigs.hip", "linalg.cpp", "gfprime.cpp", "solver.cpp"]
+SOURCES = ["kernels.hip", "chains.hip", "omega.hip", "eigs.hip", "linalg.cpp", "gfprime.cpp", "solver.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VCA_OFFLOAD_ARCH", "gfx950")

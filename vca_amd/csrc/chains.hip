@@ -1,0 +1,613 @@
+// chains.hip -- excitation chains of uneven target sectors in one set of
+// launches per Lanczos step.
+//
+// The chains of a kept state go to its N+-1 target sectors (reference:
+// build_gf_normal and its channel loops, VCA_GF_NORMAL.f90:37-104, 121-318,
+// 561-958; the continued-fraction Lanczos sp_lanc_tridiag called at
+// VCA_GF_NORMAL.f90:218-223).  On small sectors one chain, or one sector's
+// batch, cannot fill the GPU, and one launch per sector and step is bound by
+// launch latency.  Here every chain of every small target sector -- complex
+// and real vectors, sectors of any size -- advances together: a step is two
+// launches over a flat list of work items (chain, block of rows):
+//   A  k_cs_hxv:    beta_k = |u_k| from the previous update's partials (each
+//                   workgroup sums its chain's partials itself, in block order,
+//                   so all of them agree bit for bit), then H v_k and the
+//                   partials of alpha_k = <v_k|H v_k>;
+//   B  k_cs_update: alpha_k from A's partials, u_{k+1} = Hv - alpha v_k -
+//                   beta_k v_{k-1}, partials of |u_{k+1}|^2.
+// A launch only reads what earlier launches wrote; the chain-level scalars
+// (alpha, beta, norm, done flag) are written by the chain's block 0 for the
+// launches after it, and the done flag alternates between two slots by step
+// parity, so no value is read and written within one launch.  The step index
+// is the chunk base in device memory plus the kernel's step-in-chunk
+// argument, so a 16-step chunk is captured once in a HIP graph and
+// replayed; one single-lane launch advances the base after each chunk.
+//
+// Work items are ordered XCD-aware: each chain belongs to one XCD (chains
+// assigned greedily by size), and its blocks are the items i with
+// i mod 8 = that XCD, in row order, so a chain's gathers hit one L2.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <map>
+#include <numeric>
+#include <type_traits>
+
+#include "vca_internal.hpp"
+
+namespace vca {
+
+namespace {
+
+template <typename R> struct CS2;
+template <> struct CS2<double> { using T = double2; };
+template <> struct CS2<float> { using T = float2; };
+template <typename R> using ccs = typename CS2<R>::T;
+
+constexpr int kCsBlock = 256;
+constexpr int kCsEpt = 4;      // elements per lane in the update
+constexpr int kCsXcd = 8;      // XCDs of the MI355X
+constexpr int kCsChunk = 16;   // Lanczos steps per captured graph
+
+struct CsSeg {
+  const RowMeta *meta_up;  // energy, string, hop count per up row
+  const double *e_dw;
+  const uint32_t *map_dw;
+  const int32_t *cnt_dw;
+  const uint32_t *ent_up, *ent_dw;  // k-major ELL [maxnnz][dim]: col | amp << 21
+  const void *amp_up, *amp_dw;      // signed amplitudes, complex of the precision
+  int namp_up, namp_dw;             // entries of the amplitude tables
+  int dimup, dimdw;
+  int real;        // real vectors (real amplitudes)
+  int64_t n;       // elements per chain: dim (complex or real)
+  int64_t stride;  // elements between consecutive chains' vectors
+  void *u0, *u1, *hv;
+};
+
+struct CsChain {
+  int seg, local;  // segment and position in it
+  int nsteps;      // steps this chain runs; later steps are skipped
+  int nblk_h, nblk_u;
+  int64_t pofs_h, pofs_u;
+};
+
+struct CsArgs {
+  const CsSeg *segs;
+  const CsChain *chains;
+  const int2 *items;  // (chain, block) or (-1, 0) padding
+  LanczosDev *lz;     // one per chain
+  int32_t *dflag;     // [chain][2]: chain ended at or before step k, slot k & 1
+  double *pa, *pb;    // partials of alpha (A) and of |u_{k+1}|^2 (B)
+  const int32_t *kbase;
+  int i;              // step within the chunk
+  CrossTerm cross;
+};
+
+__device__ inline double cs_wave_sum(double v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Block sum returned to every lane.
+__device__ inline double cs_block_sum_all(double v) {
+  __shared__ double s_part[kCsBlock / 64];
+  __shared__ double s_total;
+  v = cs_wave_sum(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) s_part[wid] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0;
+    for (int w = 0; w < kCsBlock / 64; ++w) r += s_part[w];
+    s_total = r;
+  }
+  __syncthreads();
+  return s_total;
+}
+
+// Sum of n partials in block order, the same bits in every workgroup.
+__device__ inline double cs_sum_partials(const double *p, int n) {
+  double s = 0;
+  for (int j = threadIdx.x; j < n; j += kCsBlock) s += p[j];
+  return cs_block_sum_all(s);
+}
+
+__device__ inline double cs_cross(const CrossTerm &c, uint32_t su, uint32_t sd) {
+  if (c.norb == 1) return c.wx[0][0] * (double)__builtin_popcount(su & sd & c.site_mask);
+  double e = 0;
+  for (int io = 0; io < c.norb; ++io)
+    for (int jo = 0; jo < c.norb; ++jo)
+      e += c.wx[io][jo] * (double)__builtin_popcount((su >> io) & (sd >> jo) & c.site_mask);
+  return e;
+}
+
+template <typename R, bool REAL_AMP>
+__device__ inline void cs_cmac(R &ar, R &ai, const ccs<R> &h, const ccs<R> &y) {
+  if (REAL_AMP) {
+    ar = fma(h.x, y.x, ar);
+    ai = fma(h.x, y.y, ai);
+  } else {
+    ar = fma(h.x, y.x, ar);
+    ar = fma(-h.y, y.y, ar);
+    ai = fma(h.x, y.y, ai);
+    ai = fma(h.y, y.x, ai);
+  }
+}
+
+// Hops of one row, kCsHop at a time: the chunk's table entries, then its
+// gathered elements and amplitudes, are issued before any is summed (the
+// row's loads overlap instead of waiting one by one).  Slots past the row's
+// count are not read from the table (the ELL holds nothing there; a row
+// may have no hops at all): they take entry 0 -- element 0 of the column
+// or row 0's neighbour, amplitude 0, all in bounds -- weighted 0.
+constexpr int kCsHop = 4;
+
+template <typename R>
+__device__ inline void cs_mac(R &acc, const ccs<R> &h, R y, bool on) {
+  acc = fma(on ? h.x : (R)0, y, acc);
+}
+
+template <typename R, bool REAL_AMP>
+__device__ inline void cs_mac(R &ar, R &ai, const ccs<R> &h, const ccs<R> &y, bool on) {
+  ccs<R> hh = h;
+  if (!on) hh.x = hh.y = (R)0;
+  cs_cmac<R, REAL_AMP>(ar, ai, hh, y);
+}
+
+// H u on one row (iup, idw) of a segment, unscaled: the diagonal ed plus
+// the up-spin hops (gathers inside the column col) and the down-spin hops
+// (neighbour columns of vin).  E = R (real vectors) or ccs<R>.
+template <typename R, bool REAL_AMP, typename E>
+__device__ inline void cs_row(const CsSeg &sg, const E *vin, const E *col, int iup, int idw,
+                              int cu, int cd, const ccs<R> *s_amu, const ccs<R> *s_amd, R &ar,
+                              R &ai) {
+  const int nmax = max(cu, cd);
+  for (int q0 = 0; q0 < nmax; q0 += kCsHop) {
+    uint32_t eu[kCsHop], ed[kCsHop];
+#pragma unroll
+    for (int j = 0; j < kCsHop; ++j) {
+      const int q = q0 + j;
+      eu[j] = q < cu ? sg.ent_up[(int64_t)q * sg.dimup + iup] : 0u;
+      ed[j] = q < cd ? sg.ent_dw[(int64_t)q * sg.dimdw + idw] : 0u;
+    }
+    E yu[kCsHop], yd[kCsHop];
+    ccs<R> hu[kCsHop], hd[kCsHop];
+#pragma unroll
+    for (int j = 0; j < kCsHop; ++j) {
+      yu[j] = col[eu[j] & kColMask];
+      yd[j] = vin[(int64_t)(ed[j] & kColMask) * sg.dimup + iup];
+      hu[j] = s_amu[eu[j] >> kColBits];
+      hd[j] = s_amd[ed[j] >> kColBits];
+    }
+#pragma unroll
+    for (int j = 0; j < kCsHop; ++j) {
+      if constexpr (std::is_same<E, R>::value) {
+        cs_mac<R>(ar, hu[j], yu[j], q0 + j < cu);
+        cs_mac<R>(ar, hd[j], yd[j], q0 + j < cd);
+      } else {
+        cs_mac<R, REAL_AMP>(ar, ai, hu[j], yu[j], q0 + j < cu);
+        cs_mac<R, REAL_AMP>(ar, ai, hd[j], yd[j], q0 + j < cd);
+      }
+    }
+  }
+}
+
+// Launch A of step k: norm and done flag of u_k, then Hv = H u_k / |u_k| on
+// one block of rows (one row per lane: state r = iup + idw dimup) and the
+// block's partial of alpha_k.  The segment's amplitude tables are staged in
+// LDS (up spin at 0, down spin after it).
+template <typename R, bool REAL_AMP>
+__global__ __launch_bounds__(kCsBlock) void k_cs_hxv(const CsArgs a) {
+  using C = ccs<R>;
+  extern __shared__ __align__(16) unsigned char cs_smem[];
+  const int2 it = a.items[blockIdx.x];
+  if (it.x < 0) return;
+  const CsChain ch = a.chains[it.x];
+  const CsSeg sg = a.segs[ch.seg];
+  LanczosDev *lz = a.lz + it.x;
+  C *s_amu = reinterpret_cast<C *>(cs_smem);
+  C *s_amd = s_amu + sg.namp_up;
+  for (int t = threadIdx.x; t < sg.namp_up; t += kCsBlock)
+    s_amu[t] = static_cast<const C *>(sg.amp_up)[t];
+  for (int t = threadIdx.x; t < sg.namp_dw; t += kCsBlock)
+    s_amd[t] = static_cast<const C *>(sg.amp_dw)[t];
+  const int k = *a.kbase + a.i;
+  double nrm;
+  bool done;
+  if (k == 0) {
+    nrm = lz->nrm[0];
+    done = !(nrm > 0.0);  // O|psi> = 0: an empty channel
+    __syncthreads();
+  } else {
+    // (the block sum's barriers also order the amplitude staging)
+    const double b = sqrt(cs_sum_partials(a.pb + ch.pofs_u, ch.nblk_u));
+    done = a.dflag[2 * it.x + ((k - 1) & 1)] != 0 || !(b > 1e-300);
+    nrm = done ? 1.0 : b;
+    if (it.y == 0 && threadIdx.x == 0) {
+      lz->beta[k] = done ? 0.0 : b;
+      lz->nrm[k] = nrm;
+    }
+  }
+  done = done || k >= ch.nsteps;
+  if (it.y == 0 && threadIdx.x == 0) a.dflag[2 * it.x + (k & 1)] = done ? 1 : 0;
+  double dot = 0.0;
+  const int64_t r = (int64_t)it.y * kCsBlock + threadIdx.x;
+  if (!done && r < sg.n) {
+    const R scale = (R)(1.0 / nrm);
+    const int dimup = sg.dimup;
+    const int idw = (int)(r / dimup);
+    const int iup = (int)(r - (int64_t)idw * dimup);
+    const int64_t coff = (int64_t)ch.local * sg.stride;
+    const RowMeta mu = sg.meta_up[iup];
+    const double ed = mu.e + sg.e_dw[idw] + cs_cross(a.cross, mu.s, sg.map_dw[idw]);
+    const int nd = sg.cnt_dw[idw];
+    if (sg.real) {
+      const R *vin = static_cast<const R *>((k & 1) ? sg.u1 : sg.u0) + coff;
+      const R *col = vin + (int64_t)idw * dimup;
+      const R x = col[iup];
+      R acc = (R)ed * x, unused = 0;
+      cs_row<R, true, R>(sg, vin, col, iup, idw, mu.cnt, nd, s_amu, s_amd, acc, unused);
+      const R out = acc * scale;
+      static_cast<R *>(sg.hv)[coff + r] = out;
+      dot = (double)(x * scale) * (double)out;
+    } else {
+      const C *vin = static_cast<const C *>((k & 1) ? sg.u1 : sg.u0) + coff;
+      const C *col = vin + (int64_t)idw * dimup;
+      const C x = col[iup];
+      R ar = (R)ed * x.x, ai = (R)ed * x.y;
+      cs_row<R, REAL_AMP, C>(sg, vin, col, iup, idw, mu.cnt, nd, s_amu, s_amd, ar, ai);
+      C out;
+      out.x = ar * scale;
+      out.y = ai * scale;
+      static_cast<C *>(sg.hv)[coff + r] = out;
+      dot = (double)(x.x * scale) * (double)out.x + (double)(x.y * scale) * (double)out.y;
+    }
+  }
+  const double bs = cs_block_sum_all(dot);
+  if (threadIdx.x == 0) a.pa[ch.pofs_h + it.y] = bs;
+}
+
+template <typename E, typename R>
+__device__ inline void cs_upd(E *up, const E *uk, const E *hv, int64_t n, int64_t j0, R ca, R cb,
+                              bool first, double &ss) {
+  E u[kCsEpt], h[kCsEpt], p[kCsEpt];
+#pragma unroll
+  for (int q = 0; q < kCsEpt; ++q) {
+    const int64_t jq = j0 + q * kCsBlock;
+    const int64_t j = jq < n ? jq : n - 1;
+    u[q] = uk[j];
+    h[q] = hv[j];
+    p[q] = up[j];
+  }
+#pragma unroll
+  for (int q = 0; q < kCsEpt; ++q) {
+    const int64_t j = j0 + q * kCsBlock;
+    if (j >= n) continue;
+    E w;
+    if constexpr (std::is_same<E, R>::value) {
+      w = first ? h[q] - ca * u[q] : h[q] - ca * u[q] - cb * p[q];
+      ss += (double)w * (double)w;
+    } else {
+      if (first) {
+        w.x = h[q].x - ca * u[q].x;
+        w.y = h[q].y - ca * u[q].y;
+      } else {
+        w.x = h[q].x - ca * u[q].x - cb * p[q].x;
+        w.y = h[q].y - ca * u[q].y - cb * p[q].y;
+      }
+      ss += (double)w.x * (double)w.x + (double)w.y * (double)w.y;
+    }
+    up[j] = w;
+  }
+}
+
+// Launch B of step k: alpha_k, u_{k+1} over u_{k-1}'s buffer, and the
+// block's partial of |u_{k+1}|^2.
+template <typename R>
+__global__ __launch_bounds__(kCsBlock) void k_cs_update(const CsArgs a) {
+  using C = ccs<R>;
+  const int2 it = a.items[blockIdx.x];
+  if (it.x < 0) return;
+  const CsChain ch = a.chains[it.x];
+  const CsSeg sg = a.segs[ch.seg];
+  LanczosDev *lz = a.lz + it.x;
+  const int k = *a.kbase + a.i;
+  const bool done = a.dflag[2 * it.x + (k & 1)] != 0;
+  const double al = cs_sum_partials(a.pa + ch.pofs_h, ch.nblk_h);
+  if (it.y == 0 && threadIdx.x == 0) lz->alpha[k] = done ? 0.0 : al;
+  double ss = 0.0;
+  if (!done) {
+    const R ca = (R)(al / lz->nrm[k]);
+    const R cb = k > 0 ? (R)(lz->beta[k] / lz->nrm[k - 1]) : (R)0;
+    const int64_t coff = (int64_t)ch.local * sg.stride;
+    const int64_t j0 = (int64_t)it.y * kCsBlock * kCsEpt + threadIdx.x;
+    if (sg.real) {
+      const R *uk = static_cast<const R *>((k & 1) ? sg.u1 : sg.u0) + coff;
+      R *up = static_cast<R *>((k & 1) ? sg.u0 : sg.u1) + coff;
+      cs_upd<R, R>(up, uk, static_cast<const R *>(sg.hv) + coff, sg.n, j0, ca, cb, k == 0, ss);
+    } else {
+      const C *uk = static_cast<const C *>((k & 1) ? sg.u1 : sg.u0) + coff;
+      C *up = static_cast<C *>((k & 1) ? sg.u0 : sg.u1) + coff;
+      cs_upd<C, R>(up, uk, static_cast<const C *>(sg.hv) + coff, sg.n, j0, ca, cb, k == 0, ss);
+    }
+  }
+  const double bs = cs_block_sum_all(ss);
+  if (threadIdx.x == 0) a.pb[ch.pofs_u + it.y] = bs;
+}
+
+__global__ void k_cs_advance(int32_t *kbase, int n) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *kbase += n;
+}
+
+// (chain, block) items of every chain, XCD-aware: chains are dealt to the
+// 8 XCDs by decreasing block count, each to the least loaded; the items of
+// XCD x are i = 8 s + x, the chains of x one after another.
+std::vector<int2> cs_items(const std::vector<int> &nblk) {
+  const int nch = (int)nblk.size();
+  std::vector<int> order(nch);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int p, int q) { return nblk[p] > nblk[q]; });
+  std::vector<int64_t> load(kCsXcd, 0);
+  std::vector<std::vector<int>> own(kCsXcd);
+  for (int c : order) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    own[x].push_back(c);
+    load[x] += nblk[c];
+  }
+  for (auto &v : own) std::sort(v.begin(), v.end());
+  const int64_t len = *std::max_element(load.begin(), load.end());
+  std::vector<int2> items((size_t)len * kCsXcd, make_int2(-1, 0));
+  for (int x = 0; x < kCsXcd; ++x) {
+    int64_t s = 0;
+    for (int c : own[x])
+      for (int b = 0; b < nblk[c]; ++b, ++s) items[(size_t)s * kCsXcd + x] = make_int2(c, b);
+  }
+  return items;
+}
+
+template <typename T>
+T *cs_upload(const std::vector<T> &h, hipStream_t st) {
+  T *d = nullptr;
+  VCA_HIP(hipMalloc(&d, sizeof(T) * std::max<size_t>(1, h.size())));
+  if (!h.empty())
+    VCA_HIP(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st));
+  return d;
+}
+
+}  // namespace
+
+int64_t chain_set_bytes(const Sector &js, bool real, int prec) {
+  const size_t rsz = prec == VCA_FP64 ? sizeof(double) : sizeof(float);
+  const int64_t n = real ? js.dim + (js.dim & 1) : js.dim;
+  return 3 * n * (int64_t)(real ? rsz : 2 * rsz);
+}
+
+ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dimup_i,
+                            int dimdw_i, const void *psi, const void *psi_r,
+                            const CrossTerm &cross, bool real_amp, hipStream_t st,
+                            std::vector<ChainOut> &out) {
+  ChainSetStats stats{};
+  const int nch = (int)jobs.size();
+  out.assign(nch, ChainOut{});
+  if (nch == 0) return stats;
+  const size_t rsz = prec == VCA_FP64 ? sizeof(double) : sizeof(float);
+  // segments: the chains of one (target sector, vector kind), in job order
+  std::map<std::pair<const Sector *, bool>, int> seg_of;
+  std::vector<CsSeg> segs;
+  std::vector<std::vector<int>> seg_jobs;
+  std::vector<const Sector *> seg_sec;
+  for (int c = 0; c < nch; ++c) {
+    const auto key = std::make_pair((const Sector *)jobs[c].js, jobs[c].real);
+    auto f = seg_of.find(key);
+    if (f == seg_of.end()) {
+      f = seg_of.emplace(key, (int)segs.size()).first;
+      const Sector &js = *jobs[c].js;
+      if (js.precision != prec) fail(VCA_ERR_ARG, "chain set: sector precision differs");
+      CsSeg s{};
+      s.meta_up = js.up.meta;
+      s.e_dw = js.dw.energy;
+      s.map_dw = js.dw.map;
+      s.cnt_dw = js.dw.cnt;
+      s.ent_up = js.up.ent;
+      s.ent_dw = js.dw.ent;
+      s.amp_up = prec == VCA_FP64 ? (const void *)js.up.amp64 : (const void *)js.up.amp32;
+      s.amp_dw = prec == VCA_FP64 ? (const void *)js.dw.amp64 : (const void *)js.dw.amp32;
+      s.namp_up = 2 * std::max(1, js.up.nterms);
+      s.namp_dw = 2 * std::max(1, js.dw.nterms);
+      s.dimup = js.up.dim;
+      s.dimdw = js.dw.dim;
+      s.real = jobs[c].real ? 1 : 0;
+      s.n = js.dim;
+      // real chains: an even stride, so the excitation's pad real of an
+      // odd dimension stays inside the chain's slot
+      s.stride = jobs[c].real ? js.dim + (js.dim & 1) : js.dim;
+      segs.push_back(s);
+      seg_jobs.emplace_back();
+      seg_sec.push_back(&js);
+    }
+    seg_jobs[f->second].push_back(c);
+  }
+  const int nseg = (int)segs.size();
+  std::vector<CsChain> chains(nch);
+  std::vector<int> nblk_h(nch), nblk_u(nch);
+  int max_steps = 0;
+  int64_t ph = 0, pu = 0;
+  for (int sgi = 0; sgi < nseg; ++sgi) {
+    CsSeg &s = segs[sgi];
+    const size_t esz = s.real ? rsz : 2 * rsz;
+    const size_t bytes = esz * (size_t)s.stride * seg_jobs[sgi].size();
+    for (void **b : {&s.u0, &s.u1, &s.hv}) {
+      *b = dev_alloc_async(bytes, st);
+      stats.device_bytes += (int64_t)bytes;
+    }
+    for (size_t l = 0; l < seg_jobs[sgi].size(); ++l) {
+      const int c = seg_jobs[sgi][l];
+      CsChain &ch = chains[c];
+      ch.seg = sgi;
+      ch.local = (int)l;
+      ch.nsteps = jobs[c].nsteps;
+      ch.nblk_h = (int)((s.n + kCsBlock - 1) / kCsBlock);
+      ch.nblk_u = (int)((s.n + kCsBlock * kCsEpt - 1) / (kCsBlock * kCsEpt));
+      ch.pofs_h = ph;
+      ch.pofs_u = pu;
+      ph += ch.nblk_h;
+      pu += ch.nblk_u;
+      nblk_h[c] = ch.nblk_h;
+      nblk_u[c] = ch.nblk_u;
+      max_steps = std::max(max_steps, ch.nsteps);
+    }
+  }
+  const std::vector<int2> items_h = cs_items(nblk_h), items_u = cs_items(nblk_u);
+  // per-chain Lanczos scalars
+  const size_t cap = (size_t)max_steps + 4;
+  double *lz_arrays = static_cast<double *>(dev_alloc_async(sizeof(double) * 3 * cap * nch, st));
+  std::vector<LanczosDev> lzh(nch);
+  for (int c = 0; c < nch; ++c) {
+    lzh[c] = LanczosDev{};
+    lzh[c].alpha = lz_arrays + (size_t)c * 3 * cap;
+    lzh[c].beta = lzh[c].alpha + cap;
+    lzh[c].nrm = lzh[c].beta + cap;
+    lzh[c].y = nullptr;
+  }
+  CsSeg *d_segs = cs_upload(segs, st);
+  CsChain *d_chains = cs_upload(chains, st);
+  int2 *d_items_h = cs_upload(items_h, st);
+  int2 *d_items_u = cs_upload(items_u, st);
+  LanczosDev *d_lz = cs_upload(lzh, st);
+  int32_t *d_dflag = nullptr;
+  VCA_HIP(hipMalloc(&d_dflag, sizeof(int32_t) * 2 * nch));
+  VCA_HIP(hipMemsetAsync(d_dflag, 0, sizeof(int32_t) * 2 * nch, st));
+  VCA_HIP(hipMemsetAsync(lz_arrays, 0, sizeof(double) * 3 * cap * nch, st));
+  double *d_pa = nullptr, *d_pb = nullptr, *d_px = nullptr;
+  VCA_HIP(hipMalloc(&d_pa, sizeof(double) * std::max<int64_t>(1, ph)));
+  VCA_HIP(hipMalloc(&d_pb, sizeof(double) * std::max<int64_t>(1, pu)));
+  int32_t *d_kbase = nullptr;
+  VCA_HIP(hipMalloc(&d_kbase, sizeof(int32_t)));
+  VCA_HIP(hipMemsetAsync(d_kbase, 0, sizeof(int32_t), st));
+  // start vectors O|psi> per segment, norms into nrm[0]
+  int64_t px_need = 1;
+  for (int sgi = 0; sgi < nseg; ++sgi)
+    px_need = std::max<int64_t>(px_need, (int64_t)excite_grid(segs[sgi].n) *
+                                             (int64_t)seg_jobs[sgi].size());
+  VCA_HIP(hipMalloc(&d_px, sizeof(double) * px_need));
+  std::vector<ExcOp> ops(nch);
+  for (int c = 0; c < nch; ++c) ops[c] = jobs[c].op;
+  ExcOp *d_ops = cs_upload(ops, st);
+  for (int sgi = 0; sgi < nseg; ++sgi) {
+    const std::vector<int> &sj = seg_jobs[sgi];
+    // the chains of a segment are consecutive in neither jobs nor d_lz:
+    // stage their operators and Lanczos structs contiguously
+    std::vector<ExcOp> sops(sj.size());
+    std::vector<LanczosDev> slz(sj.size());
+    for (size_t l = 0; l < sj.size(); ++l) {
+      sops[l] = ops[sj[l]];
+      slz[l] = lzh[sj[l]];
+    }
+    ExcOp *d_sops = cs_upload(sops, st);
+    LanczosDev *d_slz = cs_upload(slz, st);
+    const Sector &js = *seg_sec[sgi];
+    launch_excite_into(js, dimup_i, dimdw_i, segs[sgi].real ? psi_r : psi, jobs[sj[0]].spin,
+                       d_sops, (int)sj.size(), segs[sgi].u0, segs[sgi].stride,
+                       segs[sgi].real != 0, d_slz, d_px, st);
+    VCA_HIP(hipStreamSynchronize(st));
+    VCA_HIP(hipFree(d_sops));
+    VCA_HIP(hipFree(d_slz));
+  }
+  CsArgs a{};
+  a.segs = d_segs;
+  a.chains = d_chains;
+  a.lz = d_lz;
+  a.dflag = d_dflag;
+  a.pa = d_pa;
+  a.pb = d_pb;
+  a.kbase = d_kbase;
+  a.cross = cross;
+  const int nih = (int)(items_h.size()), niu = (int)(items_u.size());
+  int namp = 1;
+  for (const CsSeg &sgm : segs) namp = std::max(namp, sgm.namp_up + sgm.namp_dw);
+  const size_t smem = (size_t)namp * 2 * rsz;
+  auto launch_step = [&](int i) {
+    CsArgs ah = a;
+    ah.items = d_items_h;
+    ah.i = i;
+    CsArgs au = a;
+    au.items = d_items_u;
+    au.i = i;
+    if (prec == VCA_FP64) {
+      if (real_amp)
+        hipLaunchKernelGGL((k_cs_hxv<double, true>), dim3(nih), dim3(kCsBlock), smem, st, ah);
+      else
+        hipLaunchKernelGGL((k_cs_hxv<double, false>), dim3(nih), dim3(kCsBlock), smem, st, ah);
+      hipLaunchKernelGGL(k_cs_update<double>, dim3(niu), dim3(kCsBlock), 0, st, au);
+    } else {
+      if (real_amp)
+        hipLaunchKernelGGL((k_cs_hxv<float, true>), dim3(nih), dim3(kCsBlock), smem, st, ah);
+      else
+        hipLaunchKernelGGL((k_cs_hxv<float, false>), dim3(nih), dim3(kCsBlock), smem, st, ah);
+      hipLaunchKernelGGL(k_cs_update<float>, dim3(niu), dim3(kCsBlock), 0, st, au);
+    }
+  };
+  const char *ng = std::getenv("VCA_NO_GRAPHS");
+  hipGraphExec_t ex = nullptr;
+  if (max_steps >= kCsChunk && !(ng && ng[0] == '1')) {
+    hipGraph_t graph;
+    VCA_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < kCsChunk; ++i) launch_step(i);
+    hipLaunchKernelGGL(k_cs_advance, dim3(1), dim3(1), 0, st, d_kbase, kCsChunk);
+    VCA_HIP(hipStreamEndCapture(st, &graph));
+    VCA_HIP(hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0));
+    VCA_HIP(hipGraphDestroy(graph));
+  }
+  hipEvent_t e0, e1;
+  VCA_HIP(hipEventCreate(&e0));
+  VCA_HIP(hipEventCreate(&e1));
+  VCA_HIP(hipEventRecord(e0, st));
+  int done = 0;
+  if (ex) {
+    for (; done + kCsChunk <= max_steps; done += kCsChunk) VCA_HIP(hipGraphLaunch(ex, st));
+    stats.launches += (int64_t)(done / kCsChunk) * (2 * kCsChunk + 1);
+  }
+  for (int i = 0; done < max_steps; ++done, ++i) {
+    launch_step(i);
+    stats.launches += 2;
+  }
+  VCA_HIP(hipGetLastError());
+  VCA_HIP(hipEventRecord(e1, st));
+  VCA_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  VCA_HIP(hipEventElapsedTime(&ms, e0, e1));
+  VCA_HIP(hipEventDestroy(e0));
+  VCA_HIP(hipEventDestroy(e1));
+  if (ex) VCA_HIP(hipGraphExecDestroy(ex));
+  stats.seconds = ms * 1e-3;
+  stats.steps = max_steps;
+  // alpha, beta and the start norm of every chain
+  std::vector<double> host((size_t)3 * cap * nch);
+  VCA_HIP(hipMemcpyAsync(host.data(), lz_arrays, sizeof(double) * host.size(),
+                         hipMemcpyDeviceToHost, st));
+  VCA_HIP(hipStreamSynchronize(st));
+  for (int c = 0; c < nch; ++c) {
+    const double *base = host.data() + (size_t)c * 3 * cap;
+    const int n = jobs[c].nsteps;
+    out[c].alpha.assign(base, base + n);
+    out[c].beta.assign(base + cap, base + cap + n);
+    out[c].nrm0 = base[2 * cap];
+    stats.hxv_count += n;
+    stats.hxv_states += (double)n * (double)jobs[c].js->dim;
+  }
+  for (int sgi = 0; sgi < nseg; ++sgi)
+    for (void *b : {segs[sgi].u0, segs[sgi].u1, segs[sgi].hv}) dev_free_async(b, st);
+  dev_free_async(lz_arrays, st);
+  VCA_HIP(hipStreamSynchronize(st));
+  for (void *q : {(void *)d_segs, (void *)d_chains, (void *)d_items_h, (void *)d_items_u,
+                  (void *)d_lz, (void *)d_dflag, (void *)d_pa, (void *)d_pb, (void *)d_px,
+                  (void *)d_kbase, (void *)d_ops})
+    VCA_HIP(hipFree(q));
+  return stats;
+}
+
+}  // namespace vca

@@ -2437,6 +2437,8 @@ void build_spin_tables(Sector &sec, DevSpin &sp, const SpinModel &m, int ns, int
   VCA_HIP(hipMalloc(&sp.energy, sizeof(double) * sp.dim));
   VCA_HIP(hipMalloc(&sp.cnt, sizeof(int32_t) * sp.dim));
   VCA_HIP(hipMalloc(&sp.ent, sizeof(uint32_t) * (size_t)sp.dim * sp.maxnnz));
+  // slots past a row's hop count hold 0 (column 0, amplitude index 0)
+  VCA_HIP(hipMemsetAsync(sp.ent, 0, sizeof(uint32_t) * (size_t)sp.dim * sp.maxnnz, st));
   VCA_HIP(hipMalloc(&sp.amp64, sizeof(double2) * amp64.size()));
   VCA_HIP(hipMalloc(&sp.amp32, sizeof(float2) * amp32.size()));
   VCA_HIP(hipMalloc(&d_ab, sizeof(int2) * ab.size()));
@@ -3473,11 +3475,14 @@ void launch_convert(const void *src, int src_prec, void *dst, int dst_prec, int6
   VCA_HIP(hipGetLastError());
 }
 
-void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int spin,
-                   const ExcOp *d_ops, int nb) {
+int excite_grid(int64_t dim) { return grid_for(dim); }
+
+void launch_excite_into(const Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int spin,
+                        const ExcOp *d_ops, int nb, void *out, int64_t vstride, bool real,
+                        LanczosDev *lz, double *partials, hipStream_t st) {
   ExcArgs a{};
-  a.vstride = jsec.dim;
-  a.out = jsec.buf[0];
+  a.vstride = vstride;
+  a.out = out;
   a.psi = psi;
   a.dimup_j = jsec.up.dim;
   a.dimdw_j = jsec.dw.dim;
@@ -3487,31 +3492,33 @@ void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int 
   a.map_j = a.spin == 1 ? jsec.up.map : jsec.dw.map;
   a.binom = solver_binom(jsec.solver);
   a.ops = d_ops;
-  a.lz = jsec.lz;
+  a.lz = lz;
+  a.partials = partials;
   const int grid = grid_for(jsec.dim);
-  ensure_partials(jsec, grid * nb);
-  a.partials = jsec.partials;
-  const bool ro = jsec.real_vec;
-  if (ro) a.vstride = 2 * jsec.dim;  // reals between chains (real-vector mode)
   if (jsec.precision == VCA_FP64) {
-    if (ro)
-      hipLaunchKernelGGL((k_excite<double, true>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
-                         a);
+    if (real)
+      hipLaunchKernelGGL((k_excite<double, true>), dim3(grid, nb), dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((k_excite<double, false>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
-                         a);
+      hipLaunchKernelGGL((k_excite<double, false>), dim3(grid, nb), dim3(kBlock), 0, st, a);
   } else {
-    if (ro)
-      hipLaunchKernelGGL((k_excite<float, true>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
-                         a);
+    if (real)
+      hipLaunchKernelGGL((k_excite<float, true>), dim3(grid, nb), dim3(kBlock), 0, st, a);
     else
-      hipLaunchKernelGGL((k_excite<float, false>), dim3(grid, nb), dim3(kBlock), 0, jsec.stream,
-                         a);
+      hipLaunchKernelGGL((k_excite<float, false>), dim3(grid, nb), dim3(kBlock), 0, st, a);
   }
   VCA_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_lz_start, dim3(nb), dim3(kBlock), 0, jsec.stream, jsec.lz, jsec.partials,
-                     grid);
+  hipLaunchKernelGGL(k_lz_start, dim3(nb), dim3(kBlock), 0, st, lz, partials, grid);
   VCA_HIP(hipGetLastError());
+}
+
+void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int spin,
+                   const ExcOp *d_ops, int nb) {
+  const int grid = grid_for(jsec.dim);
+  ensure_partials(jsec, grid * nb);
+  const bool ro = jsec.real_vec;
+  // reals between chains in real-vector mode: the complex stride
+  launch_excite_into(jsec, dimup_i, dimdw_i, psi, spin, d_ops, nb, jsec.buf[0],
+                     ro ? 2 * jsec.dim : jsec.dim, ro, jsec.lz, jsec.partials, jsec.stream);
 }
 
 int real_num_variants() { return kNumRealVariants; }

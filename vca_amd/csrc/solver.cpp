@@ -1265,6 +1265,14 @@ static int chain_batch_cap() {
 // degenerate states is O(1) away.
 constexpr double kRealStateTol = 1e-4;
 
+// Target sectors below this size run their chains as one chain set (all
+// such sectors of a kept state together, chains.hip); larger ones per
+// sector with their tuned H x v.  VCA_CHAIN_SET_MAX_DIM overrides (0: off).
+static int64_t chain_set_max_dim() {
+  const char *e = std::getenv("VCA_CHAIN_SET_MAX_DIM");
+  return e && *e ? std::atoll(e) : ((int64_t)1 << 17);
+}
+
 static void build_gf(Solver &s) {
   const vca_params &p = s.p;
   s.gf.clear();
@@ -1313,6 +1321,89 @@ static void build_gf(Solver &s) {
       const ChainSpec &cs = specs[c];
       if (cs.jnup < 0 || cs.jndw < 0 || cs.jnup > s.ns || cs.jndw > s.ns) continue;
       groups[{cs.jnup, cs.jndw}].push_back((int)c);
+    }
+    // pesoBZ (VCA_GF_NORMAL.f90:987-994): Boltzmann factor at finite T
+    double boltz = 1.0;
+    if (finite_t(p)) {
+      const double x = p.beta * (st.e - s.emin);
+      boltz = x < 200.0 ? std::exp(-x) : 0.0;
+    }
+    // add_to_lanczos_gf_normal (VCA_GF_NORMAL.f90:970-1022) for chain c:
+    // the tridiagonal eigensolve of its nlanc alpha/beta and the poles and
+    // weights of its channel; returns the largest excitation energy
+    auto post_chain = [&](int c, int nlanc, const double *al, const double *be,
+                          double nrm0) -> double {
+      const ChainSpec &cs = specs[c];
+      const double norm2 = nrm0 * nrm0;
+      double cmax = -std::numeric_limits<double>::infinity();
+      if (!(norm2 > 0.0)) return cmax;  // O|psi> = 0: empty channel
+      std::vector<double> w(nlanc), z(nlanc);
+      tridiag_eig_first(nlanc, al, be, w.data(), z.data());
+      const zc peso_bz =
+          zc(chan_re[cs.ichan - 1], chan_im[cs.ichan - 1]) * norm2 * boltz / s.zeta;
+      GfChan &out = res[c];
+      out.w.resize(nlanc);
+      out.p.resize(nlanc);
+      for (int j = 0; j < nlanc; ++j) {
+        const double de = w[j] - st.e;
+        cmax = std::max(cmax, de);
+        out.w[j] = peso_bz * z[j] * z[j];
+        out.p[j] = cs.isign * de;
+      }
+      return cmax;
+    };
+    // chains on small target sectors: one chain set, within half the free
+    // memory per run
+    std::vector<ChainJob> jobs;
+    std::vector<int> job_spec;
+    const int64_t cs_max = chain_set_max_dim();
+    const bool real_ham = psi_r != nullptr;
+    for (auto it = groups.begin(); it != groups.end();) {
+      Sector &js = get_sector(it->first[0], it->first[1]);
+      if (js.dim >= cs_max) {
+        ++it;
+        continue;
+      }
+      const int nlanc = (int)std::min<int64_t>(js.dim, p.lanc_ngfiter);
+      for (int c : it->second) {
+        bool real_op = real_ham;
+        for (int o = 0; o < specs[c].op.nops; ++o) real_op = real_op && specs[c].op.cim[o] == 0.0;
+        jobs.push_back(ChainJob{&js, real_op, nlanc, specs[c].op, specs[c].spin});
+        job_spec.push_back(c);
+      }
+      it = groups.erase(it);
+    }
+    for (size_t j0 = 0; j0 < jobs.size();) {
+      size_t free_b = 0, total_b = 0;
+      VCA_HIP(hipMemGetInfo(&free_b, &total_b));
+      const int64_t budget = (int64_t)(free_b / 2);
+      int64_t used = 0;
+      size_t j1 = j0;
+      while (j1 < jobs.size()) {
+        const int64_t b = chain_set_bytes(*jobs[j1].js, jobs[j1].real, prec);
+        if (j1 > j0 && used + b > budget) break;
+        used += b;
+        ++j1;
+      }
+      const std::vector<ChainJob> part(jobs.begin() + j0, jobs.begin() + j1);
+      std::vector<ChainOut> outs;
+      const ChainSetStats cst = run_chain_set(part, prec, dimup_i, dimdw_i, psi, psi_r,
+                                              sector_cross(*part[0].js), s.md.spin[0].real_amp &&
+                                              s.md.spin[1].real_amp, s.stream, outs);
+      std::vector<double> cmax(part.size());
+      parallel_for((int)part.size(), [&](int q) {
+        cmax[q] = post_chain(job_spec[j0 + q], part[q].nsteps, outs[q].alpha.data(),
+                             outs[q].beta.data(), outs[q].nrm0);
+      });
+      for (double c : cmax) s.max_exc = std::max(s.max_exc, c);
+      s.res.nchains += (int)part.size();
+      s.res.hxv_seconds += cst.seconds;
+      s.res.hxv_count += cst.hxv_count;
+      s.res.hxv_states += cst.hxv_states;
+      progress(s.t_start, "state %zu chain set: chains %zu-%zu of %zu, %d steps, %lld launches, "
+               "%.3f s", ist, j0 + 1, j1, jobs.size(), cst.steps, (long long)cst.launches,
+               cst.seconds);
+      j0 = j1;
     }
     for (auto &g : groups) {
       Sector &js = get_sector(g.first[0], g.first[1]);
@@ -1363,35 +1454,12 @@ static void build_gf(Solver &s) {
                                  hipMemcpyDeviceToHost, js.stream));
         }
         VCA_HIP(hipStreamSynchronize(js.stream));
-        // add_to_lanczos_gf_normal (VCA_GF_NORMAL.f90:970-1022): the chains'
-        // tridiagonal eigensolves run on host threads, one slice each
-        // pesoBZ (VCA_GF_NORMAL.f90:987-994): Boltzmann factor at finite T
-        double boltz = 1.0;
-        if (finite_t(p)) {
-          const double x = p.beta * (st.e - s.emin);
-          boltz = x < 200.0 ? std::exp(-x) : 0.0;
-        }
-        std::vector<double> cmax(nb, -std::numeric_limits<double>::infinity());
-        auto post = [&](int c) {
-          const ChainSpec &cs = specs[idx[b0 + c]];
-          const double norm2 = nrm0[c] * nrm0[c];
-          if (!(norm2 > 0.0)) return;  // O|psi> = 0: empty channel
-          std::vector<double> w(nlanc), z(nlanc);
-          tridiag_eig_first(nlanc, al.data() + (size_t)c * nlanc,
-                            be.data() + (size_t)c * nlanc, w.data(), z.data());
-          const zc peso_bz =
-              zc(chan_re[cs.ichan - 1], chan_im[cs.ichan - 1]) * norm2 * boltz / s.zeta;
-          GfChan &out = res[idx[b0 + c]];
-          out.w.resize(nlanc);
-          out.p.resize(nlanc);
-          for (int j = 0; j < nlanc; ++j) {
-            const double de = w[j] - st.e;
-            cmax[c] = std::max(cmax[c], de);
-            out.w[j] = peso_bz * z[j] * z[j];
-            out.p[j] = cs.isign * de;
-          }
-        };
-        parallel_for(nb, post);
+        // the chains' tridiagonal eigensolves run on host threads
+        std::vector<double> cmax(nb);
+        parallel_for(nb, [&](int c) {
+          cmax[c] = post_chain(idx[b0 + c], nlanc, al.data() + (size_t)c * nlanc,
+                               be.data() + (size_t)c * nlanc, nrm0[c]);
+        });
         for (int c = 0; c < nb; ++c) s.max_exc = std::max(s.max_exc, cmax[c]);
         s.res.nchains += nb;
         progress(s.t_start, "state %zu sector (%d,%d) %s chains %zu-%zu of %zu, %d steps",

@@ -329,6 +329,46 @@ void launch_convert(const void *src, int src_prec, void *dst, int dst_prec,
 void launch_excite(Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int spin,
                    const ExcOp *d_ops, int nb);
 CrossTerm make_cross(const ModelData &md);
+// Partial sums per chain of the excitation's norm (grid of k_excite).
+int excite_grid(int64_t dim);
+// k_excite into an explicit buffer: nb chains' start vectors O|psi> at out
+// (vstride elements apart; reals when real), their norms into lz[c].nrm[0];
+// partials holds excite_grid(jsec.dim) * nb doubles.
+void launch_excite_into(const Sector &jsec, int dimup_i, int dimdw_i, const void *psi, int spin,
+                        const ExcOp *d_ops, int nb, void *out, int64_t vstride, bool real,
+                        LanczosDev *lz, double *partials, hipStream_t st);
+
+// Chains of uneven target sectors run together (chains.hip): every chain
+// of the set advances one Lanczos step per two launches, captured in
+// 16-step graphs.
+struct ChainJob {
+  Sector *js;   // target sector (its spin tables; its precision is the set's)
+  bool real;    // real vectors (real Hamiltonian, real state, real operator)
+  int nsteps;   // Lanczos steps of the chain
+  ExcOp op;     // excitation operator on the kept state
+  int spin;     // spin the operator acts on (1 up, 2 down)
+};
+struct ChainOut {
+  std::vector<double> alpha, beta;  // nsteps each; beta[0] = 0
+  double nrm0 = 0;                  // |O psi|
+};
+struct ChainSetStats {
+  double seconds = 0;     // GPU time of the Lanczos steps
+  int64_t launches = 0;   // kernel launches of those steps
+  int64_t hxv_count = 0;  // chain steps
+  double hxv_states = 0;  // sum over chain steps of the sector dimension
+  int64_t device_bytes = 0;  // vector memory of the set
+  int steps = 0;          // steps of the longest chain
+};
+// Device bytes of one chain in a set (its three vectors).
+int64_t chain_set_bytes(const Sector &js, bool real, int prec);
+// Runs every job to its step count; psi is the kept state (complex, of the
+// set's precision) on the source sector (dimup_i x dimdw_i), psi_r its real
+// form (null when no job is real).
+ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dimup_i,
+                            int dimdw_i, const void *psi, const void *psi_r,
+                            const CrossTerm &cross, bool real_amp, hipStream_t st,
+                            std::vector<ChainOut> &out);
 
 // Lowest eigenpairs by thick-restart Lanczos (eigs.hip).
 struct EigsOut {
