@@ -36,16 +36,14 @@
 #include <numeric>
 #include <type_traits>
 
+#include "rows.hpp"
 #include "vca_internal.hpp"
 
 namespace vca {
 
 namespace {
 
-template <typename R> struct CS2;
-template <> struct CS2<double> { using T = double2; };
-template <> struct CS2<float> { using T = float2; };
-template <typename R> using ccs = typename CS2<R>::T;
+template <typename R> using ccs = rcplx<R>;
 
 constexpr int kCsBlock = 256;
 constexpr int kCsEpt = 4;      // elements per lane in the update
@@ -53,15 +51,9 @@ constexpr int kCsXcd = 8;      // XCDs of the MI355X
 constexpr int kCsChunk = 16;   // Lanczos steps per captured graph
 
 struct CsSeg {
-  const RowMeta *meta_up;  // energy, string, hop count per up row
-  const double *e_dw;
-  const uint32_t *map_dw;
-  const int32_t *cnt_dw;
-  const uint32_t *ent_up, *ent_dw;  // k-major ELL [maxnnz][dim]: col | amp << 21
-  const void *amp_up, *amp_dw;      // signed amplitudes, complex of the precision
-  int namp_up, namp_dw;             // entries of the amplitude tables
-  int dimup, dimdw;
+  RowTab tab;      // the target sector's tables
   int real;        // real vectors (real amplitudes)
+  int colmode;     // H x v items are whole columns staged in LDS (else row blocks)
   int64_t n;       // elements per chain: dim (complex or real)
   int64_t stride;  // elements between consecutive chains' vectors
   void *u0, *u1, *hv;
@@ -116,85 +108,16 @@ __device__ inline double cs_sum_partials(const double *p, int n) {
   return cs_block_sum_all(s);
 }
 
-__device__ inline double cs_cross(const CrossTerm &c, uint32_t su, uint32_t sd) {
-  if (c.norb == 1) return c.wx[0][0] * (double)__builtin_popcount(su & sd & c.site_mask);
-  double e = 0;
-  for (int io = 0; io < c.norb; ++io)
-    for (int jo = 0; jo < c.norb; ++jo)
-      e += c.wx[io][jo] * (double)__builtin_popcount((su >> io) & (sd >> jo) & c.site_mask);
-  return e;
+// Byte offset in the dynamic LDS of a staged column: after the segment's
+// amplitude tables (complex of csz bytes each), 16-byte aligned.
+__host__ __device__ inline size_t cs_col_offset(const RowTab &t, size_t csz) {
+  return (((size_t)(t.namp_up + t.namp_dw) * csz) + 15) & ~(size_t)15;
 }
 
-template <typename R, bool REAL_AMP>
-__device__ inline void cs_cmac(R &ar, R &ai, const ccs<R> &h, const ccs<R> &y) {
-  if (REAL_AMP) {
-    ar = fma(h.x, y.x, ar);
-    ai = fma(h.x, y.y, ai);
-  } else {
-    ar = fma(h.x, y.x, ar);
-    ar = fma(-h.y, y.y, ar);
-    ai = fma(h.x, y.y, ai);
-    ai = fma(h.y, y.x, ai);
-  }
-}
-
-// Hops of one row, kCsHop at a time: the chunk's table entries, then its
-// gathered elements and amplitudes, are issued before any is summed (the
-// row's loads overlap instead of waiting one by one).  Slots past the row's
-// count are not read from the table (the ELL holds nothing there; a row
-// may have no hops at all): they take entry 0 -- element 0 of the column
-// or row 0's neighbour, amplitude 0, all in bounds -- weighted 0.
-constexpr int kCsHop = 4;
-
-template <typename R>
-__device__ inline void cs_mac(R &acc, const ccs<R> &h, R y, bool on) {
-  acc = fma(on ? h.x : (R)0, y, acc);
-}
-
-template <typename R, bool REAL_AMP>
-__device__ inline void cs_mac(R &ar, R &ai, const ccs<R> &h, const ccs<R> &y, bool on) {
-  ccs<R> hh = h;
-  if (!on) hh.x = hh.y = (R)0;
-  cs_cmac<R, REAL_AMP>(ar, ai, hh, y);
-}
-
-// H u on one row (iup, idw) of a segment, unscaled: the diagonal ed plus
-// the up-spin hops (gathers inside the column col) and the down-spin hops
-// (neighbour columns of vin).  E = R (real vectors) or ccs<R>.
-template <typename R, bool REAL_AMP, typename E>
-__device__ inline void cs_row(const CsSeg &sg, const E *vin, const E *col, int iup, int idw,
-                              int cu, int cd, const ccs<R> *s_amu, const ccs<R> *s_amd, R &ar,
-                              R &ai) {
-  const int nmax = max(cu, cd);
-  for (int q0 = 0; q0 < nmax; q0 += kCsHop) {
-    uint32_t eu[kCsHop], ed[kCsHop];
-#pragma unroll
-    for (int j = 0; j < kCsHop; ++j) {
-      const int q = q0 + j;
-      eu[j] = q < cu ? sg.ent_up[(int64_t)q * sg.dimup + iup] : 0u;
-      ed[j] = q < cd ? sg.ent_dw[(int64_t)q * sg.dimdw + idw] : 0u;
-    }
-    E yu[kCsHop], yd[kCsHop];
-    ccs<R> hu[kCsHop], hd[kCsHop];
-#pragma unroll
-    for (int j = 0; j < kCsHop; ++j) {
-      yu[j] = col[eu[j] & kColMask];
-      yd[j] = vin[(int64_t)(ed[j] & kColMask) * sg.dimup + iup];
-      hu[j] = s_amu[eu[j] >> kColBits];
-      hd[j] = s_amd[ed[j] >> kColBits];
-    }
-#pragma unroll
-    for (int j = 0; j < kCsHop; ++j) {
-      if constexpr (std::is_same<E, R>::value) {
-        cs_mac<R>(ar, hu[j], yu[j], q0 + j < cu);
-        cs_mac<R>(ar, hd[j], yd[j], q0 + j < cd);
-      } else {
-        cs_mac<R, REAL_AMP>(ar, ai, hu[j], yu[j], q0 + j < cu);
-        cs_mac<R, REAL_AMP>(ar, ai, hd[j], yd[j], q0 + j < cd);
-      }
-    }
-  }
-}
+// Columns from this many rows to kCsColMax are staged whole (one workgroup
+// per column); shorter columns run as row blocks.
+constexpr int kCsColMin = 128;
+constexpr int kCsColMax = 1024;
 
 // Launch A of step k: norm and done flag of u_k, then Hv = H u_k / |u_k| on
 // one block of rows (one row per lane: state r = iup + idw dimup) and the
@@ -210,11 +133,8 @@ __global__ __launch_bounds__(kCsBlock) void k_cs_hxv(const CsArgs a) {
   const CsSeg sg = a.segs[ch.seg];
   LanczosDev *lz = a.lz + it.x;
   C *s_amu = reinterpret_cast<C *>(cs_smem);
-  C *s_amd = s_amu + sg.namp_up;
-  for (int t = threadIdx.x; t < sg.namp_up; t += kCsBlock)
-    s_amu[t] = static_cast<const C *>(sg.amp_up)[t];
-  for (int t = threadIdx.x; t < sg.namp_dw; t += kCsBlock)
-    s_amd[t] = static_cast<const C *>(sg.amp_dw)[t];
+  C *s_amd = s_amu + sg.tab.namp_up;
+  rows::stage_amps<R, kCsBlock>(sg.tab, s_amu, s_amd);
   const int k = *a.kbase + a.i;
   double nrm;
   bool done;
@@ -235,22 +155,79 @@ __global__ __launch_bounds__(kCsBlock) void k_cs_hxv(const CsArgs a) {
   done = done || k >= ch.nsteps;
   if (it.y == 0 && threadIdx.x == 0) a.dflag[2 * it.x + (k & 1)] = done ? 1 : 0;
   double dot = 0.0;
+  if (sg.colmode) {
+    // item = column it.y, staged in LDS after the amplitude tables
+    const int idw = it.y;
+    const int dimup = sg.tab.dimup;
+    const int64_t coff = (int64_t)ch.local * sg.stride;
+    const R scale = done ? (R)0 : (R)(1.0 / nrm);
+    unsigned char *s_colb = cs_smem + cs_col_offset(sg.tab, sizeof(C));
+    const int64_t cbase = coff + (int64_t)idw * dimup;
+    if (!done) {
+      if (sg.real) {
+        const R *vin = static_cast<const R *>((k & 1) ? sg.u1 : sg.u0);
+        R *s_col = reinterpret_cast<R *>(s_colb);
+        for (int f = threadIdx.x; f < dimup; f += kCsBlock) s_col[f] = vin[cbase + f];
+      } else {
+        const C *vin = static_cast<const C *>((k & 1) ? sg.u1 : sg.u0);
+        C *s_col = reinterpret_cast<C *>(s_colb);
+        for (int f = threadIdx.x; f < dimup; f += kCsBlock) s_col[f] = vin[cbase + f];
+      }
+    }
+    __syncthreads();
+    if (!done) {
+      const double e_dw = sg.tab.e_dw[idw];
+      const uint32_t s_dw = sg.tab.map_dw[idw];
+      const int nd = sg.tab.cnt_dw[idw];
+      for (int iup = threadIdx.x; iup < dimup; iup += kCsBlock) {
+        const RowMeta mu = sg.tab.meta_up[iup];
+        const double ed = mu.e + e_dw + rows::cross_e(a.cross, mu.s, s_dw);
+        if (sg.real) {
+          const R *vin = static_cast<const R *>((k & 1) ? sg.u1 : sg.u0) + coff;
+          const R *s_col = reinterpret_cast<const R *>(s_colb);
+          const R x = s_col[iup];
+          R acc = (R)ed * x, unused = 0;
+          rows::hops_col<R, true, R>(sg.tab, vin, s_col, iup, idw, mu.cnt, nd, s_amu, s_amd, acc,
+                                     unused);
+          const R out = acc * scale;
+          static_cast<R *>(sg.hv)[cbase + iup] = out;
+          dot += (double)(x * scale) * (double)out;
+        } else {
+          const C *vin = static_cast<const C *>((k & 1) ? sg.u1 : sg.u0) + coff;
+          const C *s_col = reinterpret_cast<const C *>(s_colb);
+          const C x = s_col[iup];
+          R ar = (R)ed * x.x, ai = (R)ed * x.y;
+          rows::hops_col<R, REAL_AMP, C>(sg.tab, vin, s_col, iup, idw, mu.cnt, nd, s_amu, s_amd,
+                                         ar, ai);
+          C out;
+          out.x = ar * scale;
+          out.y = ai * scale;
+          static_cast<C *>(sg.hv)[cbase + iup] = out;
+          dot += (double)(x.x * scale) * (double)out.x + (double)(x.y * scale) * (double)out.y;
+        }
+      }
+    }
+    const double bs = cs_block_sum_all(dot);
+    if (threadIdx.x == 0) a.pa[ch.pofs_h + it.y] = bs;
+    return;
+  }
   const int64_t r = (int64_t)it.y * kCsBlock + threadIdx.x;
   if (!done && r < sg.n) {
     const R scale = (R)(1.0 / nrm);
-    const int dimup = sg.dimup;
+    const int dimup = sg.tab.dimup;
     const int idw = (int)(r / dimup);
     const int iup = (int)(r - (int64_t)idw * dimup);
     const int64_t coff = (int64_t)ch.local * sg.stride;
-    const RowMeta mu = sg.meta_up[iup];
-    const double ed = mu.e + sg.e_dw[idw] + cs_cross(a.cross, mu.s, sg.map_dw[idw]);
-    const int nd = sg.cnt_dw[idw];
+    RowMeta mu;
+    const double ed = rows::diag(sg.tab, a.cross, iup, idw, mu);
+    const int nd = sg.tab.cnt_dw[idw];
     if (sg.real) {
       const R *vin = static_cast<const R *>((k & 1) ? sg.u1 : sg.u0) + coff;
       const R *col = vin + (int64_t)idw * dimup;
       const R x = col[iup];
       R acc = (R)ed * x, unused = 0;
-      cs_row<R, true, R>(sg, vin, col, iup, idw, mu.cnt, nd, s_amu, s_amd, acc, unused);
+      rows::hops<R, true, R>(sg.tab, vin, col, iup, idw, mu.cnt, nd, s_amu, s_amd, acc,
+                               unused);
       const R out = acc * scale;
       static_cast<R *>(sg.hv)[coff + r] = out;
       dot = (double)(x * scale) * (double)out;
@@ -259,7 +236,7 @@ __global__ __launch_bounds__(kCsBlock) void k_cs_hxv(const CsArgs a) {
       const C *col = vin + (int64_t)idw * dimup;
       const C x = col[iup];
       R ar = (R)ed * x.x, ai = (R)ed * x.y;
-      cs_row<R, REAL_AMP, C>(sg, vin, col, iup, idw, mu.cnt, nd, s_amu, s_amd, ar, ai);
+      rows::hops<R, REAL_AMP, C>(sg.tab, vin, col, iup, idw, mu.cnt, nd, s_amu, s_amd, ar, ai);
       C out;
       out.x = ar * scale;
       out.y = ai * scale;
@@ -408,19 +385,11 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
       const Sector &js = *jobs[c].js;
       if (js.precision != prec) fail(VCA_ERR_ARG, "chain set: sector precision differs");
       CsSeg s{};
-      s.meta_up = js.up.meta;
-      s.e_dw = js.dw.energy;
-      s.map_dw = js.dw.map;
-      s.cnt_dw = js.dw.cnt;
-      s.ent_up = js.up.ent;
-      s.ent_dw = js.dw.ent;
-      s.amp_up = prec == VCA_FP64 ? (const void *)js.up.amp64 : (const void *)js.up.amp32;
-      s.amp_dw = prec == VCA_FP64 ? (const void *)js.dw.amp64 : (const void *)js.dw.amp32;
-      s.namp_up = 2 * std::max(1, js.up.nterms);
-      s.namp_dw = 2 * std::max(1, js.dw.nterms);
-      s.dimup = js.up.dim;
-      s.dimdw = js.dw.dim;
+      s.tab = row_tab(js);
       s.real = jobs[c].real ? 1 : 0;
+      s.colmode = (js.up.dim >= kCsColMin && js.up.dim <= kCsColMax) ? 1 : 0;
+      const char *cm = std::getenv("VCA_CHAIN_SET_COLUMNS");
+      if (cm && cm[0] == '0') s.colmode = 0;
       s.n = js.dim;
       // real chains: an even stride, so the excitation's pad real of an
       // odd dimension stays inside the chain's slot
@@ -450,7 +419,7 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
       ch.seg = sgi;
       ch.local = (int)l;
       ch.nsteps = jobs[c].nsteps;
-      ch.nblk_h = (int)((s.n + kCsBlock - 1) / kCsBlock);
+      ch.nblk_h = s.colmode ? s.tab.dimdw : (int)((s.n + kCsBlock - 1) / kCsBlock);
       ch.nblk_u = (int)((s.n + kCsBlock * kCsEpt - 1) / (kCsBlock * kCsEpt));
       ch.pofs_h = ph;
       ch.pofs_u = pu;
@@ -527,9 +496,14 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
   a.kbase = d_kbase;
   a.cross = cross;
   const int nih = (int)(items_h.size()), niu = (int)(items_u.size());
-  int namp = 1;
-  for (const CsSeg &sgm : segs) namp = std::max(namp, sgm.namp_up + sgm.namp_dw);
-  const size_t smem = (size_t)namp * 2 * rsz;
+  size_t smem = 16;
+  for (const CsSeg &sgm : segs) {
+    const size_t amps = (size_t)(sgm.tab.namp_up + sgm.tab.namp_dw) * 2 * rsz;
+    const size_t col = sgm.colmode ? cs_col_offset(sgm.tab, 2 * rsz) +
+                                         (size_t)sgm.tab.dimup * (sgm.real ? rsz : 2 * rsz)
+                                   : 0;
+    smem = std::max(smem, std::max(amps, col));
+  }
   auto launch_step = [&](int i) {
     CsArgs ah = a;
     ah.items = d_items_h;

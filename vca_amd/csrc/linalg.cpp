@@ -119,61 +119,84 @@ double tridiag_lowest(int m, const double *d, const double *e) {
   return w[0];
 }
 
-void herm_eig(int n, zc *a, double *w) {
-  // a: column-major, a[i + j*n] = A(i, j).  Q accumulates the reflectors.
-  auto A = [&](int i, int j) -> zc & { return a[i + (size_t)j * n]; };
-  std::vector<zc> q((size_t)n * n, 0.0);
-  for (int i = 0; i < n; ++i) q[i + (size_t)i * n] = 1.0;
-  std::vector<zc> v(n), wv(n), u(n);
+// Householder reduction of the Hermitian (T = complex) or real symmetric
+// (T = double) matrix a to real tridiagonal form, QL on it, and the
+// eigenvectors Q D Z.  The loops run down columns (column-major storage),
+// so the O(n^3) parts stream contiguous memory.
+static inline double cnj(double x) { return x; }
+static inline zc cnj(const zc &x) { return std::conj(x); }
+static inline double sqnorm(double x) { return x * x; }
+static inline double sqnorm(const zc &x) { return std::norm(x); }
+static inline double re(double x) { return x; }
+static inline double re(const zc &x) { return x.real(); }
+
+template <typename T>
+static void herm_eig_t(int n, T *a, double *w, zc *vec_out) {
+  auto A = [&](int i, int j) -> T & { return a[i + (size_t)j * n]; };
+  std::vector<T> q((size_t)n * n, T(0));
+  for (int i = 0; i < n; ++i) q[i + (size_t)i * n] = T(1);
+  std::vector<T> v(n), wv(n), u(n), sr(n);
   for (int k = 0; k + 2 < n; ++k) {
     const int m = n - k - 1;  // length of the column below the diagonal
+    T *colk = &A(k + 1, k);
     double xnorm = 0;
-    for (int i = 0; i < m; ++i) xnorm += std::norm(A(k + 1 + i, k));
+    for (int i = 0; i < m; ++i) xnorm += sqnorm(colk[i]);
     xnorm = std::sqrt(xnorm);
     if (xnorm == 0.0) continue;
-    zc x0 = A(k + 1, k);
-    zc phase = std::abs(x0) > 0 ? x0 / std::abs(x0) : zc(1.0);
-    zc alpha = -phase * xnorm;
-    for (int i = 0; i < m; ++i) v[i] = A(k + 1 + i, k);
+    const T x0 = colk[0];
+    const double ax0 = std::abs(x0);
+    const T phase = ax0 > 0 ? x0 / ax0 : T(1);
+    const T alpha = -phase * xnorm;
+    for (int i = 0; i < m; ++i) v[i] = colk[i];
     v[0] -= alpha;
     double vn = 0;
-    for (int i = 0; i < m; ++i) vn += std::norm(v[i]);
+    for (int i = 0; i < m; ++i) vn += sqnorm(v[i]);
     vn = std::sqrt(vn);
     if (vn == 0.0) continue;
     for (int i = 0; i < m; ++i) v[i] /= vn;
-    // w = A_sub v, c = v^H w (real), u = 2 (w - c v)
-    zc c = 0;
-    for (int i = 0; i < m; ++i) {
-      zc s = 0;
-      for (int j = 0; j < m; ++j) s += A(k + 1 + i, k + 1 + j) * v[j];
-      wv[i] = s;
+    // w = A_sub v (by columns), c = v^H w (real), u = 2 (w - c v)
+    std::fill(wv.begin(), wv.begin() + m, T(0));
+    for (int j = 0; j < m; ++j) {
+      const T vj = v[j];
+      const T *cj = &A(k + 1, k + 1 + j);
+      for (int i = 0; i < m; ++i) wv[i] += cj[i] * vj;
     }
-    for (int i = 0; i < m; ++i) c += std::conj(v[i]) * wv[i];
-    for (int i = 0; i < m; ++i) u[i] = 2.0 * (wv[i] - c.real() * v[i]);
-    for (int j = 0; j < m; ++j)
-      for (int i = 0; i < m; ++i)
-        A(k + 1 + i, k + 1 + j) -= v[i] * std::conj(u[j]) + u[i] * std::conj(v[j]);
-    A(k + 1, k) = alpha;
-    A(k, k + 1) = std::conj(alpha);
+    T c = 0;
+    for (int i = 0; i < m; ++i) c += cnj(v[i]) * wv[i];
+    const double cr = re(c);
+    for (int i = 0; i < m; ++i) u[i] = 2.0 * (wv[i] - cr * v[i]);
+    for (int j = 0; j < m; ++j) {
+      T *cj = &A(k + 1, k + 1 + j);
+      const T uj = cnj(u[j]), vj = cnj(v[j]);
+      for (int i = 0; i < m; ++i) cj[i] -= v[i] * uj + u[i] * vj;
+    }
+    colk[0] = alpha;
+    A(k, k + 1) = cnj(alpha);
     for (int i = 1; i < m; ++i) {
-      A(k + 1 + i, k) = 0;
+      colk[i] = 0;
       A(k, k + 1 + i) = 0;
     }
-    // Q <- Q P on columns k+1..n-1
-    for (int r = 0; r < n; ++r) {
-      zc s = 0;
-      for (int i = 0; i < m; ++i) s += q[r + (size_t)(k + 1 + i) * n] * v[i];
-      for (int i = 0; i < m; ++i) q[r + (size_t)(k + 1 + i) * n] -= 2.0 * s * std::conj(v[i]);
+    // Q <- Q P on columns k+1..n-1: s = Q_sub v, Q_sub -= 2 s v^H
+    std::fill(sr.begin(), sr.end(), T(0));
+    for (int i = 0; i < m; ++i) {
+      const T vi = v[i];
+      const T *qc = &q[(size_t)(k + 1 + i) * n];
+      for (int r = 0; r < n; ++r) sr[r] += qc[r] * vi;
+    }
+    for (int i = 0; i < m; ++i) {
+      const T ci = 2.0 * cnj(v[i]);
+      T *qc = &q[(size_t)(k + 1 + i) * n];
+      for (int r = 0; r < n; ++r) qc[r] -= sr[r] * ci;
     }
   }
   // Real tridiagonal by a diagonal phase transform D.
   std::vector<double> d(n), f(n, 0.0);
-  std::vector<zc> ph(n);
-  ph[0] = 1.0;
-  for (int i = 0; i < n; ++i) d[i] = A(i, i).real();
+  std::vector<T> ph(n);
+  ph[0] = T(1);
+  for (int i = 0; i < n; ++i) d[i] = re(A(i, i));
   for (int i = 0; i + 1 < n; ++i) {
-    zc t = A(i + 1, i);
-    double at = std::abs(t);
+    const T t = A(i + 1, i);
+    const double at = std::abs(t);
     f[i] = at;
     ph[i + 1] = at > 0 ? ph[i] * t / at : ph[i];
   }
@@ -183,18 +206,33 @@ void herm_eig(int n, zc *a, double *w) {
   std::vector<int> idx(n);
   std::iota(idx.begin(), idx.end(), 0);
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return d[x] < d[y]; });
-  // eigenvectors: Q D Z
-  std::vector<zc> out((size_t)n * n);
+  // eigenvectors: Q D Z, column by column
+  std::vector<T> col(n);
   for (int jj = 0; jj < n; ++jj) {
-    const double *zc_ = z.data() + (size_t)idx[jj] * n;
-    for (int r = 0; r < n; ++r) {
-      zc s = 0;
-      for (int i = 0; i < n; ++i) s += q[r + (size_t)i * n] * ph[i] * zc_[i];
-      out[r + (size_t)jj * n] = s;
+    const double *zj = z.data() + (size_t)idx[jj] * n;
+    std::fill(col.begin(), col.end(), T(0));
+    for (int i = 0; i < n; ++i) {
+      const T ci = ph[i] * zj[i];
+      const T *qc = &q[(size_t)i * n];
+      for (int r = 0; r < n; ++r) col[r] += qc[r] * ci;
     }
+    for (int r = 0; r < n; ++r) vec_out[r + (size_t)jj * n] = zc(col[r]);
     w[jj] = d[idx[jj]];
   }
-  std::copy(out.begin(), out.end(), a);
+}
+
+void herm_eig(int n, zc *a, double *w) {
+  // a real symmetric matrix (real Hamiltonians) runs in real arithmetic
+  bool real = true;
+  for (size_t i = 0; i < (size_t)n * n && real; ++i) real = a[i].imag() == 0.0;
+  if (real) {
+    std::vector<double> ar((size_t)n * n);
+    for (size_t i = 0; i < ar.size(); ++i) ar[i] = a[i].real();
+    herm_eig_t<double>(n, ar.data(), w, a);
+  } else {
+    std::vector<zc> ac(a, a + (size_t)n * n);
+    herm_eig_t<zc>(n, ac.data(), w, a);
+  }
 }
 
 }  // namespace vca

@@ -48,6 +48,8 @@ void set_error(const char *fmt, ...) {
   g_err = buf;
 }
 
+std::string last_error_message() { return g_err; }
+
 void fail(int code, const char *fmt, ...) {
   char buf[1024];
   va_list ap;
@@ -93,6 +95,8 @@ struct Solver {
   bool md_valid = false;
   uint32_t *d_binom = nullptr;
   hipStream_t stream = nullptr;
+  hipStream_t batch_stream = nullptr;  // a sweep's batched eigensolves
+  hipStream_t dense_stream = nullptr;  // a sweep's dense-sector copies
   vca_results res{};
   bool solved = false;
   std::map<ElemKey, std::vector<std::vector<GfChan>>> gf;
@@ -126,6 +130,11 @@ Solver::~Solver() {
   free_states(*this);
   om.reset();
   if (d_binom) (void)hipFree(d_binom);
+  for (hipStream_t st : {batch_stream, dense_stream})
+    if (st) {
+      (void)hipStreamSynchronize(st);
+      (void)hipStreamDestroy(st);
+    }
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -147,7 +156,7 @@ Sector::~Sector() {
   for (void *q : {buf[0], buf[1], buf[2], xvec, tbuf, (void *)partials, (void *)lz,
                   (void *)lz_arrays, (void *)d_ops})
     if (q) (void)hipFree(q);
-  if (stream) (void)hipStreamDestroy(stream);
+  if (stream && own_stream) (void)hipStreamDestroy(stream);
 }
 
 CrossTerm sector_cross(const Sector &sec) {
@@ -468,7 +477,10 @@ static void hxv_tune_batch(Sector &sec, int nb) {
 }
 
 // ------------------------------------------------------------ sectors ---
-static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec) {
+// stream: a stream of the solver's for the sector's work (not destroyed
+// with the sector), or null for a stream of its own.
+static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec,
+                                           hipStream_t stream = nullptr) {
   if (!s.md_valid) fail(VCA_ERR_STATE, "Hloc not set");
   if (nup < 0 || ndw < 0 || nup > s.ns || ndw > s.ns)
     fail(VCA_ERR_ARG, "sector (%d,%d) out of range for Ns=%d", nup, ndw, s.ns);
@@ -477,7 +489,12 @@ static std::unique_ptr<Sector> make_sector(Solver &s, int nup, int ndw, int prec
   sec->nup = nup;
   sec->ndw = ndw;
   sec->precision = prec;
-  VCA_HIP(hipStreamCreate(&sec->stream));
+  if (stream) {
+    sec->stream = stream;
+    sec->own_stream = false;
+  } else {
+    VCA_HIP(hipStreamCreate(&sec->stream));
+  }
   build_spin_tables(*sec, sec->up, s.md.spin[0], s.ns, nup, s.d_binom, s.md.e_const);
   build_spin_tables(*sec, sec->dw, s.md.spin[1], s.ns, ndw, s.d_binom, 0.0);
   sec->dim = (int64_t)sec->up.dim * sec->dw.dim;
@@ -814,10 +831,11 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     std::vector<zc> h = dense_h(s, nup, ndw);
     std::vector<double> w(dim);
     herm_eig((int)dim, h.data(), w.data());
-    // copies on a stream of their own: a copy on the null stream would wait
-    // for the sector solves running concurrently on other streams
-    hipStream_t st;
-    VCA_HIP(hipStreamCreate(&st));
+    // copies on a stream other than the null stream, which would wait for
+    // the sector solves running concurrently: the sweep's shared stream, or
+    // one of their own
+    hipStream_t st = s.dense_stream;
+    if (!st) VCA_HIP(hipStreamCreate(&st));
     for (int i = 0; i < neigen; ++i) {
       evals.push_back(w[i]);
       void *d = dev_alloc_async(sizeof(double2) * dim, st);
@@ -826,7 +844,7 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
       evecs.push_back(d);
     }
     VCA_HIP(hipStreamSynchronize(st));
-    VCA_HIP(hipStreamDestroy(st));
+    if (st != s.dense_stream) VCA_HIP(hipStreamDestroy(st));
     return;
   }
   if (p.lanc_method != VCA_LANC_LANCZOS) {
@@ -882,11 +900,61 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
 // sector at the head of the list always admitted), so large sectors do not
 // run out of memory together.  VCA_SECTOR_THREADS sets the worker count
 // (default 16; 1: the sectors one after another on the calling thread).
+// Runs fn(0..n-1) on up to 16 host threads (static slices); fn must only
+// write per-index state.
+template <typename Fn>
+static void parallel_for(int n, Fn &&fn) {
+  const int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
+  const int nth = std::min(nt, n);
+  if (nth <= 1) {
+    for (int i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int t = 0; t < nth; ++t)
+    th.emplace_back([&, t]() {
+      for (int i = t; i < n; i += nth) fn(i);
+    });
+  for (auto &x : th) x.join();
+}
+
+// VCA_PROGRESS=1: one line on stderr per finished stage and chain batch,
+// with the seconds since the solve began (long sectors run for minutes).
+static bool progress_on() {
+  static const bool on = [] {
+    const char *e = std::getenv("VCA_PROGRESS");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
+static void progress(double t0, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static void progress(double t0, const char *fmt, ...) {
+  if (!progress_on()) return;
+  const double t =
+      std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+  std::fprintf(stderr, "vca [%8.2f s] ", t - t0);
+  va_list ap;
+  va_start(ap, fmt);
+  std::vfprintf(stderr, fmt, ap);
+  va_end(ap);
+  std::fputc('\n', stderr);
+  std::fflush(stderr);
+}
+
+// Thick-restart sectors below this size are solved together, their Lanczos
+// steps in shared launches (eigs_lanczos_batch); VCA_EIGS_BATCH_MAX_DIM
+// overrides (0: every sector on its own).
+static int64_t eigs_batch_max_dim() {
+  const char *e = std::getenv("VCA_EIGS_BATCH_MAX_DIM");
+  return e && *e ? std::atoll(e) : ((int64_t)1 << 17);
+}
+
 class SectorSweep {
  public:
   SectorSweep(Solver &s, const std::vector<std::pair<int, int>> &list,
               const std::vector<int> &neig)
-      : s_(s), list_(list), neig_(neig), slots_(list.size()) {
+      : s_(s), list_(list), neig_(neig), slots_(list.size()), in_batch_(list.size(), false) {
     const char *env = std::getenv("VCA_SECTOR_THREADS");
     int nt = env && *env ? std::atoi(env) : 16;
     nt = std::max(1, std::min<int>(nt, (int)list.size()));
@@ -895,6 +963,21 @@ class SectorSweep {
     size_t free_b = 0, total_b = 0;
     VCA_HIP(hipMemGetInfo(&free_b, &total_b));
     budget_ = (int64_t)(free_b / 2);
+    // the small thick-restart sectors: one batched solve on a thread of its own
+    if (s.p.lanc_method != VCA_LANC_LANCZOS) {
+      const int64_t cap = eigs_batch_max_dim();
+      Binom bn;
+      for (size_t k = 0; k < list.size(); ++k) {
+        const int64_t dim = (int64_t)bn(s.ns, list[k].first) * bn(s.ns, list[k].second);
+        const bool lanc_solve = !(neig[k] == dim || dim <= s.p.lanc_dim_threshold);
+        if (lanc_solve && dim < cap) batch_.push_back(k);
+      }
+      if (batch_.size() < 2) batch_.clear();
+      for (size_t k : batch_) in_batch_[k] = true;
+    }
+    if (!s.batch_stream) VCA_HIP(hipStreamCreate(&s.batch_stream));
+    if (!s.dense_stream) VCA_HIP(hipStreamCreate(&s.dense_stream));
+    if (!batch_.empty()) batch_thread_ = std::thread([this]() { run_batch(); });
     for (int t = 0; t < nt; ++t) workers_.emplace_back([this]() { work(); });
   }
   ~SectorSweep() {
@@ -903,6 +986,7 @@ class SectorSweep {
       abort_ = true;
     }
     cv_.notify_all();
+    if (batch_thread_.joinable()) batch_thread_.join();
     for (auto &t : workers_) t.join();
     for (auto &sl : slots_)
       for (void *d : sl.evecs)
@@ -939,6 +1023,12 @@ class SectorSweep {
     bool done = false;
     int64_t bytes = 0;
   };
+  // solved by the dense eigensolver (VCA_DIAG.f90:194)
+  bool dense(size_t k) const {
+    Binom bn;
+    const int64_t dim = (int64_t)bn(s_.ns, list_[k].first) * bn(s_.ns, list_[k].second);
+    return neig_[k] == dim || dim <= s_.p.lanc_dim_threshold;
+  }
   // device bytes a sector solve holds: the thick-restart basis, rotation
   // target and eigenvectors, or the Lanczos buffers
   int64_t need(size_t k) const {
@@ -949,6 +1039,110 @@ class SectorSweep {
     const int64_t vecs = s_.p.lanc_method == VCA_LANC_LANCZOS ? 8 : 2 * m + 1 + neig_[k];
     return vecs * dim * (int64_t)sizeof(double2);
   }
+  // The batched sectors, in chunks within half the sweep's budget: their
+  // memory is counted as the workers' is, but taken without waiting (the
+  // caller releases memory in list order, and may be waiting for a batched
+  // sector itself).
+  void run_batch() {
+    t_no_graphs = true;
+    size_t b0 = 0;
+    while (b0 < batch_.size()) {
+      int64_t bytes = 0;
+      size_t b1 = b0;
+      while (b1 < batch_.size() && (b1 == b0 || bytes + need(batch_[b1]) <= budget_ / 2))
+        bytes += need(batch_[b1++]);
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        if (abort_) return;
+        used_ += bytes;
+        for (size_t q = b0; q < b1; ++q) slots_[batch_[q]].bytes = need(batch_[q]);
+      }
+      std::vector<std::unique_ptr<Sector>> secs;
+      std::vector<EigsJob> jobs(b1 - b0);
+      int code = VCA_OK;
+      std::string msg;
+      try {
+        // the sectors' tables, built on host threads (each on its own stream)
+        secs.resize(b1 - b0);
+        std::vector<std::exception_ptr> errs(b1 - b0);
+        std::vector<std::string> emsg(b1 - b0);
+        parallel_for((int)(b1 - b0), [&](int q) {
+          const size_t k = batch_[b0 + q];
+          try {
+            secs[q] = make_sector(s_, list_[k].first, list_[k].second, VCA_FP64,
+                                  s_.batch_stream);
+          } catch (...) {
+            errs[q] = std::current_exception();
+            emsg[q] = g_err;
+          }
+        });
+        for (size_t q = 0; q < errs.size(); ++q)
+          if (errs[q]) {
+            g_err = emsg[q];
+            std::rethrow_exception(errs[q]);
+          }
+        for (size_t q = b0; q < b1; ++q) {
+          const size_t k = batch_[q];
+          const Sector &sec = *secs[q - b0];
+          EigsJob &jb = jobs[q - b0];
+          jb.sec = secs[q - b0].get();
+          jb.nev = neig_[k];
+          jb.ncv = (int)std::min<int64_t>(
+              sec.dim, (int64_t)s_.p.lanc_ncv_factor *
+                               std::max(neig_[k], s_.p.lanc_nstates_sector) +
+                           s_.p.lanc_ncv_add);
+          jb.maxiter = (int)std::min<int64_t>(sec.dim, s_.p.lanc_niter);
+          jb.tol = s_.p.lanc_tolerance;
+        }
+        progress(s_.t_start, "eigs batch: %zu sectors built", b1 - b0);
+        eigs_lanczos_batch(jobs, sector_cross(*secs[0]),
+                           s_.md.spin[0].real_amp && s_.md.spin[1].real_amp, s_.batch_stream);
+        progress(s_.t_start, "eigs batch: %zu sectors solved", b1 - b0);
+      } catch (...) {
+        code = VCA_ERR_HIP;
+        try {
+          throw;
+        } catch (const Error &e) {
+          code = e.code;
+        } catch (...) {
+        }
+        msg = g_err;
+      }
+      for (size_t q = b0; q < b1; ++q) {
+        const size_t k = batch_[q];
+        EigsJob &jb = jobs[q - b0];
+        Slot &sl = slots_[k];
+        int c = code == VCA_OK ? jb.code : code;
+        std::string m = code == VCA_OK ? jb.msg : msg;
+        if (c == VCA_OK && jb.out.nconv < jb.nev) {
+          c = VCA_ERR_NOCONV;
+          char buf[256];
+          std::snprintf(buf, sizeof buf, "sector (%d,%d): %d of %d eigenpairs converged in %d "
+                        "restarts", list_[k].first, list_[k].second, jb.out.nconv, jb.nev,
+                        jb.maxiter);
+          m = buf;
+        }
+        if (c != VCA_OK) {
+          for (void *d : jb.evecs) dev_free_async(d, s_.batch_stream);
+          jb.evecs.clear();
+          sl.err = std::make_exception_ptr(Error{c});
+          sl.msg = m;
+        } else {
+          sl.evals = std::move(jb.evals);
+          sl.evecs = std::move(jb.evecs);
+        }
+      }
+      if (s_.batch_stream) (void)hipStreamSynchronize(s_.batch_stream);
+      secs.clear();
+      {
+        std::lock_guard<std::mutex> lk(mu_);
+        for (size_t q = b0; q < b1; ++q) slots_[batch_[q]].done = true;
+      }
+      cv_.notify_all();
+      b0 = b1;
+    }
+  }
+
   void work() {
     t_no_graphs = true;
     for (;;) {
@@ -956,7 +1150,10 @@ class SectorSweep {
       {
         std::unique_lock<std::mutex> lk(mu_);
         cv_.wait(lk, [&]() {
-          return abort_ || next_ >= list_.size() || next_ < taken_ + (size_t)nthreads_;
+          while (next_ < list_.size() && in_batch_[next_]) ++next_;  // the batch's
+          // dense sectors (a few hundred states) are not held to the window
+          return abort_ || next_ >= list_.size() || next_ < taken_ + (size_t)nthreads_ ||
+                 dense(next_);
         });
         if (abort_ || next_ >= list_.size()) return;
         k = next_++;
@@ -985,6 +1182,9 @@ class SectorSweep {
   const std::vector<std::pair<int, int>> &list_;
   const std::vector<int> &neig_;
   std::vector<Slot> slots_;
+  std::vector<bool> in_batch_;  // solved by the batch thread
+  std::vector<size_t> batch_;
+  std::thread batch_thread_;
   std::vector<std::thread> workers_;
   std::mutex mu_;
   std::condition_variable cv_;
@@ -1069,6 +1269,7 @@ static void diagonalize(Solver &s) {
                     ? 1
                     : (int)std::min<int64_t>(dim, neigen_sector[{nup, ndw}]);
     }
+    progress(s.t_start, "sweep: %zu sectors", list.size());
     SectorSweep sweep(s, list, neig);
     for (size_t k = 0; k < list.size(); ++k) {
       const auto [nup, ndw] = list[k];
@@ -1098,6 +1299,7 @@ static void diagonalize(Solver &s) {
       }
       for (void *d : evecs) dev_free_async(d, s.stream);
     }
+    progress(s.t_start, "sweep: every sector taken");
     if (s.states.empty()) fail(VCA_ERR_NOCONV, "no ground state found");
     const double egs = s.states.front().e, emax = s.states.back().e;
     converged = ft ? std::exp(-p.beta * (emax - egs)) < p.cutoff : true;
@@ -1203,48 +1405,6 @@ static std::vector<ChainSpec> chain_list(const Solver &s, const KeptState &st) {
           }
       }
   return out;
-}
-
-// Runs fn(0..n-1) on up to 16 host threads (static slices); fn must only
-// write per-index state.
-template <typename Fn>
-static void parallel_for(int n, Fn &&fn) {
-  const int nt = (int)std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency()));
-  const int nth = std::min(nt, n);
-  if (nth <= 1) {
-    for (int i = 0; i < n; ++i) fn(i);
-    return;
-  }
-  std::vector<std::thread> th;
-  for (int t = 0; t < nth; ++t)
-    th.emplace_back([&, t]() {
-      for (int i = t; i < n; i += nth) fn(i);
-    });
-  for (auto &x : th) x.join();
-}
-
-// VCA_PROGRESS=1: one line on stderr per finished stage and chain batch,
-// with the seconds since the solve began (long sectors run for minutes).
-static bool progress_on() {
-  static const bool on = [] {
-    const char *e = std::getenv("VCA_PROGRESS");
-    return e && e[0] == '1';
-  }();
-  return on;
-}
-
-static void progress(double t0, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
-static void progress(double t0, const char *fmt, ...) {
-  if (!progress_on()) return;
-  const double t =
-      std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-  std::fprintf(stderr, "vca [%8.2f s] ", t - t0);
-  va_list ap;
-  va_start(ap, fmt);
-  std::vfprintf(stderr, fmt, ap);
-  va_end(ap);
-  std::fputc('\n', stderr);
-  std::fflush(stderr);
 }
 
 // VCA_CHAIN_BATCH=n: at most n chains per launch (1: one chain at a time,

@@ -236,6 +236,7 @@ struct Sector {
   int64_t dim = 0;
   DevSpin up, dw;
   hipStream_t stream = nullptr;
+  bool own_stream = true;  // false: a stream of the solver's, not destroyed here
   // Lanczos workspace (lazily allocated) for nb_cap chains of lz_cap steps
   int lz_cap = 0, nb_cap = 0;
   int hxv_variant = -1;  // launch shape of H x v picked by the autotuner
@@ -378,6 +379,24 @@ struct EigsOut {
 };
 EigsOut eigs_lanczos(Sector &sec, int nev, int ncv, double tol, int maxiter, const void *start,
                      std::vector<double> &evals, std::vector<void *> &evecs);
+// Many eigensolves at once (the small sectors of a sweep): the same
+// thick-restart solve per job, with the Lanczos steps of all jobs in shared
+// launches on stream st.  A job's failure is reported in its code and msg.
+struct EigsJob {
+  Sector *sec = nullptr;  // FP64 sector
+  int nev = 0, ncv = 0, maxiter = 0;
+  double tol = 0;
+  const void *start = nullptr;  // device start vector or null (hash vector)
+  std::vector<double> evals;
+  std::vector<void *> evecs;    // device allocations, caller frees
+  EigsOut out{};
+  int code = VCA_OK;
+  std::string msg;
+};
+void eigs_lanczos_batch(std::vector<EigsJob> &jobs, const CrossTerm &cross, bool real_amp,
+                        hipStream_t st);
+// The calling thread's last error message (set by fail / set_error).
+std::string last_error_message();
 
 // Stream-ordered device memory (hipMallocAsync / hipFreeAsync) for the
 // eigenvectors and the eigensolver workspaces: a plain hipFree waits for the
