@@ -97,6 +97,11 @@ struct Solver {
   hipStream_t stream = nullptr;
   hipStream_t batch_stream = nullptr;  // a sweep's batched eigensolves
   hipStream_t dense_stream = nullptr;  // a sweep's dense-sector copies
+  // one stream per sweep worker thread, kept for the solver's lifetime: the
+  // sectors a worker solves run on it (no stream is created and destroyed
+  // per sector, and no stream-ordered allocation or free is left on a
+  // destroyed stream)
+  std::vector<hipStream_t> worker_streams;
   vca_results res{};
   bool solved = false;
   std::map<ElemKey, std::vector<std::vector<GfChan>>> gf;
@@ -135,6 +140,10 @@ Solver::~Solver() {
       (void)hipStreamSynchronize(st);
       (void)hipStreamDestroy(st);
     }
+  for (hipStream_t st : worker_streams) {
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+  }
   if (stream) (void)hipStreamDestroy(stream);
 }
 
@@ -819,6 +828,9 @@ static bool finite_t(const vca_params &p) { return p.lanc_nstates_total > 1; }
 // dim <= lanc_dim_threshold or Neigen == dim; else the GPU plain Lanczos
 // (LANC_METHOD=lanczos, one state, FP64 or MIXED) or the GPU thick-restart
 // Lanczos (ARPACK, FP64).  Eigenvectors are device complex128 vectors.
+// The stream of the sweep worker running on this thread (null elsewhere).
+static thread_local hipStream_t t_worker_stream = nullptr;
+
 static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<double> &evals,
                          std::vector<void *> &evecs) {
   const vca_params &p = s.p;
@@ -834,7 +846,7 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
     // copies on a stream other than the null stream, which would wait for
     // the sector solves running concurrently: the sweep's shared stream, or
     // one of their own
-    hipStream_t st = s.dense_stream;
+    hipStream_t st = t_worker_stream ? t_worker_stream : s.dense_stream;
     if (!st) VCA_HIP(hipStreamCreate(&st));
     for (int i = 0; i < neigen; ++i) {
       evals.push_back(w[i]);
@@ -844,11 +856,11 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
       evecs.push_back(d);
     }
     VCA_HIP(hipStreamSynchronize(st));
-    if (st != s.dense_stream) VCA_HIP(hipStreamDestroy(st));
+    if (st != s.dense_stream && st != t_worker_stream) VCA_HIP(hipStreamDestroy(st));
     return;
   }
   if (p.lanc_method != VCA_LANC_LANCZOS) {
-    auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+    auto s64 = make_sector(s, nup, ndw, VCA_FP64, t_worker_stream);
     const int ncv = (int)std::min<int64_t>(
         dim, (int64_t)p.lanc_ncv_factor * std::max(neigen, p.lanc_nstates_sector) + p.lanc_ncv_add);
     const int maxiter = (int)std::min<int64_t>(dim, p.lanc_niter);
@@ -866,8 +878,8 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
   void *d = dev_alloc_async(sizeof(double2) * dim, s.stream);
   VCA_HIP(hipStreamSynchronize(s.stream));
   if (p.precision == VCA_MIXED) {
-    auto s32 = make_sector(s, nup, ndw, VCA_MIXED);
-    auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+    auto s32 = make_sector(s, nup, ndw, VCA_MIXED, t_worker_stream);
+    auto s64 = make_sector(s, nup, ndw, VCA_FP64, t_worker_stream);
     double e32;
     g = gs_mixed(s, *s32, *s64, p.lanc_tolerance, p.lanc_niter, &e32);
     {
@@ -878,7 +890,7 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
                            s64->stream));
     VCA_HIP(hipStreamSynchronize(s64->stream));
   } else {
-    auto s64 = make_sector(s, nup, ndw, VCA_FP64);
+    auto s64 = make_sector(s, nup, ndw, VCA_FP64, t_worker_stream);
     g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s, 0,
                    use_real_gs(*s64));
     VCA_HIP(hipMemcpyAsync(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice,
@@ -977,8 +989,13 @@ class SectorSweep {
     }
     if (!s.batch_stream) VCA_HIP(hipStreamCreate(&s.batch_stream));
     if (!s.dense_stream) VCA_HIP(hipStreamCreate(&s.dense_stream));
+    while ((int)s.worker_streams.size() < nt) {
+      hipStream_t st;
+      VCA_HIP(hipStreamCreate(&st));
+      s.worker_streams.push_back(st);
+    }
     if (!batch_.empty()) batch_thread_ = std::thread([this]() { run_batch(); });
-    for (int t = 0; t < nt; ++t) workers_.emplace_back([this]() { work(); });
+    for (int t = 0; t < nt; ++t) workers_.emplace_back([this, t]() { work(t); });
   }
   ~SectorSweep() {
     {
@@ -1143,8 +1160,9 @@ class SectorSweep {
     }
   }
 
-  void work() {
+  void work(int t) {
     t_no_graphs = true;
+    t_worker_stream = s_.worker_streams[t];
     for (;;) {
       size_t k;
       {

@@ -2,8 +2,12 @@
 
     python -m vca_amd.rocprof_summary trace  TRACE.db
     python -m vca_amd.rocprof_summary pmc    FETCH.db WRITE.db KERNELS COPY_BYTES OUT.json KEY VARIANT
+    python -m vca_amd.rocprof_summary counters PMC.db [KERNEL_SUBSTRING]
 
 `trace` prints the per-kernel table (calls, average and total duration).
+`counters` prints, per kernel (those whose name holds KERNEL_SUBSTRING),
+the dispatch count and the average per dispatch of every counter of a PMC
+pass.
 `pmc` reads a FETCH_SIZE and a WRITE_SIZE pass (KiB per dispatch), finds
 the calibration copy kernel of COPY_BYTES bytes each way (the largest
 __amd_rocclr_copyBuffer / elementwise copy dispatch) to turn the counters
@@ -85,9 +89,23 @@ def pmc_traffic(fetch_db: str, write_db: str, kernels: str, copy_bytes: int) -> 
     }
 
 
+def counter_table(db: str, sub: str = "") -> str:
+    c = sqlite3.connect(db)
+    rows = c.execute("select kernel_name, counter_name, count(*), avg(value) "
+                     "from counters_collection group by kernel_name, counter_name "
+                     "order by kernel_name, counter_name").fetchall()
+    out = ["| kernel | counter | dispatches | average per dispatch |", "|---|---|---:|---:|"]
+    for name, cn, n, avg in rows:
+        if sub in name:
+            out.append(f"| `{_short(name)}` | {cn} | {n} | {avg:.6g} |")
+    return "\n".join(out)
+
+
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
         print(trace_table(argv[1]))
+    elif len(argv) >= 2 and argv[0] == "counters":
+        print(counter_table(argv[1], argv[2] if len(argv) > 2 else ""))
     elif len(argv) >= 8 and argv[0] == "pmc":
         d = pmc_traffic(argv[1], argv[2], argv[3], int(argv[4]))
         d["variant"] = int(argv[7])
