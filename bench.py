@@ -29,7 +29,9 @@ on the sector's stream).  Reported beside it in the same JSON line:
     excitation chains of 300 steps on the N+-1 sectors, Omega on the 64x64
     k-grid) FP64 and MIXED, then G, Sigma and G0 on the default grids;
   * batched_2x5: vca_solve of the 10-site 2x5 cluster, every (Nup, Ndw)
-    sector, ARPACK-equivalent eigenpairs per sector, all chains batched;
+    sector, ARPACK-equivalent eigenpairs per sector (the 77 sectors above
+    the dense threshold in one batched eigensolve), all 760 chains as one
+    chain set (two launches per Lanczos step for every chain);
   * chain4: the 4-site chain end to end (every sector, Matsubara-sum Omega
     at beta=100 over 4096 frequencies, G on 512 real-axis points), on the
     GPU and by the oracle on one host core, with the differences;
@@ -248,8 +250,9 @@ def run_c3():
     omc = sc.vca_solve(hlc, hkc)
     r = sc.results()
     c3 = {"workload": "2x5 cluster U=6, Nspin=2, every (Nup,Ndw) sector (121, dims "
-                      "1..63,504), up to 6 eigenpairs per sector (ARPACK-equivalent), "
-                      "all chains batched per target sector",
+                      "1..63,504), up to 6 eigenpairs per sector (ARPACK-equivalent; the "
+                      "small sectors in one batched eigensolve), the 760 chains as one chain "
+                      "set (two launches per Lanczos step for all of them)",
           "seconds": time.perf_counter() - t0, "omega": omc,
           "sector_threads": int(os.environ.get("VCA_SECTOR_THREADS", "16")),
           "lanczos_step_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9,
