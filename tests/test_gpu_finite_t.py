@@ -156,18 +156,26 @@ def test_finite_t_exact_at_u0(beta, tprime):
 
 
 def test_logdet_qr_equals_pivoted_lu(monkeypatch):
-    """The FP64 log-det by Householder QR (8-row blocks) and by pivoted LU
-    give the same complex integrand (log|det| and the principal phase)."""
+    """The FP64 log-det of 8-row blocks three ways -- a matrix per lane
+    (k_logdet_lane64, the default), Householder QR with 4 lanes per matrix,
+    pivoted LU with a row per lane -- gives the same complex integrand
+    (log|det| and the principal phase), on 8 k-points (fewer than the lane
+    kernel's 32 per workgroup) and 7 frequencies (not a multiple of its 8
+    per wavefront), blocks of 4 padded to 8 rows."""
     p, hloc, hk, _, _ = M.bhz_small(nk=8)
-    _say("QR vs LU log-det, BHZ 2 sites")
+    _say("lane vs QR vs LU log-det, BHZ 2 sites")
     s = VcaSolver(p)
     s.vca_solve(hloc, hk)
     zs = np.array([0.05j, 0.7j, 3.0j, 40.0j, 0.4 + 0.3j, -2.0 + 0.02j, 5.0 * np.exp(2.0j)])
     monkeypatch.setenv("VCA_OMEGA_LU", "0")
+    f_lane = s.omega_integrand_complex(zs)
+    monkeypatch.setenv("VCA_OMEGA_LANE", "0")
     f_qr = s.omega_integrand_complex(zs)
     monkeypatch.setenv("VCA_OMEGA_LU", "1")
     f_lu = s.omega_integrand_complex(zs)
     assert np.allclose(f_qr, f_lu, rtol=1e-12, atol=1e-13), (f_qr, f_lu)
+    assert np.allclose(f_lane, f_qr, rtol=1e-12, atol=1e-13), (f_lane, f_qr)
+    assert np.allclose(f_lane.real, s.omega_integrand(zs), rtol=1e-12, atol=1e-13)
 
 
 @pytest.mark.parametrize("case", ["bhz_blocks_of_8", "chain12_block_of_16"])

@@ -777,6 +777,186 @@ __global__ __launch_bounds__(kOBlock) void k_logdet_lane8(int nk, int nz, int zc
   }
 }
 
+// --------------------------------- k x z log-det, FP64, a matrix per lane --
+// The FP64 integrand for 8-row blocks with no cross-lane step at all: as in
+// k_logdet_lane8, a wavefront holds 64 matrices (lane l: k-point l % KPW of
+// the wavefront's KPW, frequency l / KPW of the ZPI in flight), but each
+// lane also forms its own contraction X = P(z) - H_k G'(z) on the vector
+// ALUs: H_k of the block's k-points sits in LDS for the whole chunk of
+// frequencies, G'(z) and P(z) of the ZPI frequencies are staged per round,
+// and every LDS read of a wavefront touches only KPW or ZPI distinct
+// addresses (broadcast to the other lanes).  (On MI355X the FP64 matrix
+// cores run at the FP64 vector rate, so the contraction gains nothing on
+// them.)  The LU is k_logdet_lane8's in FP64: partial pivoting by compares
+// and selects, rows swapped only when a lane of the wavefront pivots, the
+// identity's unit kept exact per row, the pivot product as FP64 mantissa and
+// exponent.  Partial sums and phases as in k_logdet_lane8.
+template <int KPW, int ZPI>
+__global__ __launch_bounds__(kOBlock, 1) void k_logdet_lane64(int nk, int nz, int zchunk,
+                                                              int nspin_blk, const double2 *hk,
+                                                              const double2 *gmat,
+                                                              const double2 *pmat,
+                                                              double *partials, double2 *phase) {
+  static_assert(KPW * ZPI == 64, "one matrix per lane");
+  constexpr int NW = kOBlock / 64;
+  constexpr int KPB = NW * KPW;  // k-points per block
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int kl = lane % KPW, zl = lane / KPW;  // this lane's matrix
+  const int sb = blockIdx.z;
+  const int kb0 = blockIdx.x * KPB;
+  const int kloc = wave * KPW + kl;  // k-point within the block
+  const int k = kb0 + kloc;
+  const bool kvalid = k < nk;
+  __shared__ double2 s_h[KPB][64];
+  __shared__ double2 s_g[ZPI][64], s_p[ZPI][64];
+  __shared__ double s_red[NW][ZPI];
+  // H_k of the block's k-points (zero past nk), for the whole chunk
+  for (int e = threadIdx.x; e < KPB * 64; e += kOBlock) {
+    const int kk = e / 64, rc = e % 64;
+    double2 h = make_double2(0.0, 0.0);
+    if (kb0 + kk < nk) h = hk[((int64_t)(kb0 + kk) * nspin_blk + sb) * 64 + rc];
+    s_h[kk][rc] = h;
+  }
+  const int zbeg = blockIdx.y * zchunk;
+  const int zend = min(zbeg + zchunk, nz);
+  for (int z0 = zbeg; z0 < zend; z0 += ZPI) {
+    __syncthreads();  // H staged; the previous frequencies are done with s_g, s_p, s_red
+    for (int e = threadIdx.x; e < ZPI * 64; e += kOBlock) {
+      const int zz = e / 64, rc = e % 64;
+      const int iz = z0 + zz;
+      double2 g = make_double2(0.0, 0.0), pz = g;
+      if (iz < zend) {
+        const int64_t zoff = ((int64_t)iz * nspin_blk + sb) * kNB * kNB + (rc / 8) * kNB + rc % 8;
+        g = gmat[zoff];
+        pz = pmat[zoff];
+      }
+      s_g[zz][rc] = g;
+      s_p[zz][rc] = pz;
+    }
+    __syncthreads();
+    // X = P - H G, row by row
+    double xr[8][8], xi[8][8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      double2 h[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) h[j] = s_h[kloc][r * 8 + j];
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        const double2 pz = s_p[zl][r * 8 + c];
+        double ar = pz.x, ai = pz.y;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const double2 g = s_g[zl][j * 8 + c];
+          ar = fma(-h[j].x, g.x, ar);
+          ar = fma(h[j].y, g.y, ar);
+          ai = fma(-h[j].x, g.y, ai);
+          ai = fma(-h[j].y, g.x, ai);
+        }
+        xr[r][c] = ar;
+        xi[r][c] = ai;
+      }
+    }
+    // LU with partial pivoting of M = X + (unit of row r in column cu[r])
+    int cu[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) cu[r] = r;
+    double pm_acc = 1.0, phr = 1.0, phi = 0.0;
+    int pm_exp = 0, parity = 0;
+#pragma unroll
+    for (int kk = 0; kk < 8; ++kk) {
+      int p = kk;
+      double best;
+      {
+        const double mr = xr[kk][kk] + (cu[kk] == kk ? 1.0 : 0.0);
+        best = mr * mr + xi[kk][kk] * xi[kk][kk];
+      }
+#pragma unroll
+      for (int r = kk + 1; r < 8; ++r) {
+        const double mr = xr[r][kk] + (cu[r] == kk ? 1.0 : 0.0);
+        const double m2 = mr * mr + xi[r][kk] * xi[r][kk];
+        if (m2 > best) {
+          best = m2;
+          p = r;
+        }
+      }
+      // rows kk and p trade places; skipped when no lane of the wavefront
+      // pivots
+      if (__any(p != kk)) {
+#pragma unroll
+        for (int r = kk + 1; r < 8; ++r) {
+          const bool sw = p == r;
+#pragma unroll
+          for (int jj = kk; jj < 8; ++jj) {
+            const double ar = xr[kk][jj], br = xr[r][jj], ai = xi[kk][jj], bi = xi[r][jj];
+            xr[kk][jj] = sw ? br : ar;
+            xr[r][jj] = sw ? ar : br;
+            xi[kk][jj] = sw ? bi : ai;
+            xi[r][jj] = sw ? ai : bi;
+          }
+          const int ca = cu[kk], cb = cu[r];
+          cu[kk] = sw ? cb : ca;
+          cu[r] = sw ? ca : cb;
+        }
+        parity ^= p != kk ? 1 : 0;
+      }
+      const double pvr = xr[kk][kk] + (cu[kk] == kk ? 1.0 : 0.0);
+      const double pvi = xi[kk][kk];
+      const double pm = pvr * pvr + pvi * pvi;
+      {
+        int ex;
+        pm_acc = frexp(pm_acc * pm, &ex);
+        pm_exp += ex;
+        if (phase) {
+          const double rm = 1.0 / sqrt(pm);
+          const double ur = pvr * rm, ui = pvi * rm;
+          const double nr = phr * ur - phi * ui;
+          phi = phr * ui + phi * ur;
+          phr = nr;
+        }
+      }
+      const double inv = 1.0 / pm;
+      double pr[8], pim[8];  // the pivot row with its unit
+#pragma unroll
+      for (int jj = kk + 1; jj < 8; ++jj) {
+        pr[jj] = xr[kk][jj] + (cu[kk] == jj ? 1.0 : 0.0);
+        pim[jj] = xi[kk][jj];
+      }
+#pragma unroll
+      for (int r = kk + 1; r < 8; ++r) {
+        const double mr = xr[r][kk] + (cu[r] == kk ? 1.0 : 0.0);
+        const double mi = xi[r][kk];
+        const double lr = (mr * pvr + mi * pvi) * inv;  // l = M(r, kk) / pivot
+        const double li = (mi * pvr - mr * pvi) * inv;
+#pragma unroll
+        for (int jj = kk + 1; jj < 8; ++jj) {
+          xr[r][jj] -= lr * pr[jj] - li * pim[jj];
+          xi[r][jj] -= lr * pim[jj] + li * pr[jj];
+        }
+        if (cu[r] == kk) cu[r] = -1;  // the unit sat in the eliminated column
+      }
+    }
+    const int iz = z0 + zl;
+    double ld = 0.5 * (log(pm_acc) + (double)pm_exp * 0.69314718055994530942);
+    if (!kvalid || iz >= zend) ld = 0.0;
+    if (phase && kvalid && iz < zend) {
+      const double sg = parity ? -1.0 : 1.0;
+      phase[((int64_t)iz * nspin_blk + sb) * nk + k] = make_double2(sg * phr, sg * phi);
+    }
+    // sum over the wavefront's KPW k-points of each frequency, then the block's
+#pragma unroll
+    for (int off = KPW / 2; off > 0; off >>= 1) ld += __shfl_xor(ld, off, KPW);
+    if (kl == 0) s_red[wave][zl] = ld;
+    __syncthreads();
+    if (threadIdx.x < ZPI && z0 + (int)threadIdx.x < zend) {
+      double tsum = 0;
+      for (int w = 0; w < NW; ++w) tsum += s_red[w][threadIdx.x];
+      partials[((int64_t)(z0 + threadIdx.x) * nspin_blk + sb) * gridDim.x + blockIdx.x] = tsum;
+    }
+  }
+}
+
 __global__ __launch_bounds__(64) void k_reduce_z(int nz, int nspin_blk, int nkb, int nk,
                                                  const double *partials, double *f) {
   const int iz = blockIdx.x;
@@ -1408,7 +1588,13 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   // always makes O(1) updates (3x the rounding error of the Omega integral
   // in FP32 on the 2x2 BHZ grid).
   const char *lu_env = std::getenv("VCA_OMEGA_LU");
-  const bool use_qr = nbt == 8 && precision == VCA_FP64 && !(lu_env && lu_env[0] == '1');
+  // FP64 8-row blocks: a matrix per lane (k_logdet_lane64); VCA_OMEGA_LANE=0
+  // keeps the QR kernel
+  const char *ln_env = std::getenv("VCA_OMEGA_LANE");
+  const bool lane64 = nbt == 8 && precision == VCA_FP64 && !(lu_env && lu_env[0] == '1') &&
+                      !(ln_env && ln_env[0] == '0');
+  const bool use_qr =
+      nbt == 8 && precision == VCA_FP64 && !(lu_env && lu_env[0] == '1') && !lane64;
   // FP32: the contraction on the matrix cores (VCA_OMEGA_MFMA=0: on the
   // vector ALUs, k_logdet); 8-row blocks with one matrix per lane
   // (VCA_OMEGA_MFMA=row: a row per lane, k_logdet_mfma<8>)
@@ -1416,7 +1602,7 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   const bool mfma = precision != VCA_FP64 && !(mf_env && mf_env[0] == '0');
   const bool lane8 = mfma && nbt == 8 && !(mf_env && mf_env[0] == 'r');
   const int lanes = use_qr ? 4 : nbt;
-  const int mpb = lane8 ? (kOBlock / 64) * kLane8Kpw : kOBlock / lanes;
+  const int mpb = (lane8 || lane64) ? (kOBlock / 64) * kLane8Kpw : kOBlock / lanes;
   const int nkb = (host.nk + mpb - 1) / mpb;
   const int zchunk = 16;
   const dim3 grid(nkb, (nz + zchunk - 1) / zchunk, host.nspin_blk);
@@ -1427,7 +1613,10 @@ void OmegaDev::integrand(int nz, const double2 *z, int precision, double *f, flo
   }
   if (precision == VCA_FP64) {
     if (e0) VCA_HIP(hipEventRecord(e0, st));
-    if (use_qr)
+    if (lane64)
+      hipLaunchKernelGGL((k_logdet_lane64<kLane8Kpw, 64 / kLane8Kpw>), grid, dim3(kOBlock), 0, st,
+                         host.nk, nz, zchunk, host.nspin_blk, d_hk64, d_g64, d_p64, d_part, phase);
+    else if (use_qr)
       hipLaunchKernelGGL((k_logdet_qr<double, 8, 4>), grid, dim3(kOBlock), 0, st, host.nk, nz,
                          zchunk, host.nspin_blk, d_hk64, d_g64, d_p64, d_part, phase);
     else if (nbt == 8)
