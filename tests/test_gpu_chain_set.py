@@ -116,3 +116,24 @@ def test_chain_set_graph_replays_match_eager(monkeypatch):
         assert np.array_equal(x, y)
     for x, y in zip(out["1"][2], out["0"][2]):
         assert np.array_equal(x, y)
+
+
+def test_chain_set_row_and_column_items_agree(monkeypatch):
+    """The chain set's two H x v layouts -- row blocks, and whole columns
+    staged in LDS (the 10-site chain's target columns have 210 rows) -- and
+    the timed choice between them give the same solve to rounding."""
+    p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
+    p.lanc_ngfiter = 80
+    out = {}
+    for mode in ("0", "1", None):
+        if mode is None:
+            monkeypatch.delenv("VCA_CHAIN_SET_COLUMNS", raising=False)
+        else:
+            monkeypatch.setenv("VCA_CHAIN_SET_COLUMNS", mode)
+        _say(f"chain10: chain set, VCA_CHAIN_SET_COLUMNS={mode}")
+        s, om = _solve(p, hloc, hk, None, None)
+        out[mode] = (om, [s.vca_gf_cluster(z) for z in ZS])
+    for m in ("1", None):
+        assert abs(out[m][0] - out["0"][0]) <= 1e-12 * abs(out["0"][0])
+        for x, y in zip(out[m][1], out["0"][1]):
+            assert np.abs(x - y).max() <= 1e-10 * np.abs(y).max()

@@ -32,7 +32,9 @@
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
+#include <limits>
 #include <map>
+#include <mutex>
 #include <numeric>
 #include <type_traits>
 
@@ -387,9 +389,7 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
       CsSeg s{};
       s.tab = row_tab(js);
       s.real = jobs[c].real ? 1 : 0;
-      s.colmode = (js.up.dim >= kCsColMin && js.up.dim <= kCsColMax) ? 1 : 0;
-      const char *cm = std::getenv("VCA_CHAIN_SET_COLUMNS");
-      if (cm && cm[0] == '0') s.colmode = 0;
+      s.colmode = 0;  // set per layout below
       s.n = js.dim;
       // real chains: an even stride, so the excitation's pad real of an
       // odd dimension stays inside the chain's slot
@@ -402,7 +402,7 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
   }
   const int nseg = (int)segs.size();
   std::vector<CsChain> chains(nch);
-  std::vector<int> nblk_h(nch), nblk_u(nch);
+  std::vector<int> nblk_u(nch);
   int max_steps = 0;
   int64_t ph = 0, pu = 0;
   for (int sgi = 0; sgi < nseg; ++sgi) {
@@ -419,18 +419,73 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
       ch.seg = sgi;
       ch.local = (int)l;
       ch.nsteps = jobs[c].nsteps;
-      ch.nblk_h = s.colmode ? s.tab.dimdw : (int)((s.n + kCsBlock - 1) / kCsBlock);
       ch.nblk_u = (int)((s.n + kCsBlock * kCsEpt - 1) / (kCsBlock * kCsEpt));
-      ch.pofs_h = ph;
       ch.pofs_u = pu;
-      ph += ch.nblk_h;
       pu += ch.nblk_u;
-      nblk_h[c] = ch.nblk_h;
       nblk_u[c] = ch.nblk_u;
       max_steps = std::max(max_steps, ch.nsteps);
     }
   }
-  const std::vector<int2> items_h = cs_items(nblk_h), items_u = cs_items(nblk_u);
+  const std::vector<int2> items_u = cs_items(nblk_u);
+  // Two layouts of the H x v items: row blocks everywhere, or whole
+  // columns staged in LDS where the sector's columns have kCsColMin ..
+  // kCsColMax rows.  Both are timed on the set's first step
+  // (VCA_CHAIN_SET_COLUMNS=0/1 forces one).
+  struct Layout {
+    std::vector<CsSeg> segs;
+    std::vector<CsChain> chains;
+    std::vector<int2> items;
+    int64_t np = 0;
+    size_t smem = 16;
+    CsSeg *d_segs = nullptr;
+    CsChain *d_chains = nullptr;
+    int2 *d_items = nullptr;
+  };
+  auto make_layout = [&](bool cols) {
+    Layout L;
+    L.segs = segs;
+    L.chains = chains;
+    for (CsSeg &sgm : L.segs)
+      sgm.colmode = cols && sgm.tab.dimup >= kCsColMin && sgm.tab.dimup <= kCsColMax ? 1 : 0;
+    std::vector<int> nb(nch);
+    for (int c = 0; c < nch; ++c) {
+      CsChain &ch = L.chains[c];
+      const CsSeg &sgm = L.segs[ch.seg];
+      ch.nblk_h = sgm.colmode ? sgm.tab.dimdw : (int)((sgm.n + kCsBlock - 1) / kCsBlock);
+      ch.pofs_h = L.np;
+      L.np += ch.nblk_h;
+      nb[c] = ch.nblk_h;
+    }
+    L.items = cs_items(nb);
+    for (const CsSeg &sgm : L.segs) {
+      const size_t amps = (size_t)(sgm.tab.namp_up + sgm.tab.namp_dw) * 2 * rsz;
+      const size_t col = sgm.colmode ? cs_col_offset(sgm.tab, 2 * rsz) +
+                                           (size_t)sgm.tab.dimup * (sgm.real ? rsz : 2 * rsz)
+                                     : 0;
+      L.smem = std::max(L.smem, std::max(amps, col));
+    }
+    return L;
+  };
+  bool any_cols = false;
+  for (const CsSeg &sgm : segs)
+    any_cols = any_cols || (sgm.tab.dimup >= kCsColMin && sgm.tab.dimup <= kCsColMax);
+  const char *cm = std::getenv("VCA_CHAIN_SET_COLUMNS");
+  const bool force = cm && (cm[0] == '0' || cm[0] == '1');
+  std::vector<Layout> lay;
+  if (!any_cols || (force && cm[0] == '0')) {
+    lay.push_back(make_layout(false));
+  } else if (force) {
+    lay.push_back(make_layout(true));
+  } else {
+    lay.push_back(make_layout(true));
+    lay.push_back(make_layout(false));
+  }
+  for (Layout &L : lay) {
+    ph = std::max(ph, L.np);
+    L.d_segs = cs_upload(L.segs, st);
+    L.d_chains = cs_upload(L.chains, st);
+    L.d_items = cs_upload(L.items, st);
+  }
   // per-chain Lanczos scalars
   const size_t cap = (size_t)max_steps + 4;
   double *lz_arrays = static_cast<double *>(dev_alloc_async(sizeof(double) * 3 * cap * nch, st));
@@ -442,9 +497,6 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
     lzh[c].nrm = lzh[c].beta + cap;
     lzh[c].y = nullptr;
   }
-  CsSeg *d_segs = cs_upload(segs, st);
-  CsChain *d_chains = cs_upload(chains, st);
-  int2 *d_items_h = cs_upload(items_h, st);
   int2 *d_items_u = cs_upload(items_u, st);
   LanczosDev *d_lz = cs_upload(lzh, st);
   int32_t *d_dflag = nullptr;
@@ -487,44 +539,95 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
     VCA_HIP(hipFree(d_slz));
   }
   CsArgs a{};
-  a.segs = d_segs;
-  a.chains = d_chains;
   a.lz = d_lz;
   a.dflag = d_dflag;
   a.pa = d_pa;
   a.pb = d_pb;
   a.kbase = d_kbase;
   a.cross = cross;
-  const int nih = (int)(items_h.size()), niu = (int)(items_u.size());
-  size_t smem = 16;
-  for (const CsSeg &sgm : segs) {
-    const size_t amps = (size_t)(sgm.tab.namp_up + sgm.tab.namp_dw) * 2 * rsz;
-    const size_t col = sgm.colmode ? cs_col_offset(sgm.tab, 2 * rsz) +
-                                         (size_t)sgm.tab.dimup * (sgm.real ? rsz : 2 * rsz)
-                                   : 0;
-    smem = std::max(smem, std::max(amps, col));
-  }
-  auto launch_step = [&](int i) {
+  const int niu = (int)(items_u.size());
+  const Layout *cur = &lay[0];
+  auto launch_hxv_step = [&](const Layout &L, int i) {
     CsArgs ah = a;
-    ah.items = d_items_h;
+    ah.segs = L.d_segs;
+    ah.chains = L.d_chains;
+    ah.items = L.d_items;
     ah.i = i;
-    CsArgs au = a;
-    au.items = d_items_u;
-    au.i = i;
+    const dim3 grid((unsigned)L.items.size());
     if (prec == VCA_FP64) {
       if (real_amp)
-        hipLaunchKernelGGL((k_cs_hxv<double, true>), dim3(nih), dim3(kCsBlock), smem, st, ah);
+        hipLaunchKernelGGL((k_cs_hxv<double, true>), grid, dim3(kCsBlock), L.smem, st, ah);
       else
-        hipLaunchKernelGGL((k_cs_hxv<double, false>), dim3(nih), dim3(kCsBlock), smem, st, ah);
-      hipLaunchKernelGGL(k_cs_update<double>, dim3(niu), dim3(kCsBlock), 0, st, au);
+        hipLaunchKernelGGL((k_cs_hxv<double, false>), grid, dim3(kCsBlock), L.smem, st, ah);
     } else {
       if (real_amp)
-        hipLaunchKernelGGL((k_cs_hxv<float, true>), dim3(nih), dim3(kCsBlock), smem, st, ah);
+        hipLaunchKernelGGL((k_cs_hxv<float, true>), grid, dim3(kCsBlock), L.smem, st, ah);
       else
-        hipLaunchKernelGGL((k_cs_hxv<float, false>), dim3(nih), dim3(kCsBlock), smem, st, ah);
-      hipLaunchKernelGGL(k_cs_update<float>, dim3(niu), dim3(kCsBlock), 0, st, au);
+        hipLaunchKernelGGL((k_cs_hxv<float, false>), grid, dim3(kCsBlock), L.smem, st, ah);
     }
   };
+  auto launch_step = [&](int i) {
+    launch_hxv_step(*cur, i);
+    CsArgs au = a;
+    au.segs = cur->d_segs;
+    au.chains = cur->d_chains;
+    au.items = d_items_u;
+    au.i = i;
+    if (prec == VCA_FP64)
+      hipLaunchKernelGGL(k_cs_update<double>, dim3(niu), dim3(kCsBlock), 0, st, au);
+    else
+      hipLaunchKernelGGL(k_cs_update<float>, dim3(niu), dim3(kCsBlock), 0, st, au);
+  };
+  // the layout: launch A of step 0 timed in each (it rewrites only its own
+  // outputs -- Hv, the alpha partials, the step-0 done flags -- so the
+  // timing leaves no trace), once per set shape per process
+  if (lay.size() > 1) {
+    std::vector<int64_t> key{prec, real_amp ? 1 : 0};
+    for (int sgi = 0; sgi < nseg; ++sgi) {
+      key.push_back(segs[sgi].tab.dimup);
+      key.push_back(segs[sgi].tab.dimdw);
+      key.push_back(segs[sgi].real);
+      key.push_back((int64_t)seg_jobs[sgi].size());
+    }
+    static std::mutex mu;
+    static std::map<std::vector<int64_t>, int> tuned;
+    int pick = -1;
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      auto f = tuned.find(key);
+      if (f != tuned.end()) pick = f->second;
+    }
+    if (pick < 0) {
+      hipEvent_t t0, t1;
+      VCA_HIP(hipEventCreate(&t0));
+      VCA_HIP(hipEventCreate(&t1));
+      float best = std::numeric_limits<float>::max();
+      for (size_t li = 0; li < lay.size(); ++li) {
+        launch_hxv_step(lay[li], 0);
+        float mn = std::numeric_limits<float>::max();
+        for (int r = 0; r < 2; ++r) {
+          VCA_HIP(hipEventRecord(t0, st));
+          launch_hxv_step(lay[li], 0);
+          VCA_HIP(hipEventRecord(t1, st));
+          VCA_HIP(hipEventSynchronize(t1));
+          float ms_ = 0;
+          VCA_HIP(hipEventElapsedTime(&ms_, t0, t1));
+          mn = std::min(mn, ms_);
+        }
+        if (mn < best) {
+          best = mn;
+          pick = (int)li;
+        }
+      }
+      VCA_HIP(hipEventDestroy(t0));
+      VCA_HIP(hipEventDestroy(t1));
+      std::lock_guard<std::mutex> lk(mu);
+      tuned[key] = pick;
+    }
+    cur = &lay[pick];
+  }
+  stats.column_items = 0;
+  for (const CsSeg &sgm : cur->segs) stats.column_items += sgm.colmode ? 1 : 0;
   const char *ng = std::getenv("VCA_NO_GRAPHS");
   hipGraphExec_t ex = nullptr;
   if (max_steps >= kCsChunk && !(ng && ng[0] == '1')) {
@@ -577,7 +680,10 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
     for (void *b : {segs[sgi].u0, segs[sgi].u1, segs[sgi].hv}) dev_free_async(b, st);
   dev_free_async(lz_arrays, st);
   VCA_HIP(hipStreamSynchronize(st));
-  for (void *q : {(void *)d_segs, (void *)d_chains, (void *)d_items_h, (void *)d_items_u,
+  for (Layout &L : lay)
+    for (void *q : {(void *)L.d_segs, (void *)L.d_chains, (void *)L.d_items})
+      VCA_HIP(hipFree(q));
+  for (void *q : {(void *)d_items_u,
                   (void *)d_lz, (void *)d_dflag, (void *)d_pa, (void *)d_pb, (void *)d_px,
                   (void *)d_kbase, (void *)d_ops})
     VCA_HIP(hipFree(q));

@@ -1579,8 +1579,8 @@ static void build_gf(Solver &s) {
       s.res.hxv_count += cst.hxv_count;
       s.res.hxv_states += cst.hxv_states;
       progress(s.t_start, "state %zu chain set: chains %zu-%zu of %zu, %d steps, %lld launches, "
-               "%.3f s", ist, j0 + 1, j1, jobs.size(), cst.steps, (long long)cst.launches,
-               cst.seconds);
+               "%d segments in column items, %.3f s", ist, j0 + 1, j1, jobs.size(), cst.steps,
+               (long long)cst.launches, cst.column_items, cst.seconds);
       j0 = j1;
     }
     for (auto &g : groups) {

@@ -360,6 +360,7 @@ struct ChainSetStats {
   double hxv_states = 0;  // sum over chain steps of the sector dimension
   int64_t device_bytes = 0;  // vector memory of the set
   int steps = 0;          // steps of the longest chain
+  int column_items = 0;   // segments whose H x v items are LDS-staged columns
 };
 // Device bytes of one chain in a set (its three vectors).
 int64_t chain_set_bytes(const Sector &js, bool real, int prec);
