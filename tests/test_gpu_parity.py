@@ -98,12 +98,13 @@ def test_hxv_square_3x4_full_sector():
     assert err < 1e-12, err
 
 
-N_HXV_VARIANTS = 27  # launch shapes of the one-column kernels (kernels.hip kVariants)
+N_HXV_VARIANTS = 29  # launch shapes of the one-column kernels (kernels.hip kVariants)
 N_HXV_DIRECT = 15    # the first 15: direct form; then the transposed form (15-18
                      # whole columns, 19-20 one real plane at a time: real
                      # amplitudes only), the overlapped short-column form (21-23)
                      # and its signed-column variant (24-26: one real hopping
-                     # amplitude)
+                     # amplitude), the transposed form with complex-pair passes
+                     # (27-28: real amplitudes)
 
 
 @pytest.mark.parametrize("variant", range(N_HXV_VARIANTS))
@@ -335,11 +336,12 @@ def test_signed_column_form_in_lanczos_and_batches(prec, variant, monkeypatch):
     sec.close()
 
 
-@pytest.mark.parametrize("variant", ["16", "20"])
+@pytest.mark.parametrize("variant", ["16", "20", "27"])
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 def test_transposed_form_in_lanczos_and_batches(prec, variant, monkeypatch):
     """The transposed H x v form (forced on small sectors; variant 16 with
-    whole complex columns, 20 one real plane at a time) inside the Lanczos
+    whole complex columns, 20 one real plane at a time, 27 whole complex
+    columns through the column-pair tables) inside the Lanczos
     steps: its two gather passes' partials of alpha, the buffer
     parity and the skipping of finished chains.  Bars: the oracle's Lanczos
     coefficients (12 steps, bath model); stacked products equal to the
@@ -390,6 +392,41 @@ def test_transposed_form_in_lanczos_and_batches(prec, variant, monkeypatch):
     assert abs(out["transposed"][0] - out["direct"][0]) <= 1e-11 * abs(out["direct"][0])
     for x, y in zip(out["transposed"][1:], out["direct"][1:]):
         assert np.abs(x - y).max() <= 1e-9 * np.abs(y).max()
+
+
+def test_complex_pair_passes_equal_whole_column_passes(monkeypatch):
+    """The transposed form's complex-pair gather passes (variants 27, 28)
+    and its whole-column passes (variant 16) sum the same hops in the same
+    order (the complex64 table takes the bank-aware hop order of the
+    one-plane table the pair table is made from): MIXED products equal bit
+    for bit; FP64 (whose complex128 table keeps the original order) within
+    1e-14 of each other; the 10-site chain and the bath model, 3 stacked
+    vectors."""
+    from vca_amd.api import DeviceArray
+    monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
+    for name, (p, hloc, hk, bh, bv) in (("chain10", M.chain(nlat=10, u=4.0)),
+                                        ("bath5+5", M.bath_model(nlat=5, nb=5))):
+        s = _solver(p, hloc, bh, bv)
+        for prec in ("fp64", "mixed"):
+            dt = np.complex128 if prec == "fp64" else np.complex64
+            out = {}
+            for var in ("16", "27", "28"):
+                monkeypatch.setenv("VCA_HXV_VARIANT", var)
+                _say(f"complex-pair passes: {name} (5,5) {prec} variant {var}")
+                sec = s.sector(5, 5, prec)
+                rng = np.random.default_rng(21)
+                v = (rng.normal(size=(3, sec.dim)) + 1j * rng.normal(size=(3, sec.dim))).astype(dt)
+                dv = DeviceArray(v.shape, dt).upload(v)
+                dh = DeviceArray(v.shape, dt)
+                sec.hxv_device_batch(3, dv.ptr, dh.ptr)
+                out[var] = dh.download()
+                sec.close()
+            for var in ("27", "28"):
+                if prec == "mixed":
+                    assert np.array_equal(out[var], out["16"]), (name, var)
+                else:
+                    err = np.abs(out[var] - out["16"]).max() / np.abs(out["16"]).max()
+                    assert err < 1e-14, (name, var, err)
 
 
 @pytest.mark.parametrize("form", ["direct", "transposed2"])

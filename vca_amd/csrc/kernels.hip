@@ -1643,7 +1643,14 @@ __global__ __launch_bounds__(BS) void k_hxv_buf_real2(const HxvArgs a) {
 // hop table is DevSpin::ent4q, the one-plane table's hops addressed for the
 // pair (a mask and a shift per hop).  LDS: the amplitudes and a zero at 0,
 // then the column pair.  One partial of <v|Hv> per pair.
-template <typename R, int BS>
+//
+// CPLX: the same pass on a complex vector with real amplitudes, one column
+// per workgroup: the complex column is already the interleaved pair (real
+// part at o_col + 2 f RSZ, imaginary part after it), so it is staged as it
+// is and each hop's one 2 RSZ-byte read feeds both parts, with the sums of
+// the whole-column complex pass (k_hxv_buf, real amplitudes) in the same
+// order.  One partial of Re <v|Hv> per column.
+template <typename R, int BS, bool CPLX>
 __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
   extern __shared__ __align__(16) unsigned char smem[];
   constexpr int PF = BS == 1024 ? 4 : 2;
@@ -1652,12 +1659,14 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
   const uint32_t o_col = (uint32_t)buf_lds(a.namp_up, 0, 0, RSZ).col;
   R *s_amp = reinterpret_cast<R *>(smem);
   cplx<R> *s_pair = reinterpret_cast<cplx<R> *>(smem + o_col);
-  const int npair = (a.dimdw + 1) / 2;
+  // CPLX: complex elements of 2 reals; the column's "pair" is its two parts
+  constexpr int NR = CPLX ? 2 : 1;
+  const int npair = CPLX ? a.dimdw : (a.dimdw + 1) / 2;
   const int pair = xcd_column(blockIdx.x, npair);
-  const int idw0 = 2 * pair;
-  const bool two = idw0 + 1 < a.dimdw;
+  const int idw0 = CPLX ? pair : 2 * pair;
+  const bool two = !CPLX && idw0 + 1 < a.dimdw;
   const int idw1 = two ? idw0 + 1 : idw0;
-  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride;
+  const int64_t chain_off = (int64_t)blockIdx.y * a.vstride * NR;
   const int64_t slot = a.pbase + (int64_t)blockIdx.y * a.pstride + blockIdx.x;
   LanczosDev *lz = a.lz ? a.lz + blockIdx.y : nullptr;
   bool done = false;
@@ -1669,25 +1678,31 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
     if (k & 1) vin = static_cast<const R *>(a.v1) + chain_off;
     if (!done) scale = (R)(1.0 / lz->nrm[k]);
   }
-  const R *vcol0 = vin + (int64_t)idw0 * dimup;
-  const R *vcol1 = vin + (int64_t)idw1 * dimup;
-  R *hcol0 = static_cast<R *>(a.hv) + chain_off + (int64_t)idw0 * dimup;
-  R *hcol1 = static_cast<R *>(a.hv) + chain_off + (int64_t)idw1 * dimup;
+  const R *vcol0 = vin + (int64_t)idw0 * dimup * NR;
+  const R *vcol1 = vin + (int64_t)idw1 * dimup * NR;
+  R *hcol0 = static_cast<R *>(a.hv) + chain_off + (int64_t)idw0 * dimup * NR;
+  R *hcol1 = static_cast<R *>(a.hv) + chain_off + (int64_t)idw1 * dimup * NR;
   double dot = 0.0;
 
   if (!done) {
     const cplx<R> *g_amp = static_cast<const cplx<R> *>(a.amp_up);
     for (int t = threadIdx.x; t < a.namp_up; t += BS) s_amp[t] = g_amp[t].x;
     if (threadIdx.x == 0) s_amp[a.namp_up] = (R)0;  // padding hops
-    stage_copy<BS, cplx<R>>(
-        dimup,
-        [&](int f) {
-          cplx<R> y;
-          y.x = vcol0[f];
-          y.y = vcol1[f];
-          return y;
-        },
-        [&](int f, const cplx<R> &y) { s_pair[f] = y; });
+    if constexpr (CPLX) {
+      const cplx<R> *vc = reinterpret_cast<const cplx<R> *>(vcol0);
+      stage_copy<BS, cplx<R>>(dimup, [&](int f) { return vc[f]; },
+                              [&](int f, const cplx<R> &y) { s_pair[f] = y; });
+    } else {
+      stage_copy<BS, cplx<R>>(
+          dimup,
+          [&](int f) {
+            cplx<R> y;
+            y.x = vcol0[f];
+            y.y = vcol1[f];
+            return y;
+          },
+          [&](int f, const cplx<R> &y) { s_pair[f] = y; });
+    }
     __syncthreads();
     const double edw0 = a.skip_diag ? 0.0 : a.e_dw[idw0];
     const double edw1 = a.skip_diag ? 0.0 : a.e_dw[idw1];
@@ -1705,8 +1720,9 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
       const R x0 = xp.x, x1 = xp.y;
       R acc0 = (R)0, acc1 = (R)0;
       if (!a.skip_diag) {
-        acc0 = (R)(m.e + edw0 + cross_energy(a.cross, m.s, sdw0)) * x0;
-        acc1 = (R)(m.e + edw1 + cross_energy(a.cross, m.s, sdw1)) * x1;
+        const R e0 = (R)(m.e + edw0 + cross_energy(a.cross, m.s, sdw0));
+        acc0 = e0 * x0;
+        acc1 = (CPLX ? e0 : (R)(m.e + edw1 + cross_energy(a.cross, m.s, sdw1))) * x1;
       }
       int nc_w = (nu + 3) >> 2;
       for (int off = 32; off > 0; off >>= 1) nc_w = max(nc_w, __shfl_xor(nc_w, off, 64));
@@ -1739,11 +1755,20 @@ __global__ __launch_bounds__(BS) void k_hxv_plane2(const HxvArgs a) {
       }
       if (base < dimup) {
         const R out0 = acc0 * scale, out1 = acc1 * scale;
-        hcol0[row] = out0;
-        if (lz) dot += (double)(x0 * scale) * (double)out0;
-        if (two) {
-          hcol1[row] = out1;
-          if (lz) dot += (double)(x1 * scale) * (double)out1;
+        if constexpr (CPLX) {
+          cplx<R> o;
+          o.x = out0;
+          o.y = out1;
+          reinterpret_cast<cplx<R> *>(hcol0)[row] = o;
+          if (lz)
+            dot += (double)(x0 * scale) * (double)out0 + (double)(x1 * scale) * (double)out1;
+        } else {
+          hcol0[row] = out0;
+          if (lz) dot += (double)(x0 * scale) * (double)out0;
+          if (two) {
+            hcol1[row] = out1;
+            if (lz) dot += (double)(x1 * scale) * (double)out1;
+          }
         }
       }
     }
@@ -1794,17 +1819,36 @@ __device__ inline void acc_elem(float2 &a, const float2 &b) {
 }
 
 // E: complex element (cplx<R>) or, in real-vector mode, a real.
-// Tile edge of k_transpose: 64 for elements up to 8 bytes; 32 for 16-byte
-// (complex128) elements, whose 64 x 64 tile (66.5 KB of LDS) would leave
-// two workgroups per CU to keep the HBM streams busy.
+// Tiles are 64 x 64 for every element size (complex128: 66.5 KB of LDS,
+// two workgroups per CU), taken in super-tile order: the workgroups of one
+// launch round cover SW x SW neighbouring tiles (SW * 64 * sizeof(E) = 8 or
+// 4 KB runs of each row read or written), walking down a strip of SW tile
+// columns before the next strip, so the streams of a round stay within few
+// HBM pages.  On the 12,870 x 12,870 transposes (rows of 64 tiles in
+// launch order before): complex128 1.43 -> 1.14 ms, the add form 2.11 ->
+// 1.67 ms, FP64 reals 0.68 -> 0.57 ms.
 template <typename E>
-constexpr int tr_tile() {
-  return sizeof(E) > 8 ? 32 : kTile;
+constexpr int tr_sw() {
+  return sizeof(E) > 8 ? 8 : 16;
+}
+
+// Tile (i, j) of a ni_t x nj_t tile grid at launch position lin: strips of
+// sw tile columns (the last one narrower), each walked as sw x sw super-tiles
+// (the last one shorter), tile rows fastest inside a super-tile.  Every
+// position maps to one tile and every tile to one position.
+__device__ inline int2 tile_at(int lin, int ni_t, int nj_t, int sw) {
+  const int sj = lin / (sw * ni_t);
+  const int w = min(sw, nj_t - sj * sw);
+  const int r = lin - sj * sw * ni_t;
+  const int si = r / (sw * w);
+  const int h = min(sw, ni_t - si * sw);
+  const int q = r - si * sw * w;
+  return make_int2(si * sw + q % h, sj * sw + q / h);
 }
 
 template <typename E, bool ADD>
 __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
-  constexpr int T = tr_tile<E>();
+  constexpr int T = kTile;
   constexpr int kRows = kBlock / T;  // rows of the tile per pass of the workgroup
   constexpr int NP = T / kRows;      // passes
   __shared__ E s[T][T + 1];
@@ -1817,10 +1861,13 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
   }
   E *out = static_cast<E *>(a.out) + off;
   const int lane = threadIdx.x % T, w = threadIdx.x / T;
-  const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
+  const int2 tl = tile_at(blockIdx.x, (a.ni + T - 1) / T, (a.nj + T - 1) / T, tr_sw<E>());
+  const int i0 = tl.x * T, j0 = tl.y * T;
   // every load of the lane is issued before the first LDS store (indices
-  // clamped into the matrix, the extra elements discarded), so a tile costs
-  // one HBM round trip rather than NP; the same on the way out
+  // clamped into the matrix), so a tile costs one HBM round trip rather
+  // than NP; the same on the way out.  The LDS stores are unconditional:
+  // a clamped element lands in a tile slot whose output is not stored, and
+  // without a branch between them the loads stay in flight together.
   const int i = i0 + lane, ic = min(i, a.ni - 1);
   E x[NP];
 #pragma unroll
@@ -1829,10 +1876,7 @@ __global__ __launch_bounds__(kBlock) void k_transpose(const TrArgs a) {
     x[t] = in[(int64_t)jc * a.ni + ic];
   }
 #pragma unroll
-  for (int t = 0; t < NP; ++t) {
-    const int jj = w + kRows * t;
-    if (i < a.ni && j0 + jj < a.nj) s[jj][lane] = x[t];
-  }
+  for (int t = 0; t < NP; ++t) s[w + kRows * t][lane] = x[t];
   __syncthreads();
   const int j = j0 + lane, jc = min(j, a.nj - 1);
 #pragma unroll
@@ -1904,11 +1948,11 @@ __device__ inline void taxpy(C &x, R c, const C &u) {
 
 template <typename R, typename E>
 __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
-  constexpr int T = tr_tile<E>();
+  constexpr int T = kTile;
   constexpr int kRows = kBlock / T;  // rows of the tile per pass of the workgroup
   constexpr int NP = T / kRows;      // passes
   // elements per batch of loads in the update (registers: 4 or 5 per element)
-  constexpr int CH = sizeof(E) >= 8 ? NP / 2 : NP;
+  constexpr int CH = sizeof(E) > 8 ? NP / 4 : sizeof(E) == 8 ? NP / 2 : NP;
   __shared__ E s[T][T + 1];
   const int64_t off = (int64_t)blockIdx.z * a.vstride;
   LanczosDev *lz = a.lz + blockIdx.z;
@@ -1919,7 +1963,8 @@ __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
     const E *wt = static_cast<const E *>(a.wt) + off;
     const int ni = a.dimdw, nj = a.dimup;  // element (i = idw, j = iup)
     const int lane = threadIdx.x % T, w = threadIdx.x / T;
-    const int i0 = blockIdx.x * T, j0 = blockIdx.y * T;
+    const int2 tl = tile_at(blockIdx.x, (ni + T - 1) / T, (nj + T - 1) / T, tr_sw<E>());
+    const int i0 = tl.x * T, j0 = tl.y * T;
     // loads batched as in k_transpose (clamped indices)
     {
       const int i = i0 + lane, ic = min(i, ni - 1);
@@ -1927,10 +1972,7 @@ __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
 #pragma unroll
       for (int t = 0; t < NP; ++t) y[t] = wt[(int64_t)min(j0 + w + kRows * t, nj - 1) * ni + ic];
 #pragma unroll
-      for (int t = 0; t < NP; ++t) {
-        const int jj = w + kRows * t;
-        if (i < ni && j0 + jj < nj) s[jj][lane] = y[t];
-      }
+      for (int t = 0; t < NP; ++t) s[w + kRows * t][lane] = y[t];
     }
     __syncthreads();
     const E *hv = static_cast<const E *>(a.hv) + off;
@@ -1969,9 +2011,7 @@ __global__ __launch_bounds__(kBlock) void k_tadd_update(const TUpdArgs a) {
     }
   }
   const double b = block_sum(ss);
-  if (threadIdx.x == 0)
-    a.partials[(int64_t)blockIdx.z * gridDim.x * gridDim.y + blockIdx.y * gridDim.x +
-               blockIdx.x] = b;
+  if (threadIdx.x == 0) a.partials[(int64_t)blockIdx.z * gridDim.x + blockIdx.x] = b;
 }
 
 // ------------------------------------------------------ Lanczos update --
@@ -2562,6 +2602,7 @@ struct HxvShape {
   bool buf;    // buffer-load form (k_hxv_buf) when the column can be staged
   bool tform;  // transposed form (gather passes on v and v^T, k_transpose)
   bool plane;  // its gather passes one real plane at a time (k_hxv_plane)
+  bool cpair;  // its gather passes whole complex columns through the pair tables
   bool ov;     // overlapped short-column form (k_hxv_ov)
   bool sgn;    // ... with the signed-column up-spin table
   int bs, kr, u, cpb, grid;
@@ -2570,15 +2611,18 @@ struct HxvShape {
 // Variants of the one-column kernel the autotuner times: (BS, KR, U, form),
 // form 0 = k_hxv_col, 1 = k_hxv_buf (U fixed by KR there), 2 = transposed
 // form with k_hxv_buf passes, 3 = transposed form with k_hxv_plane passes,
-// 4 = k_hxv_ov (KR = rows per lane, NR; columns of at most NR * BS rows).
-// Forms 2 to 4 are timed only on sectors they fit (hxv_variant_ok).
+// 4 = k_hxv_ov (KR = rows per lane, NR; columns of at most NR * BS rows),
+// 5 = its signed-column variant, 6 = transposed form with complex-pair
+// passes (k_hxv_plane2<CPLX>: whole complex columns staged, real amplitudes,
+// the column-pair hop tables of both spins).
+// Forms 2 to 6 are timed only on sectors they fit (hxv_variant_ok).
 static const int kVariants[][4] = {
     {256, 4, 4, 0}, {256, 2, 8, 0},  {1024, 1, 8, 0}, {1024, 2, 8, 0}, {1024, 4, 4, 0},
     {512, 2, 8, 0}, {256, 1, 16, 0}, {512, 1, 16, 0}, {1024, 1, 16, 0}, {256, 4, 4, 1},
     {256, 2, 8, 1}, {512, 2, 8, 1},  {512, 1, 8, 1},  {1024, 1, 8, 1}, {1024, 2, 8, 1},
     {512, 1, 8, 2}, {1024, 1, 8, 2}, {512, 2, 8, 2},  {1024, 2, 8, 2}, {512, 1, 8, 3},
     {1024, 1, 8, 3}, {512, 2, 4, 4}, {1024, 1, 8, 4}, {256, 2, 4, 4}, {512, 2, 4, 5},
-    {1024, 1, 8, 5}, {256, 2, 4, 5}};
+    {1024, 1, 8, 5}, {256, 2, 4, 5}, {1024, 1, 8, 6}, {512, 1, 8, 6}};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 // The transposed form pays 9 vector lengths of HBM traffic per product
@@ -2629,6 +2673,8 @@ bool hxv_variant_ok(const Sector &sec, int v) {
            (size_t)sgn_lds(sec.up.dim, (int)esz).total <= kLdsBig &&
            (uint64_t)sec.dim * esz < (1ull << 31);
   }
+  if (form == 6)
+    return tform_fits(sec, true) && sec.up.ent4q != nullptr && sec.dw.ent4q != nullptr;
   return form < 2 || tform_fits(sec, form == 3);
 }
 
@@ -2642,8 +2688,9 @@ static HxvShape hxv_shape(const Sector &sec) {
     h.kr = kVariants[v][1];
     h.u = kVariants[v][2];
     h.buf = kVariants[v][3] != 0;
-    h.tform = kVariants[v][3] == 2 || kVariants[v][3] == 3;
+    h.tform = kVariants[v][3] == 2 || kVariants[v][3] == 3 || kVariants[v][3] == 6;
     h.plane = kVariants[v][3] == 3;
+    h.cpair = kVariants[v][3] == 6;
     h.ov = kVariants[v][3] == 4 || kVariants[v][3] == 5;
     h.sgn = kVariants[v][3] == 5;
     h.cpb = 1;
@@ -2733,14 +2780,8 @@ static bool fused_tadd(const Sector &sec) {
   return sec.real_vec ? real_uses_tform(sec) : hxv_shape(sec).tform;
 }
 
-// Tile edge of k_tadd_update for the sector's element (tr_tile).
-static int tadd_edge(const Sector &sec) {
-  const size_t rsz = sec.precision == VCA_FP64 ? sizeof(double) : sizeof(float);
-  return (sec.real_vec ? rsz : 2 * rsz) > 8 ? 32 : kTile;
-}
-
 static int tadd_tiles(const Sector &sec) {
-  const int t = tadd_edge(sec);
+  const int t = kTile;
   return ((sec.dw.dim + t - 1) / t) * ((sec.up.dim + t - 1) / t);
 }
 
@@ -2887,8 +2928,8 @@ static bool nt_streams(const Sector &sec, int nb) {
 
 template <typename E>
 static void launch_transpose(Sector &sec, const TrArgs &t, bool add, int nb) {
-  constexpr int T = tr_tile<E>();
-  const dim3 g((unsigned)((t.ni + T - 1) / T), (unsigned)((t.nj + T - 1) / T), (unsigned)nb);
+  constexpr int T = kTile;
+  const dim3 g((unsigned)(((t.ni + T - 1) / T) * ((t.nj + T - 1) / T)), 1, (unsigned)nb);
   if (add) hipLaunchKernelGGL((k_transpose<E, true>), g, dim3(kBlock), 0, sec.stream, t);
   else hipLaunchKernelGGL((k_transpose<E, false>), g, dim3(kBlock), 0, sec.stream, t);
   VCA_HIP(hipGetLastError());
@@ -2910,11 +2951,19 @@ static void launch_plane(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) 
 }
 
 // One gather pass of the transposed form: k_hxv_buf on whole complex
-// columns, or (h.plane) k_hxv_plane on the two real planes of the columns.
+// columns, (h.plane) k_hxv_plane on the two real planes of the columns, or
+// (h.cpair) k_hxv_plane2<CPLX> on whole complex columns through the
+// column-pair table.
 template <typename R>
 static void launch_buf_pass(Sector &sec, const HxvArgs &a, const HxvShape &h, bool real_amp,
                             dim3 grid) {
-  if (h.plane) {
+  if (h.cpair) {
+    constexpr int rsz = (int)sizeof(R);
+    const size_t smem = (size_t)buf_lds(a.namp_up, 0, 0, rsz).col +
+                        2 * (size_t)align16(a.dimup * rsz);
+    if (h.bs == 1024) launch_plane2<R, 1024, true>(sec, a, grid, smem);
+    else launch_plane2<R, 512, true>(sec, a, grid, smem);
+  } else if (h.plane) {
     const size_t smem = (size_t)buf_lds(a.namp_up, a.dimup, 0, (int)sizeof(R)).total;
     if (h.bs == 1024) launch_plane<R, 2, 1024>(sec, a, grid, smem);
     else launch_plane<R, 2, 512>(sec, a, grid, smem);
@@ -2942,6 +2991,7 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   a.skip_diag = 0;
   a.ncb = sec.dw.dim;
   if (h.plane) a.ent4_up = sec.up.ent4p;
+  if (h.cpair) a.ent4_up = sec.up.ent4q;
   launch_buf_pass<R>(sec, a, h, real_amp, dim3(sec.dw.dim, nb));
   // 2. vt = v^T
   TrArgs t{};
@@ -2965,7 +3015,7 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   b.cnt_up = sec.dw.cnt;
   b.ent_up = sec.dw.ent;
   b.meta_up = sec.dw.meta;
-  b.ent4_up = h.plane ? sec.dw.ent4p : sec.dw.ent4;
+  b.ent4_up = h.cpair ? sec.dw.ent4q : h.plane ? sec.dw.ent4p : sec.dw.ent4;
   b.up_nc = (sec.dw.maxnnz + 3) / 4;
   b.amp_up = fp64 ? (const void *)sec.dw.amp64 : (const void *)sec.dw.amp32;
   b.namp_up = 2 * sec.dw.nterms;
@@ -2992,10 +3042,10 @@ static void launch_tform_t(Sector &sec, HxvArgs a, const HxvShape &h, bool real_
   }
 }
 
-template <typename R>
+template <typename R, int BS = 1024, bool CPLX = false>
 static void launch_plane2(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem) {
   static std::once_flag attr_once;
-  auto fn = k_hxv_plane2<R, 1024>;
+  auto fn = k_hxv_plane2<R, BS, CPLX>;
   std::call_once(attr_once, [&]() {
     VCA_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(fn),
                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBig));
@@ -3004,7 +3054,7 @@ static void launch_plane2(Sector &sec, const HxvArgs &a, dim3 grid, size_t smem)
     if (fa.sharedSizeBytes != 0)
       fail(VCA_ERR_HIP, "k_hxv_plane2 has %zu bytes of static LDS", (size_t)fa.sharedSizeBytes);
   });
-  hipLaunchKernelGGL(fn, grid, dim3(1024), smem, sec.stream, a);
+  hipLaunchKernelGGL(fn, grid, dim3(BS), smem, sec.stream, a);
 }
 
 // One real-vector gather pass on grid workgroups per chain: one column per
@@ -3352,9 +3402,7 @@ void launch_lanczos_update(Sector &sec, bool accumulate_x, int nb) {
     const int nblk = tadd_tiles(sec);
     ensure_partials(sec, nblk * nb);
     t.partials = sec.partials;
-    const int te = tadd_edge(sec);
-    const dim3 g((unsigned)((sec.dw.dim + te - 1) / te), (unsigned)((sec.up.dim + te - 1) / te),
-                 (unsigned)nb);
+    const dim3 g((unsigned)nblk, 1, (unsigned)nb);
     const bool fp64 = sec.precision == VCA_FP64;
     if (sec.real_vec && fp64)
       hipLaunchKernelGGL((k_tadd_update<double, double>), g, dim3(kBlock), 0, sec.stream, t);
