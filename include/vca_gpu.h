@@ -149,7 +149,9 @@ typedef struct vca_results {
   int32_t finite_t;      /* 1 when lanc_nstates_total > 1                 */
   int32_t nmax;          /* finite T: Matsubara terms summed (0..nmax)    */
   int32_t spectrum_iters;/* finite T: spectrum annealing iterations       */
-  int32_t reserved;      /* alignment                                     */
+  int32_t nchains_derived; /* c_i - i c_j channels of a real H and real
+                              state taken from their two diagonal chains,
+                              not run (counted apart from nchains)      */
   int64_t npoles;        /* Lehmann poles stored, all elements            */
   int64_t grid_pole_evals; /* pole x frequency terms of vca_get_gf_grid   */
   double grid_seconds;   /* GPU time of those pole sums (events)          */
@@ -262,6 +264,18 @@ int vca_lanczos_eigs(vca_sector *sec, int32_t nev, int32_t ncv, double tol,
  * normalised), nlanc steps: alpha[nlanc], beta[nlanc] (beta[0] = 0). */
 int vca_lanczos_tridiag(vca_sector *sec, const double *v0, int32_t nlanc,
                         double *alpha, double *beta);
+/* Host only (no GPU call).  The nlanc-step tridiagonal of the sum of two
+ * chains' spectral measures: chain i has start norm ni and coefficients
+ * ai[nlanc], bi[nlanc] (bi[0] = 0) as vca_lanczos_tridiag returns them.
+ * For a real H and real a, b this is the chain of the start vector a + i b
+ * (or a - i b), which is how vca_solve takes the (c^+_i + i c^+_j) and
+ * (c_i - i c_j) channels (VCA_GF_NORMAL.f90:121-318, 561-958) of a real
+ * Hamiltonian from the c^+_i, c^+_j (c_i, c_j) chains instead of running
+ * them.  No counterpart in the reference, which runs every chain.
+ * alpha[nlanc], beta[nlanc] (beta[0] = 0); *nrm = sqrt(n1^2 + n2^2). */
+int vca_lanczos_of_sum(int32_t nlanc, const double *a1, const double *b1, double n1,
+                       const double *a2, const double *b2, double n2, double *alpha,
+                       double *beta, double *nrm);
 
 /* ---- after vca_solve: Green's function and Omega integrand ---------- */
 /* Lehmann data of element (ilat,jlat,ispin,iorb,jorb) (1-based), state and

@@ -73,8 +73,12 @@ def test_chain_set_matches_per_sector_batches(name, monkeypatch):
         res = s.results()
         out[mode] = (om, res, [s.vca_gf_cluster(z) for z in ZS])
     (om_a, ra, ga), (om_b, rb, gb) = out["per_sector"], out["chain_set"]
-    assert ra["nchains"] == rb["nchains"]
-    assert ra["hxv_count"] == rb["hxv_count"]
+    # the chain set runs real-vector chains on any sector shape, the
+    # per-sector batches only where the real H x v fits: the c_i - i c_j
+    # channels taken from diagonal chains may differ between the two
+    assert ra["nchains"] + ra["nchains_derived"] == rb["nchains"] + rb["nchains_derived"]
+    if ra["nchains_derived"] == rb["nchains_derived"]:
+        assert ra["hxv_count"] == rb["hxv_count"]
     assert ra["omegaprime"] == rb["omegaprime"]
     bar = 1e-10 if p.precision == "fp64" else 1e-5
     for z, x, y in zip(ZS, ga, gb):

@@ -63,7 +63,7 @@ class CResults(ctypes.Structure):
         ("omega_evals", ctypes.c_int64), ("omega_seconds", ctypes.c_double),
         ("zeta", ctypes.c_double), ("max_exc", ctypes.c_double),
         ("finite_t", ctypes.c_int32), ("nmax", ctypes.c_int32),
-        ("spectrum_iters", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("spectrum_iters", ctypes.c_int32), ("nchains_derived", ctypes.c_int32),
         ("npoles", ctypes.c_int64), ("grid_pole_evals", ctypes.c_int64),
         ("grid_seconds", ctypes.c_double),
     ]
@@ -103,6 +103,8 @@ _SIGS = {
     "vca_lanczos_eigs": (ctypes.c_int, [_P, _I32, _I32, ctypes.c_double, _I32, _P, _P,
                                         ctypes.POINTER(_I32)]),
     "vca_lanczos_tridiag": (ctypes.c_int, [_P, _P, _I32, _P, _P]),
+    "vca_lanczos_of_sum": (ctypes.c_int, [_I32, _P, _P, ctypes.c_double, _P, _P,
+                                          ctypes.c_double, _P, _P, _D]),
     "vca_gf_poles": (ctypes.c_int, [_P, _I32, _I32, _I32, _I32, _I32, _I32, _I32,
                                     ctypes.POINTER(_I32), _P, _P]),
     "vca_gf_cluster": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_double, _P]),

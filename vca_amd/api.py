@@ -228,6 +228,25 @@ class Sector:
         return alpha, beta
 
 
+def lanczos_of_sum(a1, b1, n1, a2, b2, n2):
+    """The n-step Lanczos tridiagonal of the sum of two chains' spectral
+    measures (host only): for a real H and real a, b, the chain of a + i b
+    from the chains of a and b (norms n1, n2; beta[0] = 0).  Returns
+    (alpha, beta, norm)."""
+    a1, b1, a2, b2 = (np.ascontiguousarray(x, dtype=np.float64) for x in (a1, b1, a2, b2))
+    n = len(a1)
+    if not (len(b1) == len(a2) == len(b2) == n):
+        raise ValueError("the two chains need the same length")
+    alpha = np.zeros(n)
+    beta = np.zeros(n)
+    nrm = ctypes.c_double()
+    check(_lib.load().vca_lanczos_of_sum(n, a1.ctypes.data, b1.ctypes.data, float(n1),
+                                         a2.ctypes.data, b2.ctypes.data, float(n2),
+                                         alpha.ctypes.data, beta.ctypes.data,
+                                         ctypes.byref(nrm)))
+    return alpha, beta, nrm.value
+
+
 def device_synchronize():
     """Waits for all work the library queued on the device."""
     check(_lib.load().vca_device_synchronize())

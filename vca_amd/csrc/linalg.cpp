@@ -113,6 +113,78 @@ void tridiag_eig_first(int n, const double *d, const double *e, double *w, doubl
   }
 }
 
+// The n-step Lanczos tridiagonal of the sum of two spectral measures, each
+// given by an n-step chain: chain i has start norm ni and coefficients
+// (ai[0..n), bi[1..n)), i.e. the n-node Gauss quadrature of its measure
+// ni^2 <u|delta(x - H)|u>.  The Lanczos coefficients alpha_0..alpha_{n-1},
+// beta_1..beta_{n-1} of a measure depend only on its moments 0..2n-1, and an
+// n-node Gauss quadrature reproduces those exactly, so the union of the two
+// quadratures (2n nodes) has the coefficients of the sum measure itself:
+// with a real H and real a, b, the measure of a + i b (or a - i b) is the
+// sum of those of a and b (the cross terms cancel, (z - H)^-1 being
+// symmetric), so the chain of the start vector a - i b is the chain of this
+// sum.  The reduction is Lanczos on diag(nodes) from the square roots of the
+// normalised weights, with full reorthogonalisation (twice), in FP64.  An
+// exhausted measure (beta below 1e-13 of the node range) ends the chain as
+// the device Lanczos does on an exact breakdown: the remaining alpha and
+// beta are 0.  Returns the start norm sqrt(n1^2 + n2^2); al and be receive
+// n values each (be[0] = 0).
+double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, const double *a2,
+                      const double *b2, double n2, double *al, double *be) {
+  const int m = 2 * n;
+  for (int k = 0; k < n; ++k) {
+    al[k] = 0.0;
+    be[k] = 0.0;
+  }
+  if (!(n1 * n1 + n2 * n2 > 0.0)) return 0.0;  // both empty
+  std::vector<double> x(m), q(m);
+  tridiag_eig_first(n, a1, b1, x.data(), q.data());
+  tridiag_eig_first(n, a2, b2, x.data() + n, q.data() + n);
+  const double w1 = n1 * n1, w2 = n2 * n2, wt = w1 + w2;
+  for (int i = 0; i < n; ++i) {
+    q[i] = std::sqrt(w1 / wt) * std::fabs(q[i]);
+    q[n + i] = std::sqrt(w2 / wt) * std::fabs(q[n + i]);
+  }
+  double xr = 0.0;
+  for (int i = 0; i < m; ++i) xr = std::max(xr, std::fabs(x[i]));
+  std::vector<double> v((size_t)n * m, 0.0), w(m);
+  // the start vector: unit norm up to the quadratures' rounding
+  double s0 = 0.0;
+  for (int i = 0; i < m; ++i) s0 += q[i] * q[i];
+  s0 = std::sqrt(s0);
+  for (int i = 0; i < m; ++i) v[i] = q[i] / s0;
+  for (int k = 0; k < n; ++k) {
+    const double *vk = v.data() + (size_t)k * m;
+    double a = 0.0;
+    for (int i = 0; i < m; ++i) {
+      w[i] = x[i] * vk[i];
+      a += vk[i] * w[i];
+    }
+    al[k] = a;
+    if (k + 1 == n) break;
+    for (int i = 0; i < m; ++i) w[i] -= a * vk[i];
+    if (k > 0) {
+      const double *vp = vk - m;  // v_{k-1}
+      for (int i = 0; i < m; ++i) w[i] -= be[k] * vp[i];
+    }
+    for (int pass = 0; pass < 2; ++pass)
+      for (int j = 0; j <= k; ++j) {
+        const double *vj = v.data() + (size_t)j * m;
+        double d = 0.0;
+        for (int i = 0; i < m; ++i) d += vj[i] * w[i];
+        for (int i = 0; i < m; ++i) w[i] -= d * vj[i];
+      }
+    double b = 0.0;
+    for (int i = 0; i < m; ++i) b += w[i] * w[i];
+    b = std::sqrt(b);
+    if (!(b > 1e-13 * xr)) break;  // exhausted: the rest stays 0
+    be[k + 1] = b;
+    double *vn = v.data() + (size_t)(k + 1) * m;
+    for (int i = 0; i < m; ++i) vn[i] = w[i] / b;
+  }
+  return std::sqrt(wt);
+}
+
 double tridiag_lowest(int m, const double *d, const double *e) {
   std::vector<double> w(m);
   tridiag_eig(m, d, e, w.data(), nullptr);

@@ -1451,11 +1451,22 @@ static int64_t chain_set_max_dim() {
   return e && *e ? std::atoll(e) : ((int64_t)1 << 17);
 }
 
+// Precision of the excitation chains: the solver's, or VCA_CHAIN_PRECISION
+// = fp64 / mixed (a MIXED solve with FP64 chains keeps its FP32 ground-state
+// Lanczos and FP32 log-dets, and its G' the accuracy of FP64 chains).
+static int chain_precision(int prec) {
+  const char *e = std::getenv("VCA_CHAIN_PRECISION");
+  if (!e || !*e) return prec;
+  if (std::strcmp(e, "fp64") == 0) return VCA_FP64;
+  if (std::strcmp(e, "mixed") == 0) return VCA_MIXED;
+  fail(VCA_ERR_ARG, "VCA_CHAIN_PRECISION=%s: fp64 or mixed", e);
+}
+
 static void build_gf(Solver &s) {
   const vca_params &p = s.p;
   s.gf.clear();
   s.max_exc = -std::numeric_limits<double>::infinity();
-  const int prec = p.precision;
+  const int prec = chain_precision(p.precision);
   std::map<std::array<int, 2>, std::unique_ptr<Sector>> cache;
   auto get_sector = [&](int nup, int ndw) -> Sector & {
     auto &slot = cache[{nup, ndw}];
@@ -1494,10 +1505,48 @@ static void build_gf(Solver &s) {
     }
     const std::vector<ChainSpec> specs = chain_list(s, st);
     std::vector<GfChan> res(specs.size());
+    const int64_t cs_max = chain_set_max_dim();
+    auto target_ok = [&](const ChainSpec &cs) {
+      return cs.jnup >= 0 && cs.jndw >= 0 && cs.jnup <= s.ns && cs.jndw <= s.ns;
+    };
+    // Channels 3 and 4, (c^+_i + i c^+_j) and (c_i - i c_j), of a real
+    // Hamiltonian and a real state: their start vector is a + i b (a - i b)
+    // with a, b the real start vectors of the diagonal chains of i and j in
+    // the same channel and target sector, and its spectral measure is the
+    // sum of theirs (the cross terms cancel: (z - H)^-1 is symmetric), so
+    // the chain's n-step tridiagonal is lanczos_of_sum of those two chains
+    // (equal to the chain run from a + i b up to rounding).  Taken so where
+    // the diagonal chains run on real vectors and the target sector is
+    // larger than the chain (no exhausted Krylov space);
+    // VCA_DERIVED_CHANNELS=0 runs every chain.
+    std::vector<std::array<int, 2>> derived(specs.size(), {-1, -1});
+    std::vector<char> keep(specs.size(), 0);  // diagonal chains the derived ones read
+    {
+      const char *dc = std::getenv("VCA_DERIVED_CHANNELS");
+      if (psi_r && !(dc && dc[0] == '0')) {
+        std::map<std::array<int, 3>, int> diag;  // (spin, channel, position) -> chain
+        for (size_t c = 0; c < specs.size(); ++c)
+          if (specs[c].op.nops == 1) diag[{specs[c].spin, specs[c].ichan, specs[c].op.pos[0]}] = (int)c;
+        for (size_t c = 0; c < specs.size(); ++c) {
+          const ChainSpec &cs = specs[c];
+          if (cs.ichan < 3 || cs.op.nops != 2 || !target_ok(cs)) continue;
+          const auto ia = diag.find({cs.spin, cs.ichan - 2, cs.op.pos[0]});
+          const auto ib = diag.find({cs.spin, cs.ichan - 2, cs.op.pos[1]});
+          if (ia == diag.end() || ib == diag.end()) continue;
+          Sector &js = get_sector(cs.jnup, cs.jndw);
+          if (js.dim <= (int64_t)p.lanc_ngfiter) continue;
+          if (js.dim >= cs_max && !real_mode_ok(js)) continue;  // complex diagonal chains
+          derived[c] = {ia->second, ib->second};
+          keep[ia->second] = keep[ib->second] = 1;
+        }
+      }
+    }
+    std::vector<std::vector<double>> kept_al(specs.size()), kept_be(specs.size());
+    std::vector<double> kept_nrm(specs.size(), 0.0);
     std::map<std::array<int, 2>, std::vector<int>> groups;
     for (size_t c = 0; c < specs.size(); ++c) {
       const ChainSpec &cs = specs[c];
-      if (cs.jnup < 0 || cs.jndw < 0 || cs.jnup > s.ns || cs.jndw > s.ns) continue;
+      if (!target_ok(cs) || derived[c][0] >= 0) continue;
       groups[{cs.jnup, cs.jndw}].push_back((int)c);
     }
     // pesoBZ (VCA_GF_NORMAL.f90:987-994): Boltzmann factor at finite T
@@ -1512,6 +1561,11 @@ static void build_gf(Solver &s) {
     auto post_chain = [&](int c, int nlanc, const double *al, const double *be,
                           double nrm0) -> double {
       const ChainSpec &cs = specs[c];
+      if (keep[c]) {
+        kept_al[c].assign(al, al + nlanc);
+        kept_be[c].assign(be, be + nlanc);
+        kept_nrm[c] = nrm0;
+      }
       const double norm2 = nrm0 * nrm0;
       double cmax = -std::numeric_limits<double>::infinity();
       if (!(norm2 > 0.0)) return cmax;  // O|psi> = 0: empty channel
@@ -1534,7 +1588,6 @@ static void build_gf(Solver &s) {
     // memory per run
     std::vector<ChainJob> jobs;
     std::vector<int> job_spec;
-    const int64_t cs_max = chain_set_max_dim();
     const bool real_ham = psi_r != nullptr;
     for (auto it = groups.begin(); it != groups.end();) {
       Sector &js = get_sector(it->first[0], it->first[1]);
@@ -1645,6 +1698,25 @@ static void build_gf(Solver &s) {
                  idx.size(), nlanc);
       }
       }  // the complex and the real-vector part
+    }
+    std::vector<int> dlist;
+    for (size_t c = 0; c < specs.size(); ++c)
+      if (derived[c][0] >= 0) dlist.push_back((int)c);
+    if (!dlist.empty()) {
+      std::vector<double> cmax(dlist.size());
+      parallel_for((int)dlist.size(), [&](int q) {
+        const int c = dlist[q], a = derived[c][0], b = derived[c][1];
+        const int n = (int)std::min(kept_al[a].size(), kept_al[b].size());
+        std::vector<double> al(n), be(n);
+        const double nrm = lanczos_of_sum(n, kept_al[a].data(), kept_be[a].data(), kept_nrm[a],
+                                          kept_al[b].data(), kept_be[b].data(), kept_nrm[b],
+                                          al.data(), be.data());
+        cmax[q] = post_chain(c, n, al.data(), be.data(), nrm);
+      });
+      for (double c : cmax) s.max_exc = std::max(s.max_exc, c);
+      s.res.nchains_derived += (int)dlist.size();
+      progress(s.t_start, "state %zu: %zu c_i - i c_j channels from their diagonal chains", ist,
+               dlist.size());
     }
     for (size_t c = 0; c < specs.size(); ++c) {
       auto &el = s.gf[specs[c].key];
@@ -2454,6 +2526,17 @@ int vca_lanczos_eigs(vca_sector *sv, int32_t nev, int32_t ncv, double tol, int32
     dev_free_async(vecs[i], s64->stream);
   }
   if (nconv) *nconv = o.nconv;
+  VCA_GUARD_END
+}
+
+int vca_lanczos_of_sum(int32_t nlanc, const double *a1, const double *b1, double n1,
+                       const double *a2, const double *b2, double n2, double *alpha,
+                       double *beta, double *nrm) {
+  VCA_GUARD_BEGIN
+  if (nlanc < 1 || !a1 || !b1 || !a2 || !b2 || !alpha || !beta || !nrm || !(n1 >= 0.0) ||
+      !(n2 >= 0.0))
+    fail(VCA_ERR_ARG, "bad argument");
+  *nrm = lanczos_of_sum(nlanc, a1, b1, n1, a2, b2, n2, alpha, beta);
   VCA_GUARD_END
 }
 

@@ -414,6 +414,8 @@ void tridiag_eig(int n, const double *d, const double *e, double *w, double *z);
 // Eigenvalues (ascending) and the first component of each eigenvector,
 // z0[j] = Z(0, j): what the continued-fraction weights need, in O(n^2).
 void tridiag_eig_first(int n, const double *d, const double *e, double *w, double *z0);
+double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, const double *a2,
+                      const double *b2, double n2, double *al, double *be);
 // Lowest eigenvalue of the leading m x m block.
 double tridiag_lowest(int m, const double *d, const double *e);
 // Complex Hermitian eigenproblem (column-major a, n x n, overwritten by the

@@ -70,7 +70,7 @@ module vca_gpu
      real(c_double)     :: hxv_states, hxv_seconds, t_diag, t_gf, t_omega
      integer(c_int64_t) :: omega_evals
      real(c_double)     :: omega_seconds, zeta, max_exc
-     integer(c_int32_t) :: finite_t, nmax, spectrum_iters, reserved
+     integer(c_int32_t) :: finite_t, nmax, spectrum_iters, nchains_derived
      integer(c_int64_t) :: npoles, grid_pole_evals
      real(c_double)     :: grid_seconds
   end type vca_results_t
