@@ -267,12 +267,16 @@ C4_FORMS = {
     # the plain FP64 GPU form: complex128 vectors everywhere, one chain per
     # launch, every Lanczos step launched eagerly (no graphs)
     "plain": ("fp64", {"VCA_CHAIN_BATCH": "1", "VCA_NO_GRAPHS": "1", "VCA_REAL_CHAINS": "0",
-                       "VCA_REAL_GS": "0"}),
+                       "VCA_REAL_GS": "0", "VCA_DERIVED_CHANNELS": "0"}),
     # FP64 with the batching form: chains of a target sector in one launch,
-    # 16-step graphs, real vectors where the operator and state are real
+    # 16-step graphs, real vectors where the operator and state are real,
+    # the c_i - i c_j channels from the diagonal chains
     "fp64": ("fp64", {}),
-    # the batched mixed-precision form
+    # the batched mixed-precision form: FP32 ground-state Lanczos refined in
+    # FP64, FP32 chains, FP32 log-dets on the matrix cores
     "mixed": ("mixed", {}),
+    # the same with FP64 chains (G' at FP64 accuracy)
+    "mixed_fp64_chains": ("mixed", {"VCA_CHAIN_PRECISION": "fp64"}),
 }
 
 
@@ -303,7 +307,8 @@ def run_c4_solve(form, save=None):
                        "16x16 k-grid",
            "form": form, "precision": prec, "knobs": env, "seconds": secs, "sft": om,
            **{k: r[k] for k in ("omegaprime", "omega_integral", "e0_mixed", "nstates",
-                                "nchains", "hxv_count", "t_diag", "t_gf", "t_omega",
+                                "nchains", "nchains_derived", "hxv_count", "t_diag", "t_gf",
+                                "t_omega",
                                 "omega_evals")},
            "hxv_gstates_per_s": r["hxv_states"] / max(r["hxv_seconds"], 1e-30) / 1e9}
     log(f"C4 solve {form}: {secs:.1f} s (diag {r['t_diag']:.1f}, chains {r['t_gf']:.1f}, Omega "
