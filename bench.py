@@ -277,6 +277,9 @@ C4_FORMS = {
     "mixed": ("mixed", {}),
     # the same with FP64 chains (G' at FP64 accuracy)
     "mixed_fp64_chains": ("mixed", {"VCA_CHAIN_PRECISION": "fp64"}),
+    # FP32 only in the ground-state Lanczos (refined in FP64): FP64 chains
+    # and FP64 log-dets
+    "mixed_gs": ("mixed", {"VCA_CHAIN_PRECISION": "fp64", "VCA_OMEGA_PRECISION": "fp64"}),
 }
 
 

@@ -2019,22 +2019,33 @@ static double omega_finite_t(Solver &s, int prec) {
   return (3.0 - p.nspin) * (sum + contour);
 }
 
+// Precision of the Omega log-dets in vca_solve: the solver's, or
+// VCA_OMEGA_PRECISION = fp64 / mixed.
+static int omega_precision(int prec) {
+  const char *e = std::getenv("VCA_OMEGA_PRECISION");
+  if (!e || !*e) return prec;
+  if (std::strcmp(e, "fp64") == 0) return VCA_FP64;
+  if (std::strcmp(e, "mixed") == 0) return VCA_MIXED;
+  fail(VCA_ERR_ARG, "VCA_OMEGA_PRECISION=%s: fp64 or mixed", e);
+}
+
 static void omega_stage(Solver &s) {
   const vca_params &p = s.p;
   prepare_omega(s);
+  const int prec = omega_precision(p.precision);
   double integral;
   if (finite_t(p)) {
-    integral = omega_finite_t(s, p.precision);
+    integral = omega_finite_t(s, prec);
   } else if (p.omega_mode == VCA_OMEGA_MATSUBARA) {
     std::vector<double2> z(p.omega_nfreq);
     for (int n = 0; n < p.omega_nfreq; ++n)
       z[n] = make_double2(0.0, (2.0 * n + 1.0) * M_PI / p.beta);
-    std::vector<double> f = integrand(s, z, p.precision);
+    std::vector<double> f = integrand(s, z, prec);
     double sum = 0;
     for (double v : f) sum += v;
     integral = (3.0 - p.nspin) * 2.0 / p.beta * sum;
   } else {
-    integral = (3.0 - p.nspin) * quad_imag_axis(s, p.precision) / M_PI;
+    integral = (3.0 - p.nspin) * quad_imag_axis(s, prec) / M_PI;
   }
   s.res.omega_integral = integral;
   s.res.sft_potential = s.res.omegaprime - integral;

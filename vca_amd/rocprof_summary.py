@@ -1,10 +1,11 @@
 """Summaries of rocprofv3 result databases (kernel trace and PMC passes).
 
-    python -m vca_amd.rocprof_summary trace  TRACE.db
+    python -m vca_amd.rocprof_summary trace  TRACE.db [ROWS]
     python -m vca_amd.rocprof_summary pmc    FETCH.db WRITE.db KERNELS COPY_BYTES OUT.json KEY VARIANT
     python -m vca_amd.rocprof_summary counters PMC.db [KERNEL_SUBSTRING]
 
-`trace` prints the per-kernel table (calls, average and total duration).
+`trace` prints the per-kernel table (calls, average and total duration),
+the ROWS (default 14) kernels of largest total time.
 `counters` prints, per kernel (those whose name holds KERNEL_SUBSTRING),
 the dispatch count and the average per dispatch of every counter of a PMC
 pass.
@@ -103,7 +104,7 @@ def counter_table(db: str, sub: str = "") -> str:
 
 def main(argv):
     if len(argv) >= 2 and argv[0] == "trace":
-        print(trace_table(argv[1]))
+        print(trace_table(argv[1], int(argv[2]) if len(argv) > 2 else 14))
     elif len(argv) >= 2 and argv[0] == "counters":
         print(counter_table(argv[1], argv[2] if len(argv) > 2 else ""))
     elif len(argv) >= 8 and argv[0] == "pmc":
