@@ -123,12 +123,19 @@ void tridiag_eig_first(int n, const double *d, const double *e, double *w, doubl
 // with a real H and real a, b, the measure of a + i b (or a - i b) is the
 // sum of those of a and b (the cross terms cancel, (z - H)^-1 being
 // symmetric), so the chain of the start vector a - i b is the chain of this
-// sum.  The reduction is Lanczos on diag(nodes) from the square roots of the
-// normalised weights, with full reorthogonalisation (twice), in FP64.  An
-// exhausted measure (beta below 1e-13 of the node range) ends the chain as
-// the device Lanczos does on an exact breakdown: the remaining alpha and
-// beta are 0.  Returns the start norm sqrt(n1^2 + n2^2); al and be receive
-// n values each (be[0] = 0).
+// sum.
+//
+// The tridiagonal of the 2n-node measure is rebuilt from its nodes and
+// weights by the Rutishauser-Kahan-Pal-Walker update (W. B. Gragg and W. J.
+// Harrod, Numer. Math. 44 (1984) 317; W. Gautschi, Orthogonal Polynomials:
+// Computation and Approximation, OUP 2004, sec. 2.2.3): the nodes enter one
+// at a time, each folded into the Jacobi matrix of the previous ones by a
+// sweep of plane rotations, O(m^2) for m nodes and stable (the same
+// coefficients as Lanczos with full reorthogonalisation, to rounding).  An
+// exhausted measure (a coefficient beta below 1e-13 of the node range) ends
+// the chain as the device Lanczos does on an exact breakdown: the remaining
+// alpha and beta are 0.  Returns the start norm sqrt(n1^2 + n2^2); al and be
+// receive n values each (be[0] = 0).
 double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, const double *a2,
                       const double *b2, double n2, double *al, double *be) {
   const int m = 2 * n;
@@ -141,46 +148,44 @@ double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, cons
   tridiag_eig_first(n, a1, b1, x.data(), q.data());
   tridiag_eig_first(n, a2, b2, x.data() + n, q.data() + n);
   const double w1 = n1 * n1, w2 = n2 * n2, wt = w1 + w2;
-  for (int i = 0; i < n; ++i) {
-    q[i] = std::sqrt(w1 / wt) * std::fabs(q[i]);
-    q[n + i] = std::sqrt(w2 / wt) * std::fabs(q[n + i]);
+  double xr = 0.0, qs = 0.0;
+  for (int i = 0; i < m; ++i) {
+    q[i] = q[i] * q[i] * (i < n ? w1 : w2);  // the node's weight
+    qs += q[i];
+    xr = std::max(xr, std::fabs(x[i]));
   }
-  double xr = 0.0;
-  for (int i = 0; i < m; ++i) xr = std::max(xr, std::fabs(x[i]));
-  std::vector<double> v((size_t)n * m, 0.0), w(m);
-  // the start vector: unit norm up to the quadratures' rounding
-  double s0 = 0.0;
-  for (int i = 0; i < m; ++i) s0 += q[i] * q[i];
-  s0 = std::sqrt(s0);
-  for (int i = 0; i < m; ++i) v[i] = q[i] / s0;
-  for (int k = 0; k < n; ++k) {
-    const double *vk = v.data() + (size_t)k * m;
-    double a = 0.0;
-    for (int i = 0; i < m; ++i) {
-      w[i] = x[i] * vk[i];
-      a += vk[i] * w[i];
-    }
-    al[k] = a;
-    if (k + 1 == n) break;
-    for (int i = 0; i < m; ++i) w[i] -= a * vk[i];
-    if (k > 0) {
-      const double *vp = vk - m;  // v_{k-1}
-      for (int i = 0; i < m; ++i) w[i] -= be[k] * vp[i];
-    }
-    for (int pass = 0; pass < 2; ++pass)
-      for (int j = 0; j <= k; ++j) {
-        const double *vj = v.data() + (size_t)j * m;
-        double d = 0.0;
-        for (int i = 0; i < m; ++i) d += vj[i] * w[i];
-        for (int i = 0; i < m; ++i) w[i] -= d * vj[i];
+  // RKPW: d = diagonal, g = squared off-diagonals (g[0]: the total weight)
+  std::vector<double> d(m, 0.0), g(m, 0.0);
+  d[0] = x[0];
+  g[0] = q[0] / qs;
+  for (int j = 1; j < m; ++j) {
+    double wn = q[j] / qs, c = 1.0, sgm = 0.0, t = 0.0;
+    const double lam = x[j];
+    for (int k = 0; k <= j; ++k) {
+      const double rho = g[k] + wn;
+      const double gk = c * rho;
+      const double sgm_prev = sgm;
+      if (rho <= 0.0) {
+        c = 1.0;
+        sgm = 0.0;
+      } else {
+        c = g[k] / rho;
+        sgm = wn / rho;
       }
-    double b = 0.0;
-    for (int i = 0; i < m; ++i) b += w[i] * w[i];
-    b = std::sqrt(b);
-    if (!(b > 1e-13 * xr)) break;  // exhausted: the rest stays 0
-    be[k + 1] = b;
-    double *vn = v.data() + (size_t)(k + 1) * m;
-    for (int i = 0; i < m; ++i) vn[i] = w[i] / b;
+      const double tk = sgm * (d[k] - lam) - c * t;
+      d[k] -= tk - t;
+      t = tk;
+      wn = sgm <= 0.0 ? sgm_prev * g[k] : t * t / sgm;
+      g[k] = gk;
+    }
+  }
+  for (int k = 0; k < n; ++k) {
+    if (k > 0) {
+      const double b = std::sqrt(std::max(g[k], 0.0));
+      if (!(b > 1e-13 * xr)) break;  // exhausted: the rest stays 0
+      be[k] = b;
+    }
+    al[k] = d[k];
   }
   return std::sqrt(wt);
 }
