@@ -135,22 +135,34 @@ void tridiag_eig_first(int n, const double *d, const double *e, double *w, doubl
 // exhausted measure (a coefficient beta below 1e-13 of the node range) ends
 // the chain as the device Lanczos does on an exact breakdown: the remaining
 // alpha and beta are 0.  Returns the start norm sqrt(n1^2 + n2^2); al and be
-// receive n values each (be[0] = 0).
-double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, const double *a2,
-                      const double *b2, double n2, double *al, double *be) {
-  const int m = 2 * n;
+// receive n values each (be[0] = 0).  lanczos_of_quadratures takes the two
+// quadratures themselves (nodes x, first eigenvector components z: weights
+// z^2, as tridiag_eig_first returns them); lanczos_of_sum the chains.
+double lanczos_of_quadratures(int n, const double *x1, const double *z1, double n1,
+                              const double *x2, const double *z2, double n2, double *al,
+                              double *be) {
   for (int k = 0; k < n; ++k) {
     al[k] = 0.0;
     be[k] = 0.0;
   }
   if (!(n1 * n1 + n2 * n2 > 0.0)) return 0.0;  // both empty
-  std::vector<double> x(m), q(m);
-  tridiag_eig_first(n, a1, b1, x.data(), q.data());
-  tridiag_eig_first(n, a2, b2, x.data() + n, q.data() + n);
-  const double w1 = n1 * n1, w2 = n2 * n2, wt = w1 + w2;
+  // the nodes of the non-empty quadratures, with their weights
+  std::vector<double> x, q;
+  x.reserve(2 * n);
+  q.reserve(2 * n);
+  for (int c = 0; c < 2; ++c) {
+    const double nc = c ? n2 : n1;
+    const double *xc = c ? x2 : x1, *zc = c ? z2 : z1;
+    if (!(nc > 0.0)) continue;
+    for (int i = 0; i < n; ++i) {
+      x.push_back(xc[i]);
+      q.push_back(zc[i] * zc[i] * nc * nc);
+    }
+  }
+  const int m = (int)x.size();
+  const double wt = n1 * n1 + n2 * n2;
   double xr = 0.0, qs = 0.0;
   for (int i = 0; i < m; ++i) {
-    q[i] = q[i] * q[i] * (i < n ? w1 : w2);  // the node's weight
     qs += q[i];
     xr = std::max(xr, std::fabs(x[i]));
   }
@@ -179,7 +191,7 @@ double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, cons
       g[k] = gk;
     }
   }
-  for (int k = 0; k < n; ++k) {
+  for (int k = 0; k < n && k < m; ++k) {
     if (k > 0) {
       const double b = std::sqrt(std::max(g[k], 0.0));
       if (!(b > 1e-13 * xr)) break;  // exhausted: the rest stays 0
@@ -188,6 +200,14 @@ double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, cons
     al[k] = d[k];
   }
   return std::sqrt(wt);
+}
+
+double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, const double *a2,
+                      const double *b2, double n2, double *al, double *be) {
+  std::vector<double> x1(n), z1(n), x2(n), z2(n);
+  tridiag_eig_first(n, a1, b1, x1.data(), z1.data());
+  tridiag_eig_first(n, a2, b2, x2.data(), z2.data());
+  return lanczos_of_quadratures(n, x1.data(), z1.data(), n1, x2.data(), z2.data(), n2, al, be);
 }
 
 double tridiag_lowest(int m, const double *d, const double *e) {

@@ -1541,7 +1541,8 @@ static void build_gf(Solver &s) {
         }
       }
     }
-    std::vector<std::vector<double>> kept_al(specs.size()), kept_be(specs.size());
+    // the diagonal chains' quadratures (Ritz values, first components)
+    std::vector<std::vector<double>> kept_x(specs.size()), kept_z(specs.size());
     std::vector<double> kept_nrm(specs.size(), 0.0);
     std::map<std::array<int, 2>, std::vector<int>> groups;
     for (size_t c = 0; c < specs.size(); ++c) {
@@ -1561,16 +1562,16 @@ static void build_gf(Solver &s) {
     auto post_chain = [&](int c, int nlanc, const double *al, const double *be,
                           double nrm0) -> double {
       const ChainSpec &cs = specs[c];
-      if (keep[c]) {
-        kept_al[c].assign(al, al + nlanc);
-        kept_be[c].assign(be, be + nlanc);
-        kept_nrm[c] = nrm0;
-      }
       const double norm2 = nrm0 * nrm0;
       double cmax = -std::numeric_limits<double>::infinity();
       if (!(norm2 > 0.0)) return cmax;  // O|psi> = 0: empty channel
       std::vector<double> w(nlanc), z(nlanc);
       tridiag_eig_first(nlanc, al, be, w.data(), z.data());
+      if (keep[c]) {
+        kept_x[c] = w;
+        kept_z[c] = z;
+        kept_nrm[c] = nrm0;
+      }
       const zc peso_bz =
           zc(chan_re[cs.ichan - 1], chan_im[cs.ichan - 1]) * norm2 * boltz / s.zeta;
       GfChan &out = res[c];
@@ -1703,14 +1704,17 @@ static void build_gf(Solver &s) {
     for (size_t c = 0; c < specs.size(); ++c)
       if (derived[c][0] >= 0) dlist.push_back((int)c);
     if (!dlist.empty()) {
-      std::vector<double> cmax(dlist.size());
+      std::vector<double> cmax(dlist.size(), -std::numeric_limits<double>::infinity());
       parallel_for((int)dlist.size(), [&](int q) {
         const int c = dlist[q], a = derived[c][0], b = derived[c][1];
-        const int n = (int)std::min(kept_al[a].size(), kept_al[b].size());
+        // an empty diagonal chain (norm 0) keeps no quadrature
+        const int n = (int)std::max(kept_x[a].size(), kept_x[b].size());
+        if (n == 0) return;  // both empty: so is the channel
         std::vector<double> al(n), be(n);
-        const double nrm = lanczos_of_sum(n, kept_al[a].data(), kept_be[a].data(), kept_nrm[a],
-                                          kept_al[b].data(), kept_be[b].data(), kept_nrm[b],
-                                          al.data(), be.data());
+        const double nrm = lanczos_of_quadratures(n, kept_x[a].data(), kept_z[a].data(),
+                                                  kept_nrm[a], kept_x[b].data(),
+                                                  kept_z[b].data(), kept_nrm[b], al.data(),
+                                                  be.data());
         cmax[q] = post_chain(c, n, al.data(), be.data(), nrm);
       });
       for (double c : cmax) s.max_exc = std::max(s.max_exc, c);

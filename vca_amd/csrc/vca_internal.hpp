@@ -416,6 +416,9 @@ void tridiag_eig(int n, const double *d, const double *e, double *w, double *z);
 void tridiag_eig_first(int n, const double *d, const double *e, double *w, double *z0);
 double lanczos_of_sum(int n, const double *a1, const double *b1, double n1, const double *a2,
                       const double *b2, double n2, double *al, double *be);
+double lanczos_of_quadratures(int n, const double *x1, const double *z1, double n1,
+                              const double *x2, const double *z2, double n2, double *al,
+                              double *be);
 // Lowest eigenvalue of the leading m x m block.
 double tridiag_lowest(int m, const double *d, const double *e);
 // Complex Hermitian eigenproblem (column-major a, n x n, overwritten by the
