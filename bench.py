@@ -600,7 +600,7 @@ def omega_section(args, cpu):
                              "integral_rel_err": errv,
                              "note": "FP32 log-dets with the contraction on the vector ALUs "
                                      "(k_logdet<float, 8>) instead of the matrix cores "
-                                     "(k_logdet_mfma<8>)"},
+                                     "(k_logdet_lane8<8, 8>)"},
         "omega_full_solve": om64,
         "full_solve_seconds": t_solve,
         "roofline": {"flops_per_eval": fl,
