@@ -1707,8 +1707,10 @@ static void build_gf(Solver &s) {
       std::vector<double> cmax(dlist.size(), -std::numeric_limits<double>::infinity());
       parallel_for((int)dlist.size(), [&](int q) {
         const int c = dlist[q], a = derived[c][0], b = derived[c][1];
-        // an empty diagonal chain (norm 0) keeps no quadrature
-        const int n = (int)std::max(kept_x[a].size(), kept_x[b].size());
+        // an empty diagonal chain (norm 0) keeps no quadrature; the two
+        // chains of one target sector have the same length
+        const size_t na = kept_x[a].size(), nb = kept_x[b].size();
+        const int n = (int)(na && nb ? std::min(na, nb) : std::max(na, nb));
         if (n == 0) return;  // both empty: so is the channel
         std::vector<double> al(n), be(n);
         const double nrm = lanczos_of_quadratures(n, kept_x[a].data(), kept_z[a].data(),
