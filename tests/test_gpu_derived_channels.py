@@ -3,13 +3,16 @@ Hamiltonian with a real ground state, taken from the two diagonal chains of
 their target sector (vca_lanczos_of_sum) instead of run.
 
 Against the oracle, which runs every chain (VCA_GF_NORMAL.f90:121-318,
-561-958): 12-step chains on target sectors of 300 states (no loss of
-orthogonality in 12 steps, so the derived tridiagonal equals the run one to
-rounding): E0 1e-10, G'(z) and Omega 1e-8, and the poles and weights of the
-derived channels 1e-9.  Against the same solve with every chain run
-(VCA_DERIVED_CHANNELS=0), 80-step chains on 52,920-state sectors, chain
-set and per-sector batches, FP64 and MIXED: G' 1e-10 (MIXED 1e-5) of its
-largest element, Omega 1e-12 (1e-8)."""
+561-958): 12-step chains on target sectors of 300 states (a chain and a
+bath model), 4-step chains on the 24-state sectors of a 2-site, 2-orbital,
+2-spin model (no loss of orthogonality in so few steps, so the derived
+tridiagonal equals the run one to rounding): E0 1e-10, G'(z) and Omega
+1e-8, and the poles and weights of the derived channels 1e-9.
+
+Against the same solve with every chain run (VCA_DERIVED_CHANNELS=0),
+80-step chains on 52,920-state sectors, chain set and per-sector batches,
+FP64 and MIXED: G' 1e-10 (MIXED 1e-5) of its largest element, Omega 1e-12
+(1e-8)."""
 import numpy as np
 import pytest
 
@@ -27,13 +30,27 @@ def _say(*a):
     print(*a, flush=True)
 
 
-@pytest.mark.parametrize("name", ["chain6", "bath3+3"])
+ELEMS = {"chain6": [(1, 2, 1, 1, 1)], "bath3+3": [(1, 2, 1, 1, 1)],
+         # two orbitals, two spins: channels between orbitals of one site
+         # and between different sites and orbitals, in each spin
+         "multiorb_real": [(1, 1, 1, 1, 2), (1, 2, 2, 2, 1), (2, 1, 1, 1, 1)]}
+
+
+@pytest.mark.parametrize("name", ["chain6", "bath3+3", "multiorb_real"])
 def test_derived_channels_against_the_oracle(name):
     if name == "chain6":
         p, hloc, hk, bh, bv = M.chain(nlat=6, u=4.0, nk=8)
-    else:
+        p.lanc_ngfiter = 12
+    elif name == "bath3+3":
         p, hloc, hk, bh, bv = M.bath_model(nlat=3, nb=3)
-    p.lanc_ngfiter = 12
+        p.lanc_ngfiter = 12
+    else:
+        # the 2-site, 2-orbital model (Ust, Jh, spin-dependent levels) with
+        # real hopping; 4-step chains on its 24-state target sectors
+        p, hloc, hk, bh, bv = M.multiorb()
+        hloc = np.asfortranarray(hloc.real.astype(np.complex128))
+        hk = np.asfortranarray(hk.real.astype(np.complex128))
+        p.lanc_ngfiter = 4
     _say(f"{name}: solve with derived channels")
     s = VcaSolver(p)
     om = s.vca_solve(hloc, hk, bh, bv)
@@ -46,15 +63,16 @@ def test_derived_channels_against_the_oracle(name):
         go = R.gf_cluster(ref["gf"], z)
         assert np.abs(g - go).max() < 1e-8 * max(1, np.abs(go).max()), z
     assert abs(om - ref["sft_potential"]) < 1e-8 * abs(ref["sft_potential"])
-    # the derived channels' Lehmann data, element (1, 2)
-    ch_ref = ref["gf"].elems[(1, 2, 1, 1, 1)][0]
-    for ichan in (3, 4):
-        w, poles = s.gf_poles(1, 2, 1, 1, 1, 1, ichan)
-        wo, po = ch_ref[ichan - 1]
-        assert len(poles) == len(po) == 12
-        o, oo = np.argsort(poles), np.argsort(po)
-        assert np.abs(poles[o] - po[oo]).max() < 1e-9 * np.abs(po).max()
-        assert np.abs(w[o] - wo[oo]).max() < 1e-9 * np.abs(wo).max()
+    # the derived channels' Lehmann data
+    for key in ELEMS[name]:
+        ch_ref = ref["gf"].elems[key][0]
+        for ichan in (3, 4):
+            w, poles = s.gf_poles(*key, 1, ichan)
+            wo, po = ch_ref[ichan - 1]
+            assert len(poles) == len(po) == p.lanc_ngfiter, (key, ichan)
+            o, oo = np.argsort(poles), np.argsort(po)
+            assert np.abs(poles[o] - po[oo]).max() < 1e-9 * np.abs(po).max(), (key, ichan)
+            assert np.abs(w[o] - wo[oo]).max() < 1e-9 * np.abs(wo).max(), (key, ichan)
 
 
 @pytest.mark.parametrize("cap", ["0", None])
