@@ -45,7 +45,8 @@ on the sector's stream).  Reported beside it in the same JSON line:
 
     python bench.py [--steps K] [--warmup W] [--gpus 1]
                     [--only c4|hxv|omega|full|c3|c1|square3x4|c4_solve]
-                    [--c4-form plain|fp64|mixed] [--save F.npz] [--compare A.npz B.npz]
+                    [--c4-form plain|fp64|mixed] [--save F.npz] [--gs-residual R]
+                    [--compare A.npz B.npz]
                     [--no-full] [--no-c3] [--no-cpu]
 """
 from __future__ import annotations
@@ -283,18 +284,23 @@ C4_FORMS = {
 }
 
 
-def run_c4_solve(form, save=None):
+def run_c4_solve(form, save=None, gs_residual=None):
     """vca_solve of the 16-site bath cluster end to end: the Lanczos ground
     state of the 165,636,900-state sector (8,8), the 240 excitation chains of
     200 steps on the 147,232,800-state sectors (9,8) and (7,8), and Omega on
     the 16x16 k-grid, in one of C4_FORMS; with save, E0, Omega and G'_ij at 64
     Matsubara and 64 real-axis points go to an .npz for the comparison of
-    two forms (each form needs minutes, so they run as separate commands)."""
+    two forms (each form needs minutes, so they run as separate commands).
+    gs_residual sets VCA_GS_RESIDUAL: the FP64 ground-state Lanczos then also
+    stops on the Ritz residual, not on |dE| alone."""
     import numpy as np
     from vca_amd import configs
     from vca_amd.api import VcaSolver
 
     prec, env = C4_FORMS[form]
+    env = dict(env)
+    if gs_residual is not None:
+        env["VCA_GS_RESIDUAL"] = repr(float(gs_residual))
     os.environ.update(env)
     os.environ.setdefault("VCA_PROGRESS", "1")
     p, hloc, hk, bh, bv = configs.config_bath_2x4(nk=16, precision=prec)
@@ -648,6 +654,8 @@ def main():
     ap.add_argument("--c4-form", choices=sorted(C4_FORMS), default="mixed",
                     help="the form of the c4_solve run")
     ap.add_argument("--save", default=None, help="c4_solve: .npz of E0, Omega and G'")
+    ap.add_argument("--gs-residual", type=float, default=None,
+                    help="c4_solve: Ritz-residual bound of the FP64 ground state (VCA_GS_RESIDUAL)")
     ap.add_argument("--compare", nargs=2, metavar=("REF", "OTHER"), default=None,
                     help="the differences of two saved c4_solve runs (no GPU)")
     ap.add_argument("--only", choices=["all", "c4", "hxv", "omega", "full", "c3", "c1",
@@ -671,7 +679,7 @@ def main():
         print(json.dumps(compare_c4_solves(*args.compare)), flush=True)
         return
     if args.only == "c4_solve":
-        print(json.dumps({"c4_solve": run_c4_solve(args.c4_form, args.save)}), flush=True)
+        print(json.dumps({"c4_solve": run_c4_solve(args.c4_form, args.save, args.gs_residual)}), flush=True)
         return
     if args.only == "omega":
         print(json.dumps({"omega": omega_section(args, cpu)}), flush=True)

@@ -646,8 +646,24 @@ static bool use_real_gs(const Sector &sec) {
 // the noise floor of an FP32 run.  real: real-vector mode (start, if given,
 // holds dim reals); the eigenvector in sec.xvec is then expanded to complex
 // unless keep_real.
+// VCA_GS_RESIDUAL=r: the FP64 ground-state Lanczos also requires the Ritz
+// residual ||H x - E x|| (beta_{m+1} |y_m|, y the lowest eigenvector of T_m)
+// to be at most r max(1, |E|) before it stops.  The |dE| < lanc_tolerance rule
+// alone (the reference's, VCA_DIAG.f90:149-185) fixes E to ~tol but the vector
+// only to ~sqrt(tol / gap), and G' is first order in the vector.  Unset: off.
+static double gs_residual_tol() {
+  const char *e = std::getenv("VCA_GS_RESIDUAL");
+  if (!e || !*e) return 0.0;
+  char *end = nullptr;
+  const double r = std::strtod(e, &end);
+  if (end == e || *end != '\0' || !(r > 0.0) || !std::isfinite(r))
+    fail(VCA_ERR_ARG, "VCA_GS_RESIDUAL=%s: a positive number", e);
+  return r;
+}
+
 static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, bool rel_tol,
-                        Solver *s, int stall = 0, bool real = false, bool keep_real = false) {
+                        Solver *s, int stall = 0, bool real = false, bool keep_real = false,
+                        double rtol = 0.0) {
   niter = (int)std::min<int64_t>(niter, sec.dim);
   ensure_lanczos(sec, niter + 2);
   RealVecScope real_mode(sec, real);
@@ -693,7 +709,16 @@ static GsOut gs_lanczos(Sector &sec, double tol, int niter, const void *start, b
       } else {
         ++since_best;
       }
-      if (de < thr || be[j + 1] == 0.0 || (stall > 0 && mm >= 20 && since_best >= stall)) {
+      bool stop = de < thr || be[j + 1] == 0.0 || (stall > 0 && mm >= 20 && since_best >= stall);
+      if (stop && rtol > 0.0 && be[j + 1] != 0.0) {
+        // y_m = the first component of the reversed tridiagonal's eigenvector
+        std::vector<double> dr(mm), er(mm + 1, 0.0), w(mm), z0(mm);
+        for (int i = 0; i < mm; ++i) dr[i] = al[mm - 1 - i];
+        for (int i = 0; i + 1 < mm; ++i) er[i + 1] = be[mm - 1 - i];
+        tridiag_eig_first(mm, dr.data(), er.data(), w.data(), z0.data());
+        stop = be[j + 1] * std::fabs(z0[0]) <= rtol * std::max(1.0, std::fabs(e));
+      }
+      if (stop) {
         m = mm;
         e_m = e;
         conv = true;
@@ -741,7 +766,8 @@ static GsOut gs_mixed(Solver &s, Sector &sec32, Sector &sec64, double tol, int n
   launch_convert(sec32.xvec, VCA_MIXED, start64, VCA_FP64,
                  real ? (sec32.dim + 1) / 2 : sec32.dim, sec32.stream);
   VCA_HIP(hipStreamSynchronize(sec32.stream));
-  GsOut g64 = gs_lanczos(sec64, tol, niter, start64, false, &s, 0, real, false);
+  GsOut g64 = gs_lanczos(sec64, tol, niter, start64, false, &s, 0, real, false,
+                         gs_residual_tol());
   VCA_HIP(hipFree(start64));
   return g64;
 }
@@ -892,7 +918,7 @@ static void solve_sector(Solver &s, int nup, int ndw, int neigen, std::vector<do
   } else {
     auto s64 = make_sector(s, nup, ndw, VCA_FP64, t_worker_stream);
     g = gs_lanczos(*s64, p.lanc_tolerance, p.lanc_niter, nullptr, false, &s, 0,
-                   use_real_gs(*s64));
+                   use_real_gs(*s64), false, gs_residual_tol());
     VCA_HIP(hipMemcpyAsync(d, s64->xvec, sizeof(double2) * dim, hipMemcpyDeviceToDevice,
                            s64->stream));
     VCA_HIP(hipStreamSynchronize(s64->stream));
@@ -1250,6 +1276,7 @@ static bool es_add_sized(std::vector<KeptState> &list, const KeptState &st, int 
 static void diagonalize(Solver &s) {
   const vca_params &p = s.p;
   const bool ft = finite_t(p);
+  (void)gs_residual_tol();  // a bad value fails here, not in a sector worker
   free_states(s);
   std::vector<std::pair<int, int>> list;
   if (s.sectors.empty()) {
