@@ -463,20 +463,28 @@ def test_real_vector_ground_state(prec, form, monkeypatch):
         assert np.linalg.norm(r) < 1e-5, (name, np.linalg.norm(r))
 
 
-def test_real_vector_ground_state_in_a_solve(monkeypatch):
+@pytest.mark.parametrize("rtol", [None, "1e-11"])
+def test_real_vector_ground_state_in_a_solve(rtol, monkeypatch):
     """A whole solve whose ground state runs on real vectors, against the
     solve on complex vectors (the oracle's own solve of this 10-site cluster
     takes over 10 minutes on one core; the pipeline tests pin smaller ones
     to it).  The two ground-state runs start from different vectors, so
-    they agree to what the Lanczos stopping rule |dE| < 1e-12 fixes, not bit
-    for bit: E0 within 1e-10 relative, Omega within 1e-8 (1.1e-10 seen), and
-    G', first order in the ground-state vector that the rule fixes to ~1e-6
-    (the eigenvector error is ~sqrt of the energy error), within 2e-6 of its
-    largest element (5.9e-7 seen).  The chains themselves add only rounding
-    (test_real_vector_chains_in_a_solve: 1e-10).  The GPU's complex path
-    matches the oracle to 1e-8 in G' because both follow the same start
-    vector."""
+    under the Lanczos stopping rule |dE| < 1e-12 alone they agree to what
+    that rule fixes, not bit for bit: E0 within 1e-10 relative, Omega within
+    1e-8 (1.1e-10 seen), and G', first order in the ground-state vector that
+    the rule fixes to ~1e-6 (the eigenvector error is ~sqrt of the energy
+    error), within 2e-6 of its largest element (5.9e-7 seen).  With the
+    Ritz-residual stop as well (VCA_GS_RESIDUAL=1e-11) both runs converge
+    the vector, and they agree to E0 1e-13, Omega 1e-12 and G' 1e-9: the
+    2e-6 is the stopping rule's, not the real vectors'.  The chains
+    themselves add only rounding (test_real_vector_chains_in_a_solve:
+    1e-10).  The GPU's complex path matches the oracle to 1e-8 in G' because
+    both follow the same start vector."""
     monkeypatch.setenv("VCA_TFORM_MIN_DIM", "0")
+    if rtol is None:
+        monkeypatch.delenv("VCA_GS_RESIDUAL", raising=False)
+    else:
+        monkeypatch.setenv("VCA_GS_RESIDUAL", rtol)
     p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
     p.lanc_ngfiter = 80
     out = {}
@@ -487,10 +495,13 @@ def test_real_vector_ground_state_in_a_solve(monkeypatch):
         om_ = sc.vca_solve(hloc, hk)
         out[real] = (om_, sc.results()["omegaprime"], sc.vca_gf_cluster(0.3j),
                      sc.vca_gf_cluster(1.0 + 0.5j))
-    assert abs(out["1"][1] - out["0"][1]) <= 1e-10 * abs(out["0"][1])
-    assert abs(out["1"][0] - out["0"][0]) <= 1e-8 * abs(out["0"][0])
+    e_bar, om_bar, g_bar = (1e-10, 1e-8, 2e-6) if rtol is None else (1e-13, 1e-12, 1e-9)
+    assert abs(out["1"][1] - out["0"][1]) <= e_bar * abs(out["0"][1])
+    assert abs(out["1"][0] - out["0"][0]) <= om_bar * abs(out["0"][0])
     for x, y in zip(out["1"][2:], out["0"][2:]):
-        assert np.abs(x - y).max() <= 2e-6 * np.abs(y).max()
+        err = np.abs(x - y).max() / np.abs(y).max()
+        _say(f"  G' real against complex ground state: {err:.1e}")
+        assert err <= g_bar
 
 
 REAL_FORMS = {
