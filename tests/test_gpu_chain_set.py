@@ -14,6 +14,7 @@ import pytest
 
 from oracle import vca_ref as R
 from vca_amd import configs
+from vca_amd._lib import VcaError
 from vca_amd.api import VcaSolver
 
 from . import _models as M
@@ -123,13 +124,15 @@ def test_chain_set_graph_replays_match_eager(monkeypatch):
 
 
 def test_chain_set_row_and_column_items_agree(monkeypatch):
-    """The chain set's two H x v layouts -- row blocks, and whole columns
-    staged in LDS (the 10-site chain's target columns have 210 rows) -- and
-    the timed choice between them give the same solve to rounding."""
+    """The chain set's H x v layouts -- row blocks, and groups of 1, 2, 4 or
+    8 whole columns staged in LDS (the 10-site chain's target columns have
+    210 rows) -- and the timed choice among them give the same solve to
+    rounding (each row's sum is the same; the alpha partials are grouped
+    differently)."""
     p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
     p.lanc_ngfiter = 80
     out = {}
-    for mode in ("0", "1", None):
+    for mode in ("0", "1", "2", "4", "8", None):
         if mode is None:
             monkeypatch.delenv("VCA_CHAIN_SET_COLUMNS", raising=False)
         else:
@@ -137,7 +140,16 @@ def test_chain_set_row_and_column_items_agree(monkeypatch):
         _say(f"chain10: chain set, VCA_CHAIN_SET_COLUMNS={mode}")
         s, om = _solve(p, hloc, hk, None, None)
         out[mode] = (om, [s.vca_gf_cluster(z) for z in ZS])
-    for m in ("1", None):
+    for m in ("1", "2", "4", "8", None):
         assert abs(out[m][0] - out["0"][0]) <= 1e-12 * abs(out["0"][0])
         for x, y in zip(out[m][1], out["0"][1]):
             assert np.abs(x - y).max() <= 1e-10 * np.abs(y).max()
+
+
+@pytest.mark.parametrize("bad", ["9", "-1", "x", "2x"])
+def test_chain_set_layout_knob_refuses_bad_values(bad, monkeypatch):
+    p, hloc, hk, _, _ = M.chain(nlat=10, u=4.0, nk=4)
+    p.lanc_ngfiter = 16
+    monkeypatch.setenv("VCA_CHAIN_SET_COLUMNS", bad)
+    with pytest.raises(VcaError):
+        _solve(p, hloc, hk, None, None)

@@ -55,7 +55,7 @@ constexpr int kCsChunk = 16;   // Lanczos steps per captured graph
 struct CsSeg {
   RowTab tab;      // the target sector's tables
   int real;        // real vectors (real amplitudes)
-  int colmode;     // H x v items are whole columns staged in LDS (else row blocks)
+  int colmode;     // H x v items: colmode whole columns staged in LDS (0: row blocks)
   int64_t n;       // elements per chain: dim (complex or real)
   int64_t stride;  // elements between consecutive chains' vectors
   void *u0, *u1, *hv;
@@ -116,10 +116,23 @@ __host__ __device__ inline size_t cs_col_offset(const RowTab &t, size_t csz) {
   return (((size_t)(t.namp_up + t.namp_dw) * csz) + 15) & ~(size_t)15;
 }
 
-// Columns from this many rows to kCsColMax are staged whole (one workgroup
-// per column); shorter columns run as row blocks.
+// Columns from this many rows to kCsColMax are staged whole (a workgroup
+// per group of G consecutive columns of one chain, at most kCsGroupRows
+// rows together); shorter columns run as row blocks.  A workgroup's fixed
+// costs (descriptors, amplitude staging, the norm's partial sums, three
+// barriers) are as long as its gathers on columns of ~200 rows, so G > 1
+// amortises them.
 constexpr int kCsColMin = 128;
 constexpr int kCsColMax = 1024;
+constexpr int kCsGroupRows = 2048;
+
+// Columns per item of a segment whose columns have dimup rows, for a layout
+// of g columns per item (0: row blocks).
+__host__ __device__ inline int cs_group(int g, int dimup) {
+  if (g <= 0 || dimup < kCsColMin || dimup > kCsColMax) return 0;
+  const int cap = kCsGroupRows / dimup;
+  return g < cap ? g : (cap > 1 ? cap : 1);
+}
 
 // Launch A of step k: norm and done flag of u_k, then Hv = H u_k / |u_k| on
 // one block of rows (one row per lane: state r = iup + idw dimup) and the
@@ -158,45 +171,48 @@ __global__ __launch_bounds__(kCsBlock) void k_cs_hxv(const CsArgs a) {
   if (it.y == 0 && threadIdx.x == 0) a.dflag[2 * it.x + (k & 1)] = done ? 1 : 0;
   double dot = 0.0;
   if (sg.colmode) {
-    // item = column it.y, staged in LDS after the amplitude tables
-    const int idw = it.y;
+    // item = columns idw0 .. idw0 + ncol - 1 of the chain (consecutive in
+    // memory), staged in LDS after the amplitude tables; lane f of the
+    // flattened rows is row iup of staged column c
     const int dimup = sg.tab.dimup;
+    const int idw0 = it.y * sg.colmode;
+    const int nrow = min(sg.colmode, sg.tab.dimdw - idw0) * dimup;
     const int64_t coff = (int64_t)ch.local * sg.stride;
     const R scale = done ? (R)0 : (R)(1.0 / nrm);
     unsigned char *s_colb = cs_smem + cs_col_offset(sg.tab, sizeof(C));
-    const int64_t cbase = coff + (int64_t)idw * dimup;
+    const int64_t cbase = coff + (int64_t)idw0 * dimup;
     if (!done) {
       if (sg.real) {
         const R *vin = static_cast<const R *>((k & 1) ? sg.u1 : sg.u0);
         R *s_col = reinterpret_cast<R *>(s_colb);
-        for (int f = threadIdx.x; f < dimup; f += kCsBlock) s_col[f] = vin[cbase + f];
+        for (int f = threadIdx.x; f < nrow; f += kCsBlock) s_col[f] = vin[cbase + f];
       } else {
         const C *vin = static_cast<const C *>((k & 1) ? sg.u1 : sg.u0);
         C *s_col = reinterpret_cast<C *>(s_colb);
-        for (int f = threadIdx.x; f < dimup; f += kCsBlock) s_col[f] = vin[cbase + f];
+        for (int f = threadIdx.x; f < nrow; f += kCsBlock) s_col[f] = vin[cbase + f];
       }
     }
     __syncthreads();
     if (!done) {
-      const double e_dw = sg.tab.e_dw[idw];
-      const uint32_t s_dw = sg.tab.map_dw[idw];
-      const int nd = sg.tab.cnt_dw[idw];
-      for (int iup = threadIdx.x; iup < dimup; iup += kCsBlock) {
+      for (int f = threadIdx.x; f < nrow; f += kCsBlock) {
+        const int c = f / dimup;
+        const int iup = f - c * dimup, idw = idw0 + c;
         const RowMeta mu = sg.tab.meta_up[iup];
-        const double ed = mu.e + e_dw + rows::cross_e(a.cross, mu.s, s_dw);
+        const double ed = mu.e + sg.tab.e_dw[idw] + rows::cross_e(a.cross, mu.s, sg.tab.map_dw[idw]);
+        const int nd = sg.tab.cnt_dw[idw];
         if (sg.real) {
           const R *vin = static_cast<const R *>((k & 1) ? sg.u1 : sg.u0) + coff;
-          const R *s_col = reinterpret_cast<const R *>(s_colb);
+          const R *s_col = reinterpret_cast<const R *>(s_colb) + c * dimup;
           const R x = s_col[iup];
           R acc = (R)ed * x, unused = 0;
           rows::hops_col<R, true, R>(sg.tab, vin, s_col, iup, idw, mu.cnt, nd, s_amu, s_amd, acc,
                                      unused);
           const R out = acc * scale;
-          static_cast<R *>(sg.hv)[cbase + iup] = out;
+          static_cast<R *>(sg.hv)[cbase + f] = out;
           dot += (double)(x * scale) * (double)out;
         } else {
           const C *vin = static_cast<const C *>((k & 1) ? sg.u1 : sg.u0) + coff;
-          const C *s_col = reinterpret_cast<const C *>(s_colb);
+          const C *s_col = reinterpret_cast<const C *>(s_colb) + c * dimup;
           const C x = s_col[iup];
           R ar = (R)ed * x.x, ai = (R)ed * x.y;
           rows::hops_col<R, REAL_AMP, C>(sg.tab, vin, s_col, iup, idw, mu.cnt, nd, s_amu, s_amd,
@@ -204,7 +220,7 @@ __global__ __launch_bounds__(kCsBlock) void k_cs_hxv(const CsArgs a) {
           C out;
           out.x = ar * scale;
           out.y = ai * scale;
-          static_cast<C *>(sg.hv)[cbase + iup] = out;
+          static_cast<C *>(sg.hv)[cbase + f] = out;
           dot += (double)(x.x * scale) * (double)out.x + (double)(x.y * scale) * (double)out.y;
         }
       }
@@ -373,6 +389,17 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
   const int nch = (int)jobs.size();
   out.assign(nch, ChainOut{});
   if (nch == 0) return stats;
+  // VCA_CHAIN_SET_COLUMNS: the H x v layout forced (checked before anything
+  // is allocated)
+  const char *cm = std::getenv("VCA_CHAIN_SET_COLUMNS");
+  int force = -1;
+  if (cm && *cm) {
+    char *end = nullptr;
+    const long v = std::strtol(cm, &end, 10);
+    if (*end != '\0' || v < 0 || v > 8)
+      fail(VCA_ERR_ARG, "VCA_CHAIN_SET_COLUMNS=%s: 0 (row blocks) or 1..8 columns per item", cm);
+    force = (int)v;
+  }
   const size_t rsz = prec == VCA_FP64 ? sizeof(double) : sizeof(float);
   // segments: the chains of one (target sector, vector kind), in job order
   std::map<std::pair<const Sector *, bool>, int> seg_of;
@@ -427,10 +454,11 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
     }
   }
   const std::vector<int2> items_u = cs_items(nblk_u);
-  // Two layouts of the H x v items: row blocks everywhere, or whole
+  // Layouts of the H x v items: row blocks everywhere, or groups of g whole
   // columns staged in LDS where the sector's columns have kCsColMin ..
-  // kCsColMax rows.  Both are timed on the set's first step
-  // (VCA_CHAIN_SET_COLUMNS=0/1 forces one).
+  // kCsColMax rows (g = 1, 2, 4, 8, fewer where g columns exceed
+  // kCsGroupRows rows).  All are timed on the set's first step
+  // (VCA_CHAIN_SET_COLUMNS=0 forces row blocks, =g groups of g columns).
   struct Layout {
     std::vector<CsSeg> segs;
     std::vector<CsChain> chains;
@@ -441,17 +469,17 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
     CsChain *d_chains = nullptr;
     int2 *d_items = nullptr;
   };
-  auto make_layout = [&](bool cols) {
+  auto make_layout = [&](int g) {
     Layout L;
     L.segs = segs;
     L.chains = chains;
-    for (CsSeg &sgm : L.segs)
-      sgm.colmode = cols && sgm.tab.dimup >= kCsColMin && sgm.tab.dimup <= kCsColMax ? 1 : 0;
+    for (CsSeg &sgm : L.segs) sgm.colmode = cs_group(g, sgm.tab.dimup);
     std::vector<int> nb(nch);
     for (int c = 0; c < nch; ++c) {
       CsChain &ch = L.chains[c];
       const CsSeg &sgm = L.segs[ch.seg];
-      ch.nblk_h = sgm.colmode ? sgm.tab.dimdw : (int)((sgm.n + kCsBlock - 1) / kCsBlock);
+      ch.nblk_h = sgm.colmode ? (sgm.tab.dimdw + sgm.colmode - 1) / sgm.colmode
+                              : (int)((sgm.n + kCsBlock - 1) / kCsBlock);
       ch.pofs_h = L.np;
       L.np += ch.nblk_h;
       nb[c] = ch.nblk_h;
@@ -460,7 +488,8 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
     for (const CsSeg &sgm : L.segs) {
       const size_t amps = (size_t)(sgm.tab.namp_up + sgm.tab.namp_dw) * 2 * rsz;
       const size_t col = sgm.colmode ? cs_col_offset(sgm.tab, 2 * rsz) +
-                                           (size_t)sgm.tab.dimup * (sgm.real ? rsz : 2 * rsz)
+                                           (size_t)sgm.colmode * sgm.tab.dimup *
+                                               (sgm.real ? rsz : 2 * rsz)
                                      : 0;
       L.smem = std::max(L.smem, std::max(amps, col));
     }
@@ -469,16 +498,14 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
   bool any_cols = false;
   for (const CsSeg &sgm : segs)
     any_cols = any_cols || (sgm.tab.dimup >= kCsColMin && sgm.tab.dimup <= kCsColMax);
-  const char *cm = std::getenv("VCA_CHAIN_SET_COLUMNS");
-  const bool force = cm && (cm[0] == '0' || cm[0] == '1');
   std::vector<Layout> lay;
-  if (!any_cols || (force && cm[0] == '0')) {
-    lay.push_back(make_layout(false));
-  } else if (force) {
-    lay.push_back(make_layout(true));
+  if (!any_cols || force == 0) {
+    lay.push_back(make_layout(0));
+  } else if (force > 0) {
+    lay.push_back(make_layout(force));
   } else {
-    lay.push_back(make_layout(true));
-    lay.push_back(make_layout(false));
+    for (int g : {4, 1, 2, 8}) lay.push_back(make_layout(g));
+    lay.push_back(make_layout(0));
   }
   for (Layout &L : lay) {
     ph = std::max(ph, L.np);
@@ -602,6 +629,7 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
       VCA_HIP(hipEventCreate(&t0));
       VCA_HIP(hipEventCreate(&t1));
       float best = std::numeric_limits<float>::max();
+      const char *verbose = std::getenv("VCA_TUNE_VERBOSE");
       for (size_t li = 0; li < lay.size(); ++li) {
         launch_hxv_step(lay[li], 0);
         float mn = std::numeric_limits<float>::max();
@@ -613,6 +641,12 @@ ChainSetStats run_chain_set(const std::vector<ChainJob> &jobs, int prec, int dim
           float ms_ = 0;
           VCA_HIP(hipEventElapsedTime(&ms_, t0, t1));
           mn = std::min(mn, ms_);
+        }
+        if (verbose && verbose[0] == '1') {
+          int g = 0;
+          for (const CsSeg &sgm : lay[li].segs) g = std::max(g, sgm.colmode);
+          std::fprintf(stderr, "vca chain set: layout %s%d: %.1f us\n", g ? "columns x" : "rows",
+                       g, mn * 1e3f);
         }
         if (mn < best) {
           best = mn;
